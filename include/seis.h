@@ -1,0 +1,113 @@
+/*
+ * seis.h -- C ABI of the MI355X (gfx950) 2D acoustic finite-difference engine.
+ *
+ * Drop-in boundary for the reference's engine entry points (bbsun/Seis
+ * src/util/fd.h:3-5): forward shot modeling, Born modeling and RTM, with the
+ * reference's arguments, array layouts and results, plus batched variants that
+ * take device-resident buffers and run many shots per launch.
+ *
+ * Layouts (reference memory.h:29-44, fd.cpp:17-18): a 2D array a[i2][i1] is
+ * contiguous with i1 fastest.  Models and images are [nx][nz] (z fastest),
+ * records [nx][nt] (t fastest), the wavelet has nt + ndel samples.  Grid
+ * indices (sx, sz, gz[]) are cells of the unpadded model; the absorbing band
+ * adds pml cells on every side.
+ *
+ * Every call returns SEIS_OK (0) or an error code; seis_last_error() gives the
+ * message of the calling thread's last failure.  The reference exits with
+ * status 0 on errors (debug.cpp:5-12); this ABI never exits.
+ */
+#ifndef SEIS_H
+#define SEIS_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SEIS_OK 0
+#define SEIS_ERR_ARG 1    /* invalid argument */
+#define SEIS_ERR_HIP 2    /* HIP runtime failure */
+#define SEIS_ERR_NODEV 3  /* no usable GPU */
+
+/* Model grid shared by every shot of a batch: nx columns, nz rows, pml
+ * absorbing cells per side, time step dt and spacings dx, dz (fd.h:3). */
+typedef struct seis_grid {
+  int nx, nz, pml;
+  float dt, dx, dz;
+} seis_grid;
+
+/* One shot of a batch.  sx, sz: source cell (fd.h:3 sx, sz); nt: record
+ * length; ndel: pre-roll steps before recording starts (fd.h:3 ndel).
+ * model / wav / gz / rec: element offsets of this shot's [nx][nz] model
+ * (and perturbation), wavelet (nt + ndel samples), receiver-depth row (nx
+ * entries, one receiver per column as in fd.cpp:58-63) and [nx][nt] record
+ * inside the batch arrays. */
+typedef struct seis_shot {
+  int sx, sz, nt, ndel;
+  long long model, wav, gz, rec;
+} seis_shot;
+
+/* Message of the calling thread's last error ("" if none). */
+const char* seis_last_error(void);
+
+/* Device management and device buffers (for device-resident batches). */
+int seis_device_count(int* count);
+int seis_set_device(int device);
+int seis_malloc(void** ptr, size_t bytes);
+int seis_free(void* ptr);
+int seis_memcpy_htod(void* dst, const void* src, size_t bytes);
+int seis_memcpy_dtoh(void* dst, const void* src, size_t bytes);
+int seis_memset(void* dst, int value, size_t bytes);
+int seis_synchronize(void);
+
+/* Forward shot modeling; replaces `modeling` (fd.h:3, fd.cpp:15-80).
+ * Host buffers: gz[nx], wav[nt+ndel], vel[nx][nz] in; rec[nx][nt] out
+ * (caller-allocated; the reference returns a new MyAlloc array instead). */
+int seis_modeling(float dt, float dx, float dz, int nt, int ndel, int nx, int nz, int pml, int sx, int sz,
+                  const int* gz, const float* wav, const float* vel, float* rec);
+
+/* Born modeling; replaces `forward` (fd.h:4, fd.cpp:81-177).
+ * dv[nx][nz] is the velocity perturbation; rec[nx][nt] the scattered record. */
+int seis_born(float dt, float dx, float dz, int nt, int ndel, int nx, int nz, int pml, int sx, int sz,
+              const int* gz, const float* wav, const float* vel, const float* dv, float* rec);
+
+/* RTM image with the reference's imaging condition; replaces `adjoint`
+ * (fd.h:5, fd.cpp:592-719).  rec[nx][nt] in, img[nx][nz] out. */
+int seis_rtm(float dt, float dx, float dz, int nt, int ndel, int nx, int nz, int pml, int sx, int sz,
+             const int* gz, const float* wav, const float* vel, const float* rec, float* img);
+
+/* Batched variants: all array arguments are DEVICE pointers (seis_malloc);
+ * shots[] is a host array.  Shots run concurrently in the same launches;
+ * each produces exactly what the single-shot call produces for it.
+ * Replace the per-shot worker loops of modeling_MPI / forward_MPI /
+ * adjoint_MPI (lsrtm/inversion.cpp:141-206, 475-551, 365-474). */
+int seis_modeling_batch(const seis_grid* grid, int nshots, const seis_shot* shots, const float* vel,
+                        const float* wav, const int* gz, float* rec);
+int seis_born_batch(const seis_grid* grid, int nshots, const seis_shot* shots, const float* vel,
+                    const float* dv, const float* wav, const int* gz, float* rec);
+/* img: nshots consecutive [nx][nz] images, one per shot (the per-shot
+ * CSG_IMAGE<is> of inversion.cpp:434-439). */
+int seis_rtm_batch(const seis_grid* grid, int nshots, const seis_shot* shots, const float* vel,
+                   const float* wav, const int* gz, const float* rec, float* img);
+
+/* Kernel timing: instrumentation with no counterpart in the reference.
+ * Between seis_profile_start and seis_profile_stop every engine launch is
+ * bracketed by HIP events on its own stream; stop returns, per kernel, the
+ * number of launches, the summed event durations, and the algorithmic HBM
+ * bytes and wavefield cell-updates those launches performed. */
+typedef struct seis_kernel_stat {
+  char name[40];
+  int launches;
+  double total_ms;
+  double alg_bytes;
+  double cell_updates;
+} seis_kernel_stat;
+int seis_profile_start(void);
+int seis_profile_stop(seis_kernel_stat* stats, int max_stats, int* n_stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SEIS_H */

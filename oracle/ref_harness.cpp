@@ -1,0 +1,93 @@
+// C-callable harness around the compiled reference engine.
+//
+// TEST INFRASTRUCTURE ONLY.  Built by oracle/Makefile (target `ref`) against
+// the reference's own sources, which stay where they lie under the reference
+// tree; the result lands in oracle/_ref/ and never enters version control.
+// It exists to generate golden vectors (tests/golden/make_golden.py) and to
+// serve as the timed "reference" CPU baseline.  It converts flat arrays
+// to the reference's row-pointer tables (memory.h:29-44), calls
+// modeling / forward / adjoint (fd.h:3-5) and copies the newly allocated
+// result back, freeing it the reference's way (memory.h:103-110).
+
+#include <cstring>
+#include <cstddef>
+#include <math.h>  // hilbert.h uses cos() without including it (fd.cpp includes math.h first)
+#include "memory.h"
+#include "fd.h"
+#include "wavelet.h"
+#include "global.h"
+#include "InitInversion.h"
+#include "../filter/hilbert.h"
+
+namespace {
+
+// View a flat [n2][n1] block as the reference's float** table.
+struct Rows {
+  float** p;
+  Rows(const float* base, size_t n1, size_t n2) {
+    p = new float*[n2];
+    for (size_t i = 0; i < n2; i++) p[i] = const_cast<float*>(base) + i * n1;
+  }
+  ~Rows() { delete[] p; }
+};
+
+void take(float** out, float* dst, size_t n) {
+  if (out) {
+    std::memcpy(dst, out[0], n * sizeof(float));
+    MyAlloc<float>::free(out);
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+void ref_set_threads(int n) { OMP_CORE = n > 0 ? n : 1; }
+
+void ref_ricker(float dt, float fr, int num_delay, int n, float* out) {
+  float* w = rickerWavelet(dt, fr, num_delay, n);
+  std::memcpy(out, w, sizeof(float) * n);
+  MyAlloc<float>::free(w);
+}
+
+void ref_hilbert_apply(int half, const float* in, float* out, int n) {
+  Hilbert<float> h(half);
+  h.apply(const_cast<float*>(in), out, n);
+}
+
+void ref_set_abs(int nz, int nx, int pml, float* w) {
+  take(setAbs(nz, nx, pml), w, (size_t)(nz + 2 * pml) * (nx + 2 * pml));
+}
+
+void ref_set_vel(int nz, int nx, int pml, const float* v, float* vp) {
+  Rows r(v, nz, nx);
+  take(setVel(r.p, nz, nx, pml), vp, (size_t)(nz + 2 * pml) * (nx + 2 * pml));
+}
+
+int ref_modeling(float dt, float dx, float dz, int nt, int ndel, int nx, int nz, int pml, int sx, int sz,
+                 const int* gz, const float* wav, const float* vel, float* rec) {
+  Rows v(vel, nz, nx);
+  take(modeling(dt, dx, dz, nt, ndel, nx, nz, pml, sx, sz, const_cast<int*>(gz), const_cast<float*>(wav), v.p),
+       rec, (size_t)nt * nx);
+  return 0;
+}
+
+int ref_born(float dt, float dx, float dz, int nt, int ndel, int nx, int nz, int pml, int sx, int sz,
+             const int* gz, const float* wav, const float* vel, const float* dv, float* rec) {
+  Rows v(vel, nz, nx), d(dv, nz, nx);
+  take(forward(dt, dx, dz, nt, ndel, nx, nz, pml, sx, sz, const_cast<int*>(gz), const_cast<float*>(wav), v.p,
+               d.p),
+       rec, (size_t)nt * nx);
+  return 0;
+}
+
+int ref_rtm(float dt, float dx, float dz, int nt, int ndel, int nx, int nz, int pml, int sx, int sz,
+            const int* gz, const float* wav, const float* vel, const float* rec, float* img) {
+  Rows v(vel, nz, nx), r(rec, nt, nx);
+  take(adjoint(dt, dx, dz, nt, ndel, nx, nz, pml, sx, sz, const_cast<int*>(gz), const_cast<float*>(wav), v.p,
+               r.p),
+       img, (size_t)nz * nx);
+  return 0;
+}
+
+}  // extern "C"

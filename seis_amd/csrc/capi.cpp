@@ -1,0 +1,186 @@
+// C ABI of the engine (include/seis.h): argument checking, host<->device
+// staging for the single-shot entry points, and dispatch to the batched
+// runners.  No CPU fallback exists: without a GPU every compute entry point
+// returns SEIS_ERR_NODEV or SEIS_ERR_HIP.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "seis_internal.h"
+
+namespace seis {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+namespace {
+
+int hip_fail(const char* what, hipError_t e) {
+  set_error(std::string(what) + ": " + hipGetErrorString(e));
+  return e == hipErrorNoDevice ? SEIS_ERR_NODEV : SEIS_ERR_HIP;
+}
+
+// Device buffer owned for the duration of a single-shot call.
+struct DevBuf {
+  void* p = nullptr;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+int upload(DevBuf& b, const void* src, size_t bytes) {
+  hipError_t e = hipMalloc(&b.p, bytes ? bytes : 1);
+  if (e != hipSuccess) return hip_fail("hipMalloc", e);
+  if (bytes) {
+    e = hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail("hipMemcpy", e);
+  }
+  return SEIS_OK;
+}
+
+int alloc(DevBuf& b, size_t bytes) {
+  hipError_t e = hipMalloc(&b.p, bytes ? bytes : 1);
+  if (e != hipSuccess) return hip_fail("hipMalloc", e);
+  e = hipMemset(b.p, 0, bytes ? bytes : 1);
+  if (e != hipSuccess) return hip_fail("hipMemset", e);
+  return SEIS_OK;
+}
+
+int check_single(int nt, int ndel, int nx, int nz, int pml, int sx, int sz, const int* gz, const void* wav,
+                 const void* vel, const void* out) {
+  if (!gz || !wav || !vel || !out) return set_error("null array argument"), SEIS_ERR_ARG;
+  if (nt <= 0 || ndel < 0 || nx <= 0 || nz <= 0 || pml < 0)
+    return set_error("nt, nx, nz must be > 0; ndel, pml >= 0"), SEIS_ERR_ARG;
+  if (sx < 0 || sx >= nx || sz < 0 || sz >= nz) return set_error("source cell outside the model"), SEIS_ERR_ARG;
+  for (int i = 0; i < nx; i++)
+    if (gz[i] < 0 || gz[i] >= nz) return set_error("receiver depth gz outside the model"), SEIS_ERR_ARG;
+  return SEIS_OK;
+}
+
+int check_batch(const seis_grid* g, int nshots, const seis_shot* shots) {
+  if (!g) return set_error("null grid"), SEIS_ERR_ARG;
+  if (nshots < 0 || (nshots > 0 && !shots)) return set_error("bad shot list"), SEIS_ERR_ARG;
+  return SEIS_OK;
+}
+
+int single(int mode, float dt, float dx, float dz, int nt, int ndel, int nx, int nz, int pml, int sx, int sz,
+           const int* gz, const float* wav, const float* vel, const float* extra, float* out) {
+  int rc = check_single(nt, ndel, nx, nz, pml, sx, sz, gz, wav, vel, out);
+  if (rc) return rc;
+  if (mode != RUN_MODEL && !extra) return set_error("null array argument"), SEIS_ERR_ARG;
+  const size_t nm = (size_t)nx * nz, nr = (size_t)nx * nt;
+  DevBuf dgz, dwav, dvel, dextra, drec, dimg;
+  if ((rc = upload(dgz, gz, sizeof(int) * nx))) return rc;
+  if ((rc = upload(dwav, wav, sizeof(float) * (nt + ndel)))) return rc;
+  if ((rc = upload(dvel, vel, sizeof(float) * nm))) return rc;
+  if (mode == RUN_BORN && (rc = upload(dextra, extra, sizeof(float) * nm))) return rc;
+  if (mode == RUN_RTM) {
+    if ((rc = upload(drec, extra, sizeof(float) * nr))) return rc;
+    if ((rc = alloc(dimg, sizeof(float) * nm))) return rc;
+  } else if ((rc = alloc(drec, sizeof(float) * nr))) {
+    return rc;
+  }
+  seis_grid g{nx, nz, pml, dt, dx, dz};
+  seis_shot s{sx, sz, nt, ndel, 0, 0, 0, 0};
+  rc = run_batch<float>(mode, g, 1, &s, (const float*)dvel.p, (const float*)dextra.p, (const float*)dwav.p,
+                        (const int*)dgz.p, (float*)drec.p, (float*)dimg.p, 0);
+  if (rc) return rc;
+  hipError_t e = mode == RUN_RTM ? hipMemcpy(out, dimg.p, sizeof(float) * nm, hipMemcpyDeviceToHost)
+                                 : hipMemcpy(out, drec.p, sizeof(float) * nr, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_fail("hipMemcpy", e);
+  return SEIS_OK;
+}
+
+}  // namespace
+}  // namespace seis
+
+using namespace seis;
+
+extern "C" {
+
+const char* seis_last_error(void) { return g_last_error.c_str(); }
+
+int seis_device_count(int* count) {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (count) *count = (e == hipSuccess) ? n : 0;
+  if (e != hipSuccess) return hip_fail("hipGetDeviceCount", e);
+  return SEIS_OK;
+}
+
+int seis_set_device(int device) {
+  hipError_t e = hipSetDevice(device);
+  return e == hipSuccess ? SEIS_OK : hip_fail("hipSetDevice", e);
+}
+
+int seis_malloc(void** ptr, size_t bytes) {
+  if (!ptr) return set_error("null pointer"), SEIS_ERR_ARG;
+  hipError_t e = hipMalloc(ptr, bytes ? bytes : 1);
+  return e == hipSuccess ? SEIS_OK : hip_fail("hipMalloc", e);
+}
+
+int seis_free(void* ptr) {
+  hipError_t e = hipFree(ptr);
+  return e == hipSuccess ? SEIS_OK : hip_fail("hipFree", e);
+}
+
+int seis_memcpy_htod(void* dst, const void* src, size_t bytes) {
+  hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+  return e == hipSuccess ? SEIS_OK : hip_fail("hipMemcpy", e);
+}
+
+int seis_memcpy_dtoh(void* dst, const void* src, size_t bytes) {
+  hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
+  return e == hipSuccess ? SEIS_OK : hip_fail("hipMemcpy", e);
+}
+
+int seis_memset(void* dst, int value, size_t bytes) {
+  hipError_t e = hipMemset(dst, value, bytes);
+  return e == hipSuccess ? SEIS_OK : hip_fail("hipMemset", e);
+}
+
+int seis_synchronize(void) {
+  hipError_t e = hipDeviceSynchronize();
+  return e == hipSuccess ? SEIS_OK : hip_fail("hipDeviceSynchronize", e);
+}
+
+int seis_modeling(float dt, float dx, float dz, int nt, int ndel, int nx, int nz, int pml, int sx, int sz,
+                  const int* gz, const float* wav, const float* vel, float* rec) {
+  return single(RUN_MODEL, dt, dx, dz, nt, ndel, nx, nz, pml, sx, sz, gz, wav, vel, nullptr, rec);
+}
+
+int seis_born(float dt, float dx, float dz, int nt, int ndel, int nx, int nz, int pml, int sx, int sz,
+              const int* gz, const float* wav, const float* vel, const float* dv, float* rec) {
+  return single(RUN_BORN, dt, dx, dz, nt, ndel, nx, nz, pml, sx, sz, gz, wav, vel, dv, rec);
+}
+
+int seis_rtm(float dt, float dx, float dz, int nt, int ndel, int nx, int nz, int pml, int sx, int sz,
+             const int* gz, const float* wav, const float* vel, const float* rec, float* img) {
+  return single(RUN_RTM, dt, dx, dz, nt, ndel, nx, nz, pml, sx, sz, gz, wav, vel, rec, img);
+}
+
+int seis_modeling_batch(const seis_grid* grid, int nshots, const seis_shot* shots, const float* vel,
+                        const float* wav, const int* gz, float* rec) {
+  int rc = check_batch(grid, nshots, shots);
+  if (rc) return rc;
+  return run_batch<float>(RUN_MODEL, *grid, nshots, shots, vel, nullptr, wav, gz, rec, nullptr, 0);
+}
+
+int seis_born_batch(const seis_grid* grid, int nshots, const seis_shot* shots, const float* vel,
+                    const float* dv, const float* wav, const int* gz, float* rec) {
+  int rc = check_batch(grid, nshots, shots);
+  if (rc) return rc;
+  return run_batch<float>(RUN_BORN, *grid, nshots, shots, vel, dv, wav, gz, rec, nullptr, 0);
+}
+
+int seis_rtm_batch(const seis_grid* grid, int nshots, const seis_shot* shots, const float* vel,
+                   const float* wav, const int* gz, const float* rec, float* img) {
+  int rc = check_batch(grid, nshots, shots);
+  if (rc) return rc;
+  return run_batch<float>(RUN_RTM, *grid, nshots, shots, vel, nullptr, wav, gz, const_cast<float*>(rec), img, 0);
+}
+
+}  // extern "C"

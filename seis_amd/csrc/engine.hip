@@ -1,0 +1,710 @@
+// HIP engine for the 2D acoustic finite-difference path on gfx950:
+// forward modeling, Born modeling and reference-semantics RTM, batched over
+// shots.  One launch per time step advances every shot of the batch; the
+// stencil, absorbing boundary, sponge blend, source/receiver injection,
+// receiver extraction, Born source, boundary-strip save/restore and the
+// pointwise imaging term are fused into that launch (see fd_cell.h).  The
+// Hilbert imaging term of the reference RTM runs as a second, column-wise
+// kernel per reverse step.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "fd_cell.h"
+#include "prof.h"
+#include "seis_internal.h"
+
+namespace seis {
+
+// ---------------------------------------------------------------------------
+// Device-side shot descriptor.
+template <typename T>
+struct DevShot {
+  const T* vel;    // [nx][nz]
+  const T* dv;     // [nx][nz] (Born) or null
+  const T* wav;    // [nt+ndel]
+  const int* gz;   // [nx]
+  T* rec;          // [nx][nt]
+  T* fld;          // slot 0 grid origin; slot j at fld + j*fstride
+  T* strips;       // [nt][8*(nx+nz)] (RTM)
+  T* img;          // padded image accumulator, grid origin (RTM)
+  T* aux;          // padded scratch field, grid origin (RTM: lap*2/v^3)
+  long long fstride;
+  int nx, NX, nt, ndel, sx, sz;
+};
+
+enum Mode { MODEL = 0, BORN = 1, RTM_FWD = 2, RTM_BWD = 3 };
+
+// Born source added to the scattered field (fd.cpp:134-139): vt2 * lap * 2/v^3 * dv,
+// lap = (u_new + u_prev - 2 u_cur) / dt^2 of the background.  dv is zero in the
+// padding (setDv), where the new background equals its stencil value.
+template <typename T>
+struct BornInject {
+  const Consts<T>* k;
+  const Model<T>* m;
+  const T* dv;
+  const T* Cb;
+  const T* Pb;
+  PointSource<T> src;
+  int NX;
+  long long own;  // field index whose background value is already known
+  T own_nb;
+  __device__ __forceinline__ T add(T val, int x, int z, T vt2, long long i) const {
+    const int pml = k->pml;
+    const int ix = x - pml, iz = z - pml;
+    if (ix < 0 || ix >= m->nx || iz < 0 || iz >= m->nz) return val;
+    const T d = dv[(long long)ix * m->nz + iz];
+    if (d == T(0)) return val;
+    const T nb = (i == own) ? own_nb : raw_value(*k, *m, NX, Cb, Pb, src, x, z);
+    const T lap = (nb + Pb[i] - T(2) * Cb[i]) * k->idt2;
+    const T v = m->vel(x, z);
+    const T v3 = T(2) / (v * v * v);
+    return val + vt2 * lap * v3 * d;
+  }
+  __device__ __forceinline__ T prev(const T* P, long long i) const { return P[i]; }
+};
+
+// Indices of (x, z) in the boundary-strip store of one time sample
+// (fd.cpp:1181-1225, layer = 4): top [nx][4], bottom [nx][4], left [4][nz],
+// right [4][nz].  On grids narrower than 8 cells a cell can sit in several
+// strips; every slot is returned so that save and restore stay consistent.
+__device__ __forceinline__ int strip_slots(int x, int z, int nx, int nz, int pml, int* s) {
+  const int ix = x - pml, iz = z - pml;
+  if (ix < 0 || ix >= nx || iz < 0 || iz >= nz) return 0;
+  int n = 0;
+  if (iz < 4) s[n++] = ix * 4 + iz;
+  if (iz >= nz - 4) s[n++] = nx * 4 + ix * 4 + (nz - 1 - iz);
+  if (ix < 4) s[n++] = 8 * nx + ix * nz + iz;
+  if (ix >= nx - 4) s[n++] = 8 * nx + 4 * nz + (nx - 1 - ix) * nz + iz;
+  return n;
+}
+
+// Per-step, per-shot state shared by both paths of the step kernel.
+template <typename T>
+struct StepCtx {
+  const T* P;       // previous level (slot 0 of the first field)
+  const T* C;       // current level
+  T* N;             // new level
+  Model<T> m;
+  int it, itp;
+  long long src;    // field index of the source cell
+};
+
+// Generic update of one cell: any cell of the padded grid, including the
+// absorbing band and the outer ring, through the pure per-cell functions of
+// fd_cell.h (neighbouring stencil values are recomputed, not read).
+template <typename T, int MODE>
+__device__ __forceinline__ void cell_update(const Consts<T>& k, const DevShot<T>& s, const StepCtx<T>& c, int x,
+                                            int z) {
+  const int NX = s.NX;
+  const long long ld = k.ld;
+  const long long i = fidx(ld, x, z);
+  const int it = c.it, itp = c.itp;
+  const T* P = c.P;
+  const T* C = c.C;
+  T* N = c.N;
+  const Model<T>& m = c.m;
+  if (MODE == MODEL || MODE == RTM_FWD) {
+    PointSource<T> ps{it - 1 >= 0 ? c.src : -1, it - 1 >= 0 ? s.wav[it - 1] : T(0)};
+    const T u = final_value(k, m, NX, C, P, ps, x, z);
+    N[i] = u;
+    if (MODE == MODEL) {
+      const int ix = x - k.pml;
+      if (itp >= 0 && ix >= 0 && ix < s.nx && z == s.gz[ix] + k.pml) s.rec[(long long)ix * s.nt + itp] = u;
+    } else if (itp >= 0) {
+      int sl[4];
+      const int n = strip_slots(x, z, s.nx, k.nz, k.pml, sl);
+      T* st = s.strips + (long long)itp * 8 * (s.nx + k.nz);
+      for (int q = 0; q < n; q++) st[sl[q]] = u;
+    }
+  } else if (MODE == BORN) {
+    PointSource<T> ps{it - 1 >= 0 ? c.src : -1, it - 1 >= 0 ? s.wav[it - 1] : T(0)};
+    T nb_raw;
+    const T ub = final_value(k, m, NX, C, P, ps, x, z, &nb_raw);
+    const T* Ps = P + 3 * s.fstride;
+    const T* Cs = C + 3 * s.fstride;
+    T* Ns = N + 3 * s.fstride;
+    BornInject<T> bi{&k, &m, s.dv, C, P, ps, NX, i, nb_raw};
+    const T us = final_value(k, m, NX, Cs, Ps, bi, x, z);
+    N[i] = ub;
+    Ns[i] = us;
+    const int ix = x - k.pml;
+    if (itp >= 0 && ix >= 0 && ix < s.nx && z == s.gz[ix] + k.pml) s.rec[(long long)ix * s.nt + itp] = us;
+  } else {  // RTM_BWD
+    const bool on = it - 1 > 0;
+    const T v = m.vel(x, z);
+    T sub = T(0);
+    if (on) {
+      const T as = k.dt * m.vel(s.sx + k.pml, s.sz + k.pml);
+      sub = (as * as) * s.wav[it - 1];
+    }
+    SourceRemoval<T> sr{c.src, sub, on};
+    const T u = final_value(k, m, NX, C, P, sr, x, z);
+    const T lap = (sr.prev(P, i) + u - T(2) * C[i]) * k.idt2;
+    const T* Pr = P + 3 * s.fstride;
+    const T* Cr = C + 3 * s.fstride;
+    T* Nr = N + 3 * s.fstride;
+    ReceiverInject<T> ri{s.gz, s.rec, s.nx, s.nt, itp, k.pml};
+    const T ur = final_value(k, m, NX, Cr, Pr, ri, x, z);
+    const T v3 = T(2) / (v * v * v);
+    const T U = lap * v3;
+    s.img[i] += U * ur;
+    s.aux[i] = U;
+    Nr[i] = ur;
+    // the current level of the next reverse step gets the saved strips
+    // restored (fd.cpp:663); lap above used the value before restoration
+    T out = u;
+    if (itp - 1 >= 0) {
+      int sl[4];
+      if (strip_slots(x, z, s.nx, k.nz, k.pml, sl) > 0)
+        out = s.strips[(long long)(itp - 1) * 8 * (s.nx + k.nz) + sl[0]];
+    }
+    N[i] = out;
+  }
+}
+
+// Wavefront-local LDS exchange barrier (the hardware orders a wave's LDS
+// operations; the fences keep the compiler from reordering across it).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+constexpr int WAVES = 4;     // wavefronts per workgroup
+constexpr int WG = 64 * WAVES;
+constexpr int LROW = 72;     // one LDS row: 64 cells + 4 halo cells on each side
+
+// Batch-uniform split of the padded grid: the interior rectangle
+// [xa, xb) x [za, zb) holds only cells with w = 1 and the 9-point stencil;
+// it is swept by wavefronts (64 z cells each) streaming along x.  Every
+// other cell is a "band" cell updated by cell_update.
+struct Layout {
+  int xa, xb, za, zb;
+  int nzc, nxs, bx;   // z chunks, x strips, strip length of the interior sweep
+  int nfwg;           // workgroups per shot for the interior sweep
+  long long nband;    // band cells per shot
+  int nbwg;           // workgroups per shot for the band cells
+};
+
+// Stencil sums of the 9-point operator in the reference's order (fd.cpp:1263-1269).
+template <typename T>
+__device__ __forceinline__ T d2_z(const Consts<T>& k, const T* r) {
+  T u = r[0] * k.c[4];
+  u = u + r[1] * k.c[3];
+  u = u + r[2] * k.c[2];
+  u = u + r[3] * k.c[1];
+  u = u + r[4] * k.c[0];
+  u = u + r[5] * k.c[1];
+  u = u + r[6] * k.c[2];
+  u = u + r[7] * k.c[3];
+  u = u + r[8] * k.c[4];
+  return u;
+}
+
+// Interior sweep of one wavefront: z = z0 + lane (cells with z >= zend idle),
+// x from x0 to x1-1.  The current level of each field is held in a 9-deep
+// register queue along x; the centre row is exchanged through LDS for the z
+// derivative.  Each cell's value is final (no absorbing band in here).
+template <typename T, int MODE>
+__device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot<T>& s, const StepCtx<T>& c, int z0,
+                                               int zend, int x0, int x1, T* lds) {
+  constexpr bool TWO = (MODE == BORN || MODE == RTM_BWD);
+  const int lane = threadIdx.x & 63;
+  const int z = z0 + lane;
+  const bool act = z < zend;
+  const int NZ = k.NZ, pml = k.pml;
+  const int zl = min(z, NZ - 1);
+  const long long ld = k.ld;
+  const T* CA = c.C;
+  const T* PA = c.P;
+  T* NA = c.N;
+  const T* CB = c.C + 3 * s.fstride;
+  const T* PB = c.P + 3 * s.fstride;
+  T* NB = c.N + 3 * s.fstride;
+  T* la = lds;
+  T* lb = lds + LROW;
+  const bool halo = lane < 4 || lane >= 60;
+  const int hz = lane < 4 ? z0 - 4 + lane : min(z0 + lane + 4, NZ - 1);
+  const int hslot = lane < 4 ? lane : lane + 8;
+  const int iz = min(z - pml, k.nz - 1);
+  const int sxp = s.sx + pml, szp = s.sz + pml;
+  T qa[9], qb[9];
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    qa[j] = CA[(long long)(x0 - 4 + j) * ld + zl];
+    if (TWO) qb[j] = CB[(long long)(x0 - 4 + j) * ld + zl];
+  }
+  // per-step scalars
+  T amp = T(0);
+  if (MODE != RTM_BWD && c.it - 1 >= 0) amp = s.wav[c.it - 1];
+  T sub = T(0);
+  if (MODE == RTM_BWD && c.it - 1 > 0) {
+    const T as = k.dt * c.m.vel(sxp, szp);
+    sub = (as * as) * s.wav[c.it - 1];
+  }
+  for (int x = x0; x < x1; x++) {
+    const long long row = (long long)x * ld;
+    qa[8] = CA[row + 4 * ld + zl];
+    if (TWO) qb[8] = CB[row + 4 * ld + zl];
+    if (halo) {
+      la[hslot] = CA[row + hz];
+      if (TWO) lb[hslot] = CB[row + hz];
+    }
+    la[lane + 4] = qa[4];
+    if (TWO) lb[lane + 4] = qb[4];
+    const int ix = x - pml;
+    const T v = s.vel[(long long)ix * k.nz + iz];
+    const T av = k.dt * v;
+    const T vt2 = av * av;
+    const T vvzz = vt2 * k.idz2;
+    const T vvxx = vt2 * k.idx2;
+    const long long i = row + zl;
+    wave_sync();
+    const T uzz = d2_z(k, la + lane);
+    T uxx = qa[0] * k.c[4];
+    uxx = uxx + qa[1] * k.c[3];
+    uxx = uxx + qa[2] * k.c[2];
+    uxx = uxx + qa[3] * k.c[1];
+    uxx = uxx + qa[4] * k.c[0];
+    uxx = uxx + qa[5] * k.c[1];
+    uxx = uxx + qa[6] * k.c[2];
+    uxx = uxx + qa[7] * k.c[3];
+    uxx = uxx + qa[8] * k.c[4];
+    T pa = PA[i];
+    if (MODE == RTM_BWD && sub != T(0) && x == sxp && z == szp) pa = pa - sub;
+    T ua = vvzz * uzz + vvxx * uxx + T(2) * qa[4] - pa;
+    if (MODE != RTM_BWD && x == sxp && z == szp && c.it - 1 >= 0) ua = ua + vt2 * amp;
+    T ub = T(0);
+    if (TWO) {
+      const T uzzb = d2_z(k, lb + lane);
+      T uxxb = qb[0] * k.c[4];
+      uxxb = uxxb + qb[1] * k.c[3];
+      uxxb = uxxb + qb[2] * k.c[2];
+      uxxb = uxxb + qb[3] * k.c[1];
+      uxxb = uxxb + qb[4] * k.c[0];
+      uxxb = uxxb + qb[5] * k.c[1];
+      uxxb = uxxb + qb[6] * k.c[2];
+      uxxb = uxxb + qb[7] * k.c[3];
+      uxxb = uxxb + qb[8] * k.c[4];
+      ub = vvzz * uzzb + vvxx * uxxb + T(2) * qb[4] - PB[i];
+    }
+    wave_sync();
+    if (act) {
+      if (MODE == MODEL || MODE == RTM_FWD) {
+        NA[i] = ua;
+        if (MODE == MODEL) {
+          if (c.itp >= 0 && z == s.gz[ix] + pml) s.rec[(long long)ix * s.nt + c.itp] = ua;
+        } else if (c.itp >= 0) {
+          int sl[4];
+          const int n = strip_slots(x, z, s.nx, k.nz, pml, sl);
+          T* st = s.strips + (long long)c.itp * 8 * (s.nx + k.nz);
+          for (int q = 0; q < n; q++) st[sl[q]] = ua;
+        }
+      } else if (MODE == BORN) {
+        const T d = s.dv[(long long)ix * k.nz + iz];
+        if (d != T(0)) {
+          const T lap = (ua + pa - T(2) * qa[4]) * k.idt2;
+          const T v3 = T(2) / (v * v * v);
+          ub = ub + vt2 * lap * v3 * d;
+        }
+        NA[i] = ua;
+        NB[i] = ub;
+        if (c.itp >= 0 && z == s.gz[ix] + pml) s.rec[(long long)ix * s.nt + c.itp] = ub;
+      } else {  // RTM_BWD: source field ua, receiver field ub
+        if (z == s.gz[ix] + pml) ub = ub + s.rec[(long long)ix * s.nt + c.itp] * vt2;
+        const T lap = (pa + ua - T(2) * qa[4]) * k.idt2;
+        const T v3 = T(2) / (v * v * v);
+        const T U = lap * v3;
+        s.img[i] += U * ub;
+        s.aux[i] = U;
+        NB[i] = ub;
+        T out = ua;
+        if (c.itp - 1 >= 0) {
+          int sl[4];
+          if (strip_slots(x, z, s.nx, k.nz, pml, sl) > 0)
+            out = s.strips[(long long)(c.itp - 1) * 8 * (s.nx + k.nz) + sl[0]];
+        }
+        NA[i] = out;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      qa[j] = qa[j + 1];
+      if (TWO) qb[j] = qb[j + 1];
+    }
+  }
+}
+
+template <typename T, int MODE>
+__global__ __launch_bounds__(WG) void step_kernel(Consts<T> k, const DevShot<T>* __restrict__ shots, Layout L,
+                                                 int step) {
+  __shared__ T lds[WAVES][2 * LROW];
+  const DevShot<T>& s = shots[blockIdx.y];
+  StepCtx<T> c;
+  if (MODE == RTM_BWD) {
+    c.it = s.nt + s.ndel - 1 - step;
+    if (c.it < s.ndel) return;
+  } else {
+    c.it = step;
+    if (c.it >= s.nt + s.ndel) return;
+  }
+  c.itp = c.it - s.ndel;
+  const int r0 = step % 3, r1 = (step + 1) % 3, r2 = (step + 2) % 3;
+  c.P = s.fld + r0 * s.fstride;
+  c.C = s.fld + r1 * s.fstride;
+  c.N = s.fld + r2 * s.fstride;
+  c.m = Model<T>{s.vel, s.nx, k.nz, k.pml};
+  c.src = fidx(k.ld, s.sx + k.pml, s.sz + k.pml);
+  if ((int)blockIdx.x < L.nfwg) {
+    const int wave = threadIdx.x >> 6;
+    const int item = blockIdx.x * WAVES + wave;
+    if (item >= L.nzc * L.nxs) return;  // whole wavefront leaves; no workgroup barrier follows
+    const int zc = item % L.nzc, xs = item / L.nzc;
+    const int z0 = L.za + zc * 64;
+    const int x0 = L.xa + xs * L.bx;
+    const int x1 = min(x0 + L.bx, L.xb);
+    interior_sweep<T, MODE>(k, s, c, z0, L.zb, x0, x1, lds[wave]);
+    return;
+  }
+  const long long j = (long long)(blockIdx.x - L.nfwg) * WG + threadIdx.x;
+  if (j >= L.nband) return;
+  const int NX = s.NX, NZ = k.NZ;
+  int x, z;
+  const long long top = (long long)NX * L.za, bot = (long long)NX * (NZ - L.zb);
+  const long long lft = (long long)L.xa * (L.zb - L.za);
+  if (j < top) {
+    x = (int)(j / L.za);
+    z = (int)(j % L.za);
+  } else if (j < top + bot) {
+    const long long q = j - top;
+    x = (int)(q / (NZ - L.zb));
+    z = L.zb + (int)(q % (NZ - L.zb));
+  } else if (j < top + bot + lft) {
+    const long long q = j - top - bot;
+    x = (int)(q / (L.zb - L.za));
+    z = L.za + (int)(q % (L.zb - L.za));
+  } else {
+    const long long q = j - top - bot - lft;
+    x = L.xb + (int)(q / (L.zb - L.za));
+    z = L.za + (int)(q % (L.zb - L.za));
+  }
+  cell_update<T, MODE>(k, s, c, x, z);
+}
+
+// Hilbert term of the reference RTM image (FaQi, fd.cpp:1272-1298 with
+// hilbert.h:42-54): per interior column, img += 1/2 (U R - H[U] H[R]) where
+// H is the 61-tap Hamming-tapered FIR applied over the full padded column.
+template <typename T>
+__global__ __launch_bounds__(256) void faqi_kernel(Consts<T> k, const DevShot<T>* __restrict__ shots, int step,
+                                                   const T* __restrict__ taps, int half, int slot_off) {
+  extern __shared__ unsigned char smem_raw[];
+  T* sU = reinterpret_cast<T*>(smem_raw);
+  const DevShot<T>& s = shots[blockIdx.y];
+  const int x = k.pml + blockIdx.x;
+  if (x >= s.NX - k.pml) return;
+  const int it = s.nt + s.ndel - 1 - step;
+  if (it < s.ndel) return;
+  const int NZ = k.NZ;
+  T* sR = sU + NZ;
+  const long long base = fidx(k.ld, x, 0);
+  const T* R = s.fld + slot_off * s.fstride;
+  for (int z = threadIdx.x; z < NZ; z += blockDim.x) {
+    sU[z] = s.aux[base + z];
+    sR[z] = R[base + z];
+  }
+  __syncthreads();
+  const T scal = T(2) / T(4);
+  for (int z = k.pml + threadIdx.x; z < NZ - k.pml; z += blockDim.x) {
+    const int i0 = max(0, z - half), i1 = min(NZ - 1, z + half);
+    T uh = T(0), rh = T(0);
+    // odd offsets only: the even taps are zero
+    int i = i0 + (((z - i0) & 1) ? 0 : 1);
+    for (; i <= i1; i += 2) {
+      const T h = taps[z - i + half];
+      uh += sU[i] * h;
+      rh += sR[i] * h;
+    }
+    s.img[base + z] += scal * (sU[z] * sR[z] - uh * rh);
+  }
+}
+
+// Writes the strips of time sample `itp` into a field (fd.cpp:1204-1223).
+template <typename T>
+__global__ void restore_strips_kernel(Consts<T> k, const DevShot<T>* __restrict__ shots, int slot, int from_end) {
+  const DevShot<T>& s = shots[blockIdx.y];
+  const int nx = s.nx, nz = k.nz, pml = k.pml;
+  const int n = 8 * (nx + nz);
+  const int itp = from_end ? s.nt - 1 : 0;
+  T* F = s.fld + slot * s.fstride;
+  const T* st = s.strips + (long long)itp * n;
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+    int x, z;
+    if (j < 4 * nx) { x = j / 4 + pml; z = pml + j % 4; }
+    else if (j < 8 * nx) { int q = j - 4 * nx; x = q / 4 + pml; z = nz + pml - 1 - q % 4; }
+    else if (j < 8 * nx + 4 * nz) { int q = j - 8 * nx; x = pml + q / nz; z = q % nz + pml; }
+    else { int q = j - 8 * nx - 4 * nz; x = nx + pml - 1 - q / nz; z = q % nz + pml; }
+    F[fidx(k.ld, x, z)] = st[j];
+  }
+}
+
+// Interior crop of a padded image: out[ix][iz] = img[ix+pml][iz+pml].
+template <typename T>
+__global__ void crop_kernel(Consts<T> k, const DevShot<T>* __restrict__ shots, T* out) {
+  const DevShot<T>& s = shots[blockIdx.y];
+  const long long n = (long long)s.nx * k.nz;
+  T* o = out + (long long)blockIdx.y * n;  // caller guarantees equal nx in a batch
+  for (long long j = blockIdx.x * (long long)blockDim.x + threadIdx.x; j < n; j += (long long)gridDim.x * blockDim.x) {
+    const int ix = (int)(j / k.nz), iz = (int)(j % k.nz);
+    o[j] = s.img[fidx(k.ld, ix + k.pml, iz + k.pml)];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host side.
+
+template <typename T>
+Consts<T> make_consts(const GridSpec& g, long long ld) {
+  Consts<T> k;
+  k.nz = g.nz;
+  k.pml = g.pml;
+  k.NZ = g.nz + 2 * g.pml;
+  k.ld = ld;
+  k.dt = (T)g.dt;
+  k.dx = (T)g.dx;
+  k.dz = (T)g.dz;
+  k.idx2 = T(1) / (k.dx * k.dx);
+  k.idz2 = T(1) / (k.dz * k.dz);
+  k.idt2 = T(1) / k.dt / k.dt;
+  const T dt2 = k.dt * k.dt;
+  k.cx = (T(1) / k.dx) * (T(1) / k.dx) * dt2;
+  k.cz = (T(1) / k.dz) * (T(1) / k.dz) * dt2;
+  k.a1z = T(1) / k.dz / T(2) / k.dt;
+  k.a1x = T(1) / k.dx / T(2) / k.dt;
+  k.a2b = T(1) / T(2) / k.dt / k.dt;
+  k.b1 = T(1) / k.dz;
+  k.b2 = T(1) / k.dx;
+  k.sq2 = std::sqrt(T(2));
+  k.c[0] = T(-205) / T(72);
+  k.c[1] = T(8) / T(5);
+  k.c[2] = T(-1) / T(5);
+  k.c[3] = T(8) / T(315);
+  k.c[4] = T(-1) / T(560);
+  return k;
+}
+
+template <typename T>
+void hilbert_taps_host(int half, T* h) {
+  h[half] = T(0);
+  for (int i = 1; i <= half; i++) {
+    T taper = (T)(0.54 + 0.46 * std::cos(3.1415926 * i / half));
+    T hi = (T)((double)(taper * T(i % 2)) * 2.0 / (3.1415926 * i));
+    h[half + i] = hi;
+    h[half - i] = -hi;
+  }
+}
+
+// Field storage geometry: margin of MX columns / MZ rows of zeros around the
+// padded grid, z stride rounded to 64 elements.
+constexpr int MX = 4, MZ = 4;
+
+long long field_ld(int NZ) { return ((long long)(NZ + 2 * MZ) + 63) / 64 * 64; }
+
+long long field_elems(int NX, long long ld) { return (long long)(NX + 2 * MX) * ld; }
+
+Layout make_layout(int NX, int NZ, int pml) {
+  Layout L;
+  L.xa = std::max(pml, 4);
+  L.xb = std::min(NX - pml, NX - 4);
+  L.za = std::max(pml, 4);
+  L.zb = std::min(NZ - pml, NZ - 4);
+  if (L.xb <= L.xa || L.zb <= L.za) {  // no interior rectangle: every cell is a band cell
+    L.xa = L.xb = 0;
+    L.za = 0;
+    L.zb = NZ;
+  }
+  L.bx = 64;
+  L.nzc = (L.zb - L.za + 63) / 64;
+  L.nxs = (L.xb - L.xa + L.bx - 1) / L.bx;
+  if (L.xb == L.xa) L.nzc = L.nxs = 0;
+  L.nfwg = (L.nzc * L.nxs + WAVES - 1) / WAVES;
+  L.nband = (long long)NX * NZ - (long long)(L.xb - L.xa) * (L.zb - L.za);
+  L.nbwg = (int)((L.nband + WG - 1) / WG);
+  return L;
+}
+
+template <typename T>
+struct BatchRun {
+  std::vector<DevShot<T>> host;
+  DevShot<T>* dev = nullptr;
+  void* pool = nullptr;
+  size_t pool_bytes = 0;
+  ~BatchRun() {
+    if (dev) (void)hipFree(dev);
+    if (pool) (void)hipFree(pool);
+  }
+};
+
+#define HIPCHECK(x)                                                            \
+  do {                                                                         \
+    hipError_t e_ = (x);                                                       \
+    if (e_ != hipSuccess) {                                                    \
+      set_error(std::string(#x) + ": " + hipGetErrorString(e_));               \
+      return SEIS_ERR_HIP;                                                     \
+    }                                                                          \
+  } while (0)
+
+int check_grid(const GridSpec& g) {
+  if (g.nx <= 0 || g.nz <= 0 || g.pml < 0) return set_error("grid: nx, nz must be > 0 and pml >= 0"), SEIS_ERR_ARG;
+  if (g.nz + 2 * g.pml < 8 || g.nx + 2 * g.pml < 8)
+    return set_error("grid: padded grid must be at least 8 x 8 cells"), SEIS_ERR_ARG;
+  if (!(g.dt > 0) || !(g.dx > 0) || !(g.dz > 0)) return set_error("grid: dt, dx, dz must be > 0"), SEIS_ERR_ARG;
+  return SEIS_OK;
+}
+
+template <typename T>
+int run_batch(int mode, const GridSpec& g, int nshots, const ShotSpec* shots, const T* vel, const T* dv,
+              const T* wav, const int* gz, T* rec, T* img, hipStream_t stream) {
+  int rc = check_grid(g);
+  if (rc) return rc;
+  if (nshots <= 0) return SEIS_OK;
+  const int NZ = g.nz + 2 * g.pml;
+  const int NX = g.nx + 2 * g.pml;
+  const long long ld = field_ld(NZ);
+  const long long fe = field_elems(NX, ld);
+  const int nslots = (mode == RUN_MODEL) ? 3 : 6;
+  const bool rtm = (mode == RUN_RTM);
+  if (rtm && (g.nx < 4 || g.nz < 4))
+    return set_error("rtm: the 4-layer boundary strips need nx >= 4 and nz >= 4"), SEIS_ERR_ARG;
+  int ntmax = 0, NTmax = 0;
+  size_t strip_elems = 0;
+  for (int s = 0; s < nshots; s++) {
+    const ShotSpec& q = shots[s];
+    if (q.nt <= 0 || q.ndel < 0) return set_error("shot: nt must be > 0 and ndel >= 0"), SEIS_ERR_ARG;
+    if (q.sx < 0 || q.sx >= g.nx || q.sz < 0 || q.sz >= g.nz)
+      return set_error("shot: source cell outside the model"), SEIS_ERR_ARG;
+    ntmax = std::max(ntmax, q.nt);
+    NTmax = std::max(NTmax, q.nt + q.ndel);
+    if (rtm) strip_elems += (size_t)q.nt * 8 * (g.nx + g.nz);
+  }
+  // per shot: nslots fields (+ image and aux for RTM), then strips
+  const size_t per_shot = (size_t)fe * (nslots + (rtm ? 2 : 0));
+  const size_t total = per_shot * nshots + strip_elems;
+  BatchRun<T> br;
+  br.pool_bytes = total * sizeof(T);
+  HIPCHECK(hipMalloc(&br.pool, br.pool_bytes));
+  HIPCHECK(hipMemsetAsync(br.pool, 0, br.pool_bytes, stream));
+  T* p = reinterpret_cast<T*>(br.pool);
+  T* strip_p = p + per_shot * nshots;
+  const long long origin = (long long)MX * ld + MZ;
+  br.host.resize(nshots);
+  for (int s = 0; s < nshots; s++) {
+    const ShotSpec& q = shots[s];
+    DevShot<T>& d = br.host[s];
+    T* base = p + per_shot * s;
+    d.vel = vel + q.model;
+    d.dv = dv ? dv + q.model : nullptr;
+    d.wav = wav + q.wav;
+    d.gz = gz + q.gz;
+    d.rec = rec + q.rec;
+    d.fld = base + origin;
+    d.fstride = fe;
+    d.img = rtm ? base + (size_t)fe * nslots + origin : nullptr;
+    d.aux = rtm ? base + (size_t)fe * (nslots + 1) + origin : nullptr;
+    d.strips = rtm ? strip_p : nullptr;
+    if (rtm) strip_p += (size_t)q.nt * 8 * (g.nx + g.nz);
+    d.nx = g.nx;
+    d.NX = NX;
+    d.nt = q.nt;
+    d.ndel = q.ndel;
+    d.sx = q.sx;
+    d.sz = q.sz;
+  }
+  HIPCHECK(hipMalloc(&br.dev, sizeof(DevShot<T>) * nshots));
+  HIPCHECK(hipMemcpyAsync(br.dev, br.host.data(), sizeof(DevShot<T>) * nshots, hipMemcpyHostToDevice, stream));
+  const Consts<T> k = make_consts<T>(g, ld);
+  const Layout L = make_layout(NX, NZ, g.pml);
+  dim3 grid(L.nfwg + L.nbwg, nshots), block(WG);
+  const double cells = (double)NX * NZ;
+  const double E = sizeof(T);
+  // shots still running at forward step st / reverse step st
+  auto active_fwd = [&](int st) {
+    int n = 0;
+    for (int s = 0; s < nshots; s++) n += st < shots[s].nt + shots[s].ndel;
+    return n;
+  };
+  auto active_bwd = [&](int st) {
+    int n = 0;
+    for (int s = 0; s < nshots; s++) n += st < shots[s].nt;
+    return n;
+  };
+  // Algorithmic HBM bytes per cell: MODEL reads vel, C, P and writes N (16 B
+  // in FP32); BORN adds the scattered C, P, N and dv (32 B for 2 updates);
+  // RTM reverse reads vel, C, P, Cr, Pr, writes N, Nr, lap*2/v^3 and
+  // read-modify-writes the image (40 B for 2 updates).
+  if (mode == RUN_MODEL) {
+    for (int st = 0; st < NTmax; st++) {
+      const double a = active_fwd(st) * cells;
+      timed_launch(stream, "step_model", 4 * E * a, a, [&] {
+        hipLaunchKernelGGL((step_kernel<T, MODEL>), grid, block, 0, stream, k, br.dev, L, st);
+      });
+    }
+  } else if (mode == RUN_BORN) {
+    for (int st = 0; st < NTmax; st++) {
+      const double a = active_fwd(st) * cells;
+      timed_launch(stream, "step_born", 8 * E * a, 2 * a, [&] {
+        hipLaunchKernelGGL((step_kernel<T, BORN>), grid, block, 0, stream, k, br.dev, L, st);
+      });
+    }
+  } else {
+    const double strip = 8.0 * (g.nx + g.nz) * E;
+    for (int st = 0; st < NTmax; st++) {
+      const int n = active_fwd(st);
+      timed_launch(stream, "step_rtm_forward", 4 * E * n * cells + strip * n, n * cells, [&] {
+        hipLaunchKernelGGL((step_kernel<T, RTM_FWD>), grid, block, 0, stream, k, br.dev, L, st);
+      });
+    }
+    // reset the six wavefields, then restore the last strips into the current level
+    for (int s = 0; s < nshots; s++)
+      HIPCHECK(hipMemsetAsync(p + per_shot * s, 0, sizeof(T) * (size_t)fe * nslots, stream));
+    hipLaunchKernelGGL((restore_strips_kernel<T>), dim3(16, nshots), dim3(256), 0, stream, k, br.dev, 1, 1);
+    const int half = 30;
+    std::vector<T> h(2 * half + 1);
+    hilbert_taps_host<T>(half, h.data());
+    T* dtaps = nullptr;
+    HIPCHECK(hipMalloc(&dtaps, sizeof(T) * h.size()));
+    HIPCHECK(hipMemcpyAsync(dtaps, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, stream));
+    const size_t smem = sizeof(T) * 2 * NZ;
+    if (smem > 64 * 1024) return set_error("rtm: padded nz too large for the column Hilbert kernel"), SEIS_ERR_ARG;
+    int maxbwd = 0;
+    for (int s = 0; s < nshots; s++) maxbwd = std::max(maxbwd, shots[s].nt);
+    const double faqi_cells = (double)g.nx * NZ;
+    for (int st = 0; st < maxbwd; st++) {
+      const int n = active_bwd(st);
+      timed_launch(stream, "step_rtm_reverse", 10 * E * n * cells + strip * n, 2 * n * cells, [&] {
+        hipLaunchKernelGGL((step_kernel<T, RTM_BWD>), grid, block, 0, stream, k, br.dev, L, st);
+      });
+      // the receiver field's new level lives in slot 3 + (st+2)%3
+      timed_launch(stream, "faqi_hilbert", 2 * E * n * faqi_cells + 2 * E * n * (double)g.nx * g.nz, 0, [&] {
+        hipLaunchKernelGGL((faqi_kernel<T>), dim3(g.nx, nshots), dim3(256), smem, stream, k, br.dev, st, dtaps,
+                           half, 3 + (st + 2) % 3);
+      });
+    }
+    hipLaunchKernelGGL((crop_kernel<T>), dim3(64, nshots), dim3(256), 0, stream, k, br.dev, img);
+    HIPCHECK(hipStreamSynchronize(stream));
+    HIPCHECK(hipFree(dtaps));
+  }
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipStreamSynchronize(stream));
+  return SEIS_OK;
+}
+
+template int run_batch<float>(int, const GridSpec&, int, const ShotSpec*, const float*, const float*, const float*,
+                              const int*, float*, float*, hipStream_t);
+
+}  // namespace seis

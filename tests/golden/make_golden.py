@@ -1,0 +1,100 @@
+"""Write the golden vectors of the 2D acoustic engine from the compiled reference.
+
+Run in the container that holds the reference tree, after `make -C oracle ref`:
+
+    python tests/golden/make_golden.py
+
+It calls the reference's own modeling / forward / adjoint (src/util/fd.h:3-5)
+through oracle/_ref/libseisref.so and stores inputs and outputs as .npz
+fixtures next to this script.  The fixtures are data only; they pin the CPU
+oracle (tests/test_oracle_golden.py) and, through it, the HIP engine.
+"""
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+from oracle_lib import Reference  # noqa: E402
+
+
+def layered(nx, nz, dz, interfaces, speeds):
+    """[nx][nz] velocity with horizontal interfaces at depths `interfaces` (m)."""
+    z = np.arange(nz) * dz
+    col = np.full(nz, speeds[0], np.float32)
+    for zi, v in zip(interfaces, speeds[1:]):
+        col[z >= zi] = v
+    return np.repeat(col[None, :], nx, axis=0).astype(np.float32)
+
+
+def small_case(seed, nx, nz, pml, nt, ndel, dt=0.001, dx=10.0, dz=10.0):
+    rng = np.random.default_rng(seed)
+    vel = layered(nx, nz, dz, [nz * dz / 3, 2 * nz * dz / 3], [1500.0, 2200.0, 3000.0])
+    vel += rng.uniform(-60, 60, vel.shape).astype(np.float32)
+    gz = rng.integers(0, 3, nx).astype(np.int32)
+    sx, sz = int(rng.integers(nx // 4, 3 * nx // 4)), int(rng.integers(0, 4))
+    fr = 15.0
+    wav = None  # filled by the reference's own Ricker
+    dv = (rng.standard_normal((nx, nz)) * (rng.random((nx, nz)) < 0.05) * 50).astype(np.float32)
+    rec_in = rng.standard_normal((nx, nt)).astype(np.float32)
+    return dict(nx=nx, nz=nz, pml=pml, nt=nt, ndel=ndel, dt=dt, dx=dx, dz=dz, sx=sx, sz=sz, fr=fr,
+                delay=25, vel=vel, gz=gz, dv=dv, rec_in=rec_in, wav=wav)
+
+
+def run_case(ref, c, what):
+    args = (c["dt"], c["dx"], c["dz"], c["nt"], c["ndel"], c["nx"], c["nz"], c["pml"], c["sx"], c["sz"], c["gz"],
+            c["wav"], c["vel"])
+    out = {}
+    if "modeling" in what:
+        out["rec"] = ref.modeling(*args)
+    if "born" in what:
+        out["born"] = ref.born(*args, c["dv"])
+    if "rtm" in what:
+        out["img"] = ref.rtm(*args, c["rec_in"])
+    return out
+
+
+def save(name, c, out, note):
+    meta = {k: c[k] for k in ("nx", "nz", "pml", "nt", "ndel", "dt", "dx", "dz", "sx", "sz", "fr", "delay")}
+    meta["note"] = note
+    arrays = {k: v for k, v in c.items() if isinstance(v, np.ndarray)}
+    arrays.update(out)
+    np.savez_compressed(os.path.join(HERE, name), meta=json.dumps(meta), **arrays)
+    print(name, {k: v.shape for k, v in arrays.items()})
+
+
+def main():
+    ref = Reference()
+    ref.set_threads(os.cpu_count() or 1)
+    cases = [
+        ("small_a.npz", small_case(11, 24, 20, 6, 80, 5)),
+        ("small_b.npz", small_case(12, 40, 33, 10, 150, 0)),
+        ("small_c.npz", small_case(13, 31, 45, 12, 120, 7)),
+        ("small_d.npz", small_case(14, 64, 48, 16, 200, 3)),
+    ]
+    for name, c in cases:
+        c["wav"] = ref.ricker(c["dt"], c["fr"], c["delay"] + c["ndel"], c["nt"] + c["ndel"])
+        out = run_case(ref, c, ("modeling", "born", "rtm"))
+        save(name, c, out, "seeded small grid; modeling, born and rtm of the reference engine")
+
+    # Plain forward modeling, 1 shot: 201 x 301 at 10 m, 3 layers 1500/2200/3000 m/s with
+    # interfaces at 600 m and 1400 m, dt = 1 ms, nt = 1500, pml 30, Ricker 15 Hz (delay 67
+    # samples, no extra pre-roll), source at (z = 20 m, x = 1500 m), 301 receivers at z = 20 m.
+    nx, nz = 301, 201
+    c1 = dict(nx=nx, nz=nz, pml=30, nt=1500, ndel=0, dt=0.001, dx=10.0, dz=10.0, sx=150, sz=2, fr=15.0,
+              delay=67)
+    c1["vel"] = layered(nx, nz, 10.0, [600.0, 1400.0], [1500.0, 2200.0, 3000.0])
+    c1["gz"] = np.full(nx, 2, np.int32)
+    c1["wav"] = ref.ricker(c1["dt"], c1["fr"], c1["delay"] + c1["ndel"], c1["nt"] + c1["ndel"])
+    t0 = time.time()
+    rec = ref.modeling(c1["dt"], c1["dx"], c1["dz"], c1["nt"], c1["ndel"], nx, nz, c1["pml"], c1["sx"], c1["sz"],
+                       c1["gz"], c1["wav"], c1["vel"])
+    print("C1 modeling %.2f s" % (time.time() - t0))
+    save("c1_modeling.npz", c1, {"rec": rec}, "single-shot FP32 forward modeling of the 3-layer model")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,57 @@
+"""The C-ABI library loads and exports every function include/*.h declares.
+
+Runs without a GPU: no compute entry point is called.
+"""
+import ctypes
+import glob
+import os
+import re
+
+import pytest
+
+from conftest import ROOT
+
+
+def declared_functions():
+    names = []
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        text = re.sub(r"//[^\n]*", "", text)
+        for m in re.finditer(r"\b(seis_[a-z0-9_]+)\s*\(", text):
+            names.append(m.group(1))
+    return sorted(set(names))
+
+
+def test_header_declares_entry_points():
+    names = declared_functions()
+    for n in ("seis_modeling", "seis_born", "seis_rtm", "seis_modeling_batch", "seis_born_batch",
+              "seis_rtm_batch", "seis_last_error"):
+        assert n in names
+
+
+def test_library_exports_every_declared_symbol():
+    import seis_amd
+    L = seis_amd.lib()
+    missing = [n for n in declared_functions() if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_python_signatures_cover_header():
+    from seis_amd._lib import exported_symbols
+    assert set(declared_functions()) <= set(exported_symbols()) | set(declared_functions())
+    missing = set(declared_functions()) - set(exported_symbols())
+    assert not missing, missing
+
+
+def test_argument_errors_without_gpu():
+    """Invalid arguments are rejected before any device work."""
+    import numpy as np
+    import seis_amd
+    from seis_amd._lib import SeisError
+    gz = np.zeros(4, np.int32)
+    with pytest.raises(SeisError):
+        seis_amd.modeling(0.001, 10, 10, 10, 0, 4, 4, 2, 9, 0, gz, np.zeros(10, np.float32),
+                          np.ones((4, 4), np.float32))  # source outside the model
+    msg = seis_amd.lib().seis_last_error().decode()
+    assert "source" in msg

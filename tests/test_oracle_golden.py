@@ -1,0 +1,63 @@
+"""The CPU oracle against the golden vectors of the compiled reference engine.
+
+The fixtures (tests/golden/*.npz) hold the outputs of the reference's own
+modeling / forward / adjoint (src/util/fd.h:3-5), written by
+tests/golden/make_golden.py.  The float32 oracle keeps the reference's operation
+order, so it must reproduce them bit for bit.
+"""
+import numpy as np
+import pytest
+
+from conftest import fixture_names, load_fixture, rel_l2
+
+SMALL = [n for n in fixture_names() if n.startswith("small_")]
+
+
+def _args(c):
+    return (c["dt"], c["dx"], c["dz"], c["nt"], c["ndel"], c["nx"], c["nz"], c["pml"], c["sx"], c["sz"], c["gz"],
+            c["wav"], c["vel"])
+
+
+@pytest.mark.parametrize("name", SMALL)
+def test_modeling_bitexact(oracle, name):
+    c = load_fixture(name)
+    rec = oracle.modeling(*_args(c))
+    assert np.array_equal(rec, c["rec"]), name
+
+
+@pytest.mark.parametrize("name", SMALL)
+def test_born_bitexact(oracle, name):
+    c = load_fixture(name)
+    rec = oracle.born(*_args(c), c["dv"])
+    assert np.array_equal(rec, c["born"]), name
+
+
+@pytest.mark.parametrize("name", SMALL)
+def test_rtm_bitexact(oracle, name):
+    c = load_fixture(name)
+    img = oracle.rtm(*_args(c), c["rec_in"])
+    assert np.array_equal(img, c["img"]), name
+
+
+def test_c1_modeling_bitexact(oracle):
+    """Single-shot FP32 forward modeling on the 201 x 301 three-layer model."""
+    c = load_fixture("c1_modeling.npz")
+    rec = oracle.modeling(*_args(c))
+    assert np.array_equal(rec, c["rec"])
+
+
+def test_c1_fp64_oracle_close(oracle):
+    """The FP64 instance agrees with the FP32 reference to FP32 rounding level."""
+    c = load_fixture("c1_modeling.npz")
+    a = list(_args(c))
+    a[11] = c["wav"].astype(np.float64)
+    a[12] = c["vel"].astype(np.float64)
+    rec64 = oracle.modeling(*a, dtype=np.float64)
+    err = rel_l2(c["rec"], rec64)
+    assert err < 1e-4, err
+
+
+def test_fixture_wavelet_is_ricker(oracle):
+    c = load_fixture("c1_modeling.npz")
+    w = oracle.ricker(c["dt"], c["fr"], c["delay"] + c["ndel"], c["nt"] + c["ndel"])
+    assert np.array_equal(w, c["wav"])
