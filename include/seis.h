@@ -77,6 +77,27 @@ int seis_born(float dt, float dx, float dz, int nt, int ndel, int nx, int nz, in
 int seis_rtm(float dt, float dx, float dz, int nt, int ndel, int nx, int nz, int pml, int sx, int sz,
              const int* gz, const float* wav, const float* vel, const float* rec, float* img);
 
+/* Ricker wavelet of the reference (wavelet.cpp:7-18): out[i] for i < n,
+ * peak frequency fr, peak at sample num_delay. */
+int seis_ricker(float dt, float fr, int num_delay, int n, float* out);
+
+/* Parameter-file driver: the reference's Inversion flow (lsrtm/inversion.cpp)
+ * over one GPU.  param_file uses the reference's grammar and keys
+ * (param_*.txt2, datastr.h:51-75); mode is
+ *   "modeling"  records of model_file (default: vfile) -> <precsg><is>.dat
+ *   "born"      Born records of perturbation model_file -> <wdir>BORN<is>.dat
+ *   "rtm"       image of the <precsg> records       (adjoint_MPI, RTM_IMG)
+ *   "rtm_born"  image of the <wdir>BORN records     (adjoint_MPI, LSRTM_STEP)
+ *   "lsrtm"     least-squares RTM, maxiter CG iterations
+ * Images are written to out_file (default <wdir>RTM_IMAGE.dat or
+ * <wdir>LSRTM_IMAGE.dat) as float32 [nx][nz]. */
+int seis_run(const char* param_file, const char* mode, const char* model_file, const char* out_file);
+
+/* Parses a parameter file like Inversion::readParamFile (inversion.cpp:38-79,
+ * grammar of parser.cpp:16-168) and writes "key = value" lines into out
+ * (out_len bytes, NUL-terminated); errors name the missing key. */
+int seis_read_params(const char* param_file, char* out, int out_len);
+
 /* Batched variants: all array arguments are DEVICE pointers (seis_malloc);
  * shots[] is a host array.  Shots run concurrently in the same launches;
  * each produces exactly what the single-shot call produces for it.
@@ -90,6 +111,43 @@ int seis_born_batch(const seis_grid* grid, int nshots, const seis_shot* shots, c
  * CSG_IMAGE<is> of inversion.cpp:434-439). */
 int seis_rtm_batch(const seis_grid* grid, int nshots, const seis_shot* shots, const float* vel,
                    const float* wav, const int* gz, const float* rec, float* img);
+
+/* Least-squares RTM.  The reference declares the LSRTM operator and solver
+ * (conope_MPI, cgsolver_MPI at lsrtm/inversion.h:81,87) but defines neither;
+ * its intended operator is Born modeling followed by migration of the Born
+ * records over all shots (inversion.cpp:420-423).  A plan holds, for a batch
+ * of shots sharing one grid, the per-step Born-source snapshots of the
+ * background (vt2 * lap * 2/v^3, fd.cpp:134-139) and applies
+ *   L   : perturbation [nx][nz] -> packed Born records (as seis_born_batch),
+ *   L^T : packed records -> image [nx][nz], the exact transpose of L summed
+ *         over shots (not the reference's imaging operator),
+ * with FP64 inner products on the GPU.  All arrays are DEVICE pointers.
+ * snapshot_kind: 0 = FP32, 1 = BF16, 2 = FP16 (16-bit forms scaled per
+ * snapshot by its max |value|).  pml must be >= 4. */
+typedef struct seis_lsrtm seis_lsrtm;
+int seis_lsrtm_create(seis_lsrtm** plan, const seis_grid* grid, int nshots, const seis_shot* shots,
+                      const float* vel, const float* wav, const int* gz, int snapshot_kind);
+/* FP64 plan (FP64 propagation and snapshots), for the dot-product test. */
+int seis_lsrtm_create_f64(seis_lsrtm** plan, const seis_grid* grid, int nshots, const seis_shot* shots,
+                          const double* vel, const double* wav, const int* gz);
+int seis_lsrtm_destroy(seis_lsrtm* plan);
+/* rec = L pert  (float, or double for an FP64 plan) */
+int seis_lsrtm_born(seis_lsrtm* plan, const void* pert, void* rec);
+/* grad = L^T rec */
+int seis_lsrtm_adjoint(seis_lsrtm* plan, const void* rec, void* grad);
+/* *out = <a, b> over n elements, accumulated in FP64 on the GPU */
+int seis_lsrtm_dot(seis_lsrtm* plan, const void* a, const void* b, long long n, double* out);
+/* Optional record mask (device, packed like the records; NULL clears): L
+ * becomes mask * L and L^T becomes L^T * mask, e.g. receiver columns times
+ * the direct-arrival mute of removeDirect (inversion.cpp:699-752).  The
+ * caller keeps the mask alive while the plan uses it. */
+int seis_lsrtm_set_mask(seis_lsrtm* plan, const void* mask);
+/* CGLS for min ||L x - dobs||^2 from x = 0: at most niter iterations, stops
+ * when ||L^T r|| <= rtol ||L^T dobs|| (rtol <= 0: never); image = x.
+ * hist (host, 2*(niter+1) doubles, may be NULL): ||r_i||^2, ||L^T r_i||^2;
+ * *iters (may be NULL) = iterations done. */
+int seis_lsrtm_cg(seis_lsrtm* plan, const void* dobs, int niter, double rtol, void* image, double* hist,
+                  int* iters);
 
 /* Kernel timing: instrumentation with no counterpart in the reference.
  * Between seis_profile_start and seis_profile_stop every engine launch is

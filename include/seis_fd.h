@@ -1,0 +1,22 @@
+// seis_fd.h -- C++ drop-in for the reference's engine header (bbsun/Seis
+// src/util/fd.h:3-5).  Same names, argument order and meaning; the result is
+// returned as a newly allocated 2D array laid out like MyAlloc<float>::alc
+// (memory.h:29-44: a table of n2 row pointers into one zeroed block of n1*n2
+// floats), so the reference's MyAlloc<float>::free (memory.h:103-110: delete[]
+// buf[0]; delete[] buf;) releases it.  The work runs on the GPU through the C
+// ABI of seis.h.  On an engine error these print the message and return NULL
+// (the reference exits with status 0 on errors instead, debug.cpp:5-12).
+#ifndef SEIS_FD_H
+#define SEIS_FD_H
+
+// rec[nx][nt] = forward shot record (fd.cpp:15-80)
+float** modeling(float dt, float dx, float dz, int nt, int ndel, int nx, int nz, int pml, int sx, int sz, int* gz,
+                 float* wav, float** vel);
+// rec[nx][nt] = Born record of perturbation dv (fd.cpp:81-177)
+float** forward(float dt, float dx, float dz, int nt, int ndel, int nx, int nz, int pml, int sx, int sz, int* gz,
+                float* wav, float** vel, float** dv);
+// img[nx][nz] = RTM image of record rec (fd.cpp:592-719)
+float** adjoint(float dt, float dx, float dz, int nt, int ndel, int nx, int nz, int pml, int sx, int sz, int* gz,
+                float* wav, float** vel, float** rec);
+
+#endif

@@ -45,6 +45,19 @@ _SIGS = {
     "seis_modeling_batch": (_I, [ctypes.POINTER(seis_grid), _I, ctypes.POINTER(seis_shot), _P, _P, _P, _P]),
     "seis_born_batch": (_I, [ctypes.POINTER(seis_grid), _I, ctypes.POINTER(seis_shot), _P, _P, _P, _P, _P]),
     "seis_rtm_batch": (_I, [ctypes.POINTER(seis_grid), _I, ctypes.POINTER(seis_shot), _P, _P, _P, _P, _P]),
+    "seis_lsrtm_create": (_I, [ctypes.POINTER(_P), ctypes.POINTER(seis_grid), _I, ctypes.POINTER(seis_shot),
+                               _P, _P, _P, _I]),
+    "seis_lsrtm_create_f64": (_I, [ctypes.POINTER(_P), ctypes.POINTER(seis_grid), _I, ctypes.POINTER(seis_shot),
+                                   _P, _P, _P]),
+    "seis_lsrtm_destroy": (_I, [_P]),
+    "seis_lsrtm_born": (_I, [_P, _P, _P]),
+    "seis_lsrtm_adjoint": (_I, [_P, _P, _P]),
+    "seis_lsrtm_dot": (_I, [_P, _P, _P, ctypes.c_longlong, ctypes.POINTER(ctypes.c_double)]),
+    "seis_lsrtm_set_mask": (_I, [_P, _P]),
+    "seis_lsrtm_cg": (_I, [_P, _P, _I, ctypes.c_double, _P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I)]),
+    "seis_ricker": (_I, [_F, _F, _I, _I, _P]),
+    "seis_run": (_I, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p]),
+    "seis_read_params": (_I, [ctypes.c_char_p, ctypes.c_char_p, _I]),
     "seis_profile_start": (_I, []),
     "seis_profile_stop": (_I, [ctypes.POINTER(seis_kernel_stat), _I, ctypes.POINTER(_I)]),
 }

@@ -1,0 +1,66 @@
+// C ABI of the parameter-file driver (include/seis.h: seis_run).
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../../include/seis.h"
+#include "../seis_internal.h"
+#include "../wavelet.h"
+#include "inversion.h"
+
+using namespace seis;
+
+extern "C" {
+
+int seis_ricker(float dt, float fr, int num_delay, int n, float* out) {
+  if (!out || n < 0) return set_error("ricker: bad argument"), SEIS_ERR_ARG;
+  ricker_host(dt, fr, num_delay, n, out);
+  return SEIS_OK;
+}
+
+int seis_read_params(const char* param_file, char* out, int out_len) {
+  if (!param_file) return set_error("seis_read_params: null file"), SEIS_ERR_ARG;
+  Params p;
+  std::string err;
+  if (!read_params(param_file, p, err)) return set_error(err), SEIS_ERR_ARG;
+  const std::string text = format_params(p);
+  if (out && out_len > 0) {
+    const size_t n = std::min(text.size(), (size_t)out_len - 1);
+    std::memcpy(out, text.data(), n);
+    out[n] = 0;
+  }
+  return SEIS_OK;
+}
+
+int seis_run(const char* param_file, const char* mode, const char* model_file, const char* out_file) {
+  if (!param_file || !mode) return set_error("seis_run: null argument"), SEIS_ERR_ARG;
+  Inversion inv;
+  if (!inv.readParamFile(param_file) || !inv.getConfig()) return set_error(inv.error()), SEIS_ERR_ARG;
+  const Params& p = inv.param;
+  const size_t n = (size_t)p.nx.val * p.nz.val;
+  std::string err;
+  const std::string m(mode);
+  if (m == "modeling" || m == "born") {
+    std::string f = model_file ? model_file : (m == "modeling" ? p.vfile.val : "");
+    if (f.empty()) return set_error("seis_run: no model file"), SEIS_ERR_ARG;
+    std::vector<float> v(n);
+    if (!read_floats(f, n, v.data(), err)) return set_error(err), SEIS_ERR_ARG;
+    const bool ok = m == "modeling" ? inv.modeling(v.data()) : inv.forward(v.data());
+    if (!ok) return set_error(inv.error()), SEIS_ERR_HIP;
+    return SEIS_OK;
+  }
+  std::vector<float> img(n);
+  if (m == "rtm" || m == "rtm_born") {
+    if (!inv.adjoint(img.data(), m == "rtm" ? RTM_IMG : LSRTM_STEP)) return set_error(inv.error()), SEIS_ERR_HIP;
+  } else if (m == "lsrtm") {
+    if (!inv.cgsolver(p.maxiter.val, 0.f, img.data())) return set_error(inv.error()), SEIS_ERR_HIP;
+  } else {
+    return set_error("seis_run: mode must be modeling, born, rtm, rtm_born or lsrtm"), SEIS_ERR_ARG;
+  }
+  const std::string f = out_file ? out_file : p.wdir.val + (m == "lsrtm" ? "LSRTM_IMAGE.dat" : "RTM_IMAGE.dat");
+  if (!write_floats(f, n, img.data(), err)) return set_error(err), SEIS_ERR_ARG;
+  return SEIS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,269 @@
+#include "inversion.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <sstream>
+
+#include "../../../include/seis.h"
+#include "../wavelet.h"
+
+namespace seis {
+
+namespace {
+
+// Device buffer owned by the driver.
+struct Dev {
+  void* p = nullptr;
+  ~Dev() {
+    if (p) seis_free(p);
+  }
+  bool alloc(size_t bytes) { return seis_malloc(&p, bytes) == SEIS_OK; }
+  bool put(const void* src, size_t bytes) { return alloc(bytes) && seis_memcpy_htod(p, src, bytes) == SEIS_OK; }
+  bool get(void* dst, size_t bytes) const { return seis_memcpy_dtoh(dst, p, bytes) == SEIS_OK; }
+};
+
+// Everything one shot needs on its model window.
+struct ShotJob {
+  int is;
+  Window w;
+  std::vector<int> igz;
+  int sx, sz;
+};
+
+std::string gpu_error(const char* what) { return std::string(what) + ": " + seis_last_error(); }
+
+}  // namespace
+
+Inversion::Inversion(int rank, int nprocs) : rank_(rank), nprocs_(nprocs) {}
+
+bool Inversion::readParamFile(const std::string& file) { return read_params(file, param, err_); }
+
+bool Inversion::getConfig() {
+  if (!read_coords(param.coordfile.val, survey, err_)) return false;
+  if (survey.ns() != param.ns.val) {
+    std::ostringstream o;
+    o << "error : ns = " << param.ns.val << " from parameter files, " << survey.ns() << " from scanning "
+      << param.coordfile.val;
+    err_ = o.str();
+    return false;
+  }
+  if (survey.ngmax() != param.ngmax.val) {
+    std::ostringstream o;
+    o << "error : ngmax = " << param.ngmax.val << " from parameter files, " << survey.ngmax() << " from scanning "
+      << param.coordfile.val;
+    err_ = o.str();
+    return false;
+  }
+  return true;
+}
+
+bool Inversion::read_model(const std::string& file, std::vector<float>& v) {
+  v.assign((size_t)param.nx.val * param.nz.val, 0.f);
+  return read_floats(file, v.size(), v.data(), err_);
+}
+
+namespace {
+
+// Runs one of the engine's batched calls for shots grouped by window width.
+// `kind`: 0 modeling, 1 Born, 2 RTM.  models / perts are global [nx][nz];
+// records are per shot [nxw][nt] on input (RTM) or output (modeling, Born);
+// images per shot [nxw][nz] (RTM).
+bool run_windows(const Params& p, int kind, std::vector<ShotJob>& jobs, const float* model, const float* pert,
+                 std::map<int, std::vector<float>>& recs, std::map<int, std::vector<float>>& imgs, std::string& err) {
+  const int nz = p.nz.val, nt = p.nt.val, ndel = p.delaycal.val;
+  const int NT = nt + ndel;
+  std::vector<float> wav(NT);
+  ricker_host(p.dt.val, p.fr.val, p.delay.val + ndel, NT, wav.data());
+  std::map<int, std::vector<int>> groups;
+  for (size_t j = 0; j < jobs.size(); j++) groups[jobs[j].w.nx].push_back((int)j);
+  for (auto& gr : groups) {
+    const int nxw = gr.first;
+    const std::vector<int>& idx = gr.second;
+    const int n = (int)idx.size();
+    const size_t M = (size_t)nxw * nz, R = (size_t)nxw * nt;
+    std::vector<float> vm(M * n), dm(kind == 1 ? M * n : 0), wv((size_t)NT * n), rc(R * n, 0.f);
+    std::vector<int> gz((size_t)nxw * n);
+    std::vector<seis_shot> sh(n);
+    for (int j = 0; j < n; j++) {
+      const ShotJob& q = jobs[idx[j]];
+      window_model(q.w, nz, model, vm.data() + M * j);
+      if (kind == 1) window_model(q.w, nz, pert, dm.data() + M * j);
+      std::copy(wav.begin(), wav.end(), wv.begin() + (size_t)NT * j);
+      std::copy(q.igz.begin(), q.igz.end(), gz.begin() + (size_t)nxw * j);
+      if (kind == 2) std::copy(recs[q.is].begin(), recs[q.is].end(), rc.begin() + R * j);
+      sh[j] = seis_shot{q.sx, q.sz, nt, ndel, (long long)(M * j), (long long)NT * j, (long long)nxw * j,
+                        (long long)(R * j)};
+    }
+    seis_grid g{nxw, nz, p.npml.val, p.dt.val, p.dx.val, p.dz.val};
+    Dev dv, dd, dw, dg, dr, di;
+    if (!dv.put(vm.data(), sizeof(float) * vm.size()) || !dw.put(wv.data(), sizeof(float) * wv.size()) ||
+        !dg.put(gz.data(), sizeof(int) * gz.size()) || !dr.put(rc.data(), sizeof(float) * rc.size()) ||
+        (kind == 1 && !dd.put(dm.data(), sizeof(float) * dm.size())) ||
+        (kind == 2 && !di.alloc(sizeof(float) * M * n))) {
+      err = gpu_error("device buffers");
+      return false;
+    }
+    int rcode;
+    if (kind == 0)
+      rcode = seis_modeling_batch(&g, n, sh.data(), (float*)dv.p, (float*)dw.p, (int*)dg.p, (float*)dr.p);
+    else if (kind == 1)
+      rcode = seis_born_batch(&g, n, sh.data(), (float*)dv.p, (float*)dd.p, (float*)dw.p, (int*)dg.p, (float*)dr.p);
+    else
+      rcode = seis_rtm_batch(&g, n, sh.data(), (float*)dv.p, (float*)dw.p, (int*)dg.p, (float*)dr.p, (float*)di.p);
+    if (rcode) {
+      err = gpu_error("engine batch");
+      return false;
+    }
+    if (kind == 2) {
+      std::vector<float> im(M * n);
+      if (!di.get(im.data(), sizeof(float) * im.size())) return err = gpu_error("download"), false;
+      for (int j = 0; j < n; j++) imgs[jobs[idx[j]].is].assign(im.begin() + M * j, im.begin() + M * (j + 1));
+    } else {
+      if (!dr.get(rc.data(), sizeof(float) * rc.size())) return err = gpu_error("download"), false;
+      for (int j = 0; j < n; j++) recs[jobs[idx[j]].is].assign(rc.begin() + R * j, rc.begin() + R * (j + 1));
+    }
+  }
+  return true;
+}
+
+bool make_job(const Params& p, const Survey& s, int is, ShotJob& q, std::string& err) {
+  q.is = is;
+  q.w = shot_window(p, s, is);
+  if (q.w.nx <= 0) return err = "empty shot window", false;
+  if (!receiver_depths(p, s, is, q.w.fx, q.w.nx, q.igz, err)) return false;
+  source_cell(p, s, is, q.w.fx, &q.sx, &q.sz);
+  if (q.sx < 0 || q.sx >= q.w.nx || q.sz < 0 || q.sz >= p.nz.val) return err = "source outside its window", false;
+  return true;
+}
+
+// Records of the engine (per window column) -> muted gathers written to file.
+bool write_gathers(const Params& p, const Survey& s, std::vector<ShotJob>& jobs,
+                   std::map<int, std::vector<float>>& recs, const float* vmute, bool born, std::string& err) {
+  const int nt = p.nt.val;
+  for (ShotJob& q : jobs) {
+    std::vector<float> csg((size_t)s.ng(q.is) * nt);
+    if (!reorder(p, s, q.is, nt, csg.data(), recs[q.is].data(), q.w.fx, q.w.nx, true, err)) return false;
+    if (!remove_direct(p, s, q.is, csg.data(), vmute, err)) return false;
+    const std::string f = born ? born_name(p, q.is) : csg_name(p, q.is);
+    if (!write_floats(f, csg.size(), csg.data(), err)) return false;
+  }
+  return true;
+}
+
+}  // namespace
+
+bool Inversion::modeling(const float* v) {
+  std::vector<ShotJob> jobs(survey.ns());
+  for (int is = 0; is < survey.ns(); is++)
+    if (!make_job(param, survey, is, jobs[is], err_)) return false;
+  std::map<int, std::vector<float>> recs, imgs;
+  if (!run_windows(param, 0, jobs, v, nullptr, recs, imgs, err_)) return false;
+  return write_gathers(param, survey, jobs, recs, v, false, err_);
+}
+
+bool Inversion::forward(const float* dv) {
+  std::vector<float> v0;
+  if (!read_model(param.v0file.val, v0)) return false;
+  std::vector<ShotJob> jobs(survey.ns());
+  for (int is = 0; is < survey.ns(); is++)
+    if (!make_job(param, survey, is, jobs[is], err_)) return false;
+  std::map<int, std::vector<float>> recs, imgs;
+  if (!run_windows(param, 1, jobs, v0.data(), dv, recs, imgs, err_)) return false;
+  return write_gathers(param, survey, jobs, recs, v0.data(), true, err_);
+}
+
+bool Inversion::adjoint(float* img, int migTag) {
+  std::vector<float> v0;
+  if (!read_model(param.v0file.val, v0)) return false;
+  const int nx = param.nx.val, nz = param.nz.val, nt = param.nt.val, ns = survey.ns();
+  std::vector<ShotJob> jobs;
+  std::map<int, std::vector<float>> recs, imgs;
+  for (int is = std::max(0, param.sbegin.val); is <= std::min(ns - 1, param.send.val); is++) {
+    ShotJob q;
+    if (!make_job(param, survey, is, q, err_)) return false;
+    std::vector<float> csg((size_t)survey.ng(is) * nt);
+    const std::string f = migTag == RTM_IMG ? csg_name(param, is) : born_name(param, is);
+    if (!read_floats(f, csg.size(), csg.data(), err_)) return false;
+    if (!remove_direct(param, survey, is, csg.data(), v0.data(), err_)) return false;
+    std::vector<float>& r = recs[is];
+    r.assign((size_t)q.w.nx * nt, 0.f);
+    if (!reorder(param, survey, is, nt, csg.data(), r.data(), q.w.fx, q.w.nx, false, err_)) return false;
+    jobs.push_back(q);
+  }
+  if (!jobs.empty() && !run_windows(param, 2, jobs, v0.data(), nullptr, recs, imgs, err_)) return false;
+  std::vector<float> mask;
+  if (param.mask.val) {
+    mask.resize((size_t)nx * nz);
+    if (!read_floats(param.maskfile.val, mask.size(), mask.data(), err_)) return false;
+  }
+  std::vector<float> g((size_t)nx * nz);
+  for (ShotJob& q : jobs) {
+    window_to_global(param, imgs[q.is].data(), g.data(), q.w.fx, param.velfx.val, q.w.nx, nx, nz, false);
+    if (param.mask.val)
+      for (size_t i = 0; i < g.size(); i++) g[i] = g[i] * mask[i];
+    if (!write_floats(shot_image_name(param, q.is), g.size(), g.data(), err_)) return false;
+  }
+  // the image is the sum of every shot's file, including shots imaged by
+  // earlier runs (inversion.cpp:460-471)
+  std::fill(img, img + (size_t)nx * nz, 0.f);
+  for (int is = 0; is < ns; is++) {
+    if (!read_floats(shot_image_name(param, is), g.size(), g.data(), err_)) return false;
+    for (size_t i = 0; i < g.size(); i++) img[i] = g[i] + img[i];
+  }
+  return true;
+}
+
+bool Inversion::cgsolver(int maxIter, float reError, float* img, std::vector<double>* rnorm) {
+  std::vector<float> v0;
+  if (!read_model(param.v0file.val, v0)) return false;
+  const int nx = param.nx.val, nz = param.nz.val, nt = param.nt.val, ns = survey.ns();
+  const int ndel = param.delaycal.val, NT = nt + ndel;
+  // every shot on the whole model; data and mask per model column
+  std::vector<float> wav(NT), d((size_t)ns * nx * nt, 0.f), mask((size_t)ns * nx * nt, 0.f);
+  ricker_host(param.dt.val, param.fr.val, param.delay.val + ndel, NT, wav.data());
+  std::vector<float> wv((size_t)NT * ns);
+  std::vector<int> gz((size_t)nx * ns);
+  std::vector<seis_shot> sh(ns);
+  for (int is = 0; is < ns; is++) {
+    std::vector<int> igz;
+    if (!receiver_depths(param, survey, is, param.velfx.val, nx, igz, err_)) return false;
+    int sx, sz;
+    source_cell(param, survey, is, param.velfx.val, &sx, &sz);
+    std::vector<float> csg((size_t)survey.ng(is) * nt), ones(csg.size(), 1.f);
+    if (!read_floats(csg_name(param, is), csg.size(), csg.data(), err_)) return false;
+    if (!remove_direct(param, survey, is, csg.data(), v0.data(), err_)) return false;
+    if (!remove_direct(param, survey, is, ones.data(), v0.data(), err_)) return false;
+    float* dr = d.data() + (size_t)is * nx * nt;
+    float* mr = mask.data() + (size_t)is * nx * nt;
+    if (!reorder(param, survey, is, nt, csg.data(), dr, param.velfx.val, nx, false, err_)) return false;
+    if (!reorder(param, survey, is, nt, ones.data(), mr, param.velfx.val, nx, false, err_)) return false;
+    std::copy(wav.begin(), wav.end(), wv.begin() + (size_t)NT * is);
+    std::copy(igz.begin(), igz.end(), gz.begin() + (size_t)nx * is);
+    sh[is] = seis_shot{sx, sz, nt, ndel, 0, (long long)NT * is, (long long)nx * is, (long long)nx * nt * is};
+  }
+  Dev dv, dw, dg, dd, dm, di;
+  if (!dv.put(v0.data(), sizeof(float) * v0.size()) || !dw.put(wv.data(), sizeof(float) * wv.size()) ||
+      !dg.put(gz.data(), sizeof(int) * gz.size()) || !dd.put(d.data(), sizeof(float) * d.size()) ||
+      !dm.put(mask.data(), sizeof(float) * mask.size()) || !di.alloc(sizeof(float) * (size_t)nx * nz))
+    return err_ = gpu_error("device buffers"), false;
+  seis_grid g{nx, nz, param.npml.val, param.dt.val, param.dx.val, param.dz.val};
+  seis_lsrtm* plan = nullptr;
+  if (seis_lsrtm_create(&plan, &g, ns, sh.data(), (float*)dv.p, (float*)dw.p, (int*)dg.p, 1))
+    return err_ = gpu_error("lsrtm plan"), false;
+  std::vector<double> hist(2 * (maxIter + 1), 0.0);
+  int done = 0;
+  int rc = seis_lsrtm_set_mask(plan, dm.p);
+  if (!rc) rc = seis_lsrtm_cg(plan, dd.p, maxIter, (double)reError, di.p, hist.data(), &done);
+  seis_lsrtm_destroy(plan);
+  if (rc) return err_ = gpu_error("lsrtm cg"), false;
+  if (!di.get(img, sizeof(float) * (size_t)nx * nz)) return err_ = gpu_error("download"), false;
+  if (rnorm) {
+    rnorm->clear();
+    for (int i = 0; i <= done; i++) rnorm->push_back(hist[2 * i]);
+  }
+  return true;
+}
+
+}  // namespace seis

@@ -1,0 +1,28 @@
+// seis_run: command-line driver over a reference-style parameter file.
+//   seis_run <param file> modeling [model file]   shot records  <precsg><is>.dat
+//   seis_run <param file> born <perturbation file>  Born records <wdir>BORN<is>.dat
+//   seis_run <param file> rtm [image file]        RTM image of the shot records
+//   seis_run <param file> rtm_born [image file]   RTM image of the Born records
+//   seis_run <param file> lsrtm [image file]      least-squares RTM (maxiter CG steps)
+// Exit status is 0 on success and 1 on error (the reference exits 0 on errors).
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../../include/seis.h"
+
+int main(int argc, char** argv) {
+  if (argc < 3) {
+    std::fprintf(stderr, "usage: %s <param file> <modeling|born|rtm|rtm_born|lsrtm> [file]\n", argv[0]);
+    return 1;
+  }
+  const std::string mode = argv[2];
+  const char* arg = argc > 3 ? argv[3] : nullptr;
+  const bool takes_model = (mode == "modeling" || mode == "born");
+  const int rc = seis_run(argv[1], argv[2], takes_model ? arg : nullptr, takes_model ? nullptr : arg);
+  if (rc != SEIS_OK) {
+    std::fprintf(stderr, "seis_run: %s\n", seis_last_error());
+    return 1;
+  }
+  return 0;
+}

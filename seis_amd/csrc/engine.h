@@ -1,0 +1,220 @@
+// Shared device-side structures of the engine's kernels.
+#pragma once
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "fd_cell.h"
+#include "seis_internal.h"
+
+namespace seis {
+
+// ---------------------------------------------------------------------------
+// Device-side shot descriptor.
+template <typename T>
+struct DevShot {
+  const T* vel;    // [nx][nz]
+  const T* dv;     // [nx][nz] (Born) or null
+  const T* wav;    // [nt+ndel]
+  const int* gz;   // [nx]
+  T* rec;          // [nx][nt]
+  T* fld;          // slot 0 grid origin; slot j at fld + j*fstride
+  T* strips;       // [nt][8*(nx+nz)] (RTM)
+  T* img;          // padded image accumulator, grid origin (RTM)
+  T* aux;          // padded scratch field, grid origin (RTM: lap*2/v^3)
+  void* snap;      // Born-source snapshots [nt+ndel][nx][nz] (LSRTM)
+  float* scale;    // per-snapshot scale [nt+ndel] (LSRTM, 16-bit snapshots)
+  T* grad;         // per-shot image gradient [nx][nz] (LSRTM)
+  long long fstride;
+  int nx, NX, nt, ndel, sx, sz;
+  int snap_kind;   // SNAP_NATIVE, SNAP_BF16 or SNAP_FP16
+};
+
+// Step kernel modes.  MODEL / BORN / RTM_* follow the reference's modeling,
+// forward and adjoint (fd.cpp:15-177, 592-719); SNAP_* run the background of
+// the Born operator and record its per-step source coefficient; BORN_S
+// propagates the scattered field from those records.
+enum Mode { MODEL = 0, BORN = 1, RTM_FWD = 2, RTM_BWD = 3, SNAP_MAX = 4, SNAP_STORE = 5, BORN_S = 6 };
+
+// Storage of the Born-source snapshots vt2 * lap * 2/v^3 (model cells only).
+// 16-bit forms store value / scale[step], with scale[step] the step's max |value|.
+enum SnapKind { SNAP_NATIVE = 0, SNAP_BF16 = 1, SNAP_FP16 = 2 };
+
+SEIS_HD inline unsigned short f32_to_bf16(float f) {
+  unsigned int u;
+  __builtin_memcpy(&u, &f, 4);
+  u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even (finite inputs)
+  return (unsigned short)(u >> 16);
+}
+
+SEIS_HD inline float bf16_to_f32(unsigned short h) {
+  unsigned int u = (unsigned int)h << 16;
+  float f;
+  __builtin_memcpy(&f, &u, 4);
+  return f;
+}
+
+SEIS_HD inline unsigned short f32_to_f16(float f) {
+  __half h = __float2half_rn(f);
+  unsigned short u;
+  __builtin_memcpy(&u, &h, 2);
+  return u;
+}
+
+SEIS_HD inline float f16_to_f32(unsigned short u) {
+  __half h;
+  __builtin_memcpy(&h, &u, 2);
+  return __half2float(h);
+}
+
+// Snapshot value of model cell j = ix*nz + iz at step it.
+template <typename T>
+__device__ __forceinline__ T snap_load(const DevShot<T>& s, int nz, int it, long long j) {
+  const long long o = (long long)it * s.nx * nz + j;
+  if (s.snap_kind == SNAP_NATIVE) return reinterpret_cast<const T*>(s.snap)[o];
+  const unsigned short h = reinterpret_cast<const unsigned short*>(s.snap)[o];
+  const float f = s.snap_kind == SNAP_BF16 ? bf16_to_f32(h) : f16_to_f32(h);
+  return (T)(f * s.scale[it]);
+}
+
+template <typename T>
+__device__ __forceinline__ void snap_store(const DevShot<T>& s, int nz, int it, long long j, T v) {
+  const long long o = (long long)it * s.nx * nz + j;
+  if (s.snap_kind == SNAP_NATIVE) {
+    reinterpret_cast<T*>(s.snap)[o] = v;
+    return;
+  }
+  const float sc = s.scale[it];
+  const float q = sc > 0.f ? (float)v / sc : 0.f;
+  reinterpret_cast<unsigned short*>(s.snap)[o] = s.snap_kind == SNAP_BF16 ? f32_to_bf16(q) : f32_to_f16(q);
+}
+
+// Indices of (x, z) in the boundary-strip store of one time sample
+// (fd.cpp:1181-1225, layer = 4): top [nx][4], bottom [nx][4], left [4][nz],
+// right [4][nz].  On grids narrower than 8 cells a cell can sit in several
+// strips; every slot is returned so that save and restore stay consistent.
+__device__ __forceinline__ int strip_slots(int x, int z, int nx, int nz, int pml, int* s) {
+  const int ix = x - pml, iz = z - pml;
+  if (ix < 0 || ix >= nx || iz < 0 || iz >= nz) return 0;
+  int n = 0;
+  if (iz < 4) s[n++] = ix * 4 + iz;
+  if (iz >= nz - 4) s[n++] = nx * 4 + ix * 4 + (nz - 1 - iz);
+  if (ix < 4) s[n++] = 8 * nx + ix * nz + iz;
+  if (ix >= nx - 4) s[n++] = 8 * nx + 4 * nz + (nx - 1 - ix) * nz + iz;
+  return n;
+}
+
+// Per-step, per-shot state shared by both paths of the step kernel.
+template <typename T>
+struct StepCtx {
+  const T* P;       // previous level (slot 0 of the first field)
+  const T* C;       // current level
+  T* N;             // new level
+  Model<T> m;
+  int it, itp;
+  long long src;    // field index of the source cell
+};
+
+// Wavefront-local LDS exchange barrier (the hardware orders a wave's LDS
+// operations; the fences keep the compiler from reordering across it).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+constexpr int WAVES = 4;     // wavefronts per workgroup
+constexpr int WG = 64 * WAVES;
+constexpr int LROW = 72;     // one LDS row: 64 cells + 4 halo cells on each side
+
+// Batch-uniform split of the padded grid: the interior rectangle
+// [xa, xb) x [za, zb) holds only cells with w = 1 and the 9-point stencil;
+// it is swept by wavefronts (64 z cells each) streaming along x.  Every
+// other cell is a "band" cell updated by cell_update.
+struct Layout {
+  int xa, xb, za, zb;
+  int nzc, nxs, bx;   // z chunks, x strips, strip length of the interior sweep
+  int nfwg;           // workgroups per shot for the interior sweep
+  long long nband;    // band cells per shot
+  int nbwg;           // workgroups per shot for the band cells
+};
+
+template <typename T>
+Consts<T> make_consts(const GridSpec& g, long long ld) {
+  Consts<T> k;
+  k.nz = g.nz;
+  k.pml = g.pml;
+  k.NZ = g.nz + 2 * g.pml;
+  k.ld = ld;
+  k.dt = (T)g.dt;
+  k.dx = (T)g.dx;
+  k.dz = (T)g.dz;
+  k.idx2 = T(1) / (k.dx * k.dx);
+  k.idz2 = T(1) / (k.dz * k.dz);
+  k.idt2 = T(1) / k.dt / k.dt;
+  const T dt2 = k.dt * k.dt;
+  k.cx = (T(1) / k.dx) * (T(1) / k.dx) * dt2;
+  k.cz = (T(1) / k.dz) * (T(1) / k.dz) * dt2;
+  k.a1z = T(1) / k.dz / T(2) / k.dt;
+  k.a1x = T(1) / k.dx / T(2) / k.dt;
+  k.a2b = T(1) / T(2) / k.dt / k.dt;
+  k.b1 = T(1) / k.dz;
+  k.b2 = T(1) / k.dx;
+  k.sq2 = std::sqrt(T(2));
+  k.c[0] = T(-205) / T(72);
+  k.c[1] = T(8) / T(5);
+  k.c[2] = T(-1) / T(5);
+  k.c[3] = T(8) / T(315);
+  k.c[4] = T(-1) / T(560);
+  return k;
+}
+
+template <typename T>
+inline void hilbert_taps_host(int half, T* h) {
+  h[half] = T(0);
+  for (int i = 1; i <= half; i++) {
+    T taper = (T)(0.54 + 0.46 * std::cos(3.1415926 * i / half));
+    T hi = (T)((double)(taper * T(i % 2)) * 2.0 / (3.1415926 * i));
+    h[half + i] = hi;
+    h[half - i] = -hi;
+  }
+}
+
+// Field storage geometry: margin of MX columns / MZ rows of zeros around the
+// padded grid, z stride rounded to 64 elements.
+constexpr int MX = 4, MZ = 4;
+
+inline long long field_ld(int NZ) { return ((long long)(NZ + 2 * MZ) + 63) / 64 * 64; }
+
+inline long long field_elems(int NX, long long ld) { return (long long)(NX + 2 * MX) * ld; }
+
+inline Layout make_layout(int NX, int NZ, int pml) {
+  Layout L;
+  L.xa = std::max(pml, 4);
+  L.xb = std::min(NX - pml, NX - 4);
+  L.za = std::max(pml, 4);
+  L.zb = std::min(NZ - pml, NZ - 4);
+  if (L.xb <= L.xa || L.zb <= L.za) {  // no interior rectangle: every cell is a band cell
+    L.xa = L.xb = 0;
+    L.za = 0;
+    L.zb = NZ;
+  }
+  L.bx = 64;
+  L.nzc = (L.zb - L.za + 63) / 64;
+  L.nxs = (L.xb - L.xa + L.bx - 1) / L.bx;
+  if (L.xb == L.xa) L.nzc = L.nxs = 0;
+  L.nfwg = (L.nzc * L.nxs + WAVES - 1) / WAVES;
+  L.nband = (long long)NX * NZ - (long long)(L.xb - L.xa) * (L.zb - L.za);
+  L.nbwg = (int)((L.nband + WG - 1) / WG);
+  return L;
+}
+
+
+// Launches one step of `mode` for every shot of a batch (engine.hip).
+template <typename T>
+void launch_step(int mode, const Consts<T>& k, const DevShot<T>* dev, const Layout& L, int nshots, int step,
+                 hipStream_t stream);
+
+}  // namespace seis

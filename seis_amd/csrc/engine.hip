@@ -15,30 +15,12 @@
 #include <string>
 #include <vector>
 
+#include "engine.h"
 #include "fd_cell.h"
 #include "prof.h"
 #include "seis_internal.h"
 
 namespace seis {
-
-// ---------------------------------------------------------------------------
-// Device-side shot descriptor.
-template <typename T>
-struct DevShot {
-  const T* vel;    // [nx][nz]
-  const T* dv;     // [nx][nz] (Born) or null
-  const T* wav;    // [nt+ndel]
-  const int* gz;   // [nx]
-  T* rec;          // [nx][nt]
-  T* fld;          // slot 0 grid origin; slot j at fld + j*fstride
-  T* strips;       // [nt][8*(nx+nz)] (RTM)
-  T* img;          // padded image accumulator, grid origin (RTM)
-  T* aux;          // padded scratch field, grid origin (RTM: lap*2/v^3)
-  long long fstride;
-  int nx, NX, nt, ndel, sx, sz;
-};
-
-enum Mode { MODEL = 0, BORN = 1, RTM_FWD = 2, RTM_BWD = 3 };
 
 // Born source added to the scattered field (fd.cpp:134-139): vt2 * lap * 2/v^3 * dv,
 // lap = (u_new + u_prev - 2 u_cur) / dt^2 of the background.  dv is zero in the
@@ -69,30 +51,20 @@ struct BornInject {
   __device__ __forceinline__ T prev(const T* P, long long i) const { return P[i]; }
 };
 
-// Indices of (x, z) in the boundary-strip store of one time sample
-// (fd.cpp:1181-1225, layer = 4): top [nx][4], bottom [nx][4], left [4][nz],
-// right [4][nz].  On grids narrower than 8 cells a cell can sit in several
-// strips; every slot is returned so that save and restore stay consistent.
-__device__ __forceinline__ int strip_slots(int x, int z, int nx, int nz, int pml, int* s) {
-  const int ix = x - pml, iz = z - pml;
-  if (ix < 0 || ix >= nx || iz < 0 || iz >= nz) return 0;
-  int n = 0;
-  if (iz < 4) s[n++] = ix * 4 + iz;
-  if (iz >= nz - 4) s[n++] = nx * 4 + ix * 4 + (nz - 1 - iz);
-  if (ix < 4) s[n++] = 8 * nx + ix * nz + iz;
-  if (ix >= nx - 4) s[n++] = 8 * nx + 4 * nz + (nx - 1 - ix) * nz + iz;
-  return n;
-}
-
-// Per-step, per-shot state shared by both paths of the step kernel.
+// Born source read back from the snapshots: S(step) * p at model cells.
 template <typename T>
-struct StepCtx {
-  const T* P;       // previous level (slot 0 of the first field)
-  const T* C;       // current level
-  T* N;             // new level
-  Model<T> m;
-  int it, itp;
-  long long src;    // field index of the source cell
+struct SnapInject {
+  const DevShot<T>* s;
+  int nz, pml, it;
+  __device__ __forceinline__ T add(T val, int x, int z, T, long long) const {
+    const int ix = x - pml, iz = z - pml;
+    if (ix < 0 || ix >= s->nx || iz < 0 || iz >= nz) return val;
+    const long long j = (long long)ix * nz + iz;
+    const T d = s->dv[j];
+    if (d == T(0)) return val;
+    return val + snap_load(*s, nz, it, j) * d;
+  }
+  __device__ __forceinline__ T prev(const T* P, long long i) const { return P[i]; }
 };
 
 // Generic update of one cell: any cell of the padded grid, including the
@@ -109,7 +81,17 @@ __device__ __forceinline__ void cell_update(const Consts<T>& k, const DevShot<T>
   const T* C = c.C;
   T* N = c.N;
   const Model<T>& m = c.m;
-  if (MODE == MODEL || MODE == RTM_FWD) {
+  if (MODE == SNAP_MAX || MODE == SNAP_STORE) {
+    // band cells lie outside the model (pml >= 4 is required): background only
+    PointSource<T> ps{it - 1 >= 0 ? c.src : -1, it - 1 >= 0 ? s.wav[it - 1] : T(0)};
+    N[i] = final_value(k, m, NX, C, P, ps, x, z);
+  } else if (MODE == BORN_S) {
+    SnapInject<T> si{&s, k.nz, k.pml, it};
+    const T u = final_value(k, m, NX, C, P, si, x, z);
+    N[i] = u;
+    const int ix = x - k.pml;
+    if (itp >= 0 && ix >= 0 && ix < s.nx && z == s.gz[ix] + k.pml) s.rec[(long long)ix * s.nt + itp] = u;
+  } else if (MODE == MODEL || MODE == RTM_FWD) {
     PointSource<T> ps{it - 1 >= 0 ? c.src : -1, it - 1 >= 0 ? s.wav[it - 1] : T(0)};
     const T u = final_value(k, m, NX, C, P, ps, x, z);
     N[i] = u;
@@ -168,30 +150,6 @@ __device__ __forceinline__ void cell_update(const Consts<T>& k, const DevShot<T>
   }
 }
 
-// Wavefront-local LDS exchange barrier (the hardware orders a wave's LDS
-// operations; the fences keep the compiler from reordering across it).
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-constexpr int WAVES = 4;     // wavefronts per workgroup
-constexpr int WG = 64 * WAVES;
-constexpr int LROW = 72;     // one LDS row: 64 cells + 4 halo cells on each side
-
-// Batch-uniform split of the padded grid: the interior rectangle
-// [xa, xb) x [za, zb) holds only cells with w = 1 and the 9-point stencil;
-// it is swept by wavefronts (64 z cells each) streaming along x.  Every
-// other cell is a "band" cell updated by cell_update.
-struct Layout {
-  int xa, xb, za, zb;
-  int nzc, nxs, bx;   // z chunks, x strips, strip length of the interior sweep
-  int nfwg;           // workgroups per shot for the interior sweep
-  long long nband;    // band cells per shot
-  int nbwg;           // workgroups per shot for the band cells
-};
-
 // Stencil sums of the 9-point operator in the reference's order (fd.cpp:1263-1269).
 template <typename T>
 __device__ __forceinline__ T d2_z(const Consts<T>& k, const T* r) {
@@ -241,8 +199,9 @@ __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot
     if (TWO) qb[j] = CB[(long long)(x0 - 4 + j) * ld + zl];
   }
   // per-step scalars
+  constexpr bool PSRC = (MODE != RTM_BWD && MODE != BORN_S);
   T amp = T(0);
-  if (MODE != RTM_BWD && c.it - 1 >= 0) amp = s.wav[c.it - 1];
+  if (PSRC && c.it - 1 >= 0) amp = s.wav[c.it - 1];
   T sub = T(0);
   if (MODE == RTM_BWD && c.it - 1 > 0) {
     const T as = k.dt * c.m.vel(sxp, szp);
@@ -279,7 +238,19 @@ __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot
     T pa = PA[i];
     if (MODE == RTM_BWD && sub != T(0) && x == sxp && z == szp) pa = pa - sub;
     T ua = vvzz * uzz + vvxx * uxx + T(2) * qa[4] - pa;
-    if (MODE != RTM_BWD && x == sxp && z == szp && c.it - 1 >= 0) ua = ua + vt2 * amp;
+    if (PSRC && x == sxp && z == szp && c.it - 1 >= 0) ua = ua + vt2 * amp;
+    if (MODE == SNAP_MAX || MODE == SNAP_STORE) {
+      // Born source coefficient of this step (fd.cpp:134-139): vt2 * lap * 2/v^3
+      const T lap = (ua + pa - T(2) * qa[4]) * k.idt2;
+      const T S = vt2 * lap * (T(2) / (v * v * v));
+      if (MODE == SNAP_MAX) {
+        float a = act ? fabsf((float)S) : 0.f;
+        for (int o = 32; o >= 1; o >>= 1) a = fmaxf(a, __shfl_xor(a, o, 64));
+        if (lane == 0) atomicMax(reinterpret_cast<unsigned int*>(s.scale) + c.it, __float_as_uint(a));
+      } else if (act) {
+        snap_store(s, k.nz, c.it, (long long)ix * k.nz + iz, S);
+      }
+    }
     T ub = T(0);
     if (TWO) {
       const T uzzb = d2_z(k, lb + lane);
@@ -296,7 +267,15 @@ __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot
     }
     wave_sync();
     if (act) {
-      if (MODE == MODEL || MODE == RTM_FWD) {
+      if (MODE == SNAP_MAX || MODE == SNAP_STORE) {
+        NA[i] = ua;
+      } else if (MODE == BORN_S) {
+        const long long j = (long long)ix * k.nz + iz;
+        const T d = s.dv[j];
+        if (d != T(0)) ua = ua + snap_load(s, k.nz, c.it, j) * d;
+        NA[i] = ua;
+        if (c.itp >= 0 && z == s.gz[ix] + pml) s.rec[(long long)ix * s.nt + c.itp] = ua;
+      } else if (MODE == MODEL || MODE == RTM_FWD) {
         NA[i] = ua;
         if (MODE == MODEL) {
           if (c.itp >= 0 && z == s.gz[ix] + pml) s.rec[(long long)ix * s.nt + c.itp] = ua;
@@ -469,74 +448,27 @@ __global__ void crop_kernel(Consts<T> k, const DevShot<T>* __restrict__ shots, T
 // Host side.
 
 template <typename T>
-Consts<T> make_consts(const GridSpec& g, long long ld) {
-  Consts<T> k;
-  k.nz = g.nz;
-  k.pml = g.pml;
-  k.NZ = g.nz + 2 * g.pml;
-  k.ld = ld;
-  k.dt = (T)g.dt;
-  k.dx = (T)g.dx;
-  k.dz = (T)g.dz;
-  k.idx2 = T(1) / (k.dx * k.dx);
-  k.idz2 = T(1) / (k.dz * k.dz);
-  k.idt2 = T(1) / k.dt / k.dt;
-  const T dt2 = k.dt * k.dt;
-  k.cx = (T(1) / k.dx) * (T(1) / k.dx) * dt2;
-  k.cz = (T(1) / k.dz) * (T(1) / k.dz) * dt2;
-  k.a1z = T(1) / k.dz / T(2) / k.dt;
-  k.a1x = T(1) / k.dx / T(2) / k.dt;
-  k.a2b = T(1) / T(2) / k.dt / k.dt;
-  k.b1 = T(1) / k.dz;
-  k.b2 = T(1) / k.dx;
-  k.sq2 = std::sqrt(T(2));
-  k.c[0] = T(-205) / T(72);
-  k.c[1] = T(8) / T(5);
-  k.c[2] = T(-1) / T(5);
-  k.c[3] = T(8) / T(315);
-  k.c[4] = T(-1) / T(560);
-  return k;
-}
-
-template <typename T>
-void hilbert_taps_host(int half, T* h) {
-  h[half] = T(0);
-  for (int i = 1; i <= half; i++) {
-    T taper = (T)(0.54 + 0.46 * std::cos(3.1415926 * i / half));
-    T hi = (T)((double)(taper * T(i % 2)) * 2.0 / (3.1415926 * i));
-    h[half + i] = hi;
-    h[half - i] = -hi;
+void launch_step(int mode, const Consts<T>& k, const DevShot<T>* dev, const Layout& L, int nshots, int step,
+                 hipStream_t stream) {
+  dim3 grid(L.nfwg + L.nbwg, nshots), block(WG);
+  switch (mode) {
+#define SEIS_CASE(M) \
+  case M: hipLaunchKernelGGL((step_kernel<T, M>), grid, block, 0, stream, k, dev, L, step); break;
+    SEIS_CASE(MODEL)
+    SEIS_CASE(BORN)
+    SEIS_CASE(RTM_FWD)
+    SEIS_CASE(RTM_BWD)
+    SEIS_CASE(SNAP_MAX)
+    SEIS_CASE(SNAP_STORE)
+    SEIS_CASE(BORN_S)
+#undef SEIS_CASE
   }
 }
 
-// Field storage geometry: margin of MX columns / MZ rows of zeros around the
-// padded grid, z stride rounded to 64 elements.
-constexpr int MX = 4, MZ = 4;
-
-long long field_ld(int NZ) { return ((long long)(NZ + 2 * MZ) + 63) / 64 * 64; }
-
-long long field_elems(int NX, long long ld) { return (long long)(NX + 2 * MX) * ld; }
-
-Layout make_layout(int NX, int NZ, int pml) {
-  Layout L;
-  L.xa = std::max(pml, 4);
-  L.xb = std::min(NX - pml, NX - 4);
-  L.za = std::max(pml, 4);
-  L.zb = std::min(NZ - pml, NZ - 4);
-  if (L.xb <= L.xa || L.zb <= L.za) {  // no interior rectangle: every cell is a band cell
-    L.xa = L.xb = 0;
-    L.za = 0;
-    L.zb = NZ;
-  }
-  L.bx = 64;
-  L.nzc = (L.zb - L.za + 63) / 64;
-  L.nxs = (L.xb - L.xa + L.bx - 1) / L.bx;
-  if (L.xb == L.xa) L.nzc = L.nxs = 0;
-  L.nfwg = (L.nzc * L.nxs + WAVES - 1) / WAVES;
-  L.nband = (long long)NX * NZ - (long long)(L.xb - L.xa) * (L.zb - L.za);
-  L.nbwg = (int)((L.nband + WG - 1) / WG);
-  return L;
-}
+template void launch_step<float>(int, const Consts<float>&, const DevShot<float>*, const Layout&, int, int,
+                                 hipStream_t);
+template void launch_step<double>(int, const Consts<double>&, const DevShot<double>*, const Layout&, int, int,
+                                  hipStream_t);
 
 template <typename T>
 struct BatchRun {

@@ -1,0 +1,284 @@
+// Exact transpose of one step of the Born scattered-field recursion, in
+// gather form.
+//
+// The forward step (fd_cell.h) maps (cur, prev, injection) to the new level
+// through: stencil (or ring formula) -> raw; one-way band update ow(raw, cur,
+// prev); blend w*raw + (1-w)*ow.  Its transpose, run backwards in time, maps
+// the adjoint levels lam^m (complete) and lam^{m+1} to lam^{m-1}:
+//
+//   lam^{m-1} = sum over cells c that read cur(.) at step m of
+//               (adjoint of raw/ow of c at level m) * d(c)/d cur
+//             + the same for prev(.) with level m+1 + record injection.
+//
+// Every term is a gather, so each cell of lam^{m-1} is a pure function of
+// lam^m, lam^{m+1} and per-band-cell products stored when those levels were
+// written (BandAdj below).  The scatter-form statement-by-statement transpose
+// in oracle/seis_oracle.cpp (born_adjoint) is the checker.
+//
+// The reference has no transpose of its Born operator (its `adjoint`,
+// fd.cpp:592-719, is an imaging operator); this is the operator pair that
+// least-squares RTM needs.
+#pragma once
+#include "fd_cell.h"
+
+namespace seis {
+
+// Position of a padded cell relative to the absorbing band, with the same
+// partition as processBoundary (fd.cpp:1092-1178) and final_value.
+struct BandCell {
+  int kind;    // 0 interior, 1 z side (top/bottom), 2 x side (left/right), 3 corner
+  int s;       // sides: +1 when the inner neighbour is at +1 along the normal
+  int sx, sz;  // corners: direction of the x and z neighbours that are read
+};
+
+SEIS_HD inline BandCell classify(int x, int z, int NX, int NZ, int pml) {
+  const int dl = x, dr = NX - 1 - x, dtp = z, db = NZ - 1 - z;
+  const int mn = imin(imin(dl, dr), imin(dtp, db));
+  BandCell b{0, 0, 0, 0};
+  if (mn >= pml) return b;
+  const bool zs = (dtp == mn || db == mn), xs = (dl == mn || dr == mn);
+  if (zs && xs) {
+    b.kind = 3;
+    b.sz = (dtp == mn) ? 1 : -1;
+    b.sx = (dl == mn) ? 1 : -1;
+  } else if (zs) {
+    b.kind = 1;
+    b.s = (dtp == mn) ? 1 : -1;
+  } else {
+    b.kind = 2;
+    b.s = (dl == mn) ? 1 : -1;
+  }
+  return b;
+}
+
+// Products stored for every band cell b of an adjoint level lam, with
+// at = -(1-w) lam(b) / den the adjoint of the one-way quantity t of b:
+//   sides  : in = (-a1+a2+2a3) at   (raw(b_in) and prev(b) coefficient)
+//            t  = -a3 at            (raw and prev tangent neighbours)
+//            c  = -2 a2 at          (cur(b) and cur(b_in))
+//   corners: in = -b1 at (raw z neighbour), t = -b2 at (raw x neighbour),
+//            c = -b3 at (cur(b))
+template <typename T>
+struct BandAdj {
+  T in, t, c;
+};
+
+template <typename T>
+SEIS_HD inline BandAdj<T> band_adj(const Consts<T>& k, const Model<T>& m, int x, int z, const BandCell& b, T lam) {
+  BandAdj<T> r{T(0), T(0), T(0)};
+  if (b.kind == 0) return r;
+  const T v = m.vel(x, z);
+  const T w = sponge_weight<T>(x, z, m.nx, m.nz, k.pml);
+  if (b.kind == 3) {
+    const T b3 = (b.sx == 1 && b.sz == 1) ? (T)(std::sqrt(2.0) / (double)v / (double)k.dt) : k.sq2 / v / k.dt;
+    const T at = -(T(1) - w) * lam / (k.b1 + k.b2 + b3);
+    r.in = -k.b1 * at;
+    r.t = -k.b2 * at;
+    r.c = -b3 * at;
+    return r;
+  }
+  const T a1 = (b.kind == 1) ? k.a1z : k.a1x;
+  const T a2 = k.a2b / v;
+  const T h = (b.kind == 1) ? k.dx : k.dz;
+  const T a3 = v / T(4) / h / h;
+  const T at = -(T(1) - w) * lam / (a1 + a2);
+  r.in = (-a1 + a2 + T(2) * a3) * at;
+  r.t = -a3 * at;
+  r.c = -T(2) * a2 * at;
+  return r;
+}
+
+// One adjoint level with its band products (all indexed like fields).
+template <typename T>
+struct AdjLevel {
+  const T* lam;
+  const T* ain;
+  const T* at;
+  const T* ac;
+};
+
+// Adjoint of the raw (pre-blend) value of cell (x, z) at a level:
+// w lam(c) + sum over the band cells that read raw(c) of their stored product.
+template <typename T>
+SEIS_HD inline T adj_raw(const Consts<T>& k, int NX, int nx, const AdjLevel<T>& L, int x, int z) {
+  const int NZ = k.NZ, pml = k.pml;
+  const long long ld = k.ld;
+  const T w = sponge_weight<T>(x, z, nx, k.nz, pml);
+  T r = w * L.lam[fidx(ld, x, z)];
+  // cells at least two cells inside the interior have no band reader
+  if (x > pml && x < NX - 1 - pml && z > pml && z < NZ - 1 - pml) return r;
+  if (z - 1 >= 0) {  // readers above: top sides and top corners
+    const BandCell b = classify(x, z - 1, NX, NZ, pml);
+    if ((b.kind == 1 && b.s == 1) || (b.kind == 3 && b.sz == 1)) r += L.ain[fidx(ld, x, z - 1)];
+    for (int e = -1; e <= 1; e += 2) {
+      if (x + e < 0 || x + e >= NX) continue;
+      const BandCell bt = classify(x + e, z - 1, NX, NZ, pml);
+      if (bt.kind == 1 && bt.s == 1) r += L.at[fidx(ld, x + e, z - 1)];
+    }
+  }
+  if (z + 1 < NZ) {  // readers below: bottom sides and bottom corners
+    const BandCell b = classify(x, z + 1, NX, NZ, pml);
+    if ((b.kind == 1 && b.s == -1) || (b.kind == 3 && b.sz == -1)) r += L.ain[fidx(ld, x, z + 1)];
+    for (int e = -1; e <= 1; e += 2) {
+      if (x + e < 0 || x + e >= NX) continue;
+      const BandCell bt = classify(x + e, z + 1, NX, NZ, pml);
+      if (bt.kind == 1 && bt.s == -1) r += L.at[fidx(ld, x + e, z + 1)];
+    }
+  }
+  if (x - 1 >= 0) {  // readers to the left: left sides and left corners
+    const BandCell b = classify(x - 1, z, NX, NZ, pml);
+    if (b.kind == 2 && b.s == 1) r += L.ain[fidx(ld, x - 1, z)];
+    if (b.kind == 3 && b.sx == 1) r += L.at[fidx(ld, x - 1, z)];
+    for (int e = -1; e <= 1; e += 2) {
+      if (z + e < 0 || z + e >= NZ) continue;
+      const BandCell bt = classify(x - 1, z + e, NX, NZ, pml);
+      if (bt.kind == 2 && bt.s == 1) r += L.at[fidx(ld, x - 1, z + e)];
+    }
+  }
+  if (x + 1 < NX) {  // readers to the right
+    const BandCell b = classify(x + 1, z, NX, NZ, pml);
+    if (b.kind == 2 && b.s == -1) r += L.ain[fidx(ld, x + 1, z)];
+    if (b.kind == 3 && b.sx == -1) r += L.at[fidx(ld, x + 1, z)];
+    for (int e = -1; e <= 1; e += 2) {
+      if (z + e < 0 || z + e >= NZ) continue;
+      const BandCell bt = classify(x + 1, z + e, NX, NZ, pml);
+      if (bt.kind == 2 && bt.s == -1) r += L.at[fidx(ld, x + 1, z + e)];
+    }
+  }
+  return r;
+}
+
+SEIS_HD inline bool is_ring(int x, int z, int NX, int NZ) { return x == 0 || z == 0 || x == NX - 1 || z == NZ - 1; }
+
+SEIS_HD inline bool high_order(int x, int z, int NX, int NZ) { return x >= 4 && x <= NX - 5 && z >= 4 && z <= NZ - 5; }
+
+// lam^{m-1}(x, z) from level m (cur terms, L1) and level m+1 (prev terms, L2),
+// without the record injection.  ar1 / ar2 give adj_raw of a cell at level m /
+// m+1 (computed directly or read from a precomputed field).  Also returns the
+// adjoint of the raw value of the cell at level m (the gradient's imaging
+// factor).
+template <typename T, class AR1, class AR2>
+SEIS_HD inline T adj_value(const Consts<T>& k, const Model<T>& m, int NX, const AdjLevel<T>& L1,
+                           const AdjLevel<T>& L2, const AR1& ar1, const AR2& ar2, int x, int z, T* raw_adj_out) {
+  const int NZ = k.NZ, pml = k.pml, nx = m.nx;
+  const long long ld = k.ld;
+  auto vt2 = [&](int xx, int zz) {
+    const T a = k.dt * m.vel(xx, zz);
+    return a * a;
+  };
+  auto vv = [&](int xx, int zz) {
+    const T v = m.vel(xx, zz);
+    return v * v;
+  };
+  T acc = T(0);
+  const T a_self = ar1(x, z);
+  if (raw_adj_out) *raw_adj_out = a_self;
+  // (i) stencil reads of cur by non-ring cells
+  if (!is_ring(x, z, NX, NZ)) {
+    const T v2 = vt2(x, z);
+    const T az = a_self * (v2 * k.idz2), ax = a_self * (v2 * k.idx2);
+    acc += T(2) * a_self;
+    if (high_order(x, z, NX, NZ)) {
+      acc += az * k.c[0] + ax * k.c[0];
+    } else {
+      acc -= T(2) * az;
+      acc -= T(2) * ax;
+    }
+  }
+  for (int d = 1; d <= 4; d++) {
+    for (int e = -1; e <= 1; e += 2) {
+      // z neighbour c = (x, z - e d) reads cur(x, z) at offset e d
+      const int cz = z - e * d;
+      if (cz >= 1 && cz <= NZ - 2 && x >= 1 && x <= NX - 2) {
+        const bool hi = high_order(x, cz, NX, NZ);
+        if (hi || d == 1) {
+          const T a = ar1(x, cz) * (vt2(x, cz) * k.idz2);
+          acc += hi ? a * k.c[d] : a;
+        }
+      }
+      const int cx = x - e * d;
+      if (cx >= 1 && cx <= NX - 2 && z >= 1 && z <= NZ - 2) {
+        const bool hi = high_order(cx, z, NX, NZ);
+        if (hi || d == 1) {
+          const T a = ar1(cx, z) * (vt2(cx, z) * k.idx2);
+          acc += hi ? a * k.c[d] : a;
+        }
+      }
+    }
+  }
+  // (ii) ring formulas reading cur (fd.cpp:1060-1091)
+  if (is_ring(x, z, NX, NZ)) {
+    const bool xe = (x == 0 || x == NX - 1), ze = (z == 0 || z == NZ - 1);
+    if (xe && ze) {
+      acc += T(2) * a_self;
+    } else {
+      const T f = ze ? k.cx : (x == 0 ? k.cz : k.cx);
+      acc += a_self * (T(2) - T(2) * (vv(x, z) * f));
+    }
+  }
+  if (z == 0 || z == NZ - 1) {  // edge-parallel neighbours on the top/bottom rows
+    for (int e = -1; e <= 1; e += 2) {
+      const int rx = x + e;
+      if (rx >= 1 && rx <= NX - 2) acc += ar1(rx, z) * (vv(rx, z) * k.cx);
+    }
+  }
+  if (x == 0 || x == NX - 1) {  // edge-parallel neighbours on the left/right columns
+    const T f = (x == 0) ? k.cz : k.cx;
+    for (int e = -1; e <= 1; e += 2) {
+      const int rz = z + e;
+      if (rz >= 1 && rz <= NZ - 2) acc += ar1(x, rz) * (vv(x, rz) * f);
+    }
+  }
+  // extrapolated normal differences: ring cell (x, 0) reads cur(x, 0..3), etc.
+  for (int side = 0; side < 2; side++) {
+    const int z0 = side == 0 ? 0 : NZ - 1, dz = side == 0 ? 1 : -1;
+    const int j = (z - z0) * dz;
+    if (j >= 0 && j <= 3) {
+      const T v1 = vv(x, z0 + dz) * k.cz, v2 = vv(x, z0 + 2 * dz) * k.cz;
+      const T cf = j == 0 ? T(2) * v1 : j == 1 ? -T(4) * v1 - v2 : j == 2 ? T(2) * v1 + T(2) * v2 : -v2;
+      acc += ar1(x, z0) * cf;
+    }
+    const int x0 = side == 0 ? 0 : NX - 1, dx = side == 0 ? 1 : -1;
+    const int i = (x - x0) * dx;
+    if (i >= 0 && i <= 3) {
+      const T v1 = vv(x0 + dx, z) * k.cx, v2 = vv(x0 + 2 * dx, z) * k.cx;
+      const T cf = i == 0 ? T(2) * v1 : i == 1 ? -T(4) * v1 - v2 : i == 2 ? T(2) * v1 + T(2) * v2 : -v2;
+      acc += ar1(x0, z) * cf;
+    }
+  }
+  // (iii) one-way reads of cur: cur(b) by band cell b, cur(b_in) by side cell b
+  const BandCell self = classify(x, z, NX, NZ, pml);
+  if (self.kind != 0) acc += L1.ac[fidx(ld, x, z)];
+  {
+    BandCell b;
+    if (z - 1 >= 0 && (b = classify(x, z - 1, NX, NZ, pml), b.kind == 1 && b.s == 1)) acc += L1.ac[fidx(ld, x, z - 1)];
+    if (z + 1 < NZ && (b = classify(x, z + 1, NX, NZ, pml), b.kind == 1 && b.s == -1)) acc += L1.ac[fidx(ld, x, z + 1)];
+    if (x - 1 >= 0 && (b = classify(x - 1, z, NX, NZ, pml), b.kind == 2 && b.s == 1)) acc += L1.ac[fidx(ld, x - 1, z)];
+    if (x + 1 < NX && (b = classify(x + 1, z, NX, NZ, pml), b.kind == 2 && b.s == -1)) acc += L1.ac[fidx(ld, x + 1, z)];
+  }
+  // (iv) -prev in the stencil and in the ring base formula, level m+1
+  acc -= ar2(x, z);
+  // (v) one-way reads of prev at level m+1
+  if (self.kind == 1 || self.kind == 2) acc += L2.ain[fidx(ld, x, z)];
+  {
+    auto pin = [&](int bx, int bz) {  // (a1+a2) at = -(1-w) lam for the inner neighbour
+      const T w = sponge_weight<T>(bx, bz, nx, k.nz, pml);
+      return -(T(1) - w) * L2.lam[fidx(ld, bx, bz)];
+    };
+    BandCell b;
+    if (z - 1 >= 0 && (b = classify(x, z - 1, NX, NZ, pml), b.kind == 1 && b.s == 1)) acc += pin(x, z - 1);
+    if (z + 1 < NZ && (b = classify(x, z + 1, NX, NZ, pml), b.kind == 1 && b.s == -1)) acc += pin(x, z + 1);
+    if (x - 1 >= 0 && (b = classify(x - 1, z, NX, NZ, pml), b.kind == 2 && b.s == 1)) acc += pin(x - 1, z);
+    if (x + 1 < NX && (b = classify(x + 1, z, NX, NZ, pml), b.kind == 2 && b.s == -1)) acc += pin(x + 1, z);
+    // tangent neighbours: z sides on the same row, x sides on the same column
+    for (int e = -1; e <= 1; e += 2) {
+      if (x + e >= 0 && x + e < NX && (b = classify(x + e, z, NX, NZ, pml), b.kind == 1))
+        acc += L2.at[fidx(ld, x + e, z)];
+      if (z + e >= 0 && z + e < NZ && (b = classify(x, z + e, NX, NZ, pml), b.kind == 2))
+        acc += L2.at[fidx(ld, x, z + e)];
+    }
+  }
+  return acc;
+}
+
+}  // namespace seis

@@ -17,7 +17,14 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <cmath>
+
+#define SEIS_HD __host__ __device__
+
 namespace seis {
+
+SEIS_HD inline int imin(int a, int b) { return a < b ? a : b; }
+SEIS_HD inline int imax(int a, int b) { return a > b ? a : b; }
 
 // Step constants shared by all shots of a batch, formed on the host in T with
 // the reference's expression order.
@@ -43,20 +50,20 @@ template <typename T>
 struct Model {
   const T* v;   // [nx][nz]
   int nx, nz, pml;
-  __device__ __forceinline__ T vel(int x, int z) const {
-    int xi = min(max(x - pml, 0), nx - 1);
-    int zi = min(max(z - pml, 0), nz - 1);
+  SEIS_HD inline T vel(int x, int z) const {
+    int xi = imin(imax(x - pml, 0), nx - 1);
+    int zi = imin(imax(z - pml, 0), nz - 1);
     return v[(long long)xi * nz + zi];
   }
 };
 
 // Linear index of (x, z) in field storage.
-__device__ __forceinline__ long long fidx(long long ld, int x, int z) { return (long long)x * ld + z; }
+SEIS_HD inline long long fidx(long long ld, int x, int z) { return (long long)x * ld + z; }
 
 // Sponge weight of setAbs (InitInversion.cpp:11-40): 1 inside, linear ramp
 // in the band, x ramp wins in the corners.
 template <typename T>
-__device__ __forceinline__ T sponge_weight(int x, int z, int nx, int nz, int pml) {
+SEIS_HD inline T sponge_weight(int x, int z, int nx, int nz, int pml) {
   int i;
   if (x < pml) i = pml - x - 1;
   else if (x >= pml + nx) i = x - pml - nx;
@@ -70,9 +77,9 @@ __device__ __forceinline__ T sponge_weight(int x, int z, int nx, int nz, int pml
 // cell and may correct the previous level (reverse-time source removal).
 struct NoInject {
   template <typename T>
-  __device__ __forceinline__ T add(T val, int, int, T, long long) const { return val; }
+  SEIS_HD inline T add(T val, int, int, T, long long) const { return val; }
   template <typename T>
-  __device__ __forceinline__ T prev(const T* P, long long i) const { return P[i]; }
+  SEIS_HD inline T prev(const T* P, long long i) const { return P[i]; }
 };
 
 // Point source u += vt2 * wav[it-1] at one cell (fd.cpp:49-51).
@@ -80,11 +87,11 @@ template <typename T>
 struct PointSource {
   long long src;  // field index of the source cell, -1 when inactive this step
   T amp;          // wav[it-1]
-  __device__ __forceinline__ T add(T val, int, int, T vt2, long long i) const {
+  SEIS_HD inline T add(T val, int, int, T vt2, long long i) const {
     return i == src ? val + vt2 * amp : val;
   }
   template <typename U>
-  __device__ __forceinline__ U prev(const U* P, long long i) const { return P[i]; }
+  SEIS_HD inline U prev(const U* P, long long i) const { return P[i]; }
 };
 
 // Reverse-time source removal from the previous level (fd.cpp:659-661):
@@ -94,8 +101,8 @@ struct SourceRemoval {
   long long src;
   T sub;  // vt2[src] * wav[it-1], 0 when inactive
   bool on;
-  __device__ __forceinline__ T add(T val, int, int, T, long long) const { return val; }
-  __device__ __forceinline__ T prev(const T* P, long long i) const {
+  SEIS_HD inline T add(T val, int, int, T, long long) const { return val; }
+  SEIS_HD inline T prev(const T* P, long long i) const {
     return (on && i == src) ? P[i] - sub : P[i];
   }
 };
@@ -107,20 +114,20 @@ struct ReceiverInject {
   const int* gz;      // [nx] receiver depth per column (model cells)
   const T* rec;       // [nx][nt]
   int nx, nt, itp, pml;
-  __device__ __forceinline__ T add(T val, int x, int z, T vt2, long long) const {
+  SEIS_HD inline T add(T val, int x, int z, T vt2, long long) const {
     int ix = x - pml;
     if (ix >= 0 && ix < nx && z == gz[ix] + pml) return val + rec[(long long)ix * nt + itp] * vt2;
     return val;
   }
   template <typename U>
-  __device__ __forceinline__ U prev(const U* P, long long i) const { return P[i]; }
+  SEIS_HD inline U prev(const U* P, long long i) const { return P[i]; }
 };
 
 // ---------------------------------------------------------------------------
 // Stencil value ("u3" of processBoundary) of cell (x, z): stencil + injection
 // inside, edge formula on the outer ring.
 template <typename T, class Inj>
-__device__ __forceinline__ T raw_value(const Consts<T>& k, const Model<T>& m, int NX, const T* C, const T* P,
+SEIS_HD inline T raw_value(const Consts<T>& k, const Model<T>& m, int NX, const T* C, const T* P,
                                        const Inj& inj, int x, int z) {
   const long long ld = k.ld;
   const int NZ = k.NZ;
@@ -200,13 +207,13 @@ __device__ __forceinline__ T raw_value(const Consts<T>& k, const Model<T>& m, in
 // New value of cell (x, z) after the absorbing boundary and the sponge blend.
 // `raw_out` receives the stencil value of the cell itself.
 template <typename T, class Inj>
-__device__ __forceinline__ T final_value(const Consts<T>& k, const Model<T>& m, int NX, const T* C, const T* P,
+SEIS_HD inline T final_value(const Consts<T>& k, const Model<T>& m, int NX, const T* C, const T* P,
                                          const Inj& inj, int x, int z, T* raw_out = nullptr) {
   const T r = raw_value(k, m, NX, C, P, inj, x, z);
   if (raw_out) *raw_out = r;
   const int pml = k.pml, NZ = k.NZ;
   const int dl = x, dr = NX - 1 - x, dtp = z, db = NZ - 1 - z;
-  const int mn = min(min(dl, dr), min(dtp, db));
+  const int mn = imin(imin(dl, dr), imin(dtp, db));
   if (mn >= pml) return r;
   const long long ld = k.ld;
   const long long i = fidx(ld, x, z);
@@ -221,7 +228,7 @@ __device__ __forceinline__ T final_value(const Consts<T>& k, const Model<T>& m, 
     const int sx = (dl == mn) ? 1 : -1;
     T b3;
     if (sx == 1 && sz == 1)
-      b3 = (T)(sqrt(2.0) / (double)v / (double)k.dt);  // fd.cpp:1176 forms sqrt(2) in double
+      b3 = (T)(std::sqrt(2.0) / (double)v / (double)k.dt);  // fd.cpp:1176 forms sqrt(2) in double
     else
       b3 = k.sq2 / v / k.dt;
     T t = -k.b1 * N(x, z + sz) - k.b2 * N(x + sx, z) + b3 * (-C[i]);

@@ -1,0 +1,672 @@
+// Least-squares RTM on gfx950: the Born operator L of the reference
+// (fd.cpp:81-177) evaluated from stored per-step Born-source snapshots, its
+// exact transpose L^T (fd_adjoint.h) with the imaging condition fused into the
+// reverse time loop, FP64 inner products on the GPU, and CGLS.
+//
+// The reference declares conope_MPI / cgsolver_MPI (lsrtm/inversion.h:81,87)
+// but defines neither; its intended normal operator is forward_MPI followed by
+// adjoint_MPI over all shots (inversion.cpp:420-423).  Here L^T is the exact
+// transpose of L, so CG minimises ||L m - d||^2 with a true gradient.
+//
+// Snapshots: the background's Born-source coefficient S^n = vt2 lap^n 2/v^3
+// of every step is computed once per plan (two background passes for 16-bit
+// storage: per-step max |S| first, then the quantized store) and reused by
+// every L and L^T application.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "engine.h"
+#include "fd_adjoint.h"
+#include "prof.h"
+#include "seis_internal.h"
+
+namespace seis {
+
+// ---------------------------------------------------------------------------
+// Field slots of one shot in the LSRTM pool.
+enum LsSlot { LS_FIELD = 0, LS_AIN = 3, LS_AT = 6, LS_AC = 9, LS_AR = 12, LS_NSLOT = 14 };
+
+// Enumeration of the cells outside an inner rectangle [xa, xb) x [za, zb) of
+// the padded grid (same order as the step kernel's band cells).
+struct Ring {
+  int xa, xb, za, zb;
+  long long n;
+};
+
+__device__ __forceinline__ void ring_cell(const Ring& R, int NX, int NZ, long long j, int* x, int* z) {
+  const long long top = (long long)NX * R.za, bot = (long long)NX * (NZ - R.zb);
+  const long long lft = (long long)R.xa * (R.zb - R.za);
+  if (j < top) {
+    *x = (int)(j / R.za);
+    *z = (int)(j % R.za);
+  } else if (j < top + bot) {
+    const long long q = j - top;
+    *x = (int)(q / (NZ - R.zb));
+    *z = R.zb + (int)(q % (NZ - R.zb));
+  } else if (j < top + bot + lft) {
+    const long long q = j - top - bot;
+    *x = (int)(q / (R.zb - R.za));
+    *z = R.za + (int)(q % (R.zb - R.za));
+  } else {
+    const long long q = j - top - bot - lft;
+    *x = R.xb + (int)(q / (R.zb - R.za));
+    *z = R.za + (int)(q % (R.zb - R.za));
+  }
+}
+
+inline Ring make_ring(int NX, int NZ, int xa, int xb, int za, int zb) {
+  Ring R;
+  if (xb <= xa || zb <= za) {
+    R.xa = R.xb = 0;
+    R.za = 0;
+    R.zb = NZ;
+  } else {
+    R.xa = xa;
+    R.xb = xb;
+    R.za = za;
+    R.zb = zb;
+  }
+  R.n = (long long)NX * NZ - (long long)(R.xb - R.xa) * (R.zb - R.za);
+  return R;
+}
+
+// Adjoint level m of a shot: slot m % 3 of the level fields and band products.
+template <typename T>
+__device__ __forceinline__ AdjLevel<T> level(const DevShot<T>& s, int lev) {
+  const int q = lev % 3;
+  const long long f = s.fstride;
+  return AdjLevel<T>{s.fld + (LS_FIELD + q) * f, s.fld + (LS_AIN + q) * f, s.fld + (LS_AT + q) * f,
+                     s.fld + (LS_AC + q) * f};
+}
+
+// Raw adjoints of level `lev` for the cells with a band reader or w < 1
+// (outside [pml+1, NX-pml-1) x [pml+1, NZ-pml-1)), stored for the step kernel.
+template <typename T>
+__global__ __launch_bounds__(256) void adj_prep_kernel(Consts<T> k, const DevShot<T>* __restrict__ shots, Ring R,
+                                                       int step) {
+  const DevShot<T>& s = shots[blockIdx.y];
+  const int lev = s.nt + s.ndel + 1 - step;
+  if (lev < 1) return;
+  const long long j = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (j >= R.n) return;
+  int x, z;
+  ring_cell(R, s.NX, k.NZ, j, &x, &z);
+  const AdjLevel<T> L = level(s, lev);
+  T* ar = s.fld + (LS_AR + lev % 2) * s.fstride;
+  ar[fidx(k.ld, x, z)] = adj_raw(k, s.NX, s.nx, L, x, z);
+}
+
+// Reads a raw adjoint: precomputed near the band, the level itself elsewhere.
+template <typename T>
+struct RawAdj {
+  const T* ar;
+  const T* lam;
+  long long ld;
+  int NX, NZ, pml;
+  __device__ __forceinline__ T operator()(int x, int z) const {
+    const long long i = fidx(ld, x, z);
+    return (x > pml && x < NX - 1 - pml && z > pml && z < NZ - 1 - pml) ? lam[i] : ar[i];
+  }
+};
+
+// Adjoint of the record injection plus the imaging condition for one cell.
+template <typename T>
+__device__ __forceinline__ T adj_finish(const Consts<T>& k, const DevShot<T>& s, int lev, int x, int z, T val,
+                                        T raw_adj) {
+  const int ix = x - k.pml, iz = z - k.pml;
+  if (ix >= 0 && ix < s.nx && iz >= 0 && iz < k.nz) {
+    const int itp = lev - 2 - s.ndel;  // the forward step lev-2 recorded level lev-1
+    if (itp >= 0 && z == s.gz[ix] + k.pml) val += s.rec[(long long)ix * s.nt + itp];
+    if (lev - 1 <= s.nt + s.ndel - 1) {
+      const long long j = (long long)ix * k.nz + iz;
+      s.grad[j] += raw_adj * snap_load(s, k.nz, lev - 1, j);
+    }
+  }
+  return val;
+}
+
+constexpr int AWG = 256;
+
+// One reverse step: lam^{lev-1} for every cell, the band products of the new
+// level, and grad += raw-adjoint(lev) * S^{lev-1}.  Interior cells (no band
+// term within the stencil reach) are swept by wavefronts streaming along x
+// like the forward kernel; the others use the general gather.
+template <typename T>
+__global__ __launch_bounds__(AWG) void adj_step_kernel(Consts<T> k, const DevShot<T>* __restrict__ shots, Layout F,
+                                                       int step) {
+  __shared__ T lds[WAVES][LROW];
+  const DevShot<T>& s = shots[blockIdx.y];
+  const int lev = s.nt + s.ndel + 1 - step;
+  if (lev < 1) return;
+  const int NX = s.NX, NZ = k.NZ, pml = k.pml;
+  const long long ld = k.ld;
+  const AdjLevel<T> L1 = level(s, lev), L2 = level(s, lev + 1);
+  const int q0 = (lev - 1) % 3;
+  T* out = s.fld + (LS_FIELD + q0) * s.fstride;
+  const Model<T> m{s.vel, s.nx, k.nz, pml};
+  if ((int)blockIdx.x < F.nfwg) {
+    const int wave = threadIdx.x >> 6;
+    const int item = blockIdx.x * WAVES + wave;
+    if (item >= F.nzc * F.nxs) return;
+    const int zc = item % F.nzc, xs = item / F.nzc;
+    const int z0 = F.za + zc * 64;
+    const int x0 = F.xa + xs * F.bx, x1 = min(x0 + F.bx, F.xb);
+    const int lane = threadIdx.x & 63;
+    const int z = z0 + lane;
+    const bool act = z < F.zb;
+    const int zl = min(z, NZ - 1);
+    T* la = lds[wave];
+    const bool halo = lane < 4 || lane >= 60;
+    const int hz = lane < 4 ? z0 - 4 + lane : min(z0 + lane + 4, NZ - 1);
+    const int hslot = lane < 4 ? lane : lane + 8;
+    const T* lam1 = L1.lam;
+    const T* lam2 = L2.lam;
+    auto vt2at = [&](int x, int zz) {
+      const T a = k.dt * m.vel(x, zz);  // clamped: idle lanes may sit past the model
+      return a * a;
+    };
+    // queues along x: qx = lam vt2 / dx^2 (rows x-4..x+4), qz and lam (rows x..x+4)
+    T qx[9], qz[5], lq[5];
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const int r = x0 - 4 + j;
+      const T l = lam1[fidx(ld, r, zl)];
+      const T v2 = vt2at(r, zl);
+      qx[j] = l * (v2 * k.idx2);
+      if (j >= 4) {
+        qz[j - 4] = l * (v2 * k.idz2);
+        lq[j - 4] = l;
+      }
+    }
+    for (int x = x0; x < x1; x++) {
+      {
+        const T l = lam1[fidx(ld, x + 4, zl)];
+        const T v2 = vt2at(x + 4, zl);
+        qx[8] = l * (v2 * k.idx2);
+        qz[4] = l * (v2 * k.idz2);
+        lq[4] = l;
+      }
+      if (halo) la[hslot] = lam1[fidx(ld, x, hz)] * (vt2at(x, hz) * k.idz2);
+      la[lane + 4] = qz[0];
+      wave_sync();
+      T sz = la[lane] * k.c[4];
+      sz += la[lane + 1] * k.c[3];
+      sz += la[lane + 2] * k.c[2];
+      sz += la[lane + 3] * k.c[1];
+      sz += la[lane + 4] * k.c[0];
+      sz += la[lane + 5] * k.c[1];
+      sz += la[lane + 6] * k.c[2];
+      sz += la[lane + 7] * k.c[3];
+      sz += la[lane + 8] * k.c[4];
+      wave_sync();
+      T sx = qx[0] * k.c[4];
+      sx += qx[1] * k.c[3];
+      sx += qx[2] * k.c[2];
+      sx += qx[3] * k.c[1];
+      sx += qx[4] * k.c[0];
+      sx += qx[5] * k.c[1];
+      sx += qx[6] * k.c[2];
+      sx += qx[7] * k.c[3];
+      sx += qx[8] * k.c[4];
+      if (act) {
+        const long long i = fidx(ld, x, z);
+        T val = T(2) * lq[0] + sz + sx - lam2[i];
+        val = adj_finish(k, s, lev, x, z, val, lq[0]);
+        out[i] = val;
+      }
+#pragma unroll
+      for (int j = 0; j < 8; j++) qx[j] = qx[j + 1];
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        qz[j] = qz[j + 1];
+        lq[j] = lq[j + 1];
+      }
+    }
+    return;
+  }
+  const long long j = (long long)(blockIdx.x - F.nfwg) * AWG + threadIdx.x;
+  if (j >= F.nband) return;
+  Ring R{F.xa, F.xb, F.za, F.zb, F.nband};
+  int x, z;
+  ring_cell(R, NX, NZ, j, &x, &z);
+  const RawAdj<T> A1{s.fld + (LS_AR + lev % 2) * s.fstride, L1.lam, ld, NX, NZ, pml};
+  const RawAdj<T> A2{s.fld + (LS_AR + (lev + 1) % 2) * s.fstride, L2.lam, ld, NX, NZ, pml};
+  T ar;
+  T val = adj_value(k, m, NX, L1, L2, A1, A2, x, z, &ar);
+  val = adj_finish(k, s, lev, x, z, val, ar);
+  const long long i = fidx(ld, x, z);
+  out[i] = val;
+  const BandCell b = classify(x, z, NX, NZ, pml);
+  if (b.kind != 0) {
+    const BandAdj<T> ba = band_adj(k, m, x, z, b, val);
+    s.fld[(LS_AIN + q0) * s.fstride + i] = ba.in;
+    s.fld[(LS_AT + q0) * s.fstride + i] = ba.t;
+    s.fld[(LS_AC + q0) * s.fstride + i] = ba.c;
+  }
+}
+
+// Sum of the per-shot gradients in shot order: out[j] = sum_s grad_s[j].
+template <typename T>
+__global__ void sum_grads_kernel(const DevShot<T>* __restrict__ shots, int nshots, long long n, T* out) {
+  for (long long j = blockIdx.x * (long long)blockDim.x + threadIdx.x; j < n; j += (long long)gridDim.x * blockDim.x) {
+    T a = T(0);
+    for (int s = 0; s < nshots; s++) a += shots[s].grad[j];
+    out[j] = a;
+  }
+}
+
+// FP64 inner product <a, b> of two FP32/FP64 vectors: fixed-order block
+// partials, then one block folds them (deterministic).
+constexpr int RB = 256;
+constexpr int RGRID = 1024;
+
+template <typename T>
+__global__ __launch_bounds__(RB) void dot_partial_kernel(const T* __restrict__ a, const T* __restrict__ b, long long n,
+                                                         double* part) {
+  __shared__ double sh[RB];
+  double acc = 0.0;
+  for (long long j = blockIdx.x * (long long)RB + threadIdx.x; j < n; j += (long long)gridDim.x * RB)
+    acc += (double)a[j] * (double)b[j];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = RB / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ __launch_bounds__(RB) void dot_final_kernel(const double* __restrict__ part, int n, double* out) {
+  __shared__ double sh[RB];
+  double acc = 0.0;
+  for (int j = threadIdx.x; j < n; j += RB) acc += part[j];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = RB / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = sh[0];
+}
+
+// y = y + alpha x (alpha in FP64, element update in FP64, stored in T)
+template <typename T>
+__global__ void axpy_kernel(T* y, const T* x, double alpha, long long n) {
+  for (long long j = blockIdx.x * (long long)blockDim.x + threadIdx.x; j < n; j += (long long)gridDim.x * blockDim.x)
+    y[j] = (T)((double)y[j] + alpha * (double)x[j]);
+}
+
+// p = s + beta p
+template <typename T>
+__global__ void xpby_kernel(T* p, const T* s, double beta, long long n) {
+  for (long long j = blockIdx.x * (long long)blockDim.x + threadIdx.x; j < n; j += (long long)gridDim.x * blockDim.x)
+    p[j] = (T)((double)s[j] + beta * (double)p[j]);
+}
+
+// y = a * b (record mask)
+template <typename T>
+__global__ void mul_kernel(T* y, const T* a, const T* b, long long n) {
+  for (long long j = blockIdx.x * (long long)blockDim.x + threadIdx.x; j < n; j += (long long)gridDim.x * blockDim.x)
+    y[j] = a[j] * b[j];
+}
+
+template <typename T>
+__global__ void fill_kernel(T* p, T v, long long n) {
+  for (long long j = blockIdx.x * (long long)blockDim.x + threadIdx.x; j < n; j += (long long)gridDim.x * blockDim.x)
+    p[j] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Plan: device-resident state of one LSRTM problem.
+
+#define LS_CHECK(x)                                                             \
+  do {                                                                          \
+    hipError_t e_ = (x);                                                        \
+    if (e_ != hipSuccess) {                                                     \
+      set_error(std::string(#x) + ": " + hipGetErrorString(e_));                \
+      return SEIS_ERR_HIP;                                                      \
+    }                                                                           \
+  } while (0)
+
+template <typename T>
+struct LsrtmPlan {
+  GridSpec g{};
+  int nshots = 0, snap_kind = 0, NX = 0, NZ = 0, NTmax = 0;
+  long long ld = 0, fe = 0, model = 0, nrec = 0;
+  std::vector<ShotSpec> spec;
+  std::vector<DevShot<T>> host;
+  DevShot<T>* dev = nullptr;
+  T* pool = nullptr;       // LS_NSLOT fields per shot
+  void* snaps = nullptr;   // snapshots of all shots
+  float* scales = nullptr;
+  T* grads = nullptr;      // per-shot gradients
+  double* part = nullptr;  // reduction scratch (RGRID partials + results)
+  T* cg = nullptr;         // CG vectors: x, s, p (model) and r, q (records)
+  const T* mask = nullptr; // optional record mask (caller-owned, device)
+  T* masked = nullptr;     // masked copy of an adjoint input
+  Consts<T> k{};
+  Layout F{};   // forward step split (interior sweep / band)
+  Layout A{};   // adjoint step split
+  Ring P{};     // cells whose raw adjoint is precomputed
+  hipStream_t stream = 0;
+
+  ~LsrtmPlan() {
+    for (void* p : {(void*)dev, (void*)pool, snaps, (void*)scales, (void*)grads, (void*)part, (void*)cg, (void*)masked})
+      if (p) (void)hipFree(p);
+  }
+
+  size_t snap_elem() const { return snap_kind == SNAP_NATIVE ? sizeof(T) : 2; }
+
+  int create(const GridSpec& grid, int ns, const ShotSpec* shots, const T* vel, const T* wav, const int* gz,
+             int kind) {
+    g = grid;
+    nshots = ns;
+    snap_kind = kind;
+    if (g.pml < 4) return set_error("lsrtm: pml must be >= 4"), SEIS_ERR_ARG;
+    if (g.nx <= 0 || g.nz <= 0 || !(g.dt > 0) || !(g.dx > 0) || !(g.dz > 0))
+      return set_error("lsrtm: bad grid"), SEIS_ERR_ARG;
+    if (kind < 0 || kind > 2) return set_error("lsrtm: snapshot kind must be 0, 1 or 2"), SEIS_ERR_ARG;
+    if (ns <= 0) return set_error("lsrtm: no shots"), SEIS_ERR_ARG;
+    NX = g.nx + 2 * g.pml;
+    NZ = g.nz + 2 * g.pml;
+    ld = field_ld(NZ);
+    fe = field_elems(NX, ld);
+    model = (long long)g.nx * g.nz;
+    spec.assign(shots, shots + ns);
+    long long snap_total = 0, scale_total = 0;
+    nrec = 0;
+    for (const ShotSpec& q : spec) {
+      if (q.nt <= 0 || q.ndel < 0 || q.sx < 0 || q.sx >= g.nx || q.sz < 0 || q.sz >= g.nz)
+        return set_error("lsrtm: bad shot"), SEIS_ERR_ARG;
+      NTmax = std::max(NTmax, q.nt + q.ndel);
+      snap_total += (long long)(q.nt + q.ndel) * model;
+      scale_total += q.nt + q.ndel;
+      nrec = std::max(nrec, q.rec + (long long)g.nx * q.nt);
+    }
+    LS_CHECK(hipMalloc(&pool, sizeof(T) * (size_t)fe * LS_NSLOT * ns));
+    LS_CHECK(hipMalloc(&snaps, snap_elem() * (size_t)snap_total));
+    LS_CHECK(hipMalloc(&scales, sizeof(float) * (size_t)scale_total));
+    LS_CHECK(hipMalloc(&grads, sizeof(T) * (size_t)model * ns));
+    LS_CHECK(hipMalloc(&part, sizeof(double) * (RGRID + 8)));
+    LS_CHECK(hipMemset(scales, 0, sizeof(float) * (size_t)scale_total));
+    k = make_consts<T>(g, ld);
+    F = make_layout(NX, NZ, g.pml);
+    {
+      // adjoint interior: every cell of the stencil reach has raw adjoint = lam
+      A = F;
+      A.xa = std::max(g.pml + 5, 8);
+      A.xb = std::min(NX - g.pml - 5, NX - 8);
+      A.za = std::max(g.pml + 5, 8);
+      A.zb = std::min(NZ - g.pml - 5, NZ - 8);
+      if (A.xb <= A.xa || A.zb <= A.za) {
+        A.xa = A.xb = 0;
+        A.za = 0;
+        A.zb = NZ;
+      }
+      A.nzc = (A.zb - A.za + 63) / 64;
+      A.nxs = (A.xb - A.xa + A.bx - 1) / A.bx;
+      if (A.xb == A.xa) A.nzc = A.nxs = 0;
+      A.nfwg = (A.nzc * A.nxs + WAVES - 1) / WAVES;
+      A.nband = (long long)NX * NZ - (long long)(A.xb - A.xa) * (A.zb - A.za);
+      A.nbwg = (int)((A.nband + AWG - 1) / AWG);
+    }
+    P = make_ring(NX, NZ, g.pml + 1, NX - g.pml - 1, g.pml + 1, NZ - g.pml - 1);
+    host.resize(ns);
+    const long long origin = (long long)MX * ld + MZ;
+    char* sp = reinterpret_cast<char*>(snaps);
+    float* sc = scales;
+    for (int s = 0; s < ns; s++) {
+      const ShotSpec& q = spec[s];
+      DevShot<T>& d = host[s];
+      std::memset(&d, 0, sizeof(d));
+      d.vel = vel + q.model;
+      d.wav = wav + q.wav;
+      d.gz = gz + q.gz;
+      d.fld = pool + (size_t)fe * LS_NSLOT * s + origin;
+      d.fstride = fe;
+      d.snap = sp;
+      d.scale = sc;
+      d.grad = grads + (size_t)model * s;
+      d.snap_kind = kind;
+      d.nx = g.nx;
+      d.NX = NX;
+      d.nt = q.nt;
+      d.ndel = q.ndel;
+      d.sx = q.sx;
+      d.sz = q.sz;
+      sp += snap_elem() * (size_t)(q.nt + q.ndel) * model;
+      sc += q.nt + q.ndel;
+    }
+    LS_CHECK(hipMalloc(&dev, sizeof(DevShot<T>) * ns));
+    LS_CHECK(hipMemcpy(dev, host.data(), sizeof(DevShot<T>) * ns, hipMemcpyHostToDevice));
+    return make_snapshots();
+  }
+
+  double cells() const { return (double)NX * NZ; }
+
+  int active_fwd(int st) const {
+    int n = 0;
+    for (const ShotSpec& q : spec) n += st < q.nt + q.ndel;
+    return n;
+  }
+
+  int zero_fields(int first, int count) {
+    for (int s = 0; s < nshots; s++)
+      LS_CHECK(hipMemsetAsync(pool + (size_t)fe * (LS_NSLOT * s + first), 0, sizeof(T) * (size_t)fe * count, stream));
+    return SEIS_OK;
+  }
+
+  int make_snapshots() {
+    const double E = sizeof(T);
+    const int passes = snap_kind == SNAP_NATIVE ? 1 : 2;
+    for (int pass = 0; pass < passes; pass++) {
+      const int mode = (passes == 2 && pass == 0) ? SNAP_MAX : SNAP_STORE;
+      int rc = zero_fields(LS_FIELD, 3);
+      if (rc) return rc;
+      for (int st = 0; st < NTmax; st++) {
+        const int n = active_fwd(st);
+        const double by = 4 * E * n * cells() + (mode == SNAP_STORE ? snap_elem() * n * (double)model : 0.0);
+        timed_launch(stream, mode == SNAP_MAX ? "lsrtm_snap_max" : "lsrtm_snap_store", by, n * cells(),
+                     [&] { launch_step<T>(mode, k, dev, F, nshots, st, stream); });
+      }
+    }
+    LS_CHECK(hipGetLastError());
+    LS_CHECK(hipStreamSynchronize(stream));
+    return SEIS_OK;
+  }
+
+  // rec (device, packed like the shot specs) = L pert (device [nx][nz])
+  int born(const T* pert, T* rec) {
+    for (int s = 0; s < nshots; s++) {
+      host[s].dv = pert;
+      host[s].rec = rec + spec[s].rec;
+    }
+    LS_CHECK(hipMemcpyAsync(dev, host.data(), sizeof(DevShot<T>) * nshots, hipMemcpyHostToDevice, stream));
+    int rc = zero_fields(LS_FIELD, 3);
+    if (rc) return rc;
+    const double E = sizeof(T);
+    for (int st = 0; st < NTmax; st++) {
+      const int n = active_fwd(st);
+      const double by = 4 * E * n * cells() + (snap_elem() + E) * n * (double)model;
+      timed_launch(stream, "lsrtm_born", by, n * cells(), [&] { launch_step<T>(BORN_S, k, dev, F, nshots, st, stream); });
+    }
+    if (mask) hipLaunchKernelGGL((mul_kernel<T>), dim3(1024), dim3(256), 0, stream, rec, rec, mask, nrec);
+    LS_CHECK(hipGetLastError());
+    LS_CHECK(hipStreamSynchronize(stream));
+    return SEIS_OK;
+  }
+
+  // grad (device [nx][nz]) = L^T rec, summed over shots
+  int adjoint(const T* rec, T* grad) {
+    if (mask) {
+      if (!masked) LS_CHECK(hipMalloc(&masked, sizeof(T) * (size_t)nrec));
+      hipLaunchKernelGGL((mul_kernel<T>), dim3(1024), dim3(256), 0, stream, masked, rec, mask, nrec);
+      rec = masked;
+    }
+    for (int s = 0; s < nshots; s++) host[s].rec = const_cast<T*>(rec) + spec[s].rec;
+    LS_CHECK(hipMemcpyAsync(dev, host.data(), sizeof(DevShot<T>) * nshots, hipMemcpyHostToDevice, stream));
+    int rc = zero_fields(0, LS_NSLOT);
+    if (rc) return rc;
+    LS_CHECK(hipMemsetAsync(grads, 0, sizeof(T) * (size_t)model * nshots, stream));
+    const double E = sizeof(T);
+    const int pwg = (int)((P.n + 255) / 256);
+    for (int st = 0; st <= NTmax; st++) {
+      int n = 0;
+      for (const ShotSpec& q : spec) n += (q.nt + q.ndel + 1 - st) >= 1;
+      timed_launch(stream, "lsrtm_adjoint_prep", 8 * E * n * (double)P.n, 0, [&] {
+        hipLaunchKernelGGL((adj_prep_kernel<T>), dim3(pwg, nshots), dim3(256), 0, stream, k, dev, P, st);
+      });
+      const double by = 4 * E * n * cells() + (snap_elem() + 2 * E) * n * (double)model;
+      timed_launch(stream, "lsrtm_adjoint", by, n * cells(), [&] {
+        hipLaunchKernelGGL((adj_step_kernel<T>), dim3(A.nfwg + A.nbwg, nshots), dim3(AWG), 0, stream, k, dev, A, st);
+      });
+    }
+    hipLaunchKernelGGL((sum_grads_kernel<T>), dim3(256), dim3(256), 0, stream, dev, nshots, model, grad);
+    LS_CHECK(hipGetLastError());
+    LS_CHECK(hipStreamSynchronize(stream));
+    return SEIS_OK;
+  }
+
+  int dot(const T* a, const T* b, long long n, double* out) {
+    hipLaunchKernelGGL((dot_partial_kernel<T>), dim3(RGRID), dim3(RB), 0, stream, a, b, n, part);
+    hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(RB), 0, stream, part, RGRID, part + RGRID);
+    LS_CHECK(hipMemcpyAsync(out, part + RGRID, sizeof(double), hipMemcpyDeviceToHost, stream));
+    LS_CHECK(hipStreamSynchronize(stream));
+    return SEIS_OK;
+  }
+
+  // CGLS on min ||L x - d||^2 from x = 0 (with the record mask, if set, in
+  // L).  hist[2*i] = ||r_i||^2, hist[2*i+1] = ||L^T r_i||^2 for i = 0..done;
+  // stops after niter iterations or when ||L^T r|| <= rtol ||L^T d||.
+  int cgls(const T* d, int niter, double rtol, T* x_out, double* hist, int* done) {
+    if (!cg) LS_CHECK(hipMalloc(&cg, sizeof(T) * (size_t)(3 * model + 2 * nrec)));
+    T *x = x_out, *s = cg, *p = cg + model, *r = cg + 2 * model, *q = cg + 2 * model + nrec;
+    const int G = 1024, B = 256;
+    hipLaunchKernelGGL((fill_kernel<T>), dim3(G), dim3(B), 0, stream, x, T(0), model);
+    LS_CHECK(hipMemcpyAsync(r, d, sizeof(T) * (size_t)nrec, hipMemcpyDeviceToDevice, stream));
+    int rc = adjoint(r, s);
+    if (rc) return rc;
+    LS_CHECK(hipMemcpyAsync(p, s, sizeof(T) * (size_t)model, hipMemcpyDeviceToDevice, stream));
+    double gamma, rr;
+    if ((rc = dot(s, s, model, &gamma)) || (rc = dot(r, r, nrec, &rr))) return rc;
+    if (hist) {
+      hist[0] = rr;
+      hist[1] = gamma;
+    }
+    if (done) *done = 0;
+    const double gamma0 = gamma;
+    for (int it = 0; it < niter; it++) {
+      if (!(gamma > 0) || (rtol > 0 && gamma <= rtol * rtol * gamma0)) break;
+      if ((rc = born(p, q))) return rc;
+      double delta;
+      if ((rc = dot(q, q, nrec, &delta))) return rc;
+      if (!(delta > 0)) break;
+      const double alpha = gamma / delta;
+      hipLaunchKernelGGL((axpy_kernel<T>), dim3(G), dim3(B), 0, stream, x, p, alpha, model);
+      hipLaunchKernelGGL((axpy_kernel<T>), dim3(G), dim3(B), 0, stream, r, q, -alpha, nrec);
+      if ((rc = adjoint(r, s))) return rc;
+      double gnew;
+      if ((rc = dot(s, s, model, &gnew)) || (rc = dot(r, r, nrec, &rr))) return rc;
+      const double beta = gnew / gamma;
+      gamma = gnew;
+      hipLaunchKernelGGL((xpby_kernel<T>), dim3(G), dim3(B), 0, stream, p, s, beta, model);
+      if (hist) {
+        hist[2 * (it + 1)] = rr;
+        hist[2 * (it + 1) + 1] = gamma;
+      }
+      if (done) *done = it + 1;
+    }
+    LS_CHECK(hipGetLastError());
+    LS_CHECK(hipStreamSynchronize(stream));
+    return SEIS_OK;
+  }
+};
+
+}  // namespace seis
+
+using namespace seis;
+
+struct seis_lsrtm {
+  int fp64 = 0;
+  LsrtmPlan<float>* f = nullptr;
+  LsrtmPlan<double>* d = nullptr;
+  ~seis_lsrtm() {
+    delete f;
+    delete d;
+  }
+};
+
+extern "C" {
+
+int seis_lsrtm_create(seis_lsrtm** plan, const seis_grid* grid, int nshots, const seis_shot* shots, const float* vel,
+                      const float* wav, const int* gz, int snapshot_kind) {
+  if (!plan || !grid || !shots || !vel || !wav || !gz) return set_error("lsrtm: null argument"), SEIS_ERR_ARG;
+  seis_lsrtm* p = new seis_lsrtm();
+  p->f = new LsrtmPlan<float>();
+  const int rc = p->f->create(*grid, nshots, shots, vel, wav, gz, snapshot_kind);
+  if (rc) {
+    delete p;
+    *plan = nullptr;
+    return rc;
+  }
+  *plan = p;
+  return SEIS_OK;
+}
+
+int seis_lsrtm_create_f64(seis_lsrtm** plan, const seis_grid* grid, int nshots, const seis_shot* shots,
+                          const double* vel, const double* wav, const int* gz) {
+  if (!plan || !grid || !shots || !vel || !wav || !gz) return set_error("lsrtm: null argument"), SEIS_ERR_ARG;
+  seis_lsrtm* p = new seis_lsrtm();
+  p->fp64 = 1;
+  p->d = new LsrtmPlan<double>();
+  const int rc = p->d->create(*grid, nshots, shots, vel, wav, gz, SNAP_NATIVE);
+  if (rc) {
+    delete p;
+    *plan = nullptr;
+    return rc;
+  }
+  *plan = p;
+  return SEIS_OK;
+}
+
+int seis_lsrtm_destroy(seis_lsrtm* plan) {
+  delete plan;
+  return SEIS_OK;
+}
+
+int seis_lsrtm_born(seis_lsrtm* plan, const void* pert, void* rec) {
+  if (!plan || !pert || !rec) return set_error("lsrtm: null argument"), SEIS_ERR_ARG;
+  return plan->fp64 ? plan->d->born((const double*)pert, (double*)rec) : plan->f->born((const float*)pert, (float*)rec);
+}
+
+int seis_lsrtm_adjoint(seis_lsrtm* plan, const void* rec, void* grad) {
+  if (!plan || !rec || !grad) return set_error("lsrtm: null argument"), SEIS_ERR_ARG;
+  return plan->fp64 ? plan->d->adjoint((const double*)rec, (double*)grad)
+                    : plan->f->adjoint((const float*)rec, (float*)grad);
+}
+
+int seis_lsrtm_dot(seis_lsrtm* plan, const void* a, const void* b, long long n, double* out) {
+  if (!plan || !a || !b || !out) return set_error("lsrtm: null argument"), SEIS_ERR_ARG;
+  return plan->fp64 ? plan->d->dot((const double*)a, (const double*)b, n, out)
+                    : plan->f->dot((const float*)a, (const float*)b, n, out);
+}
+
+int seis_lsrtm_set_mask(seis_lsrtm* plan, const void* mask) {
+  if (!plan) return set_error("lsrtm: null plan"), SEIS_ERR_ARG;
+  if (plan->fp64) plan->d->mask = (const double*)mask;
+  else plan->f->mask = (const float*)mask;
+  return SEIS_OK;
+}
+
+int seis_lsrtm_cg(seis_lsrtm* plan, const void* dobs, int niter, double rtol, void* image, double* hist,
+                  int* iters) {
+  if (!plan || !dobs || !image || niter < 0) return set_error("lsrtm: bad argument"), SEIS_ERR_ARG;
+  return plan->fp64 ? plan->d->cgls((const double*)dobs, niter, rtol, (double*)image, hist, iters)
+                    : plan->f->cgls((const float*)dobs, niter, rtol, (float*)image, hist, iters);
+}
+
+}  // extern "C"

@@ -1,0 +1,22 @@
+"""Parameter-file driver over the C ABI (seis_run, seis_read_params): the
+reference's Inversion flow (lsrtm/inversion.cpp) on one GPU."""
+import ctypes
+
+from ._lib import check, lib
+
+
+def read_params(path):
+    """Parameters of a reference-style parameter file as a dict of strings."""
+    buf = ctypes.create_string_buffer(1 << 16)
+    check(lib().seis_read_params(str(path).encode(), buf, len(buf)))
+    out = {}
+    for line in buf.value.decode().splitlines():
+        k, _, v = line.partition(" = ")
+        out[k] = v
+    return out
+
+
+def run(param_file, mode, model_file=None, out_file=None):
+    """Runs one driver mode: modeling, born, rtm, rtm_born or lsrtm."""
+    enc = lambda s: None if s is None else str(s).encode()  # noqa: E731
+    check(lib().seis_run(enc(param_file), enc(mode), enc(model_file), enc(out_file)))
