@@ -148,6 +148,19 @@ SEIS_HD inline T adj_raw(const Consts<T>& k, int NX, int nx, const AdjLevel<T>& 
   return r;
 }
 
+// Derivative of the ring's extrapolated term 2 d1 - d2 (fd.cpp:1072-1091),
+// d1 = c1 (u0 + u2 - 2 u1), d2 = c2 (u1 + u3 - 2 u2), with respect to u_j
+// (j = 0..3): 2 c1 (e0 + e2 - 2 e1) - c2 (e1 + e3 - 2 e2), e_i = [j == i].
+// Straight-line on purpose: the gfx950 backend of ROCm 7.2 placed the
+// negation of the lane-divergent `switch` form (default case -c2) under the
+// wrong execution mask, so lanes with j == 3 received +c2.  Every product
+// here is exact (factors 0, +-1, +-2), so the values equal the branchy form's.
+template <typename T>
+SEIS_HD inline T extrap_coef(int j, T c1, T c2) {
+  const T e0 = T(j == 0), e1 = T(j == 1), e2 = T(j == 2), e3 = T(j == 3);
+  return T(2) * c1 * (e0 + e2 - T(2) * e1) - c2 * (e1 + e3 - T(2) * e2);
+}
+
 SEIS_HD inline bool is_ring(int x, int z, int NX, int NZ) { return x == 0 || z == 0 || x == NX - 1 || z == NZ - 1; }
 
 SEIS_HD inline bool high_order(int x, int z, int NX, int NZ) { return x >= 4 && x <= NX - 5 && z >= 4 && z <= NZ - 5; }
@@ -230,22 +243,11 @@ SEIS_HD inline T adj_value(const Consts<T>& k, const Model<T>& m, int NX, const 
     }
   }
   // extrapolated normal differences: ring cell (x, 0) reads cur(x, 0..3), etc.
-  for (int side = 0; side < 2; side++) {
-    const int z0 = side == 0 ? 0 : NZ - 1, dz = side == 0 ? 1 : -1;
-    const int j = (z - z0) * dz;
-    if (j >= 0 && j <= 3) {
-      const T v1 = vv(x, z0 + dz) * k.cz, v2 = vv(x, z0 + 2 * dz) * k.cz;
-      const T cf = j == 0 ? T(2) * v1 : j == 1 ? -T(4) * v1 - v2 : j == 2 ? T(2) * v1 + T(2) * v2 : -v2;
-      acc += ar1(x, z0) * cf;
-    }
-    const int x0 = side == 0 ? 0 : NX - 1, dx = side == 0 ? 1 : -1;
-    const int i = (x - x0) * dx;
-    if (i >= 0 && i <= 3) {
-      const T v1 = vv(x0 + dx, z) * k.cx, v2 = vv(x0 + 2 * dx, z) * k.cx;
-      const T cf = i == 0 ? T(2) * v1 : i == 1 ? -T(4) * v1 - v2 : i == 2 ? T(2) * v1 + T(2) * v2 : -v2;
-      acc += ar1(x0, z) * cf;
-    }
-  }
+  // (the coefficient of cur at distance j from the edge; straight-line form)
+  if (z <= 3) acc += ar1(x, 0) * extrap_coef<T>(z, vv(x, 1) * k.cz, vv(x, 2) * k.cz);
+  if (NZ - 1 - z <= 3) acc += ar1(x, NZ - 1) * extrap_coef<T>(NZ - 1 - z, vv(x, NZ - 2) * k.cz, vv(x, NZ - 3) * k.cz);
+  if (x <= 3) acc += ar1(0, z) * extrap_coef<T>(x, vv(1, z) * k.cx, vv(2, z) * k.cx);
+  if (NX - 1 - x <= 3) acc += ar1(NX - 1, z) * extrap_coef<T>(NX - 1 - x, vv(NX - 2, z) * k.cx, vv(NX - 3, z) * k.cx);
   // (iii) one-way reads of cur: cur(b) by band cell b, cur(b_in) by side cell b
   const BandCell self = classify(x, z, NX, NZ, pml);
   if (self.kind != 0) acc += L1.ac[fidx(ld, x, z)];

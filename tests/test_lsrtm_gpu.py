@@ -19,7 +19,7 @@ pytestmark = pytest.mark.gpu
 
 def _setup(engine, oracle, seed=3, nx=40, nz=30, pml=8, shots=((90, 2), (120, 0), (70, 5))):
     rng = np.random.default_rng(seed)
-    dt, dx, dz = 0.001, 10.0, 10.0
+    dt, dx, dz = float(np.float32(0.001)), 10.0, 10.0  # the C ABI carries dt in FP32
     vel = (2000 + 800 * rng.random((nx, nz))).astype(np.float32)
     g = engine.Grid(nx, nz, pml, dt, dx, dz)
     sh = [engine.Shot(sx=int(rng.integers(0, nx)), sz=int(rng.integers(0, 3)), nt=nt, ndel=nd) for nt, nd in shots]
@@ -61,7 +61,7 @@ def test_adjoint_matches_oracle_transpose(engine, oracle):
               for j, s in enumerate(shots))
     p64 = LsrtmPlan(g, shots, vel.astype(np.float64), [w.astype(np.float64) for w in wavs], gzs, fp64=True)
     got = p64.adjoint(r)
-    assert rel_l2(got, ref) <= 1e-12
+    assert rel_l2(got, ref) <= 1e-12, rel_l2(got, ref)
     p32 = LsrtmPlan(g, shots, vel, wavs, gzs, snapshot=SNAP_FP32)
     got32 = p32.adjoint(r.astype(np.float32))
     assert rel_l2(got32, ref) <= 1e-4
