@@ -112,6 +112,22 @@ int seis_born_batch(const seis_grid* grid, int nshots, const seis_shot* shots, c
 int seis_rtm_batch(const seis_grid* grid, int nshots, const seis_shot* shots, const float* vel,
                    const float* wav, const int* gz, const float* rec, float* img);
 
+/* seis_rtm_batch with option bits; flags = 0 is seis_rtm_batch.
+ * SEIS_RTM_ISOLATE_BAND: the reference reconstructs the source wavefield
+ * backwards in time with the damped operator (fd.cpp:657-667).  Run that way
+ * the absorbing band grows without bound on long records in fast media
+ * (e.g. 751 x 2301 cells at 4 m, v up to 5.4 km/s, dt 0.4 ms: image values
+ * 1e9 by nt = 2500 and inf/NaN by nt = 5000, in the reference as here).  The
+ * 4-layer strips shield the interior wavefield from the band; the band
+ * reaches the image only through lap at the strip cells and through the
+ * column Hilbert filter of FaQi (fd.cpp:683-684).  With this bit the source
+ * field is held at zero outside the model area; the interior wavefields are
+ * unchanged and the image differs only within 34 cells of the model edge.
+ * No counterpart in the reference. */
+#define SEIS_RTM_ISOLATE_BAND 1
+int seis_rtm_batch_ex(const seis_grid* grid, int nshots, const seis_shot* shots, const float* vel,
+                      const float* wav, const int* gz, const float* rec, float* img, int flags);
+
 /* Least-squares RTM.  The reference declares the LSRTM operator and solver
  * (conope_MPI, cgsolver_MPI at lsrtm/inversion.h:81,87) but defines neither;
  * its intended operator is Born modeling followed by migration of the Born

@@ -45,6 +45,7 @@ _SIGS = {
     "seis_modeling_batch": (_I, [ctypes.POINTER(seis_grid), _I, ctypes.POINTER(seis_shot), _P, _P, _P, _P]),
     "seis_born_batch": (_I, [ctypes.POINTER(seis_grid), _I, ctypes.POINTER(seis_shot), _P, _P, _P, _P, _P]),
     "seis_rtm_batch": (_I, [ctypes.POINTER(seis_grid), _I, ctypes.POINTER(seis_shot), _P, _P, _P, _P, _P]),
+    "seis_rtm_batch_ex": (_I, [ctypes.POINTER(seis_grid), _I, ctypes.POINTER(seis_shot), _P, _P, _P, _P, _P, _I]),
     "seis_lsrtm_create": (_I, [ctypes.POINTER(_P), ctypes.POINTER(seis_grid), _I, ctypes.POINTER(seis_shot),
                                _P, _P, _P, _I]),
     "seis_lsrtm_create_f64": (_I, [ctypes.POINTER(_P), ctypes.POINTER(seis_grid), _I, ctypes.POINTER(seis_shot),

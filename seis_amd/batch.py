@@ -74,7 +74,10 @@ def split_records(grid, shots, rec):
     return out
 
 
-def _run(fn, grid, shots, vel, extra, wav, gz, out_elems, extra_dtype=np.float32, rec_in=None):
+RTM_ISOLATE_BAND = 1  # SEIS_RTM_ISOLATE_BAND of include/seis.h
+
+
+def _run(fn, grid, shots, vel, extra, wav, gz, out_elems, extra_dtype=np.float32, rec_in=None, flags=0):
     arr, nw, ng, nr = _layout(grid, shots)
     keep = []
     dvel, k = _dev(vel, np.float32); keep.append(k)
@@ -84,8 +87,8 @@ def _run(fn, grid, shots, vel, extra, wav, gz, out_elems, extra_dtype=np.float32
     if fn == "rtm":
         drec, k = _dev(rec_in, np.float32); keep.append(k)
         dimg = DeviceArray((len(shots) * grid.nx * grid.nz,), np.float32, zero=True)
-        check(lib().seis_rtm_batch(ctypes.byref(g), len(shots), arr, dvel.ptr, dwav.ptr, dgz.ptr, drec.ptr,
-                                   dimg.ptr))
+        check(lib().seis_rtm_batch_ex(ctypes.byref(g), len(shots), arr, dvel.ptr, dwav.ptr, dgz.ptr, drec.ptr,
+                                      dimg.ptr, int(flags)))
         return dimg.download().reshape(len(shots), grid.nx, grid.nz)
     drec = DeviceArray((nr,), np.float32, zero=True)
     if fn == "model":
@@ -109,8 +112,12 @@ def born_batch(grid, shots, vel, dv, wavs, gzs):
     return _run("born", grid, shots, vel, dv, wav, gz, None)
 
 
-def rtm_batch(grid, shots, vel, wavs, gzs, recs):
-    """Reference-semantics RTM image of every shot: [nshots][nx][nz]."""
+def rtm_batch(grid, shots, vel, wavs, gzs, recs, isolate_band=False):
+    """Reference-semantics RTM image of every shot: [nshots][nx][nz].
+    isolate_band holds the reconstructed source field at zero in the absorbing
+    band (SEIS_RTM_ISOLATE_BAND), which keeps long records in fast media
+    finite; the image then differs from the reference's only near the edges."""
     wav, gz = pack(grid, shots, wavs, gzs)
     rec = np.concatenate([np.ascontiguousarray(r, np.float32).ravel() for r in recs])
-    return _run("rtm", grid, shots, vel, None, wav, gz, None, rec_in=rec)
+    return _run("rtm", grid, shots, vel, None, wav, gz, None, rec_in=rec,
+                flags=RTM_ISOLATE_BAND if isolate_band else 0)

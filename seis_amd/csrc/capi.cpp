@@ -183,4 +183,13 @@ int seis_rtm_batch(const seis_grid* grid, int nshots, const seis_shot* shots, co
   return run_batch<float>(RUN_RTM, *grid, nshots, shots, vel, nullptr, wav, gz, const_cast<float*>(rec), img, 0);
 }
 
+int seis_rtm_batch_ex(const seis_grid* grid, int nshots, const seis_shot* shots, const float* vel,
+                      const float* wav, const int* gz, const float* rec, float* img, int flags) {
+  int rc = check_batch(grid, nshots, shots);
+  if (rc) return rc;
+  if (flags & ~SEIS_RTM_ISOLATE_BAND) return set_error("rtm: unknown option bits"), SEIS_ERR_ARG;
+  return run_batch<float>(RUN_RTM, *grid, nshots, shots, vel, nullptr, wav, gz, const_cast<float*>(rec), img, 0,
+                          flags);
+}
+
 }  // extern "C"

@@ -30,6 +30,7 @@ struct DevShot {
   long long fstride;
   int nx, NX, nt, ndel, sx, sz;
   int snap_kind;   // SNAP_NATIVE, SNAP_BF16 or SNAP_FP16
+  int rtm_flags;   // SEIS_RTM_* bits (RTM reverse pass)
 };
 
 // Step kernel modes.  MODEL / BORN / RTM_* follow the reference's modeling,

@@ -126,7 +126,12 @@ __device__ __forceinline__ void cell_update(const Consts<T>& k, const DevShot<T>
       sub = (as * as) * s.wav[it - 1];
     }
     SourceRemoval<T> sr{c.src, sub, on};
-    const T u = final_value(k, m, NX, C, P, sr, x, z);
+    // With SEIS_RTM_ISOLATE_BAND the reconstructed source field is held at
+    // zero outside the model area: run backwards, the absorbing band grows
+    // without bound, and the strips already shield the interior from it.
+    const bool band = x < k.pml || x >= k.pml + s.nx || z < k.pml || z >= k.pml + k.nz;
+    const bool iso = band && (s.rtm_flags & SEIS_RTM_ISOLATE_BAND);
+    const T u = iso ? T(0) : final_value(k, m, NX, C, P, sr, x, z);
     const T lap = (sr.prev(P, i) + u - T(2) * C[i]) * k.idt2;
     const T* Pr = P + 3 * s.fstride;
     const T* Cr = C + 3 * s.fstride;
@@ -134,9 +139,7 @@ __device__ __forceinline__ void cell_update(const Consts<T>& k, const DevShot<T>
     ReceiverInject<T> ri{s.gz, s.rec, s.nx, s.nt, itp, k.pml};
     const T ur = final_value(k, m, NX, Cr, Pr, ri, x, z);
     const T v3 = T(2) / (v * v * v);
-    const T U = lap * v3;
-    s.img[i] += U * ur;
-    s.aux[i] = U;
+    s.aux[i] = lap * v3;  // U of the imaging condition; the image pass reads it
     Nr[i] = ur;
     // the current level of the next reverse step gets the saved strips
     // restored (fd.cpp:663); lap above used the value before restoration
@@ -299,9 +302,7 @@ __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot
         if (z == s.gz[ix] + pml) ub = ub + s.rec[(long long)ix * s.nt + c.itp] * vt2;
         const T lap = (pa + ua - T(2) * qa[4]) * k.idt2;
         const T v3 = T(2) / (v * v * v);
-        const T U = lap * v3;
-        s.img[i] += U * ub;
-        s.aux[i] = U;
+        s.aux[i] = lap * v3;
         NB[i] = ub;
         T out = ua;
         if (c.itp - 1 >= 0) {
@@ -376,40 +377,63 @@ __global__ __launch_bounds__(WG) void step_kernel(Consts<T> k, const DevShot<T>*
   cell_update<T, MODE>(k, s, c, x, z);
 }
 
-// Hilbert term of the reference RTM image (FaQi, fd.cpp:1272-1298 with
-// hilbert.h:42-54): per interior column, img += 1/2 (U R - H[U] H[R]) where
-// H is the 61-tap Hamming-tapered FIR applied over the full padded column.
-template <typename T>
-__global__ __launch_bounds__(256) void faqi_kernel(Consts<T> k, const DevShot<T>* __restrict__ shots, int step,
-                                                   const T* __restrict__ taps, int half, int slot_off) {
+// The imaging condition of one reverse step for one interior column x
+// (fd.cpp:681-684): img += U R, then img += 1/2 (U R - H[U] H[R]) (FaQi,
+// fd.cpp:1272-1298), where U = lap 2/v^3 of the source field, R the new level
+// of the receiver field and H the 61-tap Hamming-tapered Hilbert FIR
+// (hilbert.h:42-69) applied over the full padded column.  The reference adds
+// U R over the whole padded grid but returns only the interior, which is all
+// that is accumulated here; the two additions stay separate roundings.
+// Even taps are zero and are skipped (adding 0 * x leaves the sum unchanged).
+struct HilbertTaps {
+  float h[61];
+};
+
+template <int HALF>
+__global__ __launch_bounds__(256) void image_kernel(Consts<float> k, const DevShot<float>* __restrict__ shots,
+                                                    int step, HilbertTaps taps, int slot_off) {
   extern __shared__ unsigned char smem_raw[];
-  T* sU = reinterpret_cast<T*>(smem_raw);
-  const DevShot<T>& s = shots[blockIdx.y];
+  float* sU = reinterpret_cast<float*>(smem_raw);
+  const DevShot<float>& s = shots[blockIdx.y];
   const int x = k.pml + blockIdx.x;
   if (x >= s.NX - k.pml) return;
   const int it = s.nt + s.ndel - 1 - step;
   if (it < s.ndel) return;
   const int NZ = k.NZ;
-  T* sR = sU + NZ;
+  float* sR = sU + NZ;
   const long long base = fidx(k.ld, x, 0);
-  const T* R = s.fld + slot_off * s.fstride;
+  const float* R = s.fld + slot_off * s.fstride;
   for (int z = threadIdx.x; z < NZ; z += blockDim.x) {
     sU[z] = s.aux[base + z];
     sR[z] = R[base + z];
   }
   __syncthreads();
-  const T scal = T(2) / T(4);
+  const float scal = 2.0f / 4.0f;
   for (int z = k.pml + threadIdx.x; z < NZ - k.pml; z += blockDim.x) {
-    const int i0 = max(0, z - half), i1 = min(NZ - 1, z + half);
-    T uh = T(0), rh = T(0);
-    // odd offsets only: the even taps are zero
-    int i = i0 + (((z - i0) & 1) ? 0 : 1);
-    for (; i <= i1; i += 2) {
-      const T h = taps[z - i + half];
-      uh += sU[i] * h;
-      rh += sR[i] * h;
+    float uh = 0.f, rh = 0.f;
+    if (z >= HALF && z + HALF <= NZ - 1) {
+      // full window: inputs z-HALF+1 .. z+HALF-1 at odd offsets, tap index
+      // z - i + HALF, summed in increasing i
+#pragma unroll
+      for (int j = 0; j < HALF; j++) {
+        const int i = z - HALF + 1 + 2 * j;
+        const float h = taps.h[2 * HALF - 1 - 2 * j];
+        uh += sU[i] * h;
+        rh += sR[i] * h;
+      }
+    } else {
+      const int i0 = max(0, z - HALF), i1 = min(NZ - 1, z + HALF);
+      for (int i = i0 + (((z - i0) & 1) ? 0 : 1); i <= i1; i += 2) {
+        const float h = taps.h[z - i + HALF];
+        uh += sU[i] * h;
+        rh += sR[i] * h;
+      }
     }
-    s.img[base + z] += scal * (sU[z] * sR[z] - uh * rh);
+    const float ur = sU[z] * sR[z];
+    float img = s.img[base + z];
+    img += ur;
+    img += scal * (ur - uh * rh);
+    s.img[base + z] = img;
   }
 }
 
@@ -501,7 +525,7 @@ int check_grid(const GridSpec& g) {
 
 template <typename T>
 int run_batch(int mode, const GridSpec& g, int nshots, const ShotSpec* shots, const T* vel, const T* dv,
-              const T* wav, const int* gz, T* rec, T* img, hipStream_t stream) {
+              const T* wav, const int* gz, T* rec, T* img, hipStream_t stream, int rtm_flags) {
   int rc = check_grid(g);
   if (rc) return rc;
   if (nshots <= 0) return SEIS_OK;
@@ -530,7 +554,9 @@ int run_batch(int mode, const GridSpec& g, int nshots, const ShotSpec* shots, co
   BatchRun<T> br;
   br.pool_bytes = total * sizeof(T);
   HIPCHECK(hipMalloc(&br.pool, br.pool_bytes));
-  HIPCHECK(hipMemsetAsync(br.pool, 0, br.pool_bytes, stream));
+  // fields and images start at zero; every strip sample is written by the
+  // forward pass before the reverse pass reads it
+  HIPCHECK(hipMemsetAsync(br.pool, 0, per_shot * nshots * sizeof(T), stream));
   T* p = reinterpret_cast<T*>(br.pool);
   T* strip_p = p + per_shot * nshots;
   const long long origin = (long long)MX * ld + MZ;
@@ -556,6 +582,7 @@ int run_batch(int mode, const GridSpec& g, int nshots, const ShotSpec* shots, co
     d.ndel = q.ndel;
     d.sx = q.sx;
     d.sz = q.sz;
+    d.rtm_flags = rtm_flags;
   }
   HIPCHECK(hipMalloc(&br.dev, sizeof(DevShot<T>) * nshots));
   HIPCHECK(hipMemcpyAsync(br.dev, br.host.data(), sizeof(DevShot<T>) * nshots, hipMemcpyHostToDevice, stream));
@@ -577,8 +604,9 @@ int run_batch(int mode, const GridSpec& g, int nshots, const ShotSpec* shots, co
   };
   // Algorithmic HBM bytes per cell: MODEL reads vel, C, P and writes N (16 B
   // in FP32); BORN adds the scattered C, P, N and dv (32 B for 2 updates);
-  // RTM reverse reads vel, C, P, Cr, Pr, writes N, Nr, lap*2/v^3 and
-  // read-modify-writes the image (40 B for 2 updates).
+  // RTM reverse reads vel, C, P, Cr, Pr and writes N, Nr and lap*2/v^3
+  // (32 B for 2 updates); the image pass reads U and R over the interior
+  // columns and read-modify-writes the interior image.
   if (mode == RUN_MODEL) {
     for (int st = 0; st < NTmax; st++) {
       const double a = active_fwd(st) * cells;
@@ -605,12 +633,9 @@ int run_batch(int mode, const GridSpec& g, int nshots, const ShotSpec* shots, co
     for (int s = 0; s < nshots; s++)
       HIPCHECK(hipMemsetAsync(p + per_shot * s, 0, sizeof(T) * (size_t)fe * nslots, stream));
     hipLaunchKernelGGL((restore_strips_kernel<T>), dim3(16, nshots), dim3(256), 0, stream, k, br.dev, 1, 1);
-    const int half = 30;
-    std::vector<T> h(2 * half + 1);
-    hilbert_taps_host<T>(half, h.data());
-    T* dtaps = nullptr;
-    HIPCHECK(hipMalloc(&dtaps, sizeof(T) * h.size()));
-    HIPCHECK(hipMemcpyAsync(dtaps, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, stream));
+    constexpr int half = 30;
+    HilbertTaps taps{};
+    hilbert_taps_host<float>(half, taps.h);
     const size_t smem = sizeof(T) * 2 * NZ;
     if (smem > 64 * 1024) return set_error("rtm: padded nz too large for the column Hilbert kernel"), SEIS_ERR_ARG;
     int maxbwd = 0;
@@ -618,18 +643,16 @@ int run_batch(int mode, const GridSpec& g, int nshots, const ShotSpec* shots, co
     const double faqi_cells = (double)g.nx * NZ;
     for (int st = 0; st < maxbwd; st++) {
       const int n = active_bwd(st);
-      timed_launch(stream, "step_rtm_reverse", 10 * E * n * cells + strip * n, 2 * n * cells, [&] {
+      timed_launch(stream, "step_rtm_reverse", 8 * E * n * cells + strip * n, 2 * n * cells, [&] {
         hipLaunchKernelGGL((step_kernel<T, RTM_BWD>), grid, block, 0, stream, k, br.dev, L, st);
       });
       // the receiver field's new level lives in slot 3 + (st+2)%3
-      timed_launch(stream, "faqi_hilbert", 2 * E * n * faqi_cells + 2 * E * n * (double)g.nx * g.nz, 0, [&] {
-        hipLaunchKernelGGL((faqi_kernel<T>), dim3(g.nx, nshots), dim3(256), smem, stream, k, br.dev, st, dtaps,
-                           half, 3 + (st + 2) % 3);
+      timed_launch(stream, "rtm_image", 2 * E * n * faqi_cells + 2 * E * n * (double)g.nx * g.nz, 0, [&] {
+        hipLaunchKernelGGL((image_kernel<half>), dim3(g.nx, nshots), dim3(256), smem, stream, k, br.dev, st, taps,
+                           3 + (st + 2) % 3);
       });
     }
     hipLaunchKernelGGL((crop_kernel<T>), dim3(64, nshots), dim3(256), 0, stream, k, br.dev, img);
-    HIPCHECK(hipStreamSynchronize(stream));
-    HIPCHECK(hipFree(dtaps));
   }
   HIPCHECK(hipGetLastError());
   HIPCHECK(hipStreamSynchronize(stream));
@@ -637,6 +660,6 @@ int run_batch(int mode, const GridSpec& g, int nshots, const ShotSpec* shots, co
 }
 
 template int run_batch<float>(int, const GridSpec&, int, const ShotSpec*, const float*, const float*, const float*,
-                              const int*, float*, float*, hipStream_t);
+                              const int*, float*, float*, hipStream_t, int);
 
 }  // namespace seis

@@ -18,6 +18,6 @@ void set_error(const std::string& msg);
 
 template <typename T>
 int run_batch(int mode, const GridSpec& g, int nshots, const ShotSpec* shots, const T* vel, const T* dv,
-              const T* wav, const int* gz, T* rec, T* img, hipStream_t stream);
+              const T* wav, const int* gz, T* rec, T* img, hipStream_t stream, int rtm_flags = 0);
 
 }  // namespace seis
