@@ -29,13 +29,18 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Gcell-updates/s (batched shots) + LSRTM s/CG-iter vs plain FP32 at stated error"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of each kernel for the 96-shot workload, from
+# rocprofv3 FETCH_SIZE / WRITE_SIZE (calibrated against a 1 GiB copy; the
+# commands and counter values are in profiles/rtm_batch_hbm_traffic.md).
+# Scaled by the number of shots in a launch.
+MEASURED_TRAFFIC_96 = {"step_rtm_forward": 3.989e9, "step_rtm_reverse": 8.761e9, "rtm_image": 2.821e9}
 
 
-def rtm_workload(nshots):
+def rtm_workload(nshots, nt=5000):
     from seis_amd import models
     nx, nz, pml = 2301, 751, 30
     dx = dz = 4.0
-    dt, nt, fr = 0.0004, 5000, 12.0
+    dt, fr = 0.0004, 12.0
     vtrue = models.layered_faulted(nx, nz, dx, dz, seed=1001)
     vmig = models.smooth_model(vtrue, 20.0)
     delay = int(round(1.0 / (fr * dt)))
@@ -85,6 +90,10 @@ def main():
     ap.add_argument("--shots", type=int, default=96)
     ap.add_argument("--cpu-nt", type=int, default=900)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--nt", type=int, default=5000,
+                    help="record length; shorter runs keep every launch's size (for counter collection)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run 8 device-to-device copies of 1 GiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the batch of shots is one GPU's share; "
@@ -93,10 +102,20 @@ def main():
 
     import seis_amd
     from seis_amd import DeviceArray
+    from seis_amd._lib import check
+    if args.copy_calibration:
+        n = 1 << 28  # 1 GiB of float32
+        a = DeviceArray((n,), np.float32, zero=True)
+        b = DeviceArray((n,), np.float32, zero=True)
+        for _ in range(8):
+            check(seis_amd.lib().seis_copy_f32(b.ptr, a.ptr, n))
+        seis_amd.synchronize()
+        log("copied 8 x %d bytes device to device" % (4 * n))
+        return
     from seis_amd.batch import _layout, Grid, Shot, RTM_ISOLATE_BAND
     import ctypes
 
-    w = rtm_workload(args.shots)
+    w = rtm_workload(args.shots, args.nt)
     nx, nz, pml, nt = w["nx"], w["nz"], w["pml"], w["nt"]
     grid = Grid(nx, nz, pml, w["dt"], w["dx"], w["dz"])
     shots = [Shot(sx=sx, sz=w["sz"], nt=nt, ndel=0, model=0) for sx in w["sxs"]]
@@ -164,12 +183,13 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded layered-faulted model; records forward-modeled on the GPU before timing)",
-        "config": {"workload": "rtm_batch: %d shots, 751x2301 cells @ 4 m, pml 30, nt=5000, dt=0.4 ms, "
-                               "Ricker 12 Hz, 480-channel split spread @ 12 m, reference imaging condition" % ns,
+        "config": {"workload": "rtm_batch: %d shots, 751x2301 cells @ 4 m, pml 30, nt=%d, dt=0.4 ms, "
+                               "Ricker 12 Hz, 480-channel split spread @ 12 m, reference imaging condition" % (ns, nt),
                    "shots": ns, "grid": [nz, nx], "padded_cells": cells, "nt": nt,
                    "propagations_per_shot": 3, "source_band": "isolated (SEIS_RTM_ISOLATE_BAND)"},
         "roofline": {"bound": "hbm", "kernel": dom[0], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": (MEASURED_TRAFFIC_96[dom[0]] * ns / 96 if dom[0] in MEASURED_TRAFFIC_96 else None),
                      "avg_launch_ms": round(dom[1]["total_ms"] / dom[1]["launches"], 4),
                      "alg_bytes_per_launch": dom[1]["alg_bytes"] / dom[1]["launches"]},
         "kernels": {k: {"launches": v["launches"], "ms": round(v["total_ms"], 2),

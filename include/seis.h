@@ -59,6 +59,10 @@ int seis_free(void* ptr);
 int seis_memcpy_htod(void* dst, const void* src, size_t bytes);
 int seis_memcpy_dtoh(void* dst, const void* src, size_t bytes);
 int seis_memset(void* dst, int value, size_t bytes);
+/* dst[i] = src[i] for n floats (device pointers, 16-byte aligned, n % 4 == 0)
+ * on the default stream: a plain streaming copy, used to calibrate HBM
+ * traffic counters.  No counterpart in the reference. */
+int seis_copy_f32(float* dst, const float* src, long long n);
 int seis_synchronize(void);
 
 /* Forward shot modeling; replaces `modeling` (fd.h:3, fd.cpp:15-80).

@@ -142,6 +142,29 @@ int seis_memset(void* dst, int value, size_t bytes) {
   return e == hipSuccess ? SEIS_OK : hip_fail("hipMemset", e);
 }
 
+}  // extern "C"
+
+namespace {
+// Plain streaming copy: 16 B per lane per iteration, grid-stride.
+__global__ __launch_bounds__(256) void copy_f32x4_kernel(float4* __restrict__ dst, const float4* __restrict__ src,
+                                                         long long n4) {
+  for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n4; i += (long long)gridDim.x * 256)
+    dst[i] = src[i];
+}
+}  // namespace
+
+extern "C" {
+
+int seis_copy_f32(float* dst, const float* src, long long n) {
+  if (n < 0 || ((n & 3) != 0)) return set_error("seis_copy_f32: n must be a non-negative multiple of 4"), SEIS_ERR_ARG;
+  if (((uintptr_t)dst | (uintptr_t)src) & 15) return set_error("seis_copy_f32: pointers must be 16-byte aligned"), SEIS_ERR_ARG;
+  if (n == 0) return SEIS_OK;
+  hipLaunchKernelGGL(copy_f32x4_kernel, dim3(4096), dim3(256), 0, 0, reinterpret_cast<float4*>(dst),
+                     reinterpret_cast<const float4*>(src), n / 4);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? SEIS_OK : hip_fail("copy_f32x4_kernel", e);
+}
+
 int seis_synchronize(void) {
   hipError_t e = hipDeviceSynchronize();
   return e == hipSuccess ? SEIS_OK : hip_fail("hipDeviceSynchronize", e);
