@@ -31,6 +31,8 @@ struct DevShot {
   int nx, NX, nt, ndel, sx, sz;
   int snap_kind;   // SNAP_NATIVE, SNAP_BF16 or SNAP_FP16
   int rtm_flags;   // SEIS_RTM_* bits (RTM reverse pass)
+  T* raw;          // stencil values of the band cells of the new level, grid
+                   // origin; second field at raw + fstride
 };
 
 // Step kernel modes.  MODEL / BORN / RTM_* follow the reference's modeling,
@@ -213,9 +215,60 @@ inline Layout make_layout(int NX, int NZ, int pml) {
 }
 
 
-// Launches one step of `mode` for every shot of a batch (engine.hip).
+// Enumeration of the cells outside an inner rectangle [xa, xb) x [za, zb) of
+// the padded grid: the rows above and below it over the full width, then the
+// columns left and right of it (z fastest within each part).
+struct Ring {
+  int xa, xb, za, zb;
+  long long n;
+};
+
+__device__ __forceinline__ void ring_cell(const Ring& R, int NX, int NZ, long long j, int* x, int* z) {
+  const long long top = (long long)NX * R.za, bot = (long long)NX * (NZ - R.zb);
+  const long long lft = (long long)R.xa * (R.zb - R.za);
+  if (j < top) {
+    *x = (int)(j / R.za);
+    *z = (int)(j % R.za);
+  } else if (j < top + bot) {
+    const long long q = j - top;
+    *x = (int)(q / (NZ - R.zb));
+    *z = R.zb + (int)(q % (NZ - R.zb));
+  } else if (j < top + bot + lft) {
+    const long long q = j - top - bot;
+    *x = (int)(q / (R.zb - R.za));
+    *z = R.za + (int)(q % (R.zb - R.za));
+  } else {
+    const long long q = j - top - bot - lft;
+    *x = R.xb + (int)(q / (R.zb - R.za));
+    *z = R.za + (int)(q % (R.zb - R.za));
+  }
+}
+
+inline Ring make_ring(int NX, int NZ, int xa, int xb, int za, int zb) {
+  Ring R;
+  if (xb <= xa || zb <= za) {
+    R.xa = R.xb = 0;
+    R.za = 0;
+    R.zb = NZ;
+  } else {
+    R.xa = xa;
+    R.xb = xb;
+    R.za = za;
+    R.zb = zb;
+  }
+  R.n = (long long)NX * NZ - (long long)(R.xb - R.xa) * (R.zb - R.za);
+  return R;
+}
+
+// The absorbing band: cells closer than pml to the outer edge.
+inline Ring band_ring(int NX, int NZ, int pml) { return make_ring(NX, NZ, pml, NX - pml, pml, NZ - pml); }
+
+// Launches one step of `mode` for every shot of a batch (engine.hip): the
+// step kernel (every cell; band cells leave their stencil value in
+// DevShot::raw) followed by the band kernel (one-way update and blend of the
+// band cells from those values).
 template <typename T>
-void launch_step(int mode, const Consts<T>& k, const DevShot<T>* dev, const Layout& L, int nshots, int step,
-                 hipStream_t stream);
+void launch_step(int mode, const Consts<T>& k, const DevShot<T>* dev, const Layout& L, const Ring& B, int nshots,
+                 int step, hipStream_t stream);
 
 }  // namespace seis

@@ -150,6 +150,109 @@ __device__ __forceinline__ void cell_update(const Consts<T>& k, const DevShot<T>
         out = s.strips[(long long)(itp - 1) * 8 * (s.nx + k.nz) + sl[0]];
     }
     N[i] = out;
+    if (edge_dist(x, z, NX, k.NZ) == k.pml) s.raw[i] = u;  // stencil value for the band (see interior_sweep)
+  }
+}
+
+// Band cells (closer than pml to the outer edge, outside the model): the
+// step kernel stores their stencil values ("u3" of processBoundary) and the
+// band kernel turns them into new values once every stencil value of the
+// level exists.  The sources, receivers, Born and snapshot terms all lie
+// inside the model, so only the plain stencil (or ring formula) is needed
+// here; SourceRemoval only changes the previous level at the source cell.
+template <typename T, int MODE>
+__device__ __forceinline__ void band_raw(const Consts<T>& k, const DevShot<T>& s, const StepCtx<T>& c, int x,
+                                         int z) {
+  const int NX = s.NX;
+  const long long i = fidx(k.ld, x, z);
+  const Model<T>& m = c.m;
+  T* RA = s.raw;
+  T* RB = s.raw + s.fstride;
+  if (MODE == RTM_BWD) {
+    if (!(s.rtm_flags & SEIS_RTM_ISOLATE_BAND)) {
+      T sub = T(0);
+      const bool on = c.it - 1 > 0;
+      if (on) {
+        const T as = k.dt * m.vel(s.sx + k.pml, s.sz + k.pml);
+        sub = (as * as) * s.wav[c.it - 1];
+      }
+      RA[i] = raw_value(k, m, NX, c.C, c.P, SourceRemoval<T>{c.src, sub, on}, x, z);
+    }
+    RB[i] = raw_value(k, m, NX, c.C + 3 * s.fstride, c.P + 3 * s.fstride, NoInject{}, x, z);
+  } else {
+    RA[i] = raw_value(k, m, NX, c.C, c.P, NoInject{}, x, z);
+    if (MODE == BORN) RB[i] = raw_value(k, m, NX, c.C + 3 * s.fstride, c.P + 3 * s.fstride, NoInject{}, x, z);
+  }
+}
+
+// Stencil value of a neighbour at the new level: stored for band cells; for
+// the others the new value itself (outside the band it is the stencil value).
+// `lim` is pml, or pml + 1 for the reconstructed RTM source field, whose new
+// level holds restored strips in the first model layer while the band reads
+// that layer's stencil values (kept in R by the step kernel).
+template <typename T>
+struct RawNeighbour {
+  const T* R;
+  const T* N;
+  long long ld;
+  int NX, NZ, lim;
+  __device__ __forceinline__ T operator()(int xx, int zz) const {
+    const long long q = fidx(ld, xx, zz);
+    return edge_dist(xx, zz, NX, NZ) < lim ? R[q] : N[q];
+  }
+};
+
+// One-way update and blend of every band cell of every shot (after the step
+// kernel of the same step).
+template <typename T, int MODE>
+__global__ __launch_bounds__(WG) void band_kernel(Consts<T> k, const DevShot<T>* __restrict__ shots, Ring B,
+                                                  int step) {
+  const DevShot<T>& s = shots[blockIdx.y];
+  int it;
+  if (MODE == RTM_BWD) {
+    it = s.nt + s.ndel - 1 - step;
+    if (it < s.ndel) return;
+  } else {
+    it = step;
+    if (it >= s.nt + s.ndel) return;
+  }
+  const long long j = (long long)blockIdx.x * WG + threadIdx.x;
+  if (j >= B.n) return;
+  const int NX = s.NX, NZ = k.NZ, pml = k.pml;
+  int x, z;
+  ring_cell(B, NX, NZ, j, &x, &z);
+  const long long ld = k.ld;
+  const long long i = fidx(ld, x, z);
+  const int r0 = step % 3, r1 = (step + 1) % 3, r2 = (step + 2) % 3;
+  const T* P = s.fld + r0 * s.fstride;
+  const T* C = s.fld + r1 * s.fstride;
+  T* N = s.fld + r2 * s.fstride;
+  const Model<T> m{s.vel, s.nx, k.nz, pml};
+  const long long f3 = 3 * s.fstride;
+  const T* RA = s.raw;
+  const T* RB = s.raw + s.fstride;
+  if (MODE == RTM_BWD) {
+    T sub = T(0);
+    const bool on = it - 1 > 0;
+    if (on) {
+      const T as = k.dt * m.vel(s.sx + pml, s.sz + pml);
+      sub = (as * as) * s.wav[it - 1];
+    }
+    const SourceRemoval<T> sr{fidx(ld, s.sx + pml, s.sz + pml), sub, on};
+    T u = T(0);
+    if (!(s.rtm_flags & SEIS_RTM_ISOLATE_BAND))
+      u = band_blend(k, m, NX, C, P, sr, x, z, RA[i], RawNeighbour<T>{RA, N, ld, NX, NZ, pml + 1});
+    const T lap = (sr.prev(P, i) + u - T(2) * C[i]) * k.idt2;
+    const T v = m.vel(x, z);
+    s.aux[i] = lap * (T(2) / (v * v * v));
+    N[i] = u;
+    N[f3 + i] = band_blend(k, m, NX, C + f3, P + f3, NoInject{}, x, z, RB[i],
+                           RawNeighbour<T>{RB, N + f3, ld, NX, NZ, pml});
+  } else {
+    N[i] = band_blend(k, m, NX, C, P, NoInject{}, x, z, RA[i], RawNeighbour<T>{RA, N, ld, NX, NZ, pml});
+    if (MODE == BORN)
+      N[f3 + i] = band_blend(k, m, NX, C + f3, P + f3, NoInject{}, x, z, RB[i],
+                             RawNeighbour<T>{RB, N + f3, ld, NX, NZ, pml});
   }
 }
 
@@ -201,6 +304,23 @@ __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot
     qa[j] = CA[(long long)(x0 - 4 + j) * ld + zl];
     if (TWO) qb[j] = CB[(long long)(x0 - 4 + j) * ld + zl];
   }
+  // The loads of column x+1 are issued before column x is computed (one
+  // column of software pipelining; the last column re-reads itself).
+  T nca, ncb = T(0), npa, npb = T(0), nv, nha = T(0), nhb = T(0);
+  auto fetch = [&](int xx) {
+    xx = min(xx, x1 - 1);
+    const long long r = (long long)xx * ld;
+    nca = CA[r + 4 * ld + zl];
+    if (TWO) ncb = CB[r + 4 * ld + zl];
+    npa = PA[r + zl];
+    if (TWO) npb = PB[r + zl];
+    nv = s.vel[(long long)(xx - pml) * k.nz + iz];
+    if (halo) {
+      nha = CA[r + hz];
+      if (TWO) nhb = CB[r + hz];
+    }
+  };
+  fetch(x0);
   // per-step scalars
   constexpr bool PSRC = (MODE != RTM_BWD && MODE != BORN_S);
   T amp = T(0);
@@ -212,16 +332,18 @@ __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot
   }
   for (int x = x0; x < x1; x++) {
     const long long row = (long long)x * ld;
-    qa[8] = CA[row + 4 * ld + zl];
-    if (TWO) qb[8] = CB[row + 4 * ld + zl];
+    qa[8] = nca;
+    if (TWO) qb[8] = ncb;
     if (halo) {
-      la[hslot] = CA[row + hz];
-      if (TWO) lb[hslot] = CB[row + hz];
+      la[hslot] = nha;
+      if (TWO) lb[hslot] = nhb;
     }
     la[lane + 4] = qa[4];
     if (TWO) lb[lane + 4] = qb[4];
     const int ix = x - pml;
-    const T v = s.vel[(long long)ix * k.nz + iz];
+    const T v = nv;
+    const T pa_cur = npa, pb_cur = npb;
+    fetch(x + 1);
     const T av = k.dt * v;
     const T vt2 = av * av;
     const T vvzz = vt2 * k.idz2;
@@ -238,7 +360,7 @@ __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot
     uxx = uxx + qa[6] * k.c[2];
     uxx = uxx + qa[7] * k.c[3];
     uxx = uxx + qa[8] * k.c[4];
-    T pa = PA[i];
+    T pa = pa_cur;
     if (MODE == RTM_BWD && sub != T(0) && x == sxp && z == szp) pa = pa - sub;
     T ua = vvzz * uzz + vvxx * uxx + T(2) * qa[4] - pa;
     if (PSRC && x == sxp && z == szp && c.it - 1 >= 0) ua = ua + vt2 * amp;
@@ -266,7 +388,7 @@ __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot
       uxxb = uxxb + qb[6] * k.c[2];
       uxxb = uxxb + qb[7] * k.c[3];
       uxxb = uxxb + qb[8] * k.c[4];
-      ub = vvzz * uzzb + vvxx * uxxb + T(2) * qb[4] - PB[i];
+      ub = vvzz * uzzb + vvxx * uxxb + T(2) * qb[4] - pb_cur;
     }
     wave_sync();
     if (act) {
@@ -311,6 +433,9 @@ __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot
             out = s.strips[(long long)(c.itp - 1) * 8 * (s.nx + k.nz) + sl[0]];
         }
         NA[i] = out;
+        // the band's one-way update reads the stencil value of this layer,
+        // not the restored strip
+        if (edge_dist(x, z, s.NX, NZ) == pml) s.raw[i] = ua;
       }
     }
 #pragma unroll
@@ -356,25 +481,11 @@ __global__ __launch_bounds__(WG) void step_kernel(Consts<T> k, const DevShot<T>*
   if (j >= L.nband) return;
   const int NX = s.NX, NZ = k.NZ;
   int x, z;
-  const long long top = (long long)NX * L.za, bot = (long long)NX * (NZ - L.zb);
-  const long long lft = (long long)L.xa * (L.zb - L.za);
-  if (j < top) {
-    x = (int)(j / L.za);
-    z = (int)(j % L.za);
-  } else if (j < top + bot) {
-    const long long q = j - top;
-    x = (int)(q / (NZ - L.zb));
-    z = L.zb + (int)(q % (NZ - L.zb));
-  } else if (j < top + bot + lft) {
-    const long long q = j - top - bot;
-    x = (int)(q / (L.zb - L.za));
-    z = L.za + (int)(q % (L.zb - L.za));
-  } else {
-    const long long q = j - top - bot - lft;
-    x = L.xb + (int)(q / (L.zb - L.za));
-    z = L.za + (int)(q % (L.zb - L.za));
-  }
-  cell_update<T, MODE>(k, s, c, x, z);
+  ring_cell(Ring{L.xa, L.xb, L.za, L.zb, L.nband}, NX, NZ, j, &x, &z);
+  if (edge_dist(x, z, NX, NZ) < k.pml)
+    band_raw<T, MODE>(k, s, c, x, z);
+  else
+    cell_update<T, MODE>(k, s, c, x, z);
 }
 
 // The imaging condition of one reverse step for one interior column x
@@ -472,12 +583,16 @@ __global__ void crop_kernel(Consts<T> k, const DevShot<T>* __restrict__ shots, T
 // Host side.
 
 template <typename T>
-void launch_step(int mode, const Consts<T>& k, const DevShot<T>* dev, const Layout& L, int nshots, int step,
-                 hipStream_t stream) {
+void launch_step(int mode, const Consts<T>& k, const DevShot<T>* dev, const Layout& L, const Ring& B, int nshots,
+                 int step, hipStream_t stream) {
   dim3 grid(L.nfwg + L.nbwg, nshots), block(WG);
+  dim3 bgrid((unsigned)((B.n + WG - 1) / WG), nshots);
   switch (mode) {
-#define SEIS_CASE(M) \
-  case M: hipLaunchKernelGGL((step_kernel<T, M>), grid, block, 0, stream, k, dev, L, step); break;
+#define SEIS_CASE(M)                                                                            \
+  case M:                                                                                       \
+    hipLaunchKernelGGL((step_kernel<T, M>), grid, block, 0, stream, k, dev, L, step);           \
+    if (B.n > 0) hipLaunchKernelGGL((band_kernel<T, M>), bgrid, block, 0, stream, k, dev, B, step); \
+    break;
     SEIS_CASE(MODEL)
     SEIS_CASE(BORN)
     SEIS_CASE(RTM_FWD)
@@ -489,10 +604,10 @@ void launch_step(int mode, const Consts<T>& k, const DevShot<T>* dev, const Layo
   }
 }
 
-template void launch_step<float>(int, const Consts<float>&, const DevShot<float>*, const Layout&, int, int,
-                                 hipStream_t);
-template void launch_step<double>(int, const Consts<double>&, const DevShot<double>*, const Layout&, int, int,
-                                  hipStream_t);
+template void launch_step<float>(int, const Consts<float>&, const DevShot<float>*, const Layout&, const Ring&, int,
+                                 int, hipStream_t);
+template void launch_step<double>(int, const Consts<double>&, const DevShot<double>*, const Layout&, const Ring&,
+                                  int, int, hipStream_t);
 
 template <typename T>
 struct BatchRun {
@@ -548,8 +663,10 @@ int run_batch(int mode, const GridSpec& g, int nshots, const ShotSpec* shots, co
     NTmax = std::max(NTmax, q.nt + q.ndel);
     if (rtm) strip_elems += (size_t)q.nt * 8 * (g.nx + g.nz);
   }
-  // per shot: nslots fields (+ image and aux for RTM), then strips
-  const size_t per_shot = (size_t)fe * (nslots + (rtm ? 2 : 0));
+  // per shot: nslots fields, the band stencil values of one or two fields
+  // (+ image and aux for RTM), then strips
+  const int nraw = (mode == RUN_MODEL) ? 1 : 2;
+  const size_t per_shot = (size_t)fe * (nslots + nraw + (rtm ? 2 : 0));
   const size_t total = per_shot * nshots + strip_elems;
   BatchRun<T> br;
   br.pool_bytes = total * sizeof(T);
@@ -572,8 +689,9 @@ int run_batch(int mode, const GridSpec& g, int nshots, const ShotSpec* shots, co
     d.rec = rec + q.rec;
     d.fld = base + origin;
     d.fstride = fe;
-    d.img = rtm ? base + (size_t)fe * nslots + origin : nullptr;
-    d.aux = rtm ? base + (size_t)fe * (nslots + 1) + origin : nullptr;
+    d.raw = base + (size_t)fe * nslots + origin;
+    d.img = rtm ? base + (size_t)fe * (nslots + nraw) + origin : nullptr;
+    d.aux = rtm ? base + (size_t)fe * (nslots + nraw + 1) + origin : nullptr;
     d.strips = rtm ? strip_p : nullptr;
     if (rtm) strip_p += (size_t)q.nt * 8 * (g.nx + g.nz);
     d.nx = g.nx;
@@ -588,7 +706,7 @@ int run_batch(int mode, const GridSpec& g, int nshots, const ShotSpec* shots, co
   HIPCHECK(hipMemcpyAsync(br.dev, br.host.data(), sizeof(DevShot<T>) * nshots, hipMemcpyHostToDevice, stream));
   const Consts<T> k = make_consts<T>(g, ld);
   const Layout L = make_layout(NX, NZ, g.pml);
-  dim3 grid(L.nfwg + L.nbwg, nshots), block(WG);
+  const Ring B = band_ring(NX, NZ, g.pml);
   const double cells = (double)NX * NZ;
   const double E = sizeof(T);
   // shots still running at forward step st / reverse step st
@@ -611,14 +729,14 @@ int run_batch(int mode, const GridSpec& g, int nshots, const ShotSpec* shots, co
     for (int st = 0; st < NTmax; st++) {
       const double a = active_fwd(st) * cells;
       timed_launch(stream, "step_model", 4 * E * a, a, [&] {
-        hipLaunchKernelGGL((step_kernel<T, MODEL>), grid, block, 0, stream, k, br.dev, L, st);
+        launch_step<T>(MODEL, k, br.dev, L, B, nshots, st, stream);
       });
     }
   } else if (mode == RUN_BORN) {
     for (int st = 0; st < NTmax; st++) {
       const double a = active_fwd(st) * cells;
       timed_launch(stream, "step_born", 8 * E * a, 2 * a, [&] {
-        hipLaunchKernelGGL((step_kernel<T, BORN>), grid, block, 0, stream, k, br.dev, L, st);
+        launch_step<T>(BORN, k, br.dev, L, B, nshots, st, stream);
       });
     }
   } else {
@@ -626,7 +744,7 @@ int run_batch(int mode, const GridSpec& g, int nshots, const ShotSpec* shots, co
     for (int st = 0; st < NTmax; st++) {
       const int n = active_fwd(st);
       timed_launch(stream, "step_rtm_forward", 4 * E * n * cells + strip * n, n * cells, [&] {
-        hipLaunchKernelGGL((step_kernel<T, RTM_FWD>), grid, block, 0, stream, k, br.dev, L, st);
+        launch_step<T>(RTM_FWD, k, br.dev, L, B, nshots, st, stream);
       });
     }
     // reset the six wavefields, then restore the last strips into the current level
@@ -644,7 +762,7 @@ int run_batch(int mode, const GridSpec& g, int nshots, const ShotSpec* shots, co
     for (int st = 0; st < maxbwd; st++) {
       const int n = active_bwd(st);
       timed_launch(stream, "step_rtm_reverse", 8 * E * n * cells + strip * n, 2 * n * cells, [&] {
-        hipLaunchKernelGGL((step_kernel<T, RTM_BWD>), grid, block, 0, stream, k, br.dev, L, st);
+        launch_step<T>(RTM_BWD, k, br.dev, L, B, nshots, st, stream);
       });
       // the receiver field's new level lives in slot 3 + (st+2)%3
       timed_launch(stream, "rtm_image", 2 * E * n * faqi_cells + 2 * E * n * (double)g.nx * g.nz, 0, [&] {

@@ -204,20 +204,24 @@ SEIS_HD inline T raw_value(const Consts<T>& k, const Model<T>& m, int NX, const 
   return inj.add(val, x, z, vt2, i);
 }
 
-// New value of cell (x, z) after the absorbing boundary and the sponge blend.
-// `raw_out` receives the stencil value of the cell itself.
-template <typename T, class Inj>
-SEIS_HD inline T final_value(const Consts<T>& k, const Model<T>& m, int NX, const T* C, const T* P,
-                                         const Inj& inj, int x, int z, T* raw_out = nullptr) {
-  const T r = raw_value(k, m, NX, C, P, inj, x, z);
-  if (raw_out) *raw_out = r;
+// Band position of a cell: the distance to the nearest outer edge; cells
+// with dist < pml are updated by the one-way layers of processBoundary.
+SEIS_HD inline int edge_dist(int x, int z, int NX, int NZ) {
+  return imin(imin(x, NX - 1 - x), imin(z, NZ - 1 - z));
+}
+
+// New value of band cell (x, z) (edge_dist < pml) from its own stencil value
+// r: the one-way update of processBoundary (fd.cpp:1092-1178) followed by the
+// sponge blend (fd.cpp:55-56).  `rawn(xx, zz)` gives the stencil value of a
+// neighbour at the same new time level.
+template <typename T, class Inj, class RawN>
+SEIS_HD inline T band_blend(const Consts<T>& k, const Model<T>& m, int NX, const T* C, const T* P, const Inj& inj,
+                            int x, int z, T r, const RawN& rawn) {
   const int pml = k.pml, NZ = k.NZ;
   const int dl = x, dr = NX - 1 - x, dtp = z, db = NZ - 1 - z;
   const int mn = imin(imin(dl, dr), imin(dtp, db));
-  if (mn >= pml) return r;
   const long long ld = k.ld;
   const long long i = fidx(ld, x, z);
-  auto N = [&](int xx, int zz) { return raw_value(k, m, NX, C, P, inj, xx, zz); };
   auto Cu = [&](int xx, int zz) { return C[fidx(ld, xx, zz)]; };
   auto Pv = [&](int xx, int zz) { return inj.prev(P, fidx(ld, xx, zz)); };
   const T v = m.vel(x, z);
@@ -231,12 +235,12 @@ SEIS_HD inline T final_value(const Consts<T>& k, const Model<T>& m, int NX, cons
       b3 = (T)(std::sqrt(2.0) / (double)v / (double)k.dt);  // fd.cpp:1176 forms sqrt(2) in double
     else
       b3 = k.sq2 / v / k.dt;
-    T t = -k.b1 * N(x, z + sz) - k.b2 * N(x + sx, z) + b3 * (-C[i]);
+    T t = -k.b1 * rawn(x, z + sz) - k.b2 * rawn(x + sx, z) + b3 * (-C[i]);
     ow = -t / (k.b1 + k.b2 + b3);
   } else if (dtp == mn || db == mn) {
     // top / bottom band: one-way update along z, fd.cpp:1092-1118
     const int s = (dtp == mn) ? 1 : -1;
-    const T nin = N(x, z + s), nip = N(x + 1, z + s), nim = N(x - 1, z + s);
+    const T nin = rawn(x, z + s), nip = rawn(x + 1, z + s), nim = rawn(x - 1, z + s);
     const T a1 = k.a1z;
     const T a2 = k.a2b / v;
     const T a3 = v / T(4) / k.dx / k.dx;
@@ -247,7 +251,7 @@ SEIS_HD inline T final_value(const Consts<T>& k, const Model<T>& m, int NX, cons
   } else {
     // left / right band: one-way update along x, fd.cpp:1119-1144
     const int s = (dl == mn) ? 1 : -1;
-    const T nin = N(x + s, z), nip = N(x + s, z + 1), nim = N(x + s, z - 1);
+    const T nin = rawn(x + s, z), nip = rawn(x + s, z + 1), nim = rawn(x + s, z - 1);
     const T a1 = k.a1x;
     const T a2 = k.a2b / v;
     const T a3 = v / T(4) / k.dz / k.dz;
@@ -258,6 +262,19 @@ SEIS_HD inline T final_value(const Consts<T>& k, const Model<T>& m, int NX, cons
   }
   const T w = sponge_weight<T>(x, z, m.nx, m.nz, pml);
   return r * w + ow * (T(1) - w);
+}
+
+// New value of cell (x, z) after the absorbing boundary and the sponge blend,
+// recomputing the neighbours' stencil values (a pure function of cur and
+// prev).  `raw_out` receives the stencil value of the cell itself.
+template <typename T, class Inj>
+SEIS_HD inline T final_value(const Consts<T>& k, const Model<T>& m, int NX, const T* C, const T* P,
+                                         const Inj& inj, int x, int z, T* raw_out = nullptr) {
+  const T r = raw_value(k, m, NX, C, P, inj, x, z);
+  if (raw_out) *raw_out = r;
+  if (edge_dist(x, z, NX, k.NZ) >= k.pml) return r;
+  auto rawn = [&](int xx, int zz) { return raw_value(k, m, NX, C, P, inj, xx, zz); };
+  return band_blend(k, m, NX, C, P, inj, x, z, r, rawn);
 }
 
 }  // namespace seis

@@ -30,50 +30,6 @@ namespace seis {
 // Field slots of one shot in the LSRTM pool.
 enum LsSlot { LS_FIELD = 0, LS_AIN = 3, LS_AT = 6, LS_AC = 9, LS_AR = 12, LS_NSLOT = 14 };
 
-// Enumeration of the cells outside an inner rectangle [xa, xb) x [za, zb) of
-// the padded grid (same order as the step kernel's band cells).
-struct Ring {
-  int xa, xb, za, zb;
-  long long n;
-};
-
-__device__ __forceinline__ void ring_cell(const Ring& R, int NX, int NZ, long long j, int* x, int* z) {
-  const long long top = (long long)NX * R.za, bot = (long long)NX * (NZ - R.zb);
-  const long long lft = (long long)R.xa * (R.zb - R.za);
-  if (j < top) {
-    *x = (int)(j / R.za);
-    *z = (int)(j % R.za);
-  } else if (j < top + bot) {
-    const long long q = j - top;
-    *x = (int)(q / (NZ - R.zb));
-    *z = R.zb + (int)(q % (NZ - R.zb));
-  } else if (j < top + bot + lft) {
-    const long long q = j - top - bot;
-    *x = (int)(q / (R.zb - R.za));
-    *z = R.za + (int)(q % (R.zb - R.za));
-  } else {
-    const long long q = j - top - bot - lft;
-    *x = R.xb + (int)(q / (R.zb - R.za));
-    *z = R.za + (int)(q % (R.zb - R.za));
-  }
-}
-
-inline Ring make_ring(int NX, int NZ, int xa, int xb, int za, int zb) {
-  Ring R;
-  if (xb <= xa || zb <= za) {
-    R.xa = R.xb = 0;
-    R.za = 0;
-    R.zb = NZ;
-  } else {
-    R.xa = xa;
-    R.xb = xb;
-    R.za = za;
-    R.zb = zb;
-  }
-  R.n = (long long)NX * NZ - (long long)(R.xb - R.xa) * (R.zb - R.za);
-  return R;
-}
-
 // Adjoint level m of a shot: slot m % 3 of the level fields and band products.
 template <typename T>
 __device__ __forceinline__ AdjLevel<T> level(const DevShot<T>& s, int lev) {
@@ -352,6 +308,7 @@ struct LsrtmPlan {
   Layout F{};   // forward step split (interior sweep / band)
   Layout A{};   // adjoint step split
   Ring P{};     // cells whose raw adjoint is precomputed
+  Ring B{};     // absorbing band of the forward steps
   hipStream_t stream = 0;
 
   ~LsrtmPlan() {
@@ -415,6 +372,7 @@ struct LsrtmPlan {
       A.nbwg = (int)((A.nband + AWG - 1) / AWG);
     }
     P = make_ring(NX, NZ, g.pml + 1, NX - g.pml - 1, g.pml + 1, NZ - g.pml - 1);
+    B = band_ring(NX, NZ, g.pml);
     host.resize(ns);
     const long long origin = (long long)MX * ld + MZ;
     char* sp = reinterpret_cast<char*>(snaps);
@@ -428,6 +386,9 @@ struct LsrtmPlan {
       d.gz = gz + q.gz;
       d.fld = pool + (size_t)fe * LS_NSLOT * s + origin;
       d.fstride = fe;
+      // the forward passes propagate one field; the band-product slots are
+      // free then (the adjoint clears every slot before use)
+      d.raw = d.fld + (size_t)fe * LS_AIN;
       d.snap = sp;
       d.scale = sc;
       d.grad = grads + (size_t)model * s;
@@ -471,7 +432,7 @@ struct LsrtmPlan {
         const int n = active_fwd(st);
         const double by = 4 * E * n * cells() + (mode == SNAP_STORE ? snap_elem() * n * (double)model : 0.0);
         timed_launch(stream, mode == SNAP_MAX ? "lsrtm_snap_max" : "lsrtm_snap_store", by, n * cells(),
-                     [&] { launch_step<T>(mode, k, dev, F, nshots, st, stream); });
+                     [&] { launch_step<T>(mode, k, dev, F, B, nshots, st, stream); });
       }
     }
     LS_CHECK(hipGetLastError());
@@ -492,7 +453,7 @@ struct LsrtmPlan {
     for (int st = 0; st < NTmax; st++) {
       const int n = active_fwd(st);
       const double by = 4 * E * n * cells() + (snap_elem() + E) * n * (double)model;
-      timed_launch(stream, "lsrtm_born", by, n * cells(), [&] { launch_step<T>(BORN_S, k, dev, F, nshots, st, stream); });
+      timed_launch(stream, "lsrtm_born", by, n * cells(), [&] { launch_step<T>(BORN_S, k, dev, F, B, nshots, st, stream); });
     }
     if (mask) hipLaunchKernelGGL((mul_kernel<T>), dim3(1024), dim3(256), 0, stream, rec, rec, mask, nrec);
     LS_CHECK(hipGetLastError());
