@@ -23,7 +23,8 @@ struct DevShot {
   T* fld;          // slot 0 grid origin; slot j at fld + j*fstride
   T* strips;       // [nt][8*(nx+nz)] (RTM)
   T* img;          // padded image accumulator, grid origin (RTM)
-  T* aux;          // padded scratch field, grid origin (RTM: lap*2/v^3)
+  T* aux;          // two padded scratch fields, grid origin, the second at
+                   // aux + fstride (RTM: lap*2/v^3 of even / odd reverse steps)
   void* snap;      // Born-source snapshots [nt+ndel][nx][nz] (LSRTM)
   float* scale;    // per-snapshot scale [nt+ndel] (LSRTM, 16-bit snapshots)
   T* grad;         // per-shot image gradient [nx][nz] (LSRTM)
@@ -118,6 +119,7 @@ struct StepCtx {
   Model<T> m;
   int it, itp;
   long long src;    // field index of the source cell
+  T* aux;           // RTM reverse: U of this step (one of two buffers)
 };
 
 // Wavefront-local LDS exchange barrier (the hardware orders a wave's LDS

@@ -139,7 +139,7 @@ __device__ __forceinline__ void cell_update(const Consts<T>& k, const DevShot<T>
     ReceiverInject<T> ri{s.gz, s.rec, s.nx, s.nt, itp, k.pml};
     const T ur = final_value(k, m, NX, Cr, Pr, ri, x, z);
     const T v3 = T(2) / (v * v * v);
-    s.aux[i] = lap * v3;  // U of the imaging condition; the image pass reads it
+    c.aux[i] = lap * v3;  // U of the imaging condition; the image pass reads it
     Nr[i] = ur;
     // the current level of the next reverse step gets the saved strips
     // restored (fd.cpp:663); lap above used the value before restoration
@@ -244,7 +244,7 @@ __global__ __launch_bounds__(WG) void band_kernel(Consts<T> k, const DevShot<T>*
       u = band_blend(k, m, NX, C, P, sr, x, z, RA[i], RawNeighbour<T>{RA, N, ld, NX, NZ, pml + 1});
     const T lap = (sr.prev(P, i) + u - T(2) * C[i]) * k.idt2;
     const T v = m.vel(x, z);
-    s.aux[i] = lap * (T(2) / (v * v * v));
+    s.aux[(step & 1) * s.fstride + i] = lap * (T(2) / (v * v * v));
     N[i] = u;
     N[f3 + i] = band_blend(k, m, NX, C + f3, P + f3, NoInject{}, x, z, RB[i],
                            RawNeighbour<T>{RB, N + f3, ld, NX, NZ, pml});
@@ -306,7 +306,11 @@ __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot
   }
   // The loads of column x+1 are issued before column x is computed (one
   // column of software pipelining; the last column re-reads itself).
-  T nca, ncb = T(0), npa, npb = T(0), nv, nha = T(0), nhb = T(0);
+  // The receiver depth of the column (and, in the reverse pass, its record
+  // sample) is fetched the same way.
+  constexpr bool GZ = (MODE == MODEL || MODE == BORN || MODE == BORN_S || MODE == RTM_BWD);
+  T nca, ncb = T(0), npa, npb = T(0), nv, nha = T(0), nhb = T(0), nrec = T(0);
+  int ngz = -1;
   auto fetch = [&](int xx) {
     xx = min(xx, x1 - 1);
     const long long r = (long long)xx * ld;
@@ -319,6 +323,8 @@ __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot
       nha = CA[r + hz];
       if (TWO) nhb = CB[r + hz];
     }
+    if (GZ) ngz = s.gz[xx - pml];
+    if (MODE == RTM_BWD) nrec = s.rec[(long long)(xx - pml) * s.nt + c.itp];
   };
   fetch(x0);
   // per-step scalars
@@ -342,7 +348,8 @@ __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot
     if (TWO) lb[lane + 4] = qb[4];
     const int ix = x - pml;
     const T v = nv;
-    const T pa_cur = npa, pb_cur = npb;
+    const T pa_cur = npa, pb_cur = npb, rec_cur = nrec;
+    const int gzr = ngz + pml;  // receiver row of this column
     fetch(x + 1);
     const T av = k.dt * v;
     const T vt2 = av * av;
@@ -399,11 +406,11 @@ __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot
         const T d = s.dv[j];
         if (d != T(0)) ua = ua + snap_load(s, k.nz, c.it, j) * d;
         NA[i] = ua;
-        if (c.itp >= 0 && z == s.gz[ix] + pml) s.rec[(long long)ix * s.nt + c.itp] = ua;
+        if (c.itp >= 0 && z == gzr) s.rec[(long long)ix * s.nt + c.itp] = ua;
       } else if (MODE == MODEL || MODE == RTM_FWD) {
         NA[i] = ua;
         if (MODE == MODEL) {
-          if (c.itp >= 0 && z == s.gz[ix] + pml) s.rec[(long long)ix * s.nt + c.itp] = ua;
+          if (c.itp >= 0 && z == gzr) s.rec[(long long)ix * s.nt + c.itp] = ua;
         } else if (c.itp >= 0) {
           int sl[4];
           const int n = strip_slots(x, z, s.nx, k.nz, pml, sl);
@@ -419,12 +426,12 @@ __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot
         }
         NA[i] = ua;
         NB[i] = ub;
-        if (c.itp >= 0 && z == s.gz[ix] + pml) s.rec[(long long)ix * s.nt + c.itp] = ub;
+        if (c.itp >= 0 && z == gzr) s.rec[(long long)ix * s.nt + c.itp] = ub;
       } else {  // RTM_BWD: source field ua, receiver field ub
-        if (z == s.gz[ix] + pml) ub = ub + s.rec[(long long)ix * s.nt + c.itp] * vt2;
+        if (z == gzr) ub = ub + rec_cur * vt2;
         const T lap = (pa + ua - T(2) * qa[4]) * k.idt2;
         const T v3 = T(2) / (v * v * v);
-        s.aux[i] = lap * v3;
+        c.aux[i] = lap * v3;
         NB[i] = ub;
         T out = ua;
         if (c.itp - 1 >= 0) {
@@ -466,6 +473,7 @@ __global__ __launch_bounds__(WG) void step_kernel(Consts<T> k, const DevShot<T>*
   c.N = s.fld + r2 * s.fstride;
   c.m = Model<T>{s.vel, s.nx, k.nz, k.pml};
   c.src = fidx(k.ld, s.sx + k.pml, s.sz + k.pml);
+  c.aux = s.aux ? s.aux + (step & 1) * s.fstride : nullptr;
   if ((int)blockIdx.x < L.nfwg) {
     const int wave = threadIdx.x >> 6;
     const int item = blockIdx.x * WAVES + wave;
@@ -501,7 +509,7 @@ struct HilbertTaps {
 };
 
 template <int HALF>
-__global__ __launch_bounds__(256) void image_kernel(Consts<float> k, const DevShot<float>* __restrict__ shots,
+__global__ __launch_bounds__(128) void image_kernel(Consts<float> k, const DevShot<float>* __restrict__ shots,
                                                     int step, HilbertTaps taps, int slot_off) {
   extern __shared__ unsigned char smem_raw[];
   float* sU = reinterpret_cast<float*>(smem_raw);
@@ -515,7 +523,7 @@ __global__ __launch_bounds__(256) void image_kernel(Consts<float> k, const DevSh
   const long long base = fidx(k.ld, x, 0);
   const float* R = s.fld + slot_off * s.fstride;
   for (int z = threadIdx.x; z < NZ; z += blockDim.x) {
-    sU[z] = s.aux[base + z];
+    sU[z] = s.aux[(step & 1) * s.fstride + base + z];
     sR[z] = R[base + z];
   }
   __syncthreads();
@@ -609,6 +617,31 @@ template void launch_step<float>(int, const Consts<float>&, const DevShot<float>
 template void launch_step<double>(int, const Consts<double>&, const DevShot<double>*, const Layout&, const Ring&,
                                   int, int, hipStream_t);
 
+// Second stream of the RTM reverse pass and its ordering events (default
+// priority; non-blocking so that it also overlaps work on the null stream).
+struct ImageStream {
+  static constexpr int NEV = 4;
+  hipStream_t s = nullptr;
+  hipEvent_t ready[NEV] = {}, done[NEV] = {};
+  int init() {
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess)
+      return set_error("rtm: hipStreamCreateWithFlags failed"), SEIS_ERR_HIP;
+    for (int i = 0; i < NEV; i++)
+      if (hipEventCreateWithFlags(&ready[i], hipEventDisableTiming) != hipSuccess ||
+          hipEventCreateWithFlags(&done[i], hipEventDisableTiming) != hipSuccess)
+        return set_error("rtm: hipEventCreateWithFlags failed"), SEIS_ERR_HIP;
+    return SEIS_OK;
+  }
+  ~ImageStream() {
+    if (s) (void)hipStreamSynchronize(s);
+    for (int i = 0; i < NEV; i++) {
+      if (ready[i]) (void)hipEventDestroy(ready[i]);
+      if (done[i]) (void)hipEventDestroy(done[i]);
+    }
+    if (s) (void)hipStreamDestroy(s);
+  }
+};
+
 template <typename T>
 struct BatchRun {
   std::vector<DevShot<T>> host;
@@ -664,9 +697,9 @@ int run_batch(int mode, const GridSpec& g, int nshots, const ShotSpec* shots, co
     if (rtm) strip_elems += (size_t)q.nt * 8 * (g.nx + g.nz);
   }
   // per shot: nslots fields, the band stencil values of one or two fields
-  // (+ image and aux for RTM), then strips
+  // (+ image and two U buffers for RTM), then strips
   const int nraw = (mode == RUN_MODEL) ? 1 : 2;
-  const size_t per_shot = (size_t)fe * (nslots + nraw + (rtm ? 2 : 0));
+  const size_t per_shot = (size_t)fe * (nslots + nraw + (rtm ? 3 : 0));
   const size_t total = per_shot * nshots + strip_elems;
   BatchRun<T> br;
   br.pool_bytes = total * sizeof(T);
@@ -759,18 +792,32 @@ int run_batch(int mode, const GridSpec& g, int nshots, const ShotSpec* shots, co
     int maxbwd = 0;
     for (int s = 0; s < nshots; s++) maxbwd = std::max(maxbwd, shots[s].nt);
     const double faqi_cells = (double)g.nx * NZ;
+    // The imaging pass of step st runs on a second stream, overlapping the
+    // reverse step st+1 (memory-bound against compute-bound).  Step st+1
+    // writes the other U buffer and does not touch the image; step st+2
+    // rewrites the U buffer of step st, so it waits for that imaging pass
+    // (which also covers the receiver slot reused at st+3).
+    ImageStream is;
+    if ((rc = is.init())) return rc;
     for (int st = 0; st < maxbwd; st++) {
       const int n = active_bwd(st);
+      if (st >= 2) HIPCHECK(hipStreamWaitEvent(stream, is.done[st % ImageStream::NEV], 0));
       timed_launch(stream, "step_rtm_reverse", 8 * E * n * cells + strip * n, 2 * n * cells, [&] {
         launch_step<T>(RTM_BWD, k, br.dev, L, B, nshots, st, stream);
       });
+      HIPCHECK(hipEventRecord(is.ready[st % ImageStream::NEV], stream));
+      HIPCHECK(hipStreamWaitEvent(is.s, is.ready[st % ImageStream::NEV], 0));
       // the receiver field's new level lives in slot 3 + (st+2)%3
-      timed_launch(stream, "rtm_image", 2 * E * n * faqi_cells + 2 * E * n * (double)g.nx * g.nz, 0, [&] {
-        hipLaunchKernelGGL((image_kernel<half>), dim3(g.nx, nshots), dim3(256), smem, stream, k, br.dev, st, taps,
+      timed_launch(is.s, "rtm_image", 2 * E * n * faqi_cells + 2 * E * n * (double)g.nx * g.nz, 0, [&] {
+        hipLaunchKernelGGL((image_kernel<half>), dim3(g.nx, nshots), dim3(128), smem, is.s, k, br.dev, st, taps,
                            3 + (st + 2) % 3);
       });
+      HIPCHECK(hipEventRecord(is.done[(st + 2) % ImageStream::NEV], is.s));
     }
+    HIPCHECK(hipEventRecord(is.ready[0], is.s));
+    HIPCHECK(hipStreamWaitEvent(stream, is.ready[0], 0));
     hipLaunchKernelGGL((crop_kernel<T>), dim3(64, nshots), dim3(256), 0, stream, k, br.dev, img);
+    HIPCHECK(hipStreamSynchronize(stream));
   }
   HIPCHECK(hipGetLastError());
   HIPCHECK(hipStreamSynchronize(stream));
