@@ -82,6 +82,132 @@ def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
+def lsrtm_bench(nshots, nt, iters, err_shots, err_iters):
+    """Seconds per CG iteration of least-squares RTM on the 376 x 1151 @ 8 m
+    workload (the RTM model at half resolution, nt = 3000, dt = 0.8 ms, 400
+    receivers per shot), with BF16 snapshots, beside the plain FP32 GPU path
+    (per iteration one Born batch and one RTM batch, as the reference's
+    forward_MPI + adjoint_MPI would), and the image error of BF16 and FP32
+    snapshots against the FP64 GPU plan on a small batch."""
+    import ctypes
+    import seis_amd
+    from seis_amd import DeviceArray, models
+    from seis_amd.batch import _layout, Grid, Shot
+    from seis_amd._lib import check
+    L = seis_amd.lib()
+    nx, nz, pml, dx, dt, fr = 1151, 376, 30, 8.0, 0.0008, 12.0
+    v = np.ascontiguousarray(models.layered_faulted(2301, 751, 4.0, 4.0, seed=1001)[::2, ::2])
+    v0 = models.smooth_model(v, 10.0)
+    dv_true = (v - v0).astype(np.float32)
+    delay = int(round(1.0 / (fr * dt)))
+    grid = Grid(nx, nz, pml, dt, dx, dx)
+    g = grid.c()
+    all_sx = [4 + 9 * i for i in range(128)]
+
+    def survey(sxs, nt_):
+        shots = [Shot(sx=sx, sz=2, nt=nt_) for sx in sxs]
+        arr, nw, ng, nr = _layout(grid, shots)
+        mask = np.zeros((len(sxs), nx, nt_), np.float32)
+        for j, sx in enumerate(sxs):
+            mask[j, max(0, sx - 200):min(nx, sx + 200)] = 1.0
+        return shots, arr, nr, mask.ravel()
+
+    def born_data(shots, arr, nr, mask, vel, wav, gz, noise_seed):
+        d_dv = DeviceArray.from_host(dv_true)
+        d_rec = DeviceArray((nr,), np.float32, zero=True)
+        check(L.seis_born_batch(ctypes.byref(g), len(shots), arr, vel.ptr, d_dv.ptr, wav.ptr, gz.ptr, d_rec.ptr))
+        d = d_rec.download() * mask
+        rms = float(np.sqrt(np.mean(d[mask > 0] ** 2)))
+        d += (0.05 * rms * np.random.default_rng(noise_seed).standard_normal(d.shape)).astype(np.float32) * mask
+        return d
+
+    out = {"workload": "lsrtm_cg: %d of the 128 shots, 376x1151 cells @ 8 m, pml 30, nt=%d, dt=0.8 ms, "
+                       "Ricker 12 Hz, 400 receivers per shot, Born data of the true perturbation + 5%% noise"
+                       % (nshots, nt)}
+    # --- timed: BF16-snapshot CGLS against the plain FP32 path
+    sxs = all_sx[::2][:nshots]
+    shots, arr, nr, mask = survey(sxs, nt)
+    ns = len(shots)
+    wav1 = seis_amd.ricker(dt, fr, delay, nt)
+    d_v0 = DeviceArray.from_host(v0)
+    d_v = DeviceArray.from_host(v)
+    d_wav = DeviceArray.from_host(np.tile(wav1, ns))
+    d_gz = DeviceArray.from_host(np.full(nx * ns, 2, np.int32))
+    d_mask = DeviceArray.from_host(mask)
+    dobs = DeviceArray.from_host(born_data(shots, arr, nr, mask, d_v0, d_wav, d_gz, 11))
+    log("lsrtm: observed data for %d shots ready" % ns)
+    t0 = time.perf_counter()
+    plan = ctypes.c_void_p()
+    check(L.seis_lsrtm_create(ctypes.byref(plan), ctypes.byref(g), ns, arr, d_v0.ptr, d_wav.ptr, d_gz.ptr, 1))
+    seis_amd.synchronize()
+    out["snapshot_s"] = round(time.perf_counter() - t0, 3)
+    check(L.seis_lsrtm_set_mask(plan, d_mask.ptr))
+    img = DeviceArray((nx, nz), np.float32, zero=True)
+    hist = (ctypes.c_double * (2 * (iters + 1)))()
+    done = ctypes.c_int(0)
+    seis_amd.profile_start()
+    t0 = time.perf_counter()
+    check(L.seis_lsrtm_cg(plan, dobs.ptr, iters, 0.0, img.ptr, hist, ctypes.byref(done)))
+    seis_amd.synchronize()
+    el = time.perf_counter() - t0
+    st = seis_amd.profile_stop()
+    out["kernels"] = {k: {"launches": v["launches"], "ms": round(v["total_ms"], 2),
+                          "GB/s": round(v["alg_bytes"] / max(v["total_ms"], 1e-9) / 1e6, 1)} for k, v in st.items()}
+    check(L.seis_lsrtm_destroy(plan))
+    h = np.array(hist[:]).reshape(-1, 2)[: done.value + 1]
+    out["cg_iterations"] = done.value
+    out["s_per_cg_iter_bf16"] = round(el / max(done.value, 1), 3)
+    out["residual_rel"] = [round(float(np.sqrt(x / h[0, 0])), 5) for x in h[:, 0]]
+    log("lsrtm: %d CG iterations (BF16 snapshots) in %.1f s" % (done.value, el))
+    # plain FP32 GPU path: Born batch + RTM batch per iteration
+    d_pert = DeviceArray.from_host(img.download())
+    d_rec = DeviceArray((nr,), np.float32, zero=True)
+    d_img = DeviceArray((ns * nx * nz,), np.float32, zero=True)
+    t0 = time.perf_counter()
+    check(L.seis_born_batch(ctypes.byref(g), ns, arr, d_v0.ptr, d_pert.ptr, d_wav.ptr, d_gz.ptr, d_rec.ptr))
+    check(L.seis_rtm_batch_ex(ctypes.byref(g), ns, arr, d_v0.ptr, d_wav.ptr, d_gz.ptr, dobs.ptr, d_img.ptr, 1))
+    seis_amd.synchronize()
+    plain = time.perf_counter() - t0
+    out["s_per_iter_plain_fp32"] = round(plain, 3)
+    out["speedup_vs_plain_fp32"] = round(plain / out["s_per_cg_iter_bf16"], 2)
+    log("lsrtm: plain FP32 path, one iteration's Born + RTM in %.1f s" % plain)
+    del d_img, d_rec, d_pert, dobs
+    # --- image error of 16/32-bit snapshots against the FP64 plan (small batch)
+    if err_shots > 0:
+        esx = all_sx[32:32 + 8 * err_shots:8][:err_shots]
+        eshots, earr, enr, emask = survey(esx, nt)
+        ew = DeviceArray.from_host(np.tile(wav1, len(eshots)))
+        egz = DeviceArray.from_host(np.full(nx * len(eshots), 2, np.int32))
+        ed = born_data(eshots, earr, enr, emask, d_v0, ew, egz, 12)
+        imgs = {}
+        for kind, name in ((0, "fp32"), (1, "bf16"), (2, "fp16"), (-1, "fp64")):
+            pl = ctypes.c_void_p()
+            if kind < 0:
+                dt64 = np.float64
+                # the plan reads these during its lifetime: keep them referenced
+                v64 = DeviceArray.from_host(v0.astype(dt64))
+                w64 = DeviceArray.from_host(np.tile(wav1, len(eshots)).astype(dt64))
+                check(L.seis_lsrtm_create_f64(ctypes.byref(pl), ctypes.byref(g), len(eshots), earr, v64.ptr, w64.ptr,
+                                              egz.ptr))
+            else:
+                dt64 = np.float32
+                check(L.seis_lsrtm_create(ctypes.byref(pl), ctypes.byref(g), len(eshots), earr, d_v0.ptr, ew.ptr,
+                                          egz.ptr, kind))
+            m = DeviceArray.from_host(emask.astype(dt64))
+            check(L.seis_lsrtm_set_mask(pl, m.ptr))
+            dd = DeviceArray.from_host(ed.astype(dt64))
+            im = DeviceArray((nx, nz), dt64, zero=True)
+            check(L.seis_lsrtm_cg(pl, dd.ptr, err_iters, 0.0, im.ptr, None, None))
+            imgs[name] = im.download().astype(np.float64)
+            check(L.seis_lsrtm_destroy(pl))
+        ref = imgs["fp64"]
+        out["image_rel_l2_vs_fp64"] = {k: float("%.3e" % (np.linalg.norm(imgs[k] - ref) / np.linalg.norm(ref)))
+                                       for k in ("fp32", "bf16", "fp16")}
+        out["error_case"] = "%d shots, %d CG iterations, same grid and nt" % (len(eshots), err_iters)
+        log("lsrtm: image errors vs FP64 %s" % out["image_rel_l2_vs_fp64"])
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,6 +218,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--nt", type=int, default=5000,
                     help="record length; shorter runs keep every launch's size (for counter collection)")
+    ap.add_argument("--lsrtm-shots", type=int, default=64, help="shots of the LSRTM timing (0 skips it)")
+    ap.add_argument("--lsrtm-nt", type=int, default=3000)
+    ap.add_argument("--lsrtm-iters", type=int, default=3)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 8 device-to-device copies of 1 GiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -198,6 +327,9 @@ def main():
         "prep_s": round(prep_s, 2),
         "image_finite": bool(np.isfinite(img).all()),
     }
+    if args.lsrtm_shots > 0:
+        del d_rec, d_img
+        out["lsrtm"] = lsrtm_bench(args.lsrtm_shots, args.lsrtm_nt, args.lsrtm_iters, 2, 3)
     if not args.no_cpu:
         threads = min(16, len(os.sched_getaffinity(0)))
         log("timing the CPU baseline on %d threads" % threads)

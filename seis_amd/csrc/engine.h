@@ -77,22 +77,22 @@ SEIS_HD inline float f16_to_f32(unsigned short u) {
 template <typename T>
 __device__ __forceinline__ T snap_load(const DevShot<T>& s, int nz, int it, long long j) {
   const long long o = (long long)it * s.nx * nz + j;
-  if (s.snap_kind == SNAP_NATIVE) return reinterpret_cast<const T*>(s.snap)[o];
-  const unsigned short h = reinterpret_cast<const unsigned short*>(s.snap)[o];
+  if (s.snap_kind == SNAP_NATIVE) return SEIS_GL(reinterpret_cast<const T*>(s.snap))[o];
+  const unsigned short h = SEIS_GL(reinterpret_cast<const unsigned short*>(s.snap))[o];
   const float f = s.snap_kind == SNAP_BF16 ? bf16_to_f32(h) : f16_to_f32(h);
-  return (T)(f * s.scale[it]);
+  return (T)(f * SEIS_GL(s.scale)[it]);
 }
 
 template <typename T>
 __device__ __forceinline__ void snap_store(const DevShot<T>& s, int nz, int it, long long j, T v) {
   const long long o = (long long)it * s.nx * nz + j;
   if (s.snap_kind == SNAP_NATIVE) {
-    reinterpret_cast<T*>(s.snap)[o] = v;
+    SEIS_GL(reinterpret_cast<T*>(s.snap))[o] = v;
     return;
   }
-  const float sc = s.scale[it];
+  const float sc = SEIS_GL(s.scale)[it];
   const float q = sc > 0.f ? (float)v / sc : 0.f;
-  reinterpret_cast<unsigned short*>(s.snap)[o] = s.snap_kind == SNAP_BF16 ? f32_to_bf16(q) : f32_to_f16(q);
+  SEIS_GL(reinterpret_cast<unsigned short*>(s.snap))[o] = s.snap_kind == SNAP_BF16 ? f32_to_bf16(q) : f32_to_f16(q);
 }
 
 // Indices of (x, z) in the boundary-strip store of one time sample

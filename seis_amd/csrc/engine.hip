@@ -198,7 +198,7 @@ struct RawNeighbour {
   int NX, NZ, lim;
   __device__ __forceinline__ T operator()(int xx, int zz) const {
     const long long q = fidx(ld, xx, zz);
-    return edge_dist(xx, zz, NX, NZ) < lim ? R[q] : N[q];
+    return edge_dist(xx, zz, NX, NZ) < lim ? SEIS_GL(R)[q] : SEIS_GL(N)[q];
   }
 };
 
@@ -285,12 +285,15 @@ __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot
   const int NZ = k.NZ, pml = k.pml;
   const int zl = min(z, NZ - 1);
   const long long ld = k.ld;
-  const T* CA = c.C;
-  const T* PA = c.P;
-  T* NA = c.N;
-  const T* CB = c.C + 3 * s.fstride;
-  const T* PB = c.P + 3 * s.fstride;
-  T* NB = c.N + 3 * s.fstride;
+  const auto CA = SEIS_GL(c.C);
+  const auto PA = SEIS_GL(c.P);
+  const auto NA = SEIS_GL(c.N);
+  const auto CB = SEIS_GL(c.C + 3 * s.fstride);
+  const auto PB = SEIS_GL(c.P + 3 * s.fstride);
+  const auto NB = SEIS_GL(c.N + 3 * s.fstride);
+  const auto VEL = SEIS_GL(s.vel);
+  const auto GZA = SEIS_GL(s.gz);
+  const auto REC = SEIS_GL(s.rec);
   T* la = lds;
   T* lb = lds + LROW;
   const bool halo = lane < 4 || lane >= 60;
@@ -318,13 +321,13 @@ __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot
     if (TWO) ncb = CB[r + 4 * ld + zl];
     npa = PA[r + zl];
     if (TWO) npb = PB[r + zl];
-    nv = s.vel[(long long)(xx - pml) * k.nz + iz];
+    nv = VEL[(long long)(xx - pml) * k.nz + iz];
     if (halo) {
       nha = CA[r + hz];
       if (TWO) nhb = CB[r + hz];
     }
-    if (GZ) ngz = s.gz[xx - pml];
-    if (MODE == RTM_BWD) nrec = s.rec[(long long)(xx - pml) * s.nt + c.itp];
+    if (GZ) ngz = GZA[xx - pml];
+    if (MODE == RTM_BWD) nrec = REC[(long long)(xx - pml) * s.nt + c.itp];
   };
   fetch(x0);
   // per-step scalars
@@ -403,22 +406,22 @@ __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot
         NA[i] = ua;
       } else if (MODE == BORN_S) {
         const long long j = (long long)ix * k.nz + iz;
-        const T d = s.dv[j];
+        const T d = SEIS_GL(s.dv)[j];
         if (d != T(0)) ua = ua + snap_load(s, k.nz, c.it, j) * d;
         NA[i] = ua;
-        if (c.itp >= 0 && z == gzr) s.rec[(long long)ix * s.nt + c.itp] = ua;
+        if (c.itp >= 0 && z == gzr) REC[(long long)ix * s.nt + c.itp] = ua;
       } else if (MODE == MODEL || MODE == RTM_FWD) {
         NA[i] = ua;
         if (MODE == MODEL) {
-          if (c.itp >= 0 && z == gzr) s.rec[(long long)ix * s.nt + c.itp] = ua;
+          if (c.itp >= 0 && z == gzr) REC[(long long)ix * s.nt + c.itp] = ua;
         } else if (c.itp >= 0) {
           int sl[4];
           const int n = strip_slots(x, z, s.nx, k.nz, pml, sl);
-          T* st = s.strips + (long long)c.itp * 8 * (s.nx + k.nz);
+          const auto st = SEIS_GL(s.strips + (long long)c.itp * 8 * (s.nx + k.nz));
           for (int q = 0; q < n; q++) st[sl[q]] = ua;
         }
       } else if (MODE == BORN) {
-        const T d = s.dv[(long long)ix * k.nz + iz];
+        const T d = SEIS_GL(s.dv)[(long long)ix * k.nz + iz];
         if (d != T(0)) {
           const T lap = (ua + pa - T(2) * qa[4]) * k.idt2;
           const T v3 = T(2) / (v * v * v);
@@ -426,23 +429,23 @@ __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot
         }
         NA[i] = ua;
         NB[i] = ub;
-        if (c.itp >= 0 && z == gzr) s.rec[(long long)ix * s.nt + c.itp] = ub;
+        if (c.itp >= 0 && z == gzr) REC[(long long)ix * s.nt + c.itp] = ub;
       } else {  // RTM_BWD: source field ua, receiver field ub
         if (z == gzr) ub = ub + rec_cur * vt2;
         const T lap = (pa + ua - T(2) * qa[4]) * k.idt2;
         const T v3 = T(2) / (v * v * v);
-        c.aux[i] = lap * v3;
+        SEIS_GL(c.aux)[i] = lap * v3;
         NB[i] = ub;
         T out = ua;
         if (c.itp - 1 >= 0) {
           int sl[4];
           if (strip_slots(x, z, s.nx, k.nz, pml, sl) > 0)
-            out = s.strips[(long long)(c.itp - 1) * 8 * (s.nx + k.nz) + sl[0]];
+            out = SEIS_GL(s.strips)[(long long)(c.itp - 1) * 8 * (s.nx + k.nz) + sl[0]];
         }
         NA[i] = out;
         // the band's one-way update reads the stencil value of this layer,
         // not the restored strip
-        if (edge_dist(x, z, s.NX, NZ) == pml) s.raw[i] = ua;
+        if (edge_dist(x, z, s.NX, NZ) == pml) SEIS_GL(s.raw)[i] = ua;
       }
     }
 #pragma unroll
@@ -521,10 +524,12 @@ __global__ __launch_bounds__(128) void image_kernel(Consts<float> k, const DevSh
   const int NZ = k.NZ;
   float* sR = sU + NZ;
   const long long base = fidx(k.ld, x, 0);
-  const float* R = s.fld + slot_off * s.fstride;
+  const auto U = SEIS_GL(s.aux + (step & 1) * s.fstride + base);
+  const auto R = SEIS_GL(s.fld + slot_off * s.fstride + base);
+  const auto IMG = SEIS_GL(s.img + base);
   for (int z = threadIdx.x; z < NZ; z += blockDim.x) {
-    sU[z] = s.aux[(step & 1) * s.fstride + base + z];
-    sR[z] = R[base + z];
+    sU[z] = U[z];
+    sR[z] = R[z];
   }
   __syncthreads();
   const float scal = 2.0f / 4.0f;
@@ -549,10 +554,10 @@ __global__ __launch_bounds__(128) void image_kernel(Consts<float> k, const DevSh
       }
     }
     const float ur = sU[z] * sR[z];
-    float img = s.img[base + z];
+    float img = IMG[z];
     img += ur;
     img += scal * (ur - uh * rh);
-    s.img[base + z] = img;
+    IMG[z] = img;
   }
 }
 

@@ -104,45 +104,45 @@ SEIS_HD inline T adj_raw(const Consts<T>& k, int NX, int nx, const AdjLevel<T>& 
   const int NZ = k.NZ, pml = k.pml;
   const long long ld = k.ld;
   const T w = sponge_weight<T>(x, z, nx, k.nz, pml);
-  T r = w * L.lam[fidx(ld, x, z)];
+  T r = w * SEIS_GL(L.lam)[fidx(ld, x, z)];
   // cells at least two cells inside the interior have no band reader
   if (x > pml && x < NX - 1 - pml && z > pml && z < NZ - 1 - pml) return r;
   if (z - 1 >= 0) {  // readers above: top sides and top corners
     const BandCell b = classify(x, z - 1, NX, NZ, pml);
-    if ((b.kind == 1 && b.s == 1) || (b.kind == 3 && b.sz == 1)) r += L.ain[fidx(ld, x, z - 1)];
+    if ((b.kind == 1 && b.s == 1) || (b.kind == 3 && b.sz == 1)) r += SEIS_GL(L.ain)[fidx(ld, x, z - 1)];
     for (int e = -1; e <= 1; e += 2) {
       if (x + e < 0 || x + e >= NX) continue;
       const BandCell bt = classify(x + e, z - 1, NX, NZ, pml);
-      if (bt.kind == 1 && bt.s == 1) r += L.at[fidx(ld, x + e, z - 1)];
+      if (bt.kind == 1 && bt.s == 1) r += SEIS_GL(L.at)[fidx(ld, x + e, z - 1)];
     }
   }
   if (z + 1 < NZ) {  // readers below: bottom sides and bottom corners
     const BandCell b = classify(x, z + 1, NX, NZ, pml);
-    if ((b.kind == 1 && b.s == -1) || (b.kind == 3 && b.sz == -1)) r += L.ain[fidx(ld, x, z + 1)];
+    if ((b.kind == 1 && b.s == -1) || (b.kind == 3 && b.sz == -1)) r += SEIS_GL(L.ain)[fidx(ld, x, z + 1)];
     for (int e = -1; e <= 1; e += 2) {
       if (x + e < 0 || x + e >= NX) continue;
       const BandCell bt = classify(x + e, z + 1, NX, NZ, pml);
-      if (bt.kind == 1 && bt.s == -1) r += L.at[fidx(ld, x + e, z + 1)];
+      if (bt.kind == 1 && bt.s == -1) r += SEIS_GL(L.at)[fidx(ld, x + e, z + 1)];
     }
   }
   if (x - 1 >= 0) {  // readers to the left: left sides and left corners
     const BandCell b = classify(x - 1, z, NX, NZ, pml);
-    if (b.kind == 2 && b.s == 1) r += L.ain[fidx(ld, x - 1, z)];
-    if (b.kind == 3 && b.sx == 1) r += L.at[fidx(ld, x - 1, z)];
+    if (b.kind == 2 && b.s == 1) r += SEIS_GL(L.ain)[fidx(ld, x - 1, z)];
+    if (b.kind == 3 && b.sx == 1) r += SEIS_GL(L.at)[fidx(ld, x - 1, z)];
     for (int e = -1; e <= 1; e += 2) {
       if (z + e < 0 || z + e >= NZ) continue;
       const BandCell bt = classify(x - 1, z + e, NX, NZ, pml);
-      if (bt.kind == 2 && bt.s == 1) r += L.at[fidx(ld, x - 1, z + e)];
+      if (bt.kind == 2 && bt.s == 1) r += SEIS_GL(L.at)[fidx(ld, x - 1, z + e)];
     }
   }
   if (x + 1 < NX) {  // readers to the right
     const BandCell b = classify(x + 1, z, NX, NZ, pml);
-    if (b.kind == 2 && b.s == -1) r += L.ain[fidx(ld, x + 1, z)];
-    if (b.kind == 3 && b.sx == -1) r += L.at[fidx(ld, x + 1, z)];
+    if (b.kind == 2 && b.s == -1) r += SEIS_GL(L.ain)[fidx(ld, x + 1, z)];
+    if (b.kind == 3 && b.sx == -1) r += SEIS_GL(L.at)[fidx(ld, x + 1, z)];
     for (int e = -1; e <= 1; e += 2) {
       if (z + e < 0 || z + e >= NZ) continue;
       const BandCell bt = classify(x + 1, z + e, NX, NZ, pml);
-      if (bt.kind == 2 && bt.s == -1) r += L.at[fidx(ld, x + 1, z + e)];
+      if (bt.kind == 2 && bt.s == -1) r += SEIS_GL(L.at)[fidx(ld, x + 1, z + e)];
     }
   }
   return r;
@@ -164,6 +164,52 @@ SEIS_HD inline T extrap_coef(int j, T c1, T c2) {
 SEIS_HD inline bool is_ring(int x, int z, int NX, int NZ) { return x == 0 || z == 0 || x == NX - 1 || z == NZ - 1; }
 
 SEIS_HD inline bool high_order(int x, int z, int NX, int NZ) { return x >= 4 && x <= NX - 5 && z >= 4 && z <= NZ - 5; }
+
+// Adjoint of the one-way band update's reads of the current level at cell
+// (x, z): cur(b) by band cell b itself and cur(b_in) by the side cell b whose
+// inner neighbour is (x, z).  Zero unless edge_dist(x, z) <= pml.
+template <typename T>
+SEIS_HD inline T adj_oneway_cur(const Consts<T>& k, int NX, const AdjLevel<T>& L1, int x, int z) {
+  const int NZ = k.NZ, pml = k.pml;
+  const long long ld = k.ld;
+  T acc = T(0);
+  const BandCell self = classify(x, z, NX, NZ, pml);
+  if (self.kind != 0) acc += SEIS_GL(L1.ac)[fidx(ld, x, z)];
+  BandCell b;
+  if (z - 1 >= 0 && (b = classify(x, z - 1, NX, NZ, pml), b.kind == 1 && b.s == 1)) acc += SEIS_GL(L1.ac)[fidx(ld, x, z - 1)];
+  if (z + 1 < NZ && (b = classify(x, z + 1, NX, NZ, pml), b.kind == 1 && b.s == -1)) acc += SEIS_GL(L1.ac)[fidx(ld, x, z + 1)];
+  if (x - 1 >= 0 && (b = classify(x - 1, z, NX, NZ, pml), b.kind == 2 && b.s == 1)) acc += SEIS_GL(L1.ac)[fidx(ld, x - 1, z)];
+  if (x + 1 < NX && (b = classify(x + 1, z, NX, NZ, pml), b.kind == 2 && b.s == -1)) acc += SEIS_GL(L1.ac)[fidx(ld, x + 1, z)];
+  return acc;
+}
+
+// Adjoint of the one-way band update's reads of the previous level (level
+// m+1) at cell (x, z).  Zero unless edge_dist(x, z) <= pml.
+template <typename T>
+SEIS_HD inline T adj_oneway_prev(const Consts<T>& k, const Model<T>& m, int NX, const AdjLevel<T>& L2, int x, int z) {
+  const int NZ = k.NZ, pml = k.pml, nx = m.nx;
+  const long long ld = k.ld;
+  T acc = T(0);
+  const BandCell self = classify(x, z, NX, NZ, pml);
+  if (self.kind == 1 || self.kind == 2) acc += SEIS_GL(L2.ain)[fidx(ld, x, z)];
+  auto pin = [&](int bx, int bz) {  // (a1+a2) at = -(1-w) lam for the inner neighbour
+    const T w = sponge_weight<T>(bx, bz, nx, k.nz, pml);
+    return -(T(1) - w) * SEIS_GL(L2.lam)[fidx(ld, bx, bz)];
+  };
+  BandCell b;
+  if (z - 1 >= 0 && (b = classify(x, z - 1, NX, NZ, pml), b.kind == 1 && b.s == 1)) acc += pin(x, z - 1);
+  if (z + 1 < NZ && (b = classify(x, z + 1, NX, NZ, pml), b.kind == 1 && b.s == -1)) acc += pin(x, z + 1);
+  if (x - 1 >= 0 && (b = classify(x - 1, z, NX, NZ, pml), b.kind == 2 && b.s == 1)) acc += pin(x - 1, z);
+  if (x + 1 < NX && (b = classify(x + 1, z, NX, NZ, pml), b.kind == 2 && b.s == -1)) acc += pin(x + 1, z);
+  // tangent neighbours: z sides on the same row, x sides on the same column
+  for (int e = -1; e <= 1; e += 2) {
+    if (x + e >= 0 && x + e < NX && (b = classify(x + e, z, NX, NZ, pml), b.kind == 1))
+      acc += SEIS_GL(L2.at)[fidx(ld, x + e, z)];
+    if (z + e >= 0 && z + e < NZ && (b = classify(x, z + e, NX, NZ, pml), b.kind == 2))
+      acc += SEIS_GL(L2.at)[fidx(ld, x, z + e)];
+  }
+  return acc;
+}
 
 // lam^{m-1}(x, z) from level m (cur terms, L1) and level m+1 (prev terms, L2),
 // without the record injection.  ar1 / ar2 give adj_raw of a cell at level m /
@@ -248,38 +294,11 @@ SEIS_HD inline T adj_value(const Consts<T>& k, const Model<T>& m, int NX, const 
   if (NZ - 1 - z <= 3) acc += ar1(x, NZ - 1) * extrap_coef<T>(NZ - 1 - z, vv(x, NZ - 2) * k.cz, vv(x, NZ - 3) * k.cz);
   if (x <= 3) acc += ar1(0, z) * extrap_coef<T>(x, vv(1, z) * k.cx, vv(2, z) * k.cx);
   if (NX - 1 - x <= 3) acc += ar1(NX - 1, z) * extrap_coef<T>(NX - 1 - x, vv(NX - 2, z) * k.cx, vv(NX - 3, z) * k.cx);
-  // (iii) one-way reads of cur: cur(b) by band cell b, cur(b_in) by side cell b
-  const BandCell self = classify(x, z, NX, NZ, pml);
-  if (self.kind != 0) acc += L1.ac[fidx(ld, x, z)];
-  {
-    BandCell b;
-    if (z - 1 >= 0 && (b = classify(x, z - 1, NX, NZ, pml), b.kind == 1 && b.s == 1)) acc += L1.ac[fidx(ld, x, z - 1)];
-    if (z + 1 < NZ && (b = classify(x, z + 1, NX, NZ, pml), b.kind == 1 && b.s == -1)) acc += L1.ac[fidx(ld, x, z + 1)];
-    if (x - 1 >= 0 && (b = classify(x - 1, z, NX, NZ, pml), b.kind == 2 && b.s == 1)) acc += L1.ac[fidx(ld, x - 1, z)];
-    if (x + 1 < NX && (b = classify(x + 1, z, NX, NZ, pml), b.kind == 2 && b.s == -1)) acc += L1.ac[fidx(ld, x + 1, z)];
-  }
-  // (iv) -prev in the stencil and in the ring base formula, level m+1
+  // (iii) one-way reads of cur, (iv) -prev of the stencil and ring base
+  // formula at level m+1, (v) one-way reads of prev at level m+1
+  acc += adj_oneway_cur(k, NX, L1, x, z);
   acc -= ar2(x, z);
-  // (v) one-way reads of prev at level m+1
-  if (self.kind == 1 || self.kind == 2) acc += L2.ain[fidx(ld, x, z)];
-  {
-    auto pin = [&](int bx, int bz) {  // (a1+a2) at = -(1-w) lam for the inner neighbour
-      const T w = sponge_weight<T>(bx, bz, nx, k.nz, pml);
-      return -(T(1) - w) * L2.lam[fidx(ld, bx, bz)];
-    };
-    BandCell b;
-    if (z - 1 >= 0 && (b = classify(x, z - 1, NX, NZ, pml), b.kind == 1 && b.s == 1)) acc += pin(x, z - 1);
-    if (z + 1 < NZ && (b = classify(x, z + 1, NX, NZ, pml), b.kind == 1 && b.s == -1)) acc += pin(x, z + 1);
-    if (x - 1 >= 0 && (b = classify(x - 1, z, NX, NZ, pml), b.kind == 2 && b.s == 1)) acc += pin(x - 1, z);
-    if (x + 1 < NX && (b = classify(x + 1, z, NX, NZ, pml), b.kind == 2 && b.s == -1)) acc += pin(x + 1, z);
-    // tangent neighbours: z sides on the same row, x sides on the same column
-    for (int e = -1; e <= 1; e += 2) {
-      if (x + e >= 0 && x + e < NX && (b = classify(x + e, z, NX, NZ, pml), b.kind == 1))
-        acc += L2.at[fidx(ld, x + e, z)];
-      if (z + e >= 0 && z + e < NZ && (b = classify(x, z + e, NX, NZ, pml), b.kind == 2))
-        acc += L2.at[fidx(ld, x, z + e)];
-    }
-  }
+  acc += adj_oneway_prev(k, m, NX, L2, x, z);
   return acc;
 }
 

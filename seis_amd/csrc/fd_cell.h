@@ -23,6 +23,25 @@
 
 namespace seis {
 
+// Global-address-space view of a device pointer.  Every array these kernels
+// touch is device (global) memory; reaching it through a generic pointer
+// compiles to FLAT instructions, whose completion the compiler can only
+// await with a full wait, which serialises each load with everything issued
+// before it.  Through the global view loads become global_load and waits
+// count only the loads they depend on.  On the host (CPU builds of these
+// headers) the view is the pointer itself.
+#if defined(__HIP_DEVICE_COMPILE__)
+template <typename T>
+using gptr = __attribute__((address_space(1))) T*;
+template <typename T>
+__device__ __forceinline__ gptr<T> gl(T* p) {
+  return (gptr<T>)p;
+}
+#define SEIS_GL(p) ::seis::gl(p)
+#else
+#define SEIS_GL(p) (p)
+#endif
+
 SEIS_HD inline int imin(int a, int b) { return a < b ? a : b; }
 SEIS_HD inline int imax(int a, int b) { return a > b ? a : b; }
 
@@ -53,7 +72,7 @@ struct Model {
   SEIS_HD inline T vel(int x, int z) const {
     int xi = imin(imax(x - pml, 0), nx - 1);
     int zi = imin(imax(z - pml, 0), nz - 1);
-    return v[(long long)xi * nz + zi];
+    return SEIS_GL(v)[(long long)xi * nz + zi];
   }
 };
 
@@ -79,7 +98,7 @@ struct NoInject {
   template <typename T>
   SEIS_HD inline T add(T val, int, int, T, long long) const { return val; }
   template <typename T>
-  SEIS_HD inline T prev(const T* P, long long i) const { return P[i]; }
+  SEIS_HD inline T prev(const T* P, long long i) const { return SEIS_GL(P)[i]; }
 };
 
 // Point source u += vt2 * wav[it-1] at one cell (fd.cpp:49-51).
@@ -91,7 +110,7 @@ struct PointSource {
     return i == src ? val + vt2 * amp : val;
   }
   template <typename U>
-  SEIS_HD inline U prev(const U* P, long long i) const { return P[i]; }
+  SEIS_HD inline U prev(const U* P, long long i) const { return SEIS_GL(P)[i]; }
 };
 
 // Reverse-time source removal from the previous level (fd.cpp:659-661):
@@ -103,7 +122,7 @@ struct SourceRemoval {
   bool on;
   SEIS_HD inline T add(T val, int, int, T, long long) const { return val; }
   SEIS_HD inline T prev(const T* P, long long i) const {
-    return (on && i == src) ? P[i] - sub : P[i];
+    return (on && i == src) ? SEIS_GL(P)[i] - sub : SEIS_GL(P)[i];
   }
 };
 
@@ -116,11 +135,11 @@ struct ReceiverInject {
   int nx, nt, itp, pml;
   SEIS_HD inline T add(T val, int x, int z, T vt2, long long) const {
     int ix = x - pml;
-    if (ix >= 0 && ix < nx && z == gz[ix] + pml) return val + rec[(long long)ix * nt + itp] * vt2;
+    if (ix >= 0 && ix < nx && z == SEIS_GL(gz)[ix] + pml) return val + SEIS_GL(rec)[(long long)ix * nt + itp] * vt2;
     return val;
   }
   template <typename U>
-  SEIS_HD inline U prev(const U* P, long long i) const { return P[i]; }
+  SEIS_HD inline U prev(const U* P, long long i) const { return SEIS_GL(P)[i]; }
 };
 
 // ---------------------------------------------------------------------------
@@ -137,7 +156,7 @@ SEIS_HD inline T raw_value(const Consts<T>& k, const Model<T>& m, int NX, const 
   if (xe || ze) {
     // ring, fd.cpp:1060-1091
     auto vv = [&](int xx, int zz) { T v = m.vel(xx, zz); return v * v; };
-    auto cu = [&](int xx, int zz) { return C[fidx(ld, xx, zz)]; };
+    auto cu = [&](int xx, int zz) { return SEIS_GL(C)[fidx(ld, xx, zz)]; };
     T r;
     const T c0 = cu(x, z);
     const T p0 = inj.prev(P, i);
@@ -174,10 +193,10 @@ SEIS_HD inline T raw_value(const Consts<T>& k, const Model<T>& m, int NX, const 
   const T vt2 = a * a;
   const T vvzz = vt2 * k.idz2;
   const T vvxx = vt2 * k.idx2;
-  const T c0 = C[i];
+  const T c0 = SEIS_GL(C)[i];
   T uzz, uxx;
   if (x >= 4 && x <= NX - 5 && z >= 4 && z <= NZ - 5) {
-    const T* cz = C + i;
+    const auto cz = SEIS_GL(C) + i;
     uzz = cz[-4] * k.c[4];
     uzz = uzz + cz[-3] * k.c[3];
     uzz = uzz + cz[-2] * k.c[2];
@@ -197,8 +216,8 @@ SEIS_HD inline T raw_value(const Consts<T>& k, const Model<T>& m, int NX, const 
     uxx = uxx + cz[3 * ld] * k.c[3];
     uxx = uxx + cz[4 * ld] * k.c[4];
   } else {
-    uzz = C[i - 1] + C[i + 1] - T(2) * c0;
-    uxx = C[i + ld] + C[i - ld] - T(2) * c0;
+    uzz = SEIS_GL(C)[i - 1] + SEIS_GL(C)[i + 1] - T(2) * c0;
+    uxx = SEIS_GL(C)[i + ld] + SEIS_GL(C)[i - ld] - T(2) * c0;
   }
   T val = vvzz * uzz + vvxx * uxx + T(2) * c0 - inj.prev(P, i);
   return inj.add(val, x, z, vt2, i);
@@ -222,7 +241,7 @@ SEIS_HD inline T band_blend(const Consts<T>& k, const Model<T>& m, int NX, const
   const int mn = imin(imin(dl, dr), imin(dtp, db));
   const long long ld = k.ld;
   const long long i = fidx(ld, x, z);
-  auto Cu = [&](int xx, int zz) { return C[fidx(ld, xx, zz)]; };
+  auto Cu = [&](int xx, int zz) { return SEIS_GL(C)[fidx(ld, xx, zz)]; };
   auto Pv = [&](int xx, int zz) { return inj.prev(P, fidx(ld, xx, zz)); };
   const T v = m.vel(x, z);
   T ow;
@@ -235,7 +254,7 @@ SEIS_HD inline T band_blend(const Consts<T>& k, const Model<T>& m, int NX, const
       b3 = (T)(std::sqrt(2.0) / (double)v / (double)k.dt);  // fd.cpp:1176 forms sqrt(2) in double
     else
       b3 = k.sq2 / v / k.dt;
-    T t = -k.b1 * rawn(x, z + sz) - k.b2 * rawn(x + sx, z) + b3 * (-C[i]);
+    T t = -k.b1 * rawn(x, z + sz) - k.b2 * rawn(x + sx, z) + b3 * (-SEIS_GL(C)[i]);
     ow = -t / (k.b1 + k.b2 + b3);
   } else if (dtp == mn || db == mn) {
     // top / bottom band: one-way update along z, fd.cpp:1092-1118
@@ -245,7 +264,7 @@ SEIS_HD inline T band_blend(const Consts<T>& k, const Model<T>& m, int NX, const
     const T a2 = k.a2b / v;
     const T a3 = v / T(4) / k.dx / k.dx;
     const T p0 = Pv(x, z), pin = Pv(x, z + s);
-    T t = -a1 * nin - a1 * (p0 - pin) + a2 * (p0 - T(2) * C[i] + nin + pin - T(2) * Cu(x, z + s)) -
+    T t = -a1 * nin - a1 * (p0 - pin) + a2 * (p0 - T(2) * SEIS_GL(C)[i] + nin + pin - T(2) * Cu(x, z + s)) -
           a3 * (nip + nim - T(2) * nin + Pv(x + 1, z) + Pv(x - 1, z) - T(2) * p0);
     ow = -t / (a1 + a2);
   } else {
@@ -256,7 +275,7 @@ SEIS_HD inline T band_blend(const Consts<T>& k, const Model<T>& m, int NX, const
     const T a2 = k.a2b / v;
     const T a3 = v / T(4) / k.dz / k.dz;
     const T p0 = Pv(x, z), pin = Pv(x + s, z);
-    T t = -a1 * nin - a1 * (p0 - pin) + a2 * (p0 - T(2) * C[i] + nin + pin - T(2) * Cu(x + s, z)) -
+    T t = -a1 * nin - a1 * (p0 - pin) + a2 * (p0 - T(2) * SEIS_GL(C)[i] + nin + pin - T(2) * Cu(x + s, z)) -
           a3 * (nip + nim - T(2) * nin + Pv(x, z + 1) + Pv(x, z - 1) - T(2) * p0);
     ow = -t / (a1 + a2);
   }

@@ -28,7 +28,7 @@ namespace seis {
 
 // ---------------------------------------------------------------------------
 // Field slots of one shot in the LSRTM pool.
-enum LsSlot { LS_FIELD = 0, LS_AIN = 3, LS_AT = 6, LS_AC = 9, LS_AR = 12, LS_NSLOT = 14 };
+enum LsSlot { LS_FIELD = 0, LS_AIN = 3, LS_AT = 6, LS_AC = 9, LS_AR = 12, LS_OW = 14, LS_NSLOT = 15 };
 
 // Adjoint level m of a shot: slot m % 3 of the level fields and band products.
 template <typename T>
@@ -52,8 +52,13 @@ __global__ __launch_bounds__(256) void adj_prep_kernel(Consts<T> k, const DevSho
   int x, z;
   ring_cell(R, s.NX, k.NZ, j, &x, &z);
   const AdjLevel<T> L = level(s, lev);
-  T* ar = s.fld + (LS_AR + lev % 2) * s.fstride;
-  ar[fidx(k.ld, x, z)] = adj_raw(k, s.NX, s.nx, L, x, z);
+  const auto ar = SEIS_GL(s.fld + (LS_AR + lev % 2) * s.fstride);
+  const long long i = fidx(k.ld, x, z);
+  ar[i] = adj_raw(k, s.NX, s.nx, L, x, z);
+  // the one-way terms of lam^{lev-1} at this cell (levels lev and lev+1 are
+  // complete now), added by the step kernel's sweep
+  const Model<T> m{s.vel, s.nx, k.nz, k.pml};
+  SEIS_GL(s.fld)[LS_OW * s.fstride + i] = adj_oneway_cur(k, s.NX, L, x, z) + adj_oneway_prev(k, m, s.NX, level(s, lev + 1), x, z);
 }
 
 // Reads a raw adjoint: precomputed near the band, the level itself elsewhere.
@@ -65,7 +70,7 @@ struct RawAdj {
   int NX, NZ, pml;
   __device__ __forceinline__ T operator()(int x, int z) const {
     const long long i = fidx(ld, x, z);
-    return (x > pml && x < NX - 1 - pml && z > pml && z < NZ - 1 - pml) ? lam[i] : ar[i];
+    return (x > pml && x < NX - 1 - pml && z > pml && z < NZ - 1 - pml) ? SEIS_GL(lam)[i] : SEIS_GL(ar)[i];
   }
 };
 
@@ -104,7 +109,13 @@ __global__ __launch_bounds__(AWG) void adj_step_kernel(Consts<T> k, const DevSho
   const int q0 = (lev - 1) % 3;
   T* out = s.fld + (LS_FIELD + q0) * s.fstride;
   const Model<T> m{s.vel, s.nx, k.nz, pml};
+  const RawAdj<T> A1{s.fld + (LS_AR + lev % 2) * s.fstride, L1.lam, ld, NX, NZ, pml};
+  const RawAdj<T> A2{s.fld + (LS_AR + (lev + 1) % 2) * s.fstride, L2.lam, ld, NX, NZ, pml};
   if ((int)blockIdx.x < F.nfwg) {
+    // Cells at least 8 from the outer edge: every stencil reader within 4
+    // cells uses the 9-point operator, so the stencil part is the 9-point
+    // gather of Q = (raw adjoint) vt2 / h^2 of the readers; cells at most
+    // pml from the edge add the one-way terms and store their band products.
     const int wave = threadIdx.x >> 6;
     const int item = blockIdx.x * WAVES + wave;
     if (item >= F.nzc * F.nxs) return;
@@ -119,18 +130,16 @@ __global__ __launch_bounds__(AWG) void adj_step_kernel(Consts<T> k, const DevSho
     const bool halo = lane < 4 || lane >= 60;
     const int hz = lane < 4 ? z0 - 4 + lane : min(z0 + lane + 4, NZ - 1);
     const int hslot = lane < 4 ? lane : lane + 8;
-    const T* lam1 = L1.lam;
-    const T* lam2 = L2.lam;
     auto vt2at = [&](int x, int zz) {
       const T a = k.dt * m.vel(x, zz);  // clamped: idle lanes may sit past the model
       return a * a;
     };
-    // queues along x: qx = lam vt2 / dx^2 (rows x-4..x+4), qz and lam (rows x..x+4)
+    // queues along x: qx = a vt2 / dx^2 (rows x-4..x+4), qz and a (rows x..x+4)
     T qx[9], qz[5], lq[5];
 #pragma unroll
     for (int j = 0; j < 8; j++) {
       const int r = x0 - 4 + j;
-      const T l = lam1[fidx(ld, r, zl)];
+      const T l = A1(r, zl);
       const T v2 = vt2at(r, zl);
       qx[j] = l * (v2 * k.idx2);
       if (j >= 4) {
@@ -138,15 +147,42 @@ __global__ __launch_bounds__(AWG) void adj_step_kernel(Consts<T> k, const DevSho
         lq[j - 4] = l;
       }
     }
-    for (int x = x0; x < x1; x++) {
-      {
-        const T l = lam1[fidx(ld, x + 4, zl)];
-        const T v2 = vt2at(x + 4, zl);
-        qx[8] = l * (v2 * k.idx2);
-        qz[4] = l * (v2 * k.idz2);
-        lq[4] = l;
+    // loads of column x+1 are issued before column x is computed, including
+    // the record sample, snapshot and gradient of its model cells
+    const int iz = z - pml;
+    const bool zin = iz >= 0 && iz < k.nz && act;
+    const int itp = lev - 2 - s.ndel;                 // record sample injected now
+    const bool grad_on = lev - 1 <= s.nt + s.ndel - 1;  // imaging with S^{lev-1}
+    T na, nv2, nh = T(0), nprev, nrec = T(0), nsnap = T(0), ngrad = T(0);
+    int ngz = -1;
+    auto fetch = [&](int xx) {
+      xx = min(xx, x1 - 1);
+      na = A1(xx + 4, zl);
+      nv2 = vt2at(xx + 4, zl);
+      if (halo) nh = A1(xx, hz) * (vt2at(xx, hz) * k.idz2);
+      nprev = A2(xx, zl);
+      const int ixx = xx - pml;
+      if (ixx >= 0 && ixx < s.nx) {
+        ngz = SEIS_GL(s.gz)[ixx];
+        if (itp >= 0) nrec = SEIS_GL(s.rec)[(long long)ixx * s.nt + itp];
+        if (grad_on && zin) {
+          const long long j = (long long)ixx * k.nz + iz;
+          nsnap = snap_load(s, k.nz, lev - 1, j);
+          ngrad = SEIS_GL(s.grad)[j];
+        }
+      } else {
+        ngz = -1;
       }
-      if (halo) la[hslot] = lam1[fidx(ld, x, hz)] * (vt2at(x, hz) * k.idz2);
+    };
+    fetch(x0);
+    for (int x = x0; x < x1; x++) {
+      qx[8] = na * (nv2 * k.idx2);
+      qz[4] = na * (nv2 * k.idz2);
+      lq[4] = na;
+      if (halo) la[hslot] = nh;
+      const T prev = nprev, rec_cur = nrec, snap_cur = nsnap, grad_cur = ngrad;
+      const int gz_cur = ngz;
+      fetch(x + 1);
       la[lane + 4] = qz[0];
       wave_sync();
       T sz = la[lane] * k.c[4];
@@ -170,9 +206,22 @@ __global__ __launch_bounds__(AWG) void adj_step_kernel(Consts<T> k, const DevSho
       sx += qx[8] * k.c[4];
       if (act) {
         const long long i = fidx(ld, x, z);
-        T val = T(2) * lq[0] + sz + sx - lam2[i];
-        val = adj_finish(k, s, lev, x, z, val, lq[0]);
-        out[i] = val;
+        T val = T(2) * lq[0] + sz + sx - prev;
+        const int ed = edge_dist(x, z, NX, NZ);
+        if (ed <= pml) val += SEIS_GL(s.fld)[LS_OW * s.fstride + i];  // one-way terms (prep kernel)
+        // adjoint of the record extraction and the imaging condition (as adj_finish)
+        const int ix = x - pml;
+        if (ix >= 0 && ix < s.nx && zin) {
+          if (itp >= 0 && z == gz_cur + pml) val += rec_cur;
+          if (grad_on) SEIS_GL(s.grad)[(long long)ix * k.nz + iz] = grad_cur + lq[0] * snap_cur;
+        }
+        SEIS_GL(out)[i] = val;
+        if (ed < pml) {
+          const BandAdj<T> ba = band_adj(k, m, x, z, classify(x, z, NX, NZ, pml), val);
+          SEIS_GL(s.fld)[(LS_AIN + q0) * s.fstride + i] = ba.in;
+          SEIS_GL(s.fld)[(LS_AT + q0) * s.fstride + i] = ba.t;
+          SEIS_GL(s.fld)[(LS_AC + q0) * s.fstride + i] = ba.c;
+        }
       }
 #pragma unroll
       for (int j = 0; j < 8; j++) qx[j] = qx[j + 1];
@@ -189,8 +238,6 @@ __global__ __launch_bounds__(AWG) void adj_step_kernel(Consts<T> k, const DevSho
   Ring R{F.xa, F.xb, F.za, F.zb, F.nband};
   int x, z;
   ring_cell(R, NX, NZ, j, &x, &z);
-  const RawAdj<T> A1{s.fld + (LS_AR + lev % 2) * s.fstride, L1.lam, ld, NX, NZ, pml};
-  const RawAdj<T> A2{s.fld + (LS_AR + (lev + 1) % 2) * s.fstride, L2.lam, ld, NX, NZ, pml};
   T ar;
   T val = adj_value(k, m, NX, L1, L2, A1, A2, x, z, &ar);
   val = adj_finish(k, s, lev, x, z, val, ar);
@@ -353,12 +400,12 @@ struct LsrtmPlan {
     k = make_consts<T>(g, ld);
     F = make_layout(NX, NZ, g.pml);
     {
-      // adjoint interior: every cell of the stencil reach has raw adjoint = lam
+      // adjoint sweep: every stencil reader within 4 cells is a 9-point cell
       A = F;
-      A.xa = std::max(g.pml + 5, 8);
-      A.xb = std::min(NX - g.pml - 5, NX - 8);
-      A.za = std::max(g.pml + 5, 8);
-      A.zb = std::min(NZ - g.pml - 5, NZ - 8);
+      A.xa = 8;
+      A.xb = NX - 8;
+      A.za = 8;
+      A.zb = NZ - 8;
       if (A.xb <= A.xa || A.zb <= A.za) {
         A.xa = A.xb = 0;
         A.za = 0;
