@@ -206,7 +206,7 @@ inline Layout make_layout(int NX, int NZ, int pml) {
     L.za = 0;
     L.zb = NZ;
   }
-  L.bx = 64;
+  L.bx = 16;  // short strips: more wavefronts in flight (measured best of 16, 32, 64)
   L.nzc = (L.zb - L.za + 63) / 64;
   L.nxs = (L.xb - L.xa + L.bx - 1) / L.bx;
   if (L.xb == L.xa) L.nzc = L.nxs = 0;
@@ -225,24 +225,28 @@ struct Ring {
   long long n;
 };
 
-__device__ __forceinline__ void ring_cell(const Ring& R, int NX, int NZ, long long j, int* x, int* z) {
-  const long long top = (long long)NX * R.za, bot = (long long)NX * (NZ - R.zb);
-  const long long lft = (long long)R.xa * (R.zb - R.za);
+// j < R.n < 2^31 (checked when the layout is made): 32-bit unsigned
+// divisions, far cheaper on the GPU than 64-bit ones.
+__device__ __forceinline__ void ring_cell(const Ring& R, int NX, int NZ, long long jl, int* x, int* z) {
+  const unsigned j = (unsigned)jl;
+  const unsigned top = (unsigned)NX * (unsigned)R.za, bot = (unsigned)NX * (unsigned)(NZ - R.zb);
+  const unsigned hz = (unsigned)(R.zb - R.za);
+  const unsigned lft = (unsigned)R.xa * hz;
   if (j < top) {
-    *x = (int)(j / R.za);
-    *z = (int)(j % R.za);
+    *x = (int)(j / (unsigned)R.za);
+    *z = (int)(j % (unsigned)R.za);
   } else if (j < top + bot) {
-    const long long q = j - top;
-    *x = (int)(q / (NZ - R.zb));
-    *z = R.zb + (int)(q % (NZ - R.zb));
+    const unsigned q = j - top, h = (unsigned)(NZ - R.zb);
+    *x = (int)(q / h);
+    *z = R.zb + (int)(q % h);
   } else if (j < top + bot + lft) {
-    const long long q = j - top - bot;
-    *x = (int)(q / (R.zb - R.za));
-    *z = R.za + (int)(q % (R.zb - R.za));
+    const unsigned q = j - top - bot;
+    *x = (int)(q / hz);
+    *z = R.za + (int)(q % hz);
   } else {
-    const long long q = j - top - bot - lft;
-    *x = R.xb + (int)(q / (R.zb - R.za));
-    *z = R.za + (int)(q % (R.zb - R.za));
+    const unsigned q = j - top - bot - lft;
+    *x = R.xb + (int)(q / hz);
+    *z = R.za + (int)(q % hz);
   }
 }
 

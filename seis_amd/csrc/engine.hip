@@ -673,6 +673,8 @@ int check_grid(const GridSpec& g) {
   if (g.nz + 2 * g.pml < 8 || g.nx + 2 * g.pml < 8)
     return set_error("grid: padded grid must be at least 8 x 8 cells"), SEIS_ERR_ARG;
   if (!(g.dt > 0) || !(g.dx > 0) || !(g.dz > 0)) return set_error("grid: dt, dx, dz must be > 0"), SEIS_ERR_ARG;
+  if ((long long)(g.nx + 2 * g.pml) * (g.nz + 2 * g.pml) >= (1LL << 31))
+    return set_error("grid: the padded grid must have fewer than 2^31 cells"), SEIS_ERR_ARG;
   return SEIS_OK;
 }
 

@@ -97,55 +97,114 @@ struct AdjLevel {
   const T* ac;
 };
 
-// Adjoint of the raw (pre-blend) value of cell (x, z) at a level:
-// w lam(c) + sum over the band cells that read raw(c) of their stored product.
-template <typename T>
-SEIS_HD inline T adj_raw(const Consts<T>& k, int NX, int nx, const AdjLevel<T>& L, int x, int z) {
-  const int NZ = k.NZ, pml = k.pml;
-  const long long ld = k.ld;
-  const T w = sponge_weight<T>(x, z, nx, k.nz, pml);
-  T r = w * SEIS_GL(L.lam)[fidx(ld, x, z)];
-  // cells at least two cells inside the interior have no band reader
-  if (x > pml && x < NX - 1 - pml && z > pml && z < NZ - 1 - pml) return r;
+// Which band-adjoint terms reach cell (x, z): one bit per (product, neighbour)
+// that adj_raw, adj_oneway_cur and adj_oneway_prev add.  The set depends only
+// on the cell's position, so a plan can compute it once per cell.
+enum BandTerm : unsigned {
+  // adj_raw: band cells that read raw(x, z)
+  BT_AIN_ZM = 1u << 0,     // ain(x, z-1): top side or top corner above
+  BT_AT_ZM_XM = 1u << 1,   // at(x-1, z-1): top side (tangent reader)
+  BT_AT_ZM_XP = 1u << 2,   // at(x+1, z-1)
+  BT_AIN_ZP = 1u << 3,     // ain(x, z+1): bottom side or bottom corner below
+  BT_AT_ZP_XM = 1u << 4,   // at(x-1, z+1)
+  BT_AT_ZP_XP = 1u << 5,   // at(x+1, z+1)
+  BT_AIN_XM = 1u << 6,     // ain(x-1, z): left side
+  BT_ATC_XM = 1u << 7,     // at(x-1, z): left corner (x neighbour read)
+  BT_AT_XM_ZM = 1u << 8,   // at(x-1, z-1): left side (tangent reader)
+  BT_AT_XM_ZP = 1u << 9,   // at(x-1, z+1)
+  BT_AIN_XP = 1u << 10,    // ain(x+1, z): right side
+  BT_ATC_XP = 1u << 11,    // at(x+1, z): right corner
+  BT_AT_XP_ZM = 1u << 12,  // at(x+1, z-1)
+  BT_AT_XP_ZP = 1u << 13,  // at(x+1, z+1)
+  // adj_oneway_cur: cur(b) by band cell b, cur(b_in) by the side cell b
+  BT_AC_SELF = 1u << 14,
+  BT_AC_ZM = 1u << 15,
+  BT_AC_ZP = 1u << 16,
+  BT_AC_XM = 1u << 17,
+  BT_AC_XP = 1u << 18,
+  // adj_oneway_prev: prev(b) and prev(b_in) of sides, tangent prev reads
+  BT_AIN_SELF = 1u << 19,
+  BT_PIN_ZM = 1u << 20,
+  BT_PIN_ZP = 1u << 21,
+  BT_PIN_XM = 1u << 22,
+  BT_PIN_XP = 1u << 23,
+  BT_TAN_XM = 1u << 24,  // at(x-1, z): z side on the same row
+  BT_TAN_ZM = 1u << 25,  // at(x, z-1): x side in the same column
+  BT_TAN_XP = 1u << 26,
+  BT_TAN_ZP = 1u << 27,
+};
+
+SEIS_HD inline unsigned band_terms(int x, int z, int NX, int NZ, int pml) {
+  unsigned mk = 0;
+  BandCell b;
   if (z - 1 >= 0) {  // readers above: top sides and top corners
-    const BandCell b = classify(x, z - 1, NX, NZ, pml);
-    if ((b.kind == 1 && b.s == 1) || (b.kind == 3 && b.sz == 1)) r += SEIS_GL(L.ain)[fidx(ld, x, z - 1)];
-    for (int e = -1; e <= 1; e += 2) {
-      if (x + e < 0 || x + e >= NX) continue;
-      const BandCell bt = classify(x + e, z - 1, NX, NZ, pml);
-      if (bt.kind == 1 && bt.s == 1) r += SEIS_GL(L.at)[fidx(ld, x + e, z - 1)];
-    }
+    b = classify(x, z - 1, NX, NZ, pml);
+    if ((b.kind == 1 && b.s == 1) || (b.kind == 3 && b.sz == 1)) mk |= BT_AIN_ZM;
+    if (b.kind == 1 && b.s == 1) mk |= BT_AC_ZM | BT_PIN_ZM;
+    if (x - 1 >= 0 && (b = classify(x - 1, z - 1, NX, NZ, pml), b.kind == 1 && b.s == 1)) mk |= BT_AT_ZM_XM;
+    if (x + 1 < NX && (b = classify(x + 1, z - 1, NX, NZ, pml), b.kind == 1 && b.s == 1)) mk |= BT_AT_ZM_XP;
+    if (classify(x, z - 1, NX, NZ, pml).kind == 2) mk |= BT_TAN_ZM;
   }
   if (z + 1 < NZ) {  // readers below: bottom sides and bottom corners
-    const BandCell b = classify(x, z + 1, NX, NZ, pml);
-    if ((b.kind == 1 && b.s == -1) || (b.kind == 3 && b.sz == -1)) r += SEIS_GL(L.ain)[fidx(ld, x, z + 1)];
-    for (int e = -1; e <= 1; e += 2) {
-      if (x + e < 0 || x + e >= NX) continue;
-      const BandCell bt = classify(x + e, z + 1, NX, NZ, pml);
-      if (bt.kind == 1 && bt.s == -1) r += SEIS_GL(L.at)[fidx(ld, x + e, z + 1)];
-    }
+    b = classify(x, z + 1, NX, NZ, pml);
+    if ((b.kind == 1 && b.s == -1) || (b.kind == 3 && b.sz == -1)) mk |= BT_AIN_ZP;
+    if (b.kind == 1 && b.s == -1) mk |= BT_AC_ZP | BT_PIN_ZP;
+    if (x - 1 >= 0 && (b = classify(x - 1, z + 1, NX, NZ, pml), b.kind == 1 && b.s == -1)) mk |= BT_AT_ZP_XM;
+    if (x + 1 < NX && (b = classify(x + 1, z + 1, NX, NZ, pml), b.kind == 1 && b.s == -1)) mk |= BT_AT_ZP_XP;
+    if (classify(x, z + 1, NX, NZ, pml).kind == 2) mk |= BT_TAN_ZP;
   }
   if (x - 1 >= 0) {  // readers to the left: left sides and left corners
-    const BandCell b = classify(x - 1, z, NX, NZ, pml);
-    if (b.kind == 2 && b.s == 1) r += SEIS_GL(L.ain)[fidx(ld, x - 1, z)];
-    if (b.kind == 3 && b.sx == 1) r += SEIS_GL(L.at)[fidx(ld, x - 1, z)];
-    for (int e = -1; e <= 1; e += 2) {
-      if (z + e < 0 || z + e >= NZ) continue;
-      const BandCell bt = classify(x - 1, z + e, NX, NZ, pml);
-      if (bt.kind == 2 && bt.s == 1) r += SEIS_GL(L.at)[fidx(ld, x - 1, z + e)];
-    }
+    b = classify(x - 1, z, NX, NZ, pml);
+    if (b.kind == 2 && b.s == 1) mk |= BT_AIN_XM | BT_AC_XM | BT_PIN_XM;
+    if (b.kind == 3 && b.sx == 1) mk |= BT_ATC_XM;
+    if (b.kind == 1) mk |= BT_TAN_XM;
+    if (z - 1 >= 0 && (b = classify(x - 1, z - 1, NX, NZ, pml), b.kind == 2 && b.s == 1)) mk |= BT_AT_XM_ZM;
+    if (z + 1 < NZ && (b = classify(x - 1, z + 1, NX, NZ, pml), b.kind == 2 && b.s == 1)) mk |= BT_AT_XM_ZP;
   }
   if (x + 1 < NX) {  // readers to the right
-    const BandCell b = classify(x + 1, z, NX, NZ, pml);
-    if (b.kind == 2 && b.s == -1) r += SEIS_GL(L.ain)[fidx(ld, x + 1, z)];
-    if (b.kind == 3 && b.sx == -1) r += SEIS_GL(L.at)[fidx(ld, x + 1, z)];
-    for (int e = -1; e <= 1; e += 2) {
-      if (z + e < 0 || z + e >= NZ) continue;
-      const BandCell bt = classify(x + 1, z + e, NX, NZ, pml);
-      if (bt.kind == 2 && bt.s == -1) r += SEIS_GL(L.at)[fidx(ld, x + 1, z + e)];
-    }
+    b = classify(x + 1, z, NX, NZ, pml);
+    if (b.kind == 2 && b.s == -1) mk |= BT_AIN_XP | BT_AC_XP | BT_PIN_XP;
+    if (b.kind == 3 && b.sx == -1) mk |= BT_ATC_XP;
+    if (b.kind == 1) mk |= BT_TAN_XP;
+    if (z - 1 >= 0 && (b = classify(x + 1, z - 1, NX, NZ, pml), b.kind == 2 && b.s == -1)) mk |= BT_AT_XP_ZM;
+    if (z + 1 < NZ && (b = classify(x + 1, z + 1, NX, NZ, pml), b.kind == 2 && b.s == -1)) mk |= BT_AT_XP_ZP;
   }
+  const BandCell self = classify(x, z, NX, NZ, pml);
+  if (self.kind != 0) mk |= BT_AC_SELF;
+  if (self.kind == 1 || self.kind == 2) mk |= BT_AIN_SELF;
+  return mk;
+}
+
+// Adjoint of the raw (pre-blend) value of cell (x, z) at a level:
+// w lam(c) + sum over the band cells that read raw(c) of their stored
+// product; `mk` = band_terms(x, z, ...).
+template <typename T>
+SEIS_HD inline T adj_raw_m(const Consts<T>& k, int nx, const AdjLevel<T>& L, int x, int z, unsigned mk) {
+  const long long ld = k.ld;
+  const T w = sponge_weight<T>(x, z, nx, k.nz, k.pml);
+  auto ain = [&](int xx, int zz) { return SEIS_GL(L.ain)[fidx(ld, xx, zz)]; };
+  auto at = [&](int xx, int zz) { return SEIS_GL(L.at)[fidx(ld, xx, zz)]; };
+  T r = w * SEIS_GL(L.lam)[fidx(ld, x, z)];
+  if (mk & BT_AIN_ZM) r += ain(x, z - 1);
+  if (mk & BT_AT_ZM_XM) r += at(x - 1, z - 1);
+  if (mk & BT_AT_ZM_XP) r += at(x + 1, z - 1);
+  if (mk & BT_AIN_ZP) r += ain(x, z + 1);
+  if (mk & BT_AT_ZP_XM) r += at(x - 1, z + 1);
+  if (mk & BT_AT_ZP_XP) r += at(x + 1, z + 1);
+  if (mk & BT_AIN_XM) r += ain(x - 1, z);
+  if (mk & BT_ATC_XM) r += at(x - 1, z);
+  if (mk & BT_AT_XM_ZM) r += at(x - 1, z - 1);
+  if (mk & BT_AT_XM_ZP) r += at(x - 1, z + 1);
+  if (mk & BT_AIN_XP) r += ain(x + 1, z);
+  if (mk & BT_ATC_XP) r += at(x + 1, z);
+  if (mk & BT_AT_XP_ZM) r += at(x + 1, z - 1);
+  if (mk & BT_AT_XP_ZP) r += at(x + 1, z + 1);
   return r;
+}
+
+template <typename T>
+SEIS_HD inline T adj_raw(const Consts<T>& k, int NX, int nx, const AdjLevel<T>& L, int x, int z) {
+  return adj_raw_m(k, nx, L, x, z, band_terms(x, z, NX, k.NZ, k.pml));
 }
 
 // Derivative of the ring's extrapolated term 2 d1 - d2 (fd.cpp:1072-1091),
@@ -169,46 +228,51 @@ SEIS_HD inline bool high_order(int x, int z, int NX, int NZ) { return x >= 4 && 
 // (x, z): cur(b) by band cell b itself and cur(b_in) by the side cell b whose
 // inner neighbour is (x, z).  Zero unless edge_dist(x, z) <= pml.
 template <typename T>
-SEIS_HD inline T adj_oneway_cur(const Consts<T>& k, int NX, const AdjLevel<T>& L1, int x, int z) {
-  const int NZ = k.NZ, pml = k.pml;
+SEIS_HD inline T adj_oneway_cur_m(const Consts<T>& k, const AdjLevel<T>& L1, int x, int z, unsigned mk) {
   const long long ld = k.ld;
+  auto ac = [&](int xx, int zz) { return SEIS_GL(L1.ac)[fidx(ld, xx, zz)]; };
   T acc = T(0);
-  const BandCell self = classify(x, z, NX, NZ, pml);
-  if (self.kind != 0) acc += SEIS_GL(L1.ac)[fidx(ld, x, z)];
-  BandCell b;
-  if (z - 1 >= 0 && (b = classify(x, z - 1, NX, NZ, pml), b.kind == 1 && b.s == 1)) acc += SEIS_GL(L1.ac)[fidx(ld, x, z - 1)];
-  if (z + 1 < NZ && (b = classify(x, z + 1, NX, NZ, pml), b.kind == 1 && b.s == -1)) acc += SEIS_GL(L1.ac)[fidx(ld, x, z + 1)];
-  if (x - 1 >= 0 && (b = classify(x - 1, z, NX, NZ, pml), b.kind == 2 && b.s == 1)) acc += SEIS_GL(L1.ac)[fidx(ld, x - 1, z)];
-  if (x + 1 < NX && (b = classify(x + 1, z, NX, NZ, pml), b.kind == 2 && b.s == -1)) acc += SEIS_GL(L1.ac)[fidx(ld, x + 1, z)];
+  if (mk & BT_AC_SELF) acc += ac(x, z);
+  if (mk & BT_AC_ZM) acc += ac(x, z - 1);
+  if (mk & BT_AC_ZP) acc += ac(x, z + 1);
+  if (mk & BT_AC_XM) acc += ac(x - 1, z);
+  if (mk & BT_AC_XP) acc += ac(x + 1, z);
   return acc;
 }
 
 // Adjoint of the one-way band update's reads of the previous level (level
 // m+1) at cell (x, z).  Zero unless edge_dist(x, z) <= pml.
 template <typename T>
-SEIS_HD inline T adj_oneway_prev(const Consts<T>& k, const Model<T>& m, int NX, const AdjLevel<T>& L2, int x, int z) {
-  const int NZ = k.NZ, pml = k.pml, nx = m.nx;
+SEIS_HD inline T adj_oneway_prev_m(const Consts<T>& k, int nx, const AdjLevel<T>& L2, int x, int z, unsigned mk) {
+  const int pml = k.pml;
   const long long ld = k.ld;
   T acc = T(0);
-  const BandCell self = classify(x, z, NX, NZ, pml);
-  if (self.kind == 1 || self.kind == 2) acc += SEIS_GL(L2.ain)[fidx(ld, x, z)];
+  if (mk & BT_AIN_SELF) acc += SEIS_GL(L2.ain)[fidx(ld, x, z)];
   auto pin = [&](int bx, int bz) {  // (a1+a2) at = -(1-w) lam for the inner neighbour
     const T w = sponge_weight<T>(bx, bz, nx, k.nz, pml);
     return -(T(1) - w) * SEIS_GL(L2.lam)[fidx(ld, bx, bz)];
   };
-  BandCell b;
-  if (z - 1 >= 0 && (b = classify(x, z - 1, NX, NZ, pml), b.kind == 1 && b.s == 1)) acc += pin(x, z - 1);
-  if (z + 1 < NZ && (b = classify(x, z + 1, NX, NZ, pml), b.kind == 1 && b.s == -1)) acc += pin(x, z + 1);
-  if (x - 1 >= 0 && (b = classify(x - 1, z, NX, NZ, pml), b.kind == 2 && b.s == 1)) acc += pin(x - 1, z);
-  if (x + 1 < NX && (b = classify(x + 1, z, NX, NZ, pml), b.kind == 2 && b.s == -1)) acc += pin(x + 1, z);
+  if (mk & BT_PIN_ZM) acc += pin(x, z - 1);
+  if (mk & BT_PIN_ZP) acc += pin(x, z + 1);
+  if (mk & BT_PIN_XM) acc += pin(x - 1, z);
+  if (mk & BT_PIN_XP) acc += pin(x + 1, z);
   // tangent neighbours: z sides on the same row, x sides on the same column
-  for (int e = -1; e <= 1; e += 2) {
-    if (x + e >= 0 && x + e < NX && (b = classify(x + e, z, NX, NZ, pml), b.kind == 1))
-      acc += SEIS_GL(L2.at)[fidx(ld, x + e, z)];
-    if (z + e >= 0 && z + e < NZ && (b = classify(x, z + e, NX, NZ, pml), b.kind == 2))
-      acc += SEIS_GL(L2.at)[fidx(ld, x, z + e)];
-  }
+  auto at = [&](int xx, int zz) { return SEIS_GL(L2.at)[fidx(ld, xx, zz)]; };
+  if (mk & BT_TAN_XM) acc += at(x - 1, z);
+  if (mk & BT_TAN_ZM) acc += at(x, z - 1);
+  if (mk & BT_TAN_XP) acc += at(x + 1, z);
+  if (mk & BT_TAN_ZP) acc += at(x, z + 1);
   return acc;
+}
+
+template <typename T>
+SEIS_HD inline T adj_oneway_cur(const Consts<T>& k, int NX, const AdjLevel<T>& L1, int x, int z) {
+  return adj_oneway_cur_m(k, L1, x, z, band_terms(x, z, NX, k.NZ, k.pml));
+}
+
+template <typename T>
+SEIS_HD inline T adj_oneway_prev(const Consts<T>& k, const Model<T>& m, int NX, const AdjLevel<T>& L2, int x, int z) {
+  return adj_oneway_prev_m(k, m.nx, L2, x, z, band_terms(x, z, NX, k.NZ, k.pml));
 }
 
 // lam^{m-1}(x, z) from level m (cur terms, L1) and level m+1 (prev terms, L2),

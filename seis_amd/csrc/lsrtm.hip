@@ -41,9 +41,22 @@ __device__ __forceinline__ AdjLevel<T> level(const DevShot<T>& s, int lev) {
 
 // Raw adjoints of level `lev` for the cells with a band reader or w < 1
 // (outside [pml+1, NX-pml-1) x [pml+1, NZ-pml-1)), stored for the step kernel.
+// Band-term masks (band_terms) of the ring cells, computed once per plan.
+__global__ void band_terms_kernel(Ring R, int NX, int NZ, int pml, unsigned* __restrict__ out) {
+  const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= R.n) return;
+  int x, z;
+  ring_cell(R, NX, NZ, j, &x, &z);
+  out[j] = band_terms(x, z, NX, NZ, pml);
+}
+
+// Raw adjoints of level `lev` for the cells with a band reader or w < 1
+// (outside [pml+1, NX-pml-1) x [pml+1, NZ-pml-1)), and the one-way terms of
+// lam^{lev-1} at those cells (levels lev and lev+1 are complete now), both
+// stored for the step kernel.  `masks` holds band_terms of each ring cell.
 template <typename T>
 __global__ __launch_bounds__(256) void adj_prep_kernel(Consts<T> k, const DevShot<T>* __restrict__ shots, Ring R,
-                                                       int step) {
+                                                       const unsigned* __restrict__ masks, int step) {
   const DevShot<T>& s = shots[blockIdx.y];
   const int lev = s.nt + s.ndel + 1 - step;
   if (lev < 1) return;
@@ -51,14 +64,12 @@ __global__ __launch_bounds__(256) void adj_prep_kernel(Consts<T> k, const DevSho
   if (j >= R.n) return;
   int x, z;
   ring_cell(R, s.NX, k.NZ, j, &x, &z);
+  const unsigned mk = SEIS_GL(masks)[j];
   const AdjLevel<T> L = level(s, lev);
-  const auto ar = SEIS_GL(s.fld + (LS_AR + lev % 2) * s.fstride);
   const long long i = fidx(k.ld, x, z);
-  ar[i] = adj_raw(k, s.NX, s.nx, L, x, z);
-  // the one-way terms of lam^{lev-1} at this cell (levels lev and lev+1 are
-  // complete now), added by the step kernel's sweep
-  const Model<T> m{s.vel, s.nx, k.nz, k.pml};
-  SEIS_GL(s.fld)[LS_OW * s.fstride + i] = adj_oneway_cur(k, s.NX, L, x, z) + adj_oneway_prev(k, m, s.NX, level(s, lev + 1), x, z);
+  SEIS_GL(s.fld + (LS_AR + lev % 2) * s.fstride)[i] = adj_raw_m(k, s.nx, L, x, z, mk);
+  SEIS_GL(s.fld)[LS_OW * s.fstride + i] =
+      adj_oneway_cur_m(k, L, x, z, mk) + adj_oneway_prev_m(k, s.nx, level(s, lev + 1), x, z, mk);
 }
 
 // Reads a raw adjoint: precomputed near the band, the level itself elsewhere.
@@ -351,6 +362,7 @@ struct LsrtmPlan {
   T* cg = nullptr;         // CG vectors: x, s, p (model) and r, q (records)
   const T* mask = nullptr; // optional record mask (caller-owned, device)
   T* masked = nullptr;     // masked copy of an adjoint input
+  unsigned* pmask = nullptr;  // band_terms of the ring cells P
   Consts<T> k{};
   Layout F{};   // forward step split (interior sweep / band)
   Layout A{};   // adjoint step split
@@ -359,7 +371,7 @@ struct LsrtmPlan {
   hipStream_t stream = 0;
 
   ~LsrtmPlan() {
-    for (void* p : {(void*)dev, (void*)pool, snaps, (void*)scales, (void*)grads, (void*)part, (void*)cg, (void*)masked})
+    for (void* p : {(void*)dev, (void*)pool, snaps, (void*)scales, (void*)grads, (void*)part, (void*)cg, (void*)masked, (void*)pmask})
       if (p) (void)hipFree(p);
   }
 
@@ -373,6 +385,8 @@ struct LsrtmPlan {
     if (g.pml < 4) return set_error("lsrtm: pml must be >= 4"), SEIS_ERR_ARG;
     if (g.nx <= 0 || g.nz <= 0 || !(g.dt > 0) || !(g.dx > 0) || !(g.dz > 0))
       return set_error("lsrtm: bad grid"), SEIS_ERR_ARG;
+    if ((long long)(g.nx + 2 * g.pml) * (g.nz + 2 * g.pml) >= (1LL << 31))
+      return set_error("lsrtm: the padded grid must have fewer than 2^31 cells"), SEIS_ERR_ARG;
     if (kind < 0 || kind > 2) return set_error("lsrtm: snapshot kind must be 0, 1 or 2"), SEIS_ERR_ARG;
     if (ns <= 0) return set_error("lsrtm: no shots"), SEIS_ERR_ARG;
     NX = g.nx + 2 * g.pml;
@@ -420,6 +434,11 @@ struct LsrtmPlan {
     }
     P = make_ring(NX, NZ, g.pml + 1, NX - g.pml - 1, g.pml + 1, NZ - g.pml - 1);
     B = band_ring(NX, NZ, g.pml);
+    if (P.n > 0) {
+      LS_CHECK(hipMalloc(&pmask, sizeof(unsigned) * (size_t)P.n));
+      hipLaunchKernelGGL(band_terms_kernel, dim3((unsigned)((P.n + 255) / 256)), dim3(256), 0, stream, P, NX, NZ,
+                         g.pml, pmask);
+    }
     host.resize(ns);
     const long long origin = (long long)MX * ld + MZ;
     char* sp = reinterpret_cast<char*>(snaps);
@@ -526,7 +545,7 @@ struct LsrtmPlan {
       int n = 0;
       for (const ShotSpec& q : spec) n += (q.nt + q.ndel + 1 - st) >= 1;
       timed_launch(stream, "lsrtm_adjoint_prep", 8 * E * n * (double)P.n, 0, [&] {
-        hipLaunchKernelGGL((adj_prep_kernel<T>), dim3(pwg, nshots), dim3(256), 0, stream, k, dev, P, st);
+        hipLaunchKernelGGL((adj_prep_kernel<T>), dim3(pwg, nshots), dim3(256), 0, stream, k, dev, P, pmask, st);
       });
       const double by = 4 * E * n * cells() + (snap_elem() + 2 * E) * n * (double)model;
       timed_launch(stream, "lsrtm_adjoint", by, n * cells(), [&] {
