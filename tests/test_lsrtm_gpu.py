@@ -91,3 +91,57 @@ def test_16bit_snapshots_close_to_fp32(engine, oracle, kind, tol):
     ref = LsrtmPlan(g, shots, vel, wavs, gzs, snapshot=SNAP_FP32).born(dv)
     got = LsrtmPlan(g, shots, vel, wavs, gzs, snapshot=kind).born(dv)
     assert rel_l2(got, ref) <= tol
+
+
+def test_cgls_matches_textbook_cgls_on_explicit_operator(engine, oracle):
+    """The GPU CGLS (FP64 plan) against CGLS written out in numpy on the
+    explicit matrix of the same operator (columns L e_j from the plan's own
+    Born), and L^T against that matrix's transpose; after as many iterations
+    as unknowns the residual is the least-squares minimum."""
+    from seis_amd.lsrtm import LsrtmPlan
+    print("case cgls_explicit 6x5 nt=60", flush=True)
+    nx, nz, pml, nt = 6, 5, 4, 60
+    rng = np.random.default_rng(8)
+    dt, dx, dz = float(np.float32(0.001)), 10.0, 10.0
+    vel = 2000.0 + 500.0 * rng.random((nx, nz))
+    g = engine.Grid(nx, nz, pml, dt, dx, dz)
+    shots = [engine.Shot(sx=2, sz=1, nt=nt)]
+    plan = LsrtmPlan(g, shots, vel, [oracle.ricker(dt, 30.0, 15, nt)], [np.zeros(nx, np.int32)], fp64=True)
+    n = nx * nz
+    Lm = np.stack([plan.born(np.eye(n)[j].reshape(nx, nz)) for j in range(n)], axis=1)
+    d = rng.standard_normal(plan.nrec)
+    # the transpose of the explicit matrix
+    lt = plan.adjoint(d).ravel()
+    assert np.linalg.norm(lt - Lm.T @ d) <= 1e-12 * np.linalg.norm(Lm.T @ d)
+    # textbook CGLS from x = 0
+    k = 10
+    x = np.zeros(n)
+    r = d.copy()
+    s = Lm.T @ r
+    p = s.copy()
+    gamma = s @ s
+    ref = [(r @ r, gamma)]
+    for _ in range(k):
+        q = Lm @ p
+        alpha = gamma / (q @ q)
+        x += alpha * p
+        r -= alpha * q
+        s = Lm.T @ r
+        gnew = s @ s
+        p = s + (gnew / gamma) * p
+        gamma = gnew
+        ref.append((r @ r, gamma))
+    img, hist = plan.cg(d, niter=k)
+    assert hist.shape == (k + 1, 2)
+    assert np.allclose(hist, np.array(ref), rtol=1e-9, atol=0)
+    # the iterate is more sensitive than the residuals on this ill-conditioned
+    # operator (the two sides sum their dot products in different orders)
+    assert np.linalg.norm(img.ravel() - x) <= 1e-6 * np.linalg.norm(x)
+    # n iterations come down to the least-squares minimum (never below it;
+    # finite-precision CG on this conditioning stays a few 1e-6 above it)
+    img_n, hist_n = plan.cg(d, niter=n)
+    xls = np.linalg.lstsq(Lm, d, rcond=None)[0]
+    rmin = np.sum((d - Lm @ xls) ** 2)
+    assert rmin * (1 - 1e-10) <= hist_n[-1, 0] <= rmin * (1 + 1e-4)
+    assert np.all(np.diff(hist_n[:, 0]) <= 1e-12 * hist_n[0, 0])  # CGLS residuals never increase
+    plan.close()
