@@ -169,6 +169,17 @@ int seis_lsrtm_set_mask(seis_lsrtm* plan, const void* mask);
 int seis_lsrtm_cg(seis_lsrtm* plan, const void* dobs, int niter, double rtol, void* image, double* hist,
                   int* iters);
 
+/* Strip-length autotuning of the interior sweeps (no counterpart in the
+ * reference).  On by default: the first run of a configuration (kind of
+ * pass, padded grid, shots per launch, device) times a few trial launches of
+ * each candidate strip length on its zeroed fields and keeps the fastest for
+ * the process.  seis_set_tuning(0) turns it off (fixed default) and forgets
+ * the table; seis_tuned_strip reports the choice for a configuration (0 when
+ * none was made).  kind: "forward", "born", "rtm", "lsrtm_forward",
+ * "lsrtm_adjoint". */
+int seis_set_tuning(int on);
+int seis_tuned_strip(const char* kind, int nx_padded, int nz_padded, int nshots, int* bx);
+
 /* Kernel timing: instrumentation with no counterpart in the reference.
  * Between seis_profile_start and seis_profile_stop every engine launch is
  * bracketed by HIP events on its own stream; stop returns, per kernel, the

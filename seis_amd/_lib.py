@@ -39,6 +39,8 @@ _SIGS = {
     "seis_memcpy_dtoh": (_I, [_P, _P, _SZ]),
     "seis_memset": (_I, [_P, _I, _SZ]),
     "seis_copy_f32": (_I, [_P, _P, ctypes.c_longlong]),
+    "seis_set_tuning": (_I, [_I]),
+    "seis_tuned_strip": (_I, [ctypes.c_char_p, _I, _I, _I, ctypes.POINTER(_I)]),
     "seis_synchronize": (_I, []),
     "seis_modeling": (_I, [_F, _F, _F, _I, _I, _I, _I, _I, _I, _I, _P, _P, _P, _P]),
     "seis_born": (_I, [_F, _F, _F, _I, _I, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P]),

@@ -195,7 +195,17 @@ inline long long field_ld(int NZ) { return ((long long)(NZ + 2 * MZ) + 63) / 64 
 
 inline long long field_elems(int NX, long long ld) { return (long long)(NX + 2 * MX) * ld; }
 
-inline Layout make_layout(int NX, int NZ, int pml) {
+// Strip length bx of the interior sweep: the derived counts.
+inline Layout with_strip(Layout L, int bx) {
+  L.bx = bx;
+  L.nzc = (L.zb - L.za + 63) / 64;
+  L.nxs = (L.xb - L.xa + L.bx - 1) / L.bx;
+  if (L.xb == L.xa) L.nzc = L.nxs = 0;
+  L.nfwg = (L.nzc * L.nxs + WAVES - 1) / WAVES;
+  return L;
+}
+
+inline Layout make_layout(int NX, int NZ, int pml, int bx = 16) {
   Layout L;
   L.xa = std::max(pml, 4);
   L.xb = std::min(NX - pml, NX - 4);
@@ -206,16 +216,17 @@ inline Layout make_layout(int NX, int NZ, int pml) {
     L.za = 0;
     L.zb = NZ;
   }
-  L.bx = 16;  // short strips: more wavefronts in flight (measured best of 16, 32, 64)
-  L.nzc = (L.zb - L.za + 63) / 64;
-  L.nxs = (L.xb - L.xa + L.bx - 1) / L.bx;
-  if (L.xb == L.xa) L.nzc = L.nxs = 0;
-  L.nfwg = (L.nzc * L.nxs + WAVES - 1) / WAVES;
   L.nband = (long long)NX * NZ - (long long)(L.xb - L.xa) * (L.zb - L.za);
   L.nbwg = (int)((L.nband + WG - 1) / WG);
-  return L;
+  return with_strip(L, bx);
 }
 
+// Strip-length autotuning (tune.cpp).  `trial(bx)` enqueues a few launches
+// with strip length bx on `stream`; the first time a configuration `key` is
+// seen each candidate is timed and the fastest kept for the process.  With
+// tuning off (seis_set_tuning) the default strip length is used.
+int tuned_strip(const char* kind, int NX, int NZ, int nshots, hipStream_t stream, void (*trial)(void*, int),
+                void* ctx);
 
 // Enumeration of the cells outside an inner rectangle [xa, xb) x [za, zb) of
 // the padded grid: the rows above and below it over the full width, then the

@@ -745,8 +745,30 @@ int run_batch(int mode, const GridSpec& g, int nshots, const ShotSpec* shots, co
   HIPCHECK(hipMalloc(&br.dev, sizeof(DevShot<T>) * nshots));
   HIPCHECK(hipMemcpyAsync(br.dev, br.host.data(), sizeof(DevShot<T>) * nshots, hipMemcpyHostToDevice, stream));
   const Consts<T> k = make_consts<T>(g, ld);
-  const Layout L = make_layout(NX, NZ, g.pml);
+  Layout L = make_layout(NX, NZ, g.pml);
   const Ring B = band_ring(NX, NZ, g.pml);
+  {
+    // strip length: timed on the zeroed fields with this pass's step kernel
+    // at step 0 (no injection yet, so the fields stay zero; records and strip
+    // samples written here are rewritten by the real step 0)
+    struct Trial {
+      int mode;
+      const Consts<T>* k;
+      const DevShot<T>* dev;
+      Layout L;
+      Ring B;
+      int nshots;
+      hipStream_t s;
+    };
+    const int proxy = mode == RUN_MODEL ? MODEL : mode == RUN_BORN ? BORN : RTM_FWD;
+    Trial tr{proxy, &k, br.dev, L, B, nshots, stream};
+    auto fn = [](void* c, int bx) {
+      const Trial* t = static_cast<const Trial*>(c);
+      launch_step<T>(t->mode, *t->k, t->dev, with_strip(t->L, bx), t->B, t->nshots, 0, t->s);
+    };
+    const char* kind = mode == RUN_MODEL ? "forward" : mode == RUN_BORN ? "born" : "rtm";
+    L = with_strip(L, tuned_strip(kind, NX, NZ, nshots, stream, fn, &tr));
+  }
   const double cells = (double)NX * NZ;
   const double E = sizeof(T);
   // shots still running at forward step st / reverse step st

@@ -368,6 +368,7 @@ struct LsrtmPlan {
   Layout A{};   // adjoint step split
   Ring P{};     // cells whose raw adjoint is precomputed
   Ring B{};     // absorbing band of the forward steps
+  bool adj_tuned = false;
   hipStream_t stream = 0;
 
   ~LsrtmPlan() {
@@ -425,12 +426,9 @@ struct LsrtmPlan {
         A.za = 0;
         A.zb = NZ;
       }
-      A.nzc = (A.zb - A.za + 63) / 64;
-      A.nxs = (A.xb - A.xa + A.bx - 1) / A.bx;
-      if (A.xb == A.xa) A.nzc = A.nxs = 0;
-      A.nfwg = (A.nzc * A.nxs + WAVES - 1) / WAVES;
       A.nband = (long long)NX * NZ - (long long)(A.xb - A.xa) * (A.zb - A.za);
       A.nbwg = (int)((A.nband + AWG - 1) / AWG);
+      A = with_strip(A, A.bx);
     }
     P = make_ring(NX, NZ, g.pml + 1, NX - g.pml - 1, g.pml + 1, NZ - g.pml - 1);
     B = band_ring(NX, NZ, g.pml);
@@ -470,6 +468,17 @@ struct LsrtmPlan {
     }
     LS_CHECK(hipMalloc(&dev, sizeof(DevShot<T>) * ns));
     LS_CHECK(hipMemcpy(dev, host.data(), sizeof(DevShot<T>) * ns, hipMemcpyHostToDevice));
+    {
+      // strip length of the forward passes: the background step with the
+      // max-abs reduction at step 0 on zeroed fields (writes zeros only)
+      int rc = zero_fields(0, LS_NSLOT);
+      if (rc) return rc;
+      auto fn = [](void* c, int bx) {
+        LsrtmPlan* p = static_cast<LsrtmPlan*>(c);
+        launch_step<T>(SNAP_MAX, p->k, p->dev, with_strip(p->F, bx), p->B, p->nshots, 0, p->stream);
+      };
+      F = with_strip(F, tuned_strip("lsrtm_forward", NX, NZ, nshots, stream, fn, this));
+    }
     return make_snapshots();
   }
 
@@ -536,6 +545,18 @@ struct LsrtmPlan {
     }
     for (int s = 0; s < nshots; s++) host[s].rec = const_cast<T*>(rec) + spec[s].rec;
     LS_CHECK(hipMemcpyAsync(dev, host.data(), sizeof(DevShot<T>) * nshots, hipMemcpyHostToDevice, stream));
+    if (!adj_tuned) {
+      // strip length of the adjoint sweep: its step 0 (no gradient update
+      // yet); the fields it writes are cleared below
+      auto fn = [](void* c, int bx) {
+        LsrtmPlan* p = static_cast<LsrtmPlan*>(c);
+        const Layout a = with_strip(p->A, bx);
+        hipLaunchKernelGGL((adj_step_kernel<T>), dim3(a.nfwg + a.nbwg, p->nshots), dim3(AWG), 0, p->stream, p->k,
+                           p->dev, a, 0);
+      };
+      A = with_strip(A, tuned_strip("lsrtm_adjoint", NX, NZ, nshots, stream, fn, this));
+      adj_tuned = true;
+    }
     int rc = zero_fields(0, LS_NSLOT);
     if (rc) return rc;
     LS_CHECK(hipMemsetAsync(grads, 0, sizeof(T) * (size_t)model * nshots, stream));
