@@ -19,4 +19,9 @@ float** forward(float dt, float dx, float dz, int nt, int ndel, int nx, int nz, 
 float** adjoint(float dt, float dx, float dz, int nt, int ndel, int nx, int nz, int pml, int sx, int sz, int* gz,
                 float* wav, float** vel, float** rec);
 
+// Releases a result of the three functions above exactly as the reference's
+// MyAlloc<float>::free does (delete[] buf[0]; delete[] buf;), for callers that
+// are not C++ (no counterpart in the reference; NULL is ignored).
+extern "C" void seis_fd_free(float** buf);
+
 #endif

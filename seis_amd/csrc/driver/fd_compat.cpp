@@ -77,3 +77,5 @@ float** adjoint(float dt, float dx, float dz, int nt, int ndel, int nx, int nz, 
   if (!img) return nullptr;
   return finish(seis_rtm(dt, dx, dz, nt, ndel, nx, nz, pml, sx, sz, gz, wav, v.p, r.p, img[0]), img);
 }
+
+extern "C" void seis_fd_free(float** buf) { free2(buf); }
