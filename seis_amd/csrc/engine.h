@@ -110,6 +110,17 @@ __device__ __forceinline__ int strip_slots(int x, int z, int nx, int nz, int pml
   return n;
 }
 
+// The first slot strip_slots would return, or -1 (no private array).
+__device__ __forceinline__ int first_strip_slot(int x, int z, int nx, int nz, int pml) {
+  const int ix = x - pml, iz = z - pml;
+  if (ix < 0 || ix >= nx || iz < 0 || iz >= nz) return -1;
+  if (iz < 4) return ix * 4 + iz;
+  if (iz >= nz - 4) return nx * 4 + ix * 4 + (nz - 1 - iz);
+  if (ix < 4) return 8 * nx + ix * nz + iz;
+  if (ix >= nx - 4) return 8 * nx + 4 * nz + (nx - 1 - ix) * nz + iz;
+  return -1;
+}
+
 // Per-step, per-shot state shared by both paths of the step kernel.
 template <typename T>
 struct StepCtx {
@@ -195,13 +206,25 @@ inline long long field_ld(int NZ) { return ((long long)(NZ + 2 * MZ) + 63) / 64 
 
 inline long long field_elems(int NX, long long ld) { return (long long)(NX + 2 * MX) * ld; }
 
-// Strip length bx of the interior sweep: the derived counts.
+// XCDs of the MI355X.  The dispatcher deals the workgroups of a launch
+// round-robin over them, so along one row of the grid, blocks b and b' run on
+// the same XCD (and share its L2) when b = b' mod NXCD.
+constexpr int NXCD = 8;
+
+// Interior tile of block b of the sweep (nfwg a multiple of NXCD): each XCD
+// takes one contiguous run of tiles, so neighbouring strips of a z chunk,
+// which read 8 columns in common, meet in the same L2.
+__device__ __forceinline__ int xcd_tile(int b, int nfwg) { return (b % NXCD) * (nfwg / NXCD) + b / NXCD; }
+
+// Strip length bx of the interior sweep: the derived counts (the sweep's
+// workgroup count is padded to a multiple of NXCD; the padding blocks exit).
 inline Layout with_strip(Layout L, int bx) {
   L.bx = bx;
   L.nzc = (L.zb - L.za + 63) / 64;
   L.nxs = (L.xb - L.xa + L.bx - 1) / L.bx;
   if (L.xb == L.xa) L.nzc = L.nxs = 0;
   L.nfwg = (L.nzc * L.nxs + WAVES - 1) / WAVES;
+  L.nfwg = (L.nfwg + NXCD - 1) / NXCD * NXCD;
   return L;
 }
 

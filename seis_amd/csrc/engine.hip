@@ -145,9 +145,8 @@ __device__ __forceinline__ void cell_update(const Consts<T>& k, const DevShot<T>
     // restored (fd.cpp:663); lap above used the value before restoration
     T out = u;
     if (itp - 1 >= 0) {
-      int sl[4];
-      if (strip_slots(x, z, s.nx, k.nz, k.pml, sl) > 0)
-        out = s.strips[(long long)(itp - 1) * 8 * (s.nx + k.nz) + sl[0]];
+      const int sl = first_strip_slot(x, z, s.nx, k.nz, k.pml);
+      if (sl >= 0) out = s.strips[(long long)(itp - 1) * 8 * (s.nx + k.nz) + sl];
     }
     N[i] = out;
     if (edge_dist(x, z, NX, k.NZ) == k.pml) s.raw[i] = u;  // stencil value for the band (see interior_sweep)
@@ -438,9 +437,8 @@ __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot
         NB[i] = ub;
         T out = ua;
         if (c.itp - 1 >= 0) {
-          int sl[4];
-          if (strip_slots(x, z, s.nx, k.nz, pml, sl) > 0)
-            out = SEIS_GL(s.strips)[(long long)(c.itp - 1) * 8 * (s.nx + k.nz) + sl[0]];
+          const int sl = first_strip_slot(x, z, s.nx, k.nz, pml);
+          if (sl >= 0) out = SEIS_GL(s.strips)[(long long)(c.itp - 1) * 8 * (s.nx + k.nz) + sl];
         }
         NA[i] = out;
         // the band's one-way update reads the stencil value of this layer,
@@ -479,7 +477,7 @@ __global__ __launch_bounds__(WG) void step_kernel(Consts<T> k, const DevShot<T>*
   c.aux = s.aux ? s.aux + (step & 1) * s.fstride : nullptr;
   if ((int)blockIdx.x < L.nfwg) {
     const int wave = threadIdx.x >> 6;
-    const int item = blockIdx.x * WAVES + wave;
+    const int item = xcd_tile(blockIdx.x, L.nfwg) * WAVES + wave;
     if (item >= L.nzc * L.nxs) return;  // whole wavefront leaves; no workgroup barrier follows
     const int zc = item % L.nzc, xs = item / L.nzc;
     const int z0 = L.za + zc * 64;

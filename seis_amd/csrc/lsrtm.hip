@@ -128,7 +128,7 @@ __global__ __launch_bounds__(AWG) void adj_step_kernel(Consts<T> k, const DevSho
     // gather of Q = (raw adjoint) vt2 / h^2 of the readers; cells at most
     // pml from the edge add the one-way terms and store their band products.
     const int wave = threadIdx.x >> 6;
-    const int item = blockIdx.x * WAVES + wave;
+    const int item = xcd_tile(blockIdx.x, F.nfwg) * WAVES + wave;
     if (item >= F.nzc * F.nxs) return;
     const int zc = item % F.nzc, xs = item / F.nzc;
     const int z0 = F.za + zc * 64;
