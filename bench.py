@@ -17,6 +17,7 @@ algorithmic bytes), and the reference's CPU path timed on this host on a
 bounded sample of the same workload.
 """
 import argparse
+import contextlib
 import json
 import os
 import sys
@@ -70,7 +71,8 @@ def cpu_baseline(w, nt_sample, threads):
     sx = w["sxs"][len(w["sxs"]) // 2]
     rec[spread_mask(w, sx)] = np.random.default_rng(5).standard_normal((int(spread_mask(w, sx).sum()), nt_sample))
     t0 = time.perf_counter()
-    lib.rtm(w["dt"], w["dx"], w["dz"], nt_sample, 0, nx, nz, pml, sx, w["sz"], gz, wav, w["vmig"], rec)
+    with stdout_to_stderr():
+        lib.rtm(w["dt"], w["dx"], w["dz"], nt_sample, 0, nx, nz, pml, sx, w["sz"], gz, wav, w["vmig"], rec)
     el = time.perf_counter() - t0
     cells = (nx + 2 * pml) * (nz + 2 * pml)
     updates = 3.0 * nt_sample * cells
@@ -80,6 +82,58 @@ def cpu_baseline(w, nt_sample, threads):
 
 def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+@contextlib.contextmanager
+def stdout_to_stderr():
+    """The compiled reference prints its progress to stdout (fd 1); the
+    bench's stdout carries only the JSON line."""
+    sys.stdout.flush()
+    saved = os.dup(1)
+    os.dup2(2, 1)
+    try:
+        yield
+    finally:
+        sys.stdout.flush()
+        os.dup2(saved, 1)
+        os.close(saved)
+
+
+def c1_single_shot(calls, with_cpu, threads):
+    """The reference's CPU-runnable case: one-shot forward modeling of the
+    3-layer model, 201 x 301 cells @ 10 m, pml 30, nt = 1500, through the
+    single-shot host-buffer entry point (fd.h modeling), with the time loop
+    replayed as a HIP graph and launched step by step."""
+    import seis_amd
+    nx, nz, pml, nt, dt, dx = 301, 201, 30, 1500, 0.001, 10.0
+    vel = np.empty((nx, nz), np.float32)
+    z = np.arange(nz) * dx
+    vel[:] = np.where(z < 600.0, 1500.0, np.where(z < 1400.0, 2200.0, 3000.0)).astype(np.float32)
+    gz = np.full(nx, 2, np.int32)
+    wav = seis_amd.ricker(dt, 15.0, 67, nt)
+    args = (dt, dx, dx, nt, 0, nx, nz, pml, 150, 2, gz, wav, vel)
+    out = {"workload": "modeling, 1 shot, 201x301 cells @ 10 m, pml 30, nt=1500, 3-layer model, host buffers "
+                       "in and out (fd.h modeling)"}
+    rec = seis_amd.modeling(*args)  # warm-up (and the strip-length trial)
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        seis_amd.modeling(*args)
+    out["ms_per_call"] = round((time.perf_counter() - t0) * 1e3 / calls, 3)
+    cells = (nx + 2 * pml) * (nz + 2 * pml) * nt
+    out["gcell_updates_per_s"] = round(cells / (out["ms_per_call"] * 1e-3) / 1e9, 2)
+    if with_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib
+        kind = "reference" if oracle_lib.have_reference() else "port"
+        lib = oracle_lib.Reference() if kind == "reference" else oracle_lib.Oracle()
+        lib.set_threads(threads)
+        t0 = time.perf_counter()
+        with stdout_to_stderr():
+            ref = lib.modeling(*args)
+        out["cpu_ms_per_call"] = round((time.perf_counter() - t0) * 1e3, 1)
+        out["cpu"] = {"kind": kind, "cores": threads}
+        out["equals_cpu"] = bool(np.array_equal(ref, rec))
+    return out
 
 
 def lsrtm_bench(nshots, nt, iters, err_shots, err_iters):
@@ -229,6 +283,7 @@ def main():
     ap.add_argument("--lsrtm-shots", type=int, default=64, help="shots of the LSRTM timing (0 skips it)")
     ap.add_argument("--lsrtm-nt", type=int, default=3000)
     ap.add_argument("--lsrtm-iters", type=int, default=3)
+    ap.add_argument("--c1-calls", type=int, default=10, help="timed calls of the C1 single-shot case (0 skips it)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 8 device-to-device copies of 1 GiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -253,6 +308,11 @@ def main():
     import ctypes
 
     w = rtm_workload(args.shots, args.nt)
+    if not args.no_cpu and args.cpu_nt <= w["delay"]:
+        # the reference's engine exits the process when the wavelet delay
+        # does not fit the record
+        print("--cpu-nt must exceed the wavelet delay (%d samples)" % w["delay"], file=sys.stderr)
+        sys.exit(2)
     nx, nz, pml, nt = w["nx"], w["nz"], w["pml"], w["nt"]
     grid = Grid(nx, nz, pml, w["dt"], w["dx"], w["dz"])
     shots = [Shot(sx=sx, sz=w["sz"], nt=nt, ndel=0, model=0) for sx in w["sxs"]]
@@ -335,11 +395,14 @@ def main():
         "prep_s": round(prep_s, 2),
         "image_finite": bool(np.isfinite(img).all()),
     }
+    threads = min(16, len(os.sched_getaffinity(0)))
+    if args.c1_calls > 0:
+        log("C1 single-shot modeling, %d calls per mode" % args.c1_calls)
+        out["c1_single_shot"] = c1_single_shot(args.c1_calls, not args.no_cpu, threads)
     if args.lsrtm_shots > 0:
         del d_rec, d_img
         out["lsrtm"] = lsrtm_bench(args.lsrtm_shots, args.lsrtm_nt, args.lsrtm_iters, 2, 3)
     if not args.no_cpu:
-        threads = min(16, len(os.sched_getaffinity(0)))
         log("timing the CPU baseline on %d threads" % threads)
         cb = cpu_baseline(w, args.cpu_nt, threads)
         cb["gpu_over_cpu"] = round(value / cb["value"], 1)
