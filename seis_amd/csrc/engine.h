@@ -23,8 +23,9 @@ struct DevShot {
   T* fld;          // slot 0 grid origin; slot j at fld + j*fstride
   T* strips;       // [nt][8*(nx+nz)] (RTM)
   T* img;          // padded image accumulator, grid origin (RTM)
-  T* aux;          // two padded scratch fields, grid origin, the second at
-                   // aux + fstride (RTM: lap*2/v^3 of even / odd reverse steps)
+  T* aux;          // NAUX padded scratch fields, grid origin, field j at
+                   // aux + j*fstride (RTM: lap*2/v^3 of reverse step st in
+                   // field st % NAUX)
   void* snap;      // Born-source snapshots [nt+ndel][nx][nz] (LSRTM)
   float* scale;    // per-snapshot scale [nt+ndel] (LSRTM, 16-bit snapshots)
   T* grad;         // per-shot image gradient [nx][nz] (LSRTM)
@@ -35,6 +36,10 @@ struct DevShot {
   T* raw;          // stencil values of the band cells of the new level, grid
                    // origin; second field at raw + fstride
 };
+
+// U buffers of the RTM reverse pass: the imaging pass applies two
+// consecutive steps at once while the next two steps run.
+constexpr int NAUX = 4;
 
 // Step kernel modes.  MODEL / BORN / RTM_* follow the reference's modeling,
 // forward and adjoint (fd.cpp:15-177, 592-719); SNAP_* run the background of
@@ -130,7 +135,7 @@ struct StepCtx {
   Model<T> m;
   int it, itp;
   long long src;    // field index of the source cell
-  T* aux;           // RTM reverse: U of this step (one of two buffers)
+  T* aux;           // RTM reverse: U of this step (one of NAUX buffers)
 };
 
 // Wavefront-local LDS exchange barrier (the hardware orders a wave's LDS

@@ -243,7 +243,7 @@ __global__ __launch_bounds__(WG) void band_kernel(Consts<T> k, const DevShot<T>*
       u = band_blend(k, m, NX, C, P, sr, x, z, RA[i], RawNeighbour<T>{RA, N, ld, NX, NZ, pml + 1});
     const T lap = (sr.prev(P, i) + u - T(2) * C[i]) * k.idt2;
     const T v = m.vel(x, z);
-    s.aux[(step & 1) * s.fstride + i] = lap * (T(2) / (v * v * v));
+    s.aux[(step % NAUX) * s.fstride + i] = lap * (T(2) / (v * v * v));
     N[i] = u;
     N[f3 + i] = band_blend(k, m, NX, C + f3, P + f3, NoInject{}, x, z, RB[i],
                            RawNeighbour<T>{RB, N + f3, ld, NX, NZ, pml});
@@ -474,7 +474,7 @@ __global__ __launch_bounds__(WG) void step_kernel(Consts<T> k, const DevShot<T>*
   c.N = s.fld + r2 * s.fstride;
   c.m = Model<T>{s.vel, s.nx, k.nz, k.pml};
   c.src = fidx(k.ld, s.sx + k.pml, s.sz + k.pml);
-  c.aux = s.aux ? s.aux + (step & 1) * s.fstride : nullptr;
+  c.aux = s.aux ? s.aux + (step % NAUX) * s.fstride : nullptr;
   if ((int)blockIdx.x < L.nfwg) {
     const int wave = threadIdx.x >> 6;
     const int item = xcd_tile(blockIdx.x, L.nfwg) * WAVES + wave;
@@ -509,52 +509,74 @@ struct HilbertTaps {
   float h[61];
 };
 
+// One step's term at row z of a column held in LDS (sU, sR over NZ rows),
+// added to img in the reference's order.
+template <int HALF>
+__device__ __forceinline__ float image_term(const HilbertTaps& taps, const float* sU, const float* sR, int z, int NZ,
+                                            float img) {
+  float uh = 0.f, rh = 0.f;
+  if (z >= HALF && z + HALF <= NZ - 1) {
+    // full window: inputs z-HALF+1 .. z+HALF-1 at odd offsets, tap index
+    // z - i + HALF, summed in increasing i
+#pragma unroll
+    for (int j = 0; j < HALF; j++) {
+      const int i = z - HALF + 1 + 2 * j;
+      const float h = taps.h[2 * HALF - 1 - 2 * j];
+      uh += sU[i] * h;
+      rh += sR[i] * h;
+    }
+  } else {
+    const int i0 = max(0, z - HALF), i1 = min(NZ - 1, z + HALF);
+    for (int i = i0 + (((z - i0) & 1) ? 0 : 1); i <= i1; i += 2) {
+      const float h = taps.h[z - i + HALF];
+      uh += sU[i] * h;
+      rh += sR[i] * h;
+    }
+  }
+  const float scal = 2.0f / 4.0f;
+  const float ur = sU[z] * sR[z];
+  img += ur;
+  img += scal * (ur - uh * rh);
+  return img;
+}
+
+// Reverse steps `step` and, when `pair`, step + 1 applied to one interior
+// column (the image is read and written once for both).  U of step st is in
+// aux field st % NAUX; R of step st is in field slot 3 + (st + 2) % 3.
 template <int HALF>
 __global__ __launch_bounds__(128) void image_kernel(Consts<float> k, const DevShot<float>* __restrict__ shots,
-                                                    int step, HilbertTaps taps, int slot_off) {
+                                                    int step, HilbertTaps taps, int pair) {
   extern __shared__ unsigned char smem_raw[];
-  float* sU = reinterpret_cast<float*>(smem_raw);
+  float* sU0 = reinterpret_cast<float*>(smem_raw);
   const DevShot<float>& s = shots[blockIdx.y];
   const int x = k.pml + blockIdx.x;
   if (x >= s.NX - k.pml) return;
   const int it = s.nt + s.ndel - 1 - step;
   if (it < s.ndel) return;
+  const bool two = pair && it - 1 >= s.ndel;
   const int NZ = k.NZ;
-  float* sR = sU + NZ;
+  float* sR0 = sU0 + NZ;
+  float* sU1 = sR0 + NZ;
+  float* sR1 = sU1 + NZ;
   const long long base = fidx(k.ld, x, 0);
-  const auto U = SEIS_GL(s.aux + (step & 1) * s.fstride + base);
-  const auto R = SEIS_GL(s.fld + slot_off * s.fstride + base);
+  const auto U0 = SEIS_GL(s.aux + (step % NAUX) * s.fstride + base);
+  const auto R0 = SEIS_GL(s.fld + (3 + (step + 2) % 3) * s.fstride + base);
+  const auto U1 = SEIS_GL(s.aux + ((step + 1) % NAUX) * s.fstride + base);
+  const auto R1 = SEIS_GL(s.fld + (3 + (step + 3) % 3) * s.fstride + base);
   const auto IMG = SEIS_GL(s.img + base);
   for (int z = threadIdx.x; z < NZ; z += blockDim.x) {
-    sU[z] = U[z];
-    sR[z] = R[z];
+    sU0[z] = U0[z];
+    sR0[z] = R0[z];
+    if (two) {
+      sU1[z] = U1[z];
+      sR1[z] = R1[z];
+    }
   }
   __syncthreads();
-  const float scal = 2.0f / 4.0f;
   for (int z = k.pml + threadIdx.x; z < NZ - k.pml; z += blockDim.x) {
-    float uh = 0.f, rh = 0.f;
-    if (z >= HALF && z + HALF <= NZ - 1) {
-      // full window: inputs z-HALF+1 .. z+HALF-1 at odd offsets, tap index
-      // z - i + HALF, summed in increasing i
-#pragma unroll
-      for (int j = 0; j < HALF; j++) {
-        const int i = z - HALF + 1 + 2 * j;
-        const float h = taps.h[2 * HALF - 1 - 2 * j];
-        uh += sU[i] * h;
-        rh += sR[i] * h;
-      }
-    } else {
-      const int i0 = max(0, z - HALF), i1 = min(NZ - 1, z + HALF);
-      for (int i = i0 + (((z - i0) & 1) ? 0 : 1); i <= i1; i += 2) {
-        const float h = taps.h[z - i + HALF];
-        uh += sU[i] * h;
-        rh += sR[i] * h;
-      }
-    }
-    const float ur = sU[z] * sR[z];
     float img = IMG[z];
-    img += ur;
-    img += scal * (ur - uh * rh);
+    img = image_term<HALF>(taps, sU0, sR0, z, NZ, img);
+    if (two) img = image_term<HALF>(taps, sU1, sR1, z, NZ, img);
     IMG[z] = img;
   }
 }
@@ -704,7 +726,7 @@ int run_batch(int mode, const GridSpec& g, int nshots, const ShotSpec* shots, co
   // per shot: nslots fields, the band stencil values of one or two fields
   // (+ image and two U buffers for RTM), then strips
   const int nraw = (mode == RUN_MODEL) ? 1 : 2;
-  const size_t per_shot = (size_t)fe * (nslots + nraw + (rtm ? 3 : 0));
+  const size_t per_shot = (size_t)fe * (nslots + nraw + (rtm ? 1 + NAUX : 0));
   const size_t total = per_shot * nshots + strip_elems;
   BatchRun<T> br;
   br.pool_bytes = total * sizeof(T);
@@ -814,32 +836,39 @@ int run_batch(int mode, const GridSpec& g, int nshots, const ShotSpec* shots, co
     constexpr int half = 30;
     HilbertTaps taps{};
     hilbert_taps_host<float>(half, taps.h);
-    const size_t smem = sizeof(T) * 2 * NZ;
-    if (smem > 64 * 1024) return set_error("rtm: padded nz too large for the column Hilbert kernel"), SEIS_ERR_ARG;
+    const size_t smem1 = sizeof(T) * 2 * NZ;
+    if (smem1 > 64 * 1024) return set_error("rtm: padded nz too large for the column Hilbert kernel"), SEIS_ERR_ARG;
+    // two reverse steps per imaging pass where four columns fit in LDS
+    const int per = 2 * smem1 <= 64 * 1024 ? 2 : 1;
+    const size_t smem = per * smem1;
     int maxbwd = 0;
     for (int s = 0; s < nshots; s++) maxbwd = std::max(maxbwd, shots[s].nt);
     const double faqi_cells = (double)g.nx * NZ;
-    // The imaging pass of step st runs on a second stream, overlapping the
-    // reverse step st+1 (memory-bound against compute-bound).  Step st+1
-    // writes the other U buffer and does not touch the image; step st+2
-    // rewrites the U buffer of step st, so it waits for that imaging pass
-    // (which also covers the receiver slot reused at st+3).
+    // Imaging pass q covers reverse steps [per q, per q + per) and runs on a
+    // second stream after the last of them, beside the next reverse step.
+    // Step st rewrites the receiver slot of step st-3 (and the U buffer of
+    // step st-4), so it first waits for the pass that covers step st-3.
     ImageStream is;
     if ((rc = is.init())) return rc;
     for (int st = 0; st < maxbwd; st++) {
       const int n = active_bwd(st);
-      if (st >= 2) HIPCHECK(hipStreamWaitEvent(stream, is.done[st % ImageStream::NEV], 0));
+      if (st >= 3) HIPCHECK(hipStreamWaitEvent(stream, is.done[((st - 3) / per) % ImageStream::NEV], 0));
       timed_launch(stream, "step_rtm_reverse", 8 * E * n * cells + strip * n, 2 * n * cells, [&] {
         launch_step<T>(RTM_BWD, k, br.dev, L, B, nshots, st, stream);
       });
-      HIPCHECK(hipEventRecord(is.ready[st % ImageStream::NEV], stream));
-      HIPCHECK(hipStreamWaitEvent(is.s, is.ready[st % ImageStream::NEV], 0));
-      // the receiver field's new level lives in slot 3 + (st+2)%3
-      timed_launch(is.s, "rtm_image", 2 * E * n * faqi_cells + 2 * E * n * (double)g.nx * g.nz, 0, [&] {
-        hipLaunchKernelGGL((image_kernel<half>), dim3(g.nx, nshots), dim3(128), smem, is.s, k, br.dev, st, taps,
-                           3 + (st + 2) % 3);
-      });
-      HIPCHECK(hipEventRecord(is.done[(st + 2) % ImageStream::NEV], is.s));
+      if ((st + 1) % per != 0 && st != maxbwd - 1) continue;
+      const int q = st / per, first = q * per;
+      const int pair = st > first;
+      const int n0 = active_bwd(first);
+      HIPCHECK(hipEventRecord(is.ready[q % ImageStream::NEV], stream));
+      HIPCHECK(hipStreamWaitEvent(is.s, is.ready[q % ImageStream::NEV], 0));
+      // per step U and R over the interior columns; the image once
+      timed_launch(is.s, "rtm_image", 2 * E * faqi_cells * (n0 + (pair ? n : 0)) + 2 * E * n0 * (double)g.nx * g.nz,
+                   0, [&] {
+                     hipLaunchKernelGGL((image_kernel<half>), dim3(g.nx, nshots), dim3(128), smem, is.s, k, br.dev,
+                                        first, taps, pair);
+                   });
+      HIPCHECK(hipEventRecord(is.done[q % ImageStream::NEV], is.s));
     }
     HIPCHECK(hipEventRecord(is.ready[0], is.s));
     HIPCHECK(hipStreamWaitEvent(stream, is.ready[0], 0));

@@ -122,3 +122,23 @@ def test_rtm_isolated_band_equals_reference_interior(oracle):
     assert np.isfinite(img).all()
     assert np.array_equal(img[m:nx - m, m:nz - m], ref[m:nx - m, m:nz - m])
     assert not np.array_equal(img, ref)  # the edge rows do see the change
+
+
+def test_rtm_odd_record_lengths_match_oracle(engine, oracle):
+    """The imaging pass applies two reverse steps at a time: shots whose
+    record ends inside a pair (odd nt next to even nt) and a batch whose
+    longest record is odd (a final single step) image as the reference does."""
+    from seis_amd.batch import Grid, Shot, rtm_batch
+    print("case rtm_odd_nt", flush=True)
+    rng = np.random.default_rng(17)
+    nx, nz, pml, dt = 40, 36, 8, 0.001
+    g = Grid(nx, nz, pml, dt, 10.0, 10.0)
+    v = (1900.0 + rng.uniform(0, 500, (nx, nz))).astype(np.float32)
+    shots = [Shot(sx=int(rng.integers(0, nx)), sz=1, nt=nt, ndel=nd) for nt, nd in [(101, 0), (64, 3), (77, 6)]]
+    wavs = [oracle.ricker(dt, 25.0, 15 + s.ndel, s.nt + s.ndel) for s in shots]
+    gzs = [rng.integers(0, 4, nx).astype(np.int32) for _ in shots]
+    recs = [rng.standard_normal((nx, s.nt)).astype(np.float32) for s in shots]
+    imgs = rtm_batch(g, shots, v, wavs, gzs, recs)
+    for j, s in enumerate(shots):
+        ref = oracle.rtm(dt, 10.0, 10.0, s.nt, s.ndel, nx, nz, pml, s.sx, s.sz, gzs[j], wavs[j], v, recs[j])
+        assert np.array_equal(imgs[j], ref), j
