@@ -81,6 +81,13 @@ int seis_born(float dt, float dx, float dz, int nt, int ndel, int nx, int nz, in
 int seis_rtm(float dt, float dx, float dz, int nt, int ndel, int nx, int nz, int pml, int sx, int sz,
              const int* gz, const float* wav, const float* vel, const float* rec, float* img);
 
+/* Source illumination; replaces `illum` (fd.h:10, fd.cpp:275-339): the
+ * modeling loop without receivers, ss[ix][iz] = sum over steps it > ndel of
+ * u^2 after the absorbing boundary.  wav[nt+ndel], vel[nx][nz] in;
+ * ss[nx][nz] out. */
+int seis_illum(float dt, float dx, float dz, int nt, int ndel, int nx, int nz, int pml, int sx, int sz,
+               const float* wav, const float* vel, float* ss);
+
 /* Ricker wavelet of the reference (wavelet.cpp:7-18): out[i] for i < n,
  * peak frequency fr, peak at sample num_delay. */
 int seis_ricker(float dt, float fr, int num_delay, int n, float* out);
@@ -93,8 +100,10 @@ int seis_ricker(float dt, float fr, int num_delay, int n, float* out);
  *   "rtm"       image of the <precsg> records       (adjoint_MPI, RTM_IMG)
  *   "rtm_born"  image of the <wdir>BORN records     (adjoint_MPI, LSRTM_STEP)
  *   "lsrtm"     least-squares RTM, maxiter CG iterations
- * Images are written to out_file (default <wdir>RTM_IMAGE.dat or
- * <wdir>LSRTM_IMAGE.dat) as float32 [nx][nz]. */
+ *   "illum"     source illumination of every shot -> <wdir>ILLUM<is>.dat,
+ *               and their sum (illum_MPI)
+ * Images are written to out_file (default <wdir>RTM_IMAGE.dat,
+ * <wdir>LSRTM_IMAGE.dat or <wdir>ILLUM.dat) as float32 [nx][nz]. */
 int seis_run(const char* param_file, const char* mode, const char* model_file, const char* out_file);
 
 /* Parses a parameter file like Inversion::readParamFile (inversion.cpp:38-79,
@@ -115,6 +124,12 @@ int seis_born_batch(const seis_grid* grid, int nshots, const seis_shot* shots, c
  * CSG_IMAGE<is> of inversion.cpp:434-439). */
 int seis_rtm_batch(const seis_grid* grid, int nshots, const seis_shot* shots, const float* vel,
                    const float* wav, const int* gz, const float* rec, float* img);
+
+/* ss: nshots consecutive [nx][nz] illuminations, one per shot (the
+ * per-shot ILLUM<is> of illum_MPI, inversion.cpp:207-290, which sums them);
+ * the shots' gz and rec offsets are not used. */
+int seis_illum_batch(const seis_grid* grid, int nshots, const seis_shot* shots, const float* vel,
+                     const float* wav, float* ss);
 
 /* seis_rtm_batch with option bits; flags = 0 is seis_rtm_batch.
  * SEIS_RTM_ISOLATE_BAND: the reference reconstructs the source wavefield

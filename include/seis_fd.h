@@ -1,5 +1,5 @@
 // seis_fd.h -- C++ drop-in for the reference's engine header (bbsun/Seis
-// src/util/fd.h:3-5).  Same names, argument order and meaning; the result is
+// src/util/fd.h:3-5, 10).  Same names, argument order and meaning; the result is
 // returned as a newly allocated 2D array laid out like MyAlloc<float>::alc
 // (memory.h:29-44: a table of n2 row pointers into one zeroed block of n1*n2
 // floats), so the reference's MyAlloc<float>::free (memory.h:103-110: delete[]
@@ -19,7 +19,12 @@ float** forward(float dt, float dx, float dz, int nt, int ndel, int nx, int nz, 
 float** adjoint(float dt, float dx, float dz, int nt, int ndel, int nx, int nz, int pml, int sx, int sz, int* gz,
                 float* wav, float** vel, float** rec);
 
-// Releases a result of the three functions above exactly as the reference's
+// ss[nx][nz] = source illumination, sum of u^2 over steps it > ndel
+// (fd.cpp:275-339)
+float** illum(float dt, float dx, float dz, int nt, int ndel, int nx, int nz, int pml, int sx, int sz, float* wav,
+              float** vel);
+
+// Releases a result of the functions above exactly as the reference's
 // MyAlloc<float>::free does (delete[] buf[0]; delete[] buf;), for callers that
 // are not C++ (no counterpart in the reference; NULL is ignored).
 extern "C" void seis_fd_free(float** buf);

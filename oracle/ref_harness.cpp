@@ -6,7 +6,7 @@
 // It exists to generate golden vectors (tests/golden/make_golden.py) and to
 // serve as the timed "reference" CPU baseline.  It converts flat arrays
 // to the reference's row-pointer tables (memory.h:29-44), calls
-// modeling / forward / adjoint (fd.h:3-5) and copies the newly allocated
+// modeling / forward / adjoint / illum (fd.h:3-5,10) and copies the newly allocated
 // result back, freeing it the reference's way (memory.h:103-110).
 
 #include <cstring>
@@ -69,6 +69,13 @@ int ref_modeling(float dt, float dx, float dz, int nt, int ndel, int nx, int nz,
   Rows v(vel, nz, nx);
   take(modeling(dt, dx, dz, nt, ndel, nx, nz, pml, sx, sz, const_cast<int*>(gz), const_cast<float*>(wav), v.p),
        rec, (size_t)nt * nx);
+  return 0;
+}
+
+int ref_illum(float dt, float dx, float dz, int nt, int ndel, int nx, int nz, int pml, int sx, int sz,
+              const float* wav, const float* vel, float* ss) {
+  Rows v(vel, nz, nx);
+  take(illum(dt, dx, dz, nt, ndel, nx, nz, pml, sx, sz, const_cast<float*>(wav), v.p), ss, (size_t)nx * nz);
   return 0;
 }
 

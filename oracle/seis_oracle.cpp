@@ -365,6 +365,35 @@ void modeling(T dt, T dx, T dz, int nt, int ndel, int nx, int nz, int pml, int s
   }
 }
 
+// Source illumination, fd.cpp:275-339: the modeling loop without receivers;
+// after the blend of every step it > ndel (strictly), ss += u2 * u2 at every
+// cell.  The reference sums over the padded grid and returns the interior;
+// only the interior is accumulated here.  ss is [nx][nz].
+template <typename T>
+void illum(T dt, T dx, T dz, int nt, int ndel, int nx, int nz, int pml, int sx, int sz, const T* wav, const T* v,
+           T* ss) {
+  Geom g(nx, nz, pml);
+  Coefs<T> k(g, dt, dx, dz, v, false);
+  std::vector<T> a(g.cells()), b(g.cells()), c(g.cells()), ow(g.cells());
+  T *u0 = a.data(), *u1 = b.data(), *u2 = c.data();
+  const size_t src = g.at(sx + pml, sz + pml);
+  for (size_t i = 0; i < (size_t)nx * nz; i++) ss[i] = T(0);
+  const int NT = nt + ndel;
+  for (int it = 0; it < NT; it++) {
+    stencil_step(g, u0, u1, u2, k.vvzz.data(), k.vvxx.data());
+    if (it - 1 >= 0) u2[src] += k.vt2[src] * wav[it - 1];
+    habc_update(g, dt, dx, dz, k.vel.data(), ow.data(), u2, u1, u0);
+    blend(g, u2, ow.data(), k.w.data());
+    if (it > ndel)
+      for (int ix = 0; ix < nx; ix++)
+        for (int iz = 0; iz < nz; iz++) {
+          const T u = u2[g.at(ix + pml, iz + pml)];
+          ss[(size_t)ix * nz + iz] += u * u;
+        }
+    T* t = u0; u0 = u1; u1 = u2; u2 = t;
+  }
+}
+
 // Born (linearized) modeling, fd.cpp:81-177: the background field drives a
 // scattered field through vt2 * lap(u) * 2/v^3 * dv, lap taken from the
 // background's three latest levels; the scattered field is recorded.
@@ -733,6 +762,11 @@ int oracle_get_threads(void) { return g_threads; }
   int oracle_modeling_##SUF(T dt, T dx, T dz, int nt, int ndel, int nx, int nz, int pml, int sx,      \
                             int sz, const int* gz, const T* wav, const T* vel, T* rec) {             \
     modeling<T>(dt, dx, dz, nt, ndel, nx, nz, pml, sx, sz, gz, wav, vel, rec);                        \
+    return 0;                                                                                         \
+  }                                                                                                   \
+  int oracle_illum_##SUF(T dt, T dx, T dz, int nt, int ndel, int nx, int nz, int pml, int sx, int sz, \
+                         const T* wav, const T* vel, T* ss) {                                         \
+    illum<T>(dt, dx, dz, nt, ndel, nx, nz, pml, sx, sz, wav, vel, ss);                               \
     return 0;                                                                                         \
   }                                                                                                   \
   int oracle_born_##SUF(T dt, T dx, T dz, int nt, int ndel, int nx, int nz, int pml, int sx, int sz,  \

@@ -121,3 +121,16 @@ def rtm_batch(grid, shots, vel, wavs, gzs, recs, isolate_band=False):
     rec = np.concatenate([np.ascontiguousarray(r, np.float32).ravel() for r in recs])
     return _run("rtm", grid, shots, vel, None, wav, gz, None, rec_in=rec,
                 flags=RTM_ISOLATE_BAND if isolate_band else 0)
+
+
+def illum_batch(grid, shots, vel, wavs):
+    """Source illumination of every shot: [nshots][nx][nz] (illum_MPI's
+    per-shot ILLUM<is>, which it then sums)."""
+    wav = np.concatenate([np.asarray(w, np.float32)[: s.nt + s.ndel] for w, s in zip(wavs, shots)])
+    arr, nw, ng, nr = _layout(grid, shots)
+    dvel, kv = _dev(vel, np.float32)
+    dwav, kw = _dev(wav, np.float32)
+    dss = DeviceArray((len(shots) * grid.nx * grid.nz,), np.float32, zero=True)
+    g = grid.c()
+    check(lib().seis_illum_batch(ctypes.byref(g), len(shots), arr, dvel.ptr, dwav.ptr, dss.ptr))
+    return dss.download().reshape(len(shots), grid.nx, grid.nz)

@@ -185,6 +185,34 @@ int seis_rtm(float dt, float dx, float dz, int nt, int ndel, int nx, int nz, int
   return single(RUN_RTM, dt, dx, dz, nt, ndel, nx, nz, pml, sx, sz, gz, wav, vel, rec, img);
 }
 
+int seis_illum(float dt, float dx, float dz, int nt, int ndel, int nx, int nz, int pml, int sx, int sz,
+               const float* wav, const float* vel, float* ss) {
+  if (!wav || !vel || !ss) return set_error("null array argument"), SEIS_ERR_ARG;
+  if (nt <= 0 || ndel < 0 || nx <= 0 || nz <= 0 || pml < 0)
+    return set_error("nt, nx, nz must be > 0; ndel, pml >= 0"), SEIS_ERR_ARG;
+  if (sx < 0 || sx >= nx || sz < 0 || sz >= nz) return set_error("source cell outside the model"), SEIS_ERR_ARG;
+  const size_t nm = (size_t)nx * nz;
+  DevBuf dwav, dvel, dss;
+  int rc;
+  if ((rc = upload(dwav, wav, sizeof(float) * (nt + ndel)))) return rc;
+  if ((rc = upload(dvel, vel, sizeof(float) * nm))) return rc;
+  if ((rc = alloc(dss, sizeof(float) * nm))) return rc;
+  seis_grid g{nx, nz, pml, dt, dx, dz};
+  seis_shot s{sx, sz, nt, ndel, 0, 0, 0, 0};
+  rc = run_batch<float>(RUN_ILLUM, g, 1, &s, (const float*)dvel.p, nullptr, (const float*)dwav.p, nullptr, nullptr,
+                        (float*)dss.p, 0);
+  if (rc) return rc;
+  hipError_t e = hipMemcpy(ss, dss.p, sizeof(float) * nm, hipMemcpyDeviceToHost);
+  return e == hipSuccess ? SEIS_OK : hip_fail("hipMemcpy", e);
+}
+
+int seis_illum_batch(const seis_grid* grid, int nshots, const seis_shot* shots, const float* vel, const float* wav,
+                     float* ss) {
+  int rc = check_batch(grid, nshots, shots);
+  if (rc) return rc;
+  return run_batch<float>(RUN_ILLUM, *grid, nshots, shots, vel, nullptr, wav, nullptr, nullptr, ss, 0);
+}
+
 int seis_modeling_batch(const seis_grid* grid, int nshots, const seis_shot* shots, const float* vel,
                         const float* wav, const int* gz, float* rec) {
   int rc = check_batch(grid, nshots, shots);

@@ -55,10 +55,14 @@ int seis_run(const char* param_file, const char* mode, const char* model_file, c
     if (!inv.adjoint(img.data(), m == "rtm" ? RTM_IMG : LSRTM_STEP)) return set_error(inv.error()), SEIS_ERR_HIP;
   } else if (m == "lsrtm") {
     if (!inv.cgsolver(p.maxiter.val, 0.f, img.data())) return set_error(inv.error()), SEIS_ERR_HIP;
+  } else if (m == "illum") {
+    if (!inv.illum(img.data())) return set_error(inv.error()), SEIS_ERR_HIP;
   } else {
-    return set_error("seis_run: mode must be modeling, born, rtm, rtm_born or lsrtm"), SEIS_ERR_ARG;
+    return set_error("seis_run: mode must be modeling, born, rtm, rtm_born, lsrtm or illum"), SEIS_ERR_ARG;
   }
-  const std::string f = out_file ? out_file : p.wdir.val + (m == "lsrtm" ? "LSRTM_IMAGE.dat" : "RTM_IMAGE.dat");
+  const std::string f = out_file ? out_file
+                                 : p.wdir.val + (m == "lsrtm" ? "LSRTM_IMAGE.dat" : m == "illum" ? "ILLUM.dat"
+                                                                                               : "RTM_IMAGE.dat");
   if (!write_floats(f, n, img.data(), err)) return set_error(err), SEIS_ERR_ARG;
   return SEIS_OK;
 }

@@ -78,4 +78,12 @@ float** adjoint(float dt, float dx, float dz, int nt, int ndel, int nx, int nz, 
   return finish(seis_rtm(dt, dx, dz, nt, ndel, nx, nz, pml, sx, sz, gz, wav, v.p, r.p, img[0]), img);
 }
 
+float** illum(float dt, float dx, float dz, int nt, int ndel, int nx, int nz, int pml, int sx, int sz, float* wav,
+              float** vel) {
+  Flat v(vel, nz, nx);
+  float** ss = alloc2(nz, nx);
+  if (!ss) return nullptr;
+  return finish(seis_illum(dt, dx, dz, nt, ndel, nx, nz, pml, sx, sz, wav, v.p, ss[0]), ss);
+}
+
 extern "C" void seis_fd_free(float** buf) { free2(buf); }

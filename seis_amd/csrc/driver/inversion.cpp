@@ -67,9 +67,9 @@ bool Inversion::read_model(const std::string& file, std::vector<float>& v) {
 namespace {
 
 // Runs one of the engine's batched calls for shots grouped by window width.
-// `kind`: 0 modeling, 1 Born, 2 RTM.  models / perts are global [nx][nz];
-// records are per shot [nxw][nt] on input (RTM) or output (modeling, Born);
-// images per shot [nxw][nz] (RTM).
+// `kind`: 0 modeling, 1 Born, 2 RTM, 3 illumination.  models / perts are
+// global [nx][nz]; records are per shot [nxw][nt] on input (RTM) or output
+// (modeling, Born); images per shot [nxw][nz] (RTM, illumination).
 bool run_windows(const Params& p, int kind, std::vector<ShotJob>& jobs, const float* model, const float* pert,
                  std::map<int, std::vector<float>>& recs, std::map<int, std::vector<float>>& imgs, std::string& err) {
   const int nz = p.nz.val, nt = p.nt.val, ndel = p.delaycal.val;
@@ -101,7 +101,7 @@ bool run_windows(const Params& p, int kind, std::vector<ShotJob>& jobs, const fl
     if (!dv.put(vm.data(), sizeof(float) * vm.size()) || !dw.put(wv.data(), sizeof(float) * wv.size()) ||
         !dg.put(gz.data(), sizeof(int) * gz.size()) || !dr.put(rc.data(), sizeof(float) * rc.size()) ||
         (kind == 1 && !dd.put(dm.data(), sizeof(float) * dm.size())) ||
-        (kind == 2 && !di.alloc(sizeof(float) * M * n))) {
+        (kind >= 2 && !di.alloc(sizeof(float) * M * n))) {
       err = gpu_error("device buffers");
       return false;
     }
@@ -110,13 +110,15 @@ bool run_windows(const Params& p, int kind, std::vector<ShotJob>& jobs, const fl
       rcode = seis_modeling_batch(&g, n, sh.data(), (float*)dv.p, (float*)dw.p, (int*)dg.p, (float*)dr.p);
     else if (kind == 1)
       rcode = seis_born_batch(&g, n, sh.data(), (float*)dv.p, (float*)dd.p, (float*)dw.p, (int*)dg.p, (float*)dr.p);
-    else
+    else if (kind == 2)
       rcode = seis_rtm_batch(&g, n, sh.data(), (float*)dv.p, (float*)dw.p, (int*)dg.p, (float*)dr.p, (float*)di.p);
+    else
+      rcode = seis_illum_batch(&g, n, sh.data(), (float*)dv.p, (float*)dw.p, (float*)di.p);
     if (rcode) {
       err = gpu_error("engine batch");
       return false;
     }
-    if (kind == 2) {
+    if (kind >= 2) {
       std::vector<float> im(M * n);
       if (!di.get(im.data(), sizeof(float) * im.size())) return err = gpu_error("download"), false;
       for (int j = 0; j < n; j++) imgs[jobs[idx[j]].is].assign(im.begin() + M * j, im.begin() + M * (j + 1));
@@ -211,6 +213,25 @@ bool Inversion::adjoint(float* img, int migTag) {
   for (int is = 0; is < ns; is++) {
     if (!read_floats(shot_image_name(param, is), g.size(), g.data(), err_)) return false;
     for (size_t i = 0; i < g.size(); i++) img[i] = g[i] + img[i];
+  }
+  return true;
+}
+
+bool Inversion::illum(float* ss) {
+  std::vector<float> v0;
+  if (!read_model(param.v0file.val, v0)) return false;
+  const int nx = param.nx.val, nz = param.nz.val, ns = survey.ns();
+  std::vector<ShotJob> jobs(ns);
+  for (int is = 0; is < ns; is++)
+    if (!make_job(param, survey, is, jobs[is], err_)) return false;
+  std::map<int, std::vector<float>> recs, imgs;
+  if (ns > 0 && !run_windows(param, 3, jobs, v0.data(), nullptr, recs, imgs, err_)) return false;
+  std::fill(ss, ss + (size_t)nx * nz, 0.f);
+  std::vector<float> g((size_t)nx * nz);
+  for (ShotJob& q : jobs) {
+    window_to_global(param, imgs[q.is].data(), g.data(), q.w.fx, param.velfx.val, q.w.nx, nx, nz, false);
+    if (!write_floats(illum_name(param, q.is), g.size(), g.data(), err_)) return false;
+    for (size_t i = 0; i < g.size(); i++) ss[i] = g[i] + ss[i];
   }
   return true;
 }

@@ -33,6 +33,10 @@ class Inversion {
   // <wdir>CSG_IMAGE<is>.dat, then all ns per-shot images are summed
   // (adjoint_MPI, inversion.cpp:365-474).
   bool adjoint(float* img, int migTag);
+  // Source illumination [nx][nz] of every shot's window through the v0file
+  // background -> <wdir>ILLUM<is>.dat, summed over all ns shots in shot order
+  // (illum_MPI, inversion.cpp:207-290).
+  bool illum(float* ss);
   // Least-squares RTM of the CSG records by CG on the exact Born operator
   // (the declared conope_MPI / cgsolver_MPI, inversion.h:81,87): maxIter
   // iterations or until ||L^T r|| / ||L^T d|| < reError.  img [nx][nz].

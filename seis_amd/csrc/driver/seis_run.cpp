@@ -4,6 +4,7 @@
 //   seis_run <param file> rtm [image file]        RTM image of the shot records
 //   seis_run <param file> rtm_born [image file]   RTM image of the Born records
 //   seis_run <param file> lsrtm [image file]      least-squares RTM (maxiter CG steps)
+//   seis_run <param file> illum [image file]      source illumination (per shot <wdir>ILLUM<is>.dat, summed)
 // Exit status is 0 on success and 1 on error (the reference exits 0 on errors).
 #include <cstdio>
 #include <cstring>
@@ -13,7 +14,7 @@
 
 int main(int argc, char** argv) {
   if (argc < 3) {
-    std::fprintf(stderr, "usage: %s <param file> <modeling|born|rtm|rtm_born|lsrtm> [file]\n", argv[0]);
+    std::fprintf(stderr, "usage: %s <param file> <modeling|born|rtm|rtm_born|lsrtm|illum> [file]\n", argv[0]);
     return 1;
   }
   const std::string mode = argv[2];

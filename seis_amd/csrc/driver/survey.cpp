@@ -257,5 +257,6 @@ std::string join(const std::string& a, const char* b, int is) {
 std::string csg_name(const Params& p, int is) { return join(p.precsg.val, "", is); }
 std::string born_name(const Params& p, int is) { return join(p.wdir.val, "BORN", is); }
 std::string shot_image_name(const Params& p, int is) { return join(p.wdir.val, "CSG_IMAGE", is); }
+std::string illum_name(const Params& p, int is) { return join(p.wdir.val, "ILLUM", is); }
 
 }  // namespace seis

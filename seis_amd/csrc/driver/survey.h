@@ -67,5 +67,6 @@ bool remove_direct(const Params& p, const Survey& s, int is, float* csg, const f
 std::string csg_name(const Params& p, int is);         // <precsg><is>.dat
 std::string born_name(const Params& p, int is);        // <wdir>BORN<is>.dat
 std::string shot_image_name(const Params& p, int is);  // <wdir>CSG_IMAGE<is>.dat
+std::string illum_name(const Params& p, int is);       // <wdir>ILLUM<is>.dat
 
 }  // namespace seis

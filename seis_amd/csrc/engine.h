@@ -22,7 +22,7 @@ struct DevShot {
   T* rec;          // [nx][nt]
   T* fld;          // slot 0 grid origin; slot j at fld + j*fstride
   T* strips;       // [nt][8*(nx+nz)] (RTM)
-  T* img;          // padded image accumulator, grid origin (RTM)
+  T* img;          // padded image (RTM) or illumination (ILLUM) accumulator, grid origin
   T* aux;          // NAUX padded scratch fields, grid origin, field j at
                    // aux + j*fstride (RTM: lap*2/v^3 of reverse step st in
                    // field st % NAUX)
@@ -44,8 +44,10 @@ constexpr int NAUX = 4;
 // Step kernel modes.  MODEL / BORN / RTM_* follow the reference's modeling,
 // forward and adjoint (fd.cpp:15-177, 592-719); SNAP_* run the background of
 // the Born operator and record its per-step source coefficient; BORN_S
-// propagates the scattered field from those records.
-enum Mode { MODEL = 0, BORN = 1, RTM_FWD = 2, RTM_BWD = 3, SNAP_MAX = 4, SNAP_STORE = 5, BORN_S = 6 };
+// propagates the scattered field from those records; ILLUM is the source
+// illumination of illum (fd.cpp:275-339): modeling without receivers that
+// accumulates u^2 of the model cells into DevShot::img.
+enum Mode { MODEL = 0, BORN = 1, RTM_FWD = 2, RTM_BWD = 3, SNAP_MAX = 4, SNAP_STORE = 5, BORN_S = 6, ILLUM = 7 };
 
 // Storage of the Born-source snapshots vt2 * lap * 2/v^3 (model cells only).
 // 16-bit forms store value / scale[step], with scale[step] the step's max |value|.

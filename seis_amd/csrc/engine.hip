@@ -91,11 +91,14 @@ __device__ __forceinline__ void cell_update(const Consts<T>& k, const DevShot<T>
     N[i] = u;
     const int ix = x - k.pml;
     if (itp >= 0 && ix >= 0 && ix < s.nx && z == s.gz[ix] + k.pml) s.rec[(long long)ix * s.nt + itp] = u;
-  } else if (MODE == MODEL || MODE == RTM_FWD) {
+  } else if (MODE == MODEL || MODE == RTM_FWD || MODE == ILLUM) {
     PointSource<T> ps{it - 1 >= 0 ? c.src : -1, it - 1 >= 0 ? s.wav[it - 1] : T(0)};
     const T u = final_value(k, m, NX, C, P, ps, x, z);
     N[i] = u;
-    if (MODE == MODEL) {
+    if (MODE == ILLUM) {
+      const int ix = x - k.pml, iz = z - k.pml;
+      if (it > s.ndel && ix >= 0 && ix < s.nx && iz >= 0 && iz < k.nz) s.img[i] += u * u;
+    } else if (MODE == MODEL) {
       const int ix = x - k.pml;
       if (itp >= 0 && ix >= 0 && ix < s.nx && z == s.gz[ix] + k.pml) s.rec[(long long)ix * s.nt + itp] = u;
     } else if (itp >= 0) {
@@ -311,8 +314,10 @@ __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot
   // The receiver depth of the column (and, in the reverse pass, its record
   // sample) is fetched the same way.
   constexpr bool GZ = (MODE == MODEL || MODE == BORN || MODE == BORN_S || MODE == RTM_BWD);
-  T nca, ncb = T(0), npa, npb = T(0), nv, nha = T(0), nhb = T(0), nrec = T(0);
+  T nca, ncb = T(0), npa, npb = T(0), nv, nha = T(0), nhb = T(0), nrec = T(0), nss = T(0);
   int ngz = -1;
+  const bool acc = MODE == ILLUM && c.it > s.ndel && act;  // accumulate u^2 (interior cells are model cells)
+  const auto SSA = SEIS_GL(s.img);
   auto fetch = [&](int xx) {
     xx = min(xx, x1 - 1);
     const long long r = (long long)xx * ld;
@@ -327,6 +332,7 @@ __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot
     }
     if (GZ) ngz = GZA[xx - pml];
     if (MODE == RTM_BWD) nrec = REC[(long long)(xx - pml) * s.nt + c.itp];
+    if (acc) nss = SSA[r + zl];
   };
   fetch(x0);
   // per-step scalars
@@ -350,7 +356,7 @@ __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot
     if (TWO) lb[lane + 4] = qb[4];
     const int ix = x - pml;
     const T v = nv;
-    const T pa_cur = npa, pb_cur = npb, rec_cur = nrec;
+    const T pa_cur = npa, pb_cur = npb, rec_cur = nrec, ss_cur = nss;
     const int gzr = ngz + pml;  // receiver row of this column
     fetch(x + 1);
     const T av = k.dt * v;
@@ -409,9 +415,11 @@ __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot
         if (d != T(0)) ua = ua + snap_load(s, k.nz, c.it, j) * d;
         NA[i] = ua;
         if (c.itp >= 0 && z == gzr) REC[(long long)ix * s.nt + c.itp] = ua;
-      } else if (MODE == MODEL || MODE == RTM_FWD) {
+      } else if (MODE == MODEL || MODE == RTM_FWD || MODE == ILLUM) {
         NA[i] = ua;
-        if (MODE == MODEL) {
+        if (MODE == ILLUM) {
+          if (acc) SSA[i] = ss_cur + ua * ua;
+        } else if (MODE == MODEL) {
           if (c.itp >= 0 && z == gzr) REC[(long long)ix * s.nt + c.itp] = ua;
         } else if (c.itp >= 0) {
           int sl[4];
@@ -633,6 +641,7 @@ void launch_step(int mode, const Consts<T>& k, const DevShot<T>* dev, const Layo
     SEIS_CASE(SNAP_MAX)
     SEIS_CASE(SNAP_STORE)
     SEIS_CASE(BORN_S)
+    SEIS_CASE(ILLUM)
 #undef SEIS_CASE
   }
 }
@@ -708,7 +717,8 @@ int run_batch(int mode, const GridSpec& g, int nshots, const ShotSpec* shots, co
   const int NX = g.nx + 2 * g.pml;
   const long long ld = field_ld(NZ);
   const long long fe = field_elems(NX, ld);
-  const int nslots = (mode == RUN_MODEL) ? 3 : 6;
+  const bool ilm = (mode == RUN_ILLUM);
+  const int nslots = (mode == RUN_MODEL || ilm) ? 3 : 6;
   const bool rtm = (mode == RUN_RTM);
   if (rtm && (g.nx < 4 || g.nz < 4))
     return set_error("rtm: the 4-layer boundary strips need nx >= 4 and nz >= 4"), SEIS_ERR_ARG;
@@ -724,9 +734,10 @@ int run_batch(int mode, const GridSpec& g, int nshots, const ShotSpec* shots, co
     if (rtm) strip_elems += (size_t)q.nt * 8 * (g.nx + g.nz);
   }
   // per shot: nslots fields, the band stencil values of one or two fields
-  // (+ image and two U buffers for RTM), then strips
-  const int nraw = (mode == RUN_MODEL) ? 1 : 2;
-  const size_t per_shot = (size_t)fe * (nslots + nraw + (rtm ? 1 + NAUX : 0));
+  // (+ image and NAUX U buffers for RTM, + the illumination for ILLUM), then
+  // strips
+  const int nraw = (mode == RUN_MODEL || ilm) ? 1 : 2;
+  const size_t per_shot = (size_t)fe * (nslots + nraw + (rtm ? 1 + NAUX : ilm ? 1 : 0));
   const size_t total = per_shot * nshots + strip_elems;
   BatchRun<T> br;
   br.pool_bytes = total * sizeof(T);
@@ -745,12 +756,12 @@ int run_batch(int mode, const GridSpec& g, int nshots, const ShotSpec* shots, co
     d.vel = vel + q.model;
     d.dv = dv ? dv + q.model : nullptr;
     d.wav = wav + q.wav;
-    d.gz = gz + q.gz;
-    d.rec = rec + q.rec;
+    d.gz = gz ? gz + q.gz : nullptr;  // illumination has no receivers
+    d.rec = rec ? rec + q.rec : nullptr;
     d.fld = base + origin;
     d.fstride = fe;
     d.raw = base + (size_t)fe * nslots + origin;
-    d.img = rtm ? base + (size_t)fe * (nslots + nraw) + origin : nullptr;
+    d.img = (rtm || ilm) ? base + (size_t)fe * (nslots + nraw) + origin : nullptr;
     d.aux = rtm ? base + (size_t)fe * (nslots + nraw + 1) + origin : nullptr;
     d.strips = rtm ? strip_p : nullptr;
     if (rtm) strip_p += (size_t)q.nt * 8 * (g.nx + g.nz);
@@ -780,13 +791,14 @@ int run_batch(int mode, const GridSpec& g, int nshots, const ShotSpec* shots, co
       int nshots;
       hipStream_t s;
     };
-    const int proxy = mode == RUN_MODEL ? MODEL : mode == RUN_BORN ? BORN : RTM_FWD;
+    // (an ILLUM trial at step 0 accumulates nothing: step 0 never exceeds ndel)
+    const int proxy = mode == RUN_MODEL ? MODEL : mode == RUN_BORN ? BORN : ilm ? ILLUM : RTM_FWD;
     Trial tr{proxy, &k, br.dev, L, B, nshots, stream};
     auto fn = [](void* c, int bx) {
       const Trial* t = static_cast<const Trial*>(c);
       launch_step<T>(t->mode, *t->k, t->dev, with_strip(t->L, bx), t->B, t->nshots, 0, t->s);
     };
-    const char* kind = mode == RUN_MODEL ? "forward" : mode == RUN_BORN ? "born" : "rtm";
+    const char* kind = mode == RUN_MODEL ? "forward" : mode == RUN_BORN ? "born" : ilm ? "illum" : "rtm";
     L = with_strip(L, tuned_strip(kind, NX, NZ, nshots, stream, fn, &tr));
   }
   const double cells = (double)NX * NZ;
@@ -814,6 +826,16 @@ int run_batch(int mode, const GridSpec& g, int nshots, const ShotSpec* shots, co
         launch_step<T>(MODEL, k, br.dev, L, B, nshots, st, stream);
       });
     }
+  } else if (ilm) {
+    // MODEL's bytes plus the read-modify-write of u^2 at the model cells
+    const double model_cells = (double)g.nx * g.nz;
+    for (int st = 0; st < NTmax; st++) {
+      const double a = active_fwd(st);
+      timed_launch(stream, "step_illum", 4 * E * a * cells + 2 * E * a * model_cells, a * cells, [&] {
+        launch_step<T>(ILLUM, k, br.dev, L, B, nshots, st, stream);
+      });
+    }
+    hipLaunchKernelGGL((crop_kernel<T>), dim3(64, nshots), dim3(256), 0, stream, k, br.dev, img);
   } else if (mode == RUN_BORN) {
     for (int st = 0; st < NTmax; st++) {
       const double a = active_fwd(st) * cells;

@@ -11,7 +11,7 @@ namespace seis {
 using GridSpec = seis_grid;
 using ShotSpec = seis_shot;
 
-enum RunMode { RUN_MODEL = 0, RUN_BORN = 1, RUN_RTM = 2 };
+enum RunMode { RUN_MODEL = 0, RUN_BORN = 1, RUN_RTM = 2, RUN_ILLUM = 3 };
 
 // Records the calling thread's last error message.
 void set_error(const std::string& msg);

@@ -1,4 +1,4 @@
-"""Host mirror of the reference engine entry points (src/util/fd.h:3-5).
+"""Host mirror of the reference engine entry points (src/util/fd.h:3-5, 10).
 
 Same argument order and meaning as the reference; arrays are numpy arrays in
 the reference layouts (vel/dv [nx][nz], rec [nx][nt], wav [nt+ndel], gz [nx]).
@@ -57,3 +57,17 @@ def adjoint(dt, dx, dz, nt, ndel, nx, nz, pml, sx, sz, gz, wav, vel, rec):
     img = np.zeros((nx, nz), np.float32)
     check(lib().seis_rtm(dt, dx, dz, nt, ndel, nx, nz, pml, sx, sz, _p(gz), _p(wav), _p(vel), _p(rec), _p(img)))
     return img
+
+
+def illum(dt, dx, dz, nt, ndel, nx, nz, pml, sx, sz, wav, vel):
+    """Source illumination ss[nx][nz] = sum over steps it > ndel of u^2
+    (fd.cpp:275-339; no receivers)."""
+    wav = np.ascontiguousarray(wav, np.float32)
+    vel = np.ascontiguousarray(vel, np.float32)
+    if wav.shape[0] < nt + ndel:
+        raise ValueError("wav must have nt + ndel samples")
+    if vel.shape != (nx, nz):
+        raise ValueError("vel must be [nx][nz]")
+    ss = np.zeros((nx, nz), np.float32)
+    check(lib().seis_illum(dt, dx, dz, nt, ndel, nx, nz, pml, sx, sz, _p(wav), _p(vel), _p(ss)))
+    return ss

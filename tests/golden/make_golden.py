@@ -4,7 +4,7 @@ Run in the container that holds the reference tree, after `make -C oracle ref`:
 
     python tests/golden/make_golden.py
 
-It calls the reference's own modeling / forward / adjoint (src/util/fd.h:3-5)
+It calls the reference's own modeling / forward / adjoint / illum (src/util/fd.h:3-5,10)
 through oracle/_ref/libseisref.so and stores inputs and outputs as .npz
 fixtures next to this script.  The fixtures are data only; they pin the CPU
 oracle (tests/test_oracle_golden.py) and, through it, the HIP engine.
@@ -54,6 +54,9 @@ def run_case(ref, c, what):
         out["born"] = ref.born(*args, c["dv"])
     if "rtm" in what:
         out["img"] = ref.rtm(*args, c["rec_in"])
+    if "illum" in what:
+        out["ss"] = ref.illum(c["dt"], c["dx"], c["dz"], c["nt"], c["ndel"], c["nx"], c["nz"], c["pml"], c["sx"],
+                              c["sz"], c["wav"], c["vel"])
     return out
 
 
@@ -77,8 +80,8 @@ def main():
     ]
     for name, c in cases:
         c["wav"] = ref.ricker(c["dt"], c["fr"], c["delay"] + c["ndel"], c["nt"] + c["ndel"])
-        out = run_case(ref, c, ("modeling", "born", "rtm"))
-        save(name, c, out, "seeded small grid; modeling, born and rtm of the reference engine")
+        out = run_case(ref, c, ("modeling", "born", "rtm", "illum"))
+        save(name, c, out, "seeded small grid; modeling, born, rtm and illum of the reference engine")
 
     # Plain forward modeling, 1 shot: 201 x 301 at 10 m, 3 layers 1500/2200/3000 m/s with
     # interfaces at 600 m and 1400 m, dt = 1 ms, nt = 1500, pml 30, Ricker 15 Hz (delay 67

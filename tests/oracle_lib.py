@@ -87,6 +87,18 @@ class _Lib:
         return img
 
 
+    def illum(self, dt, dx, dz, nt, ndel, nx, nz, pml, sx, sz, wav, vel, dtype=np.float32):
+        """Source illumination ss[nx][nz] (fd.h:10; no receivers)."""
+        S = self._scalar(dtype)
+        wav = np.ascontiguousarray(wav, dtype)
+        vel = np.ascontiguousarray(vel, dtype)
+        assert wav.shape[0] >= nt + ndel and vel.shape == (nx, nz)
+        ss = np.zeros((nx, nz), dtype)
+        self._fn("illum", dtype)(S(dt), S(dx), S(dz), _I(nt), _I(ndel), _I(nx), _I(nz), _I(pml), _I(sx), _I(sz),
+                                 _ptr(wav), _ptr(vel), _ptr(ss))
+        return ss
+
+
 class Oracle(_Lib):
     def __init__(self, path=ORACLE_SO):
         super().__init__(path, "oracle_", True)

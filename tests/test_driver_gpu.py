@@ -108,3 +108,25 @@ def test_modeling_rtm_lsrtm_files(engine, oracle, tmp_path):
     driver.run(param, "lsrtm")
     ls = np.fromfile(tmp_path / "WDIR" / "LSRTM_IMAGE.dat", np.float32).reshape(NX, NZ)
     assert np.isfinite(ls).all() and np.abs(ls).max() > 0
+
+
+def test_illum_files(engine, oracle, tmp_path):
+    """seis_run illum (illum_MPI): per-shot windowed illumination through the
+    v0file background, moved to the global grid -> <wdir>ILLUM<is>.dat, and
+    their sum in shot order, against the glue restatement over the oracle."""
+    from seis_amd import driver
+    print("case driver_illum", flush=True)
+    P, shots, v, v0 = _survey(tmp_path)
+    param = tmp_path / "param.txt"
+    wav = oracle.ricker(P["dt"], P["fr"], P["delay"], NT)
+    driver.run(param, "illum")
+    total = np.fromfile(tmp_path / "WDIR" / "ILLUM.dat", np.float32).reshape(NX, NZ)
+    ref = np.zeros((NX, NZ), np.float32)
+    for is_, shot in enumerate(shots):
+        fx, nxw, vw, igz, isx, isz = _window_case(P, shot, v0)
+        ss = oracle.illum(P["dt"], P["dx"], P["dz"], NT, 0, nxw, NZ, P["npml"], isx, isz, wav, vw)
+        part = G.to_global(P, ss, fx, 0.0, NX, NZ)
+        got = np.fromfile(tmp_path / "WDIR" / ("ILLUM%d.dat" % is_), np.float32).reshape(NX, NZ)
+        assert np.array_equal(got, part), is_
+        ref = part + ref
+    assert np.array_equal(total, ref)

@@ -1,6 +1,6 @@
 """The fd.h-signature entry points (include/seis_fd.h) on the GPU: the exact
 C++ symbols a reference build links against in place of util/fd.cpp's
-modeling / forward / adjoint (fd.h:3-5), called with the reference's
+modeling / forward / adjoint / illum (fd.h:3-5, 10), called with the reference's
 MyAlloc-style float** inputs, against the reference's golden vectors."""
 import ctypes
 
@@ -36,7 +36,7 @@ def test_fd_h_entry_points_match_reference(engine, name):
     c = load_fixture(name)
     print("case fd.h", name, flush=True)
     sym = {"modeling": "_Z8modelingfffiiiiiiiPiPfPS0_", "forward": "_Z7forwardfffiiiiiiiPiPfPS0_S1_",
-           "adjoint": "_Z7adjointfffiiiiiiiPiPfPS0_S1_"}
+           "adjoint": "_Z7adjointfffiiiiiiiPiPfPS0_S1_", "illum": "_Z5illumfffiiiiiiiPfPS_"}
     fns = {}
     for k, mangled in sym.items():
         f = getattr(lib, mangled)
@@ -60,4 +60,7 @@ def test_fd_h_entry_points_match_reference(engine, name):
     img = fns["adjoint"](*head, gzp, wavp, vel_t, rin_t)
     assert np.array_equal(_result(img, c["nx"], c["nz"]), c["img"])
     lib.seis_fd_free(img)
+    ss = fns["illum"](*head, wavp, vel_t)
+    assert np.array_equal(_result(ss, c["nx"], c["nz"]), c["ss"])
+    lib.seis_fd_free(ss)
     del keep, dv_a, rin_a

@@ -1,7 +1,7 @@
 """The CPU oracle against the golden vectors of the compiled reference engine.
 
 The fixtures (tests/golden/*.npz) hold the outputs of the reference's own
-modeling / forward / adjoint (src/util/fd.h:3-5), written by
+modeling / forward / adjoint / illum (src/util/fd.h:3-5, 10), written by
 tests/golden/make_golden.py.  The float32 oracle keeps the reference's operation
 order, so it must reproduce them bit for bit.
 """
@@ -37,6 +37,38 @@ def test_rtm_bitexact(oracle, name):
     c = load_fixture(name)
     img = oracle.rtm(*_args(c), c["rec_in"])
     assert np.array_equal(img, c["img"]), name
+
+
+def _illum_args(c):
+    return (c["dt"], c["dx"], c["dz"], c["nt"], c["ndel"], c["nx"], c["nz"], c["pml"], c["sx"], c["sz"], c["wav"],
+            c["vel"])
+
+
+@pytest.mark.parametrize("name", SMALL)
+def test_illum_bitexact(oracle, name):
+    c = load_fixture(name)
+    ss = oracle.illum(*_illum_args(c))
+    assert np.array_equal(ss, c["ss"]), name
+
+
+@pytest.mark.parametrize("name", SMALL)
+def test_illum_is_sum_of_squared_modeling_samples(oracle, name):
+    """Independent of the illumination code: at each receiver cell the
+    illumination is the float32 running sum of the squared record samples of
+    steps it > ndel (sample index itp = it - ndel >= 1), because modeling
+    records the same field after the same boundary update.  Pins the strict
+    inequality, the time alignment and the interior crop."""
+    c = load_fixture(name)
+    rec = oracle.modeling(*_args(c))
+    ss = oracle.illum(*_illum_args(c))
+    for ix in range(c["nx"]):
+        acc = np.float32(0)
+        for v in rec[ix, 1:]:
+            acc = np.float32(acc + np.float32(v * v))
+        assert ss[ix, c["gz"][ix]] == acc, ix
+    # a sum from it >= ndel would add rec[:, 0]^2: the fixtures with a
+    # pre-roll (ndel > 0) have nonzero first samples, so the loop above sees it
+    assert c["ndel"] == 0 or np.count_nonzero(rec[:, 0]) > 0
 
 
 def test_c1_modeling_bitexact(oracle):
