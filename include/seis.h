@@ -106,6 +106,21 @@ int seis_ricker(float dt, float fr, int num_delay, int n, float* out);
  * <wdir>LSRTM_IMAGE.dat or <wdir>ILLUM.dat) as float32 [nx][nz]. */
 int seis_run(const char* param_file, const char* mode, const char* model_file, const char* out_file);
 
+/* SU interop.  seis_write_su writes n2 traces of n1 float samples
+ * (data[n2][n1]) as the reference's writeSu does (io.cpp:34-58: d1 = d2 = 1,
+ * f2 = 1, dt = 1000, fldr = ep = 1, tracl = tracr = cdp = cdpt = trace
+ * number); n2 = 0 writes the single trace data[n1] of its one-trace form
+ * (io.cpp:8-33, whose dt of 1000000 lands in the 16-bit field as 16960).
+ * seis_su_split does what the reference's get_info utility does
+ * (zzz/get_info/get_info.c:32-214): traces of su_file grouped by the header
+ * key ("fldr" default, "ep", "cdp" or "sx") go to <dir>CSG<i>.dat, the
+ * coordinates file gets "sx sz gx gz" per trace (sx with get_info's rand()
+ * jitter in [0, 2), sz = selev, gz = gelev) and info_file the per-shot
+ * statistics table. */
+int seis_write_su(const char* path, int n1, int n2, const float* data);
+int seis_su_split(const char* su_file, const char* info_file, const char* coord_file, const char* dir,
+                  const char* key);
+
 /* Parses a parameter file like Inversion::readParamFile (inversion.cpp:38-79,
  * grammar of parser.cpp:16-168) and writes "key = value" lines into out
  * (out_len bytes, NUL-terminated); errors name the missing key. */

@@ -18,6 +18,7 @@
 #include "global.h"
 #include "InitInversion.h"
 #include "../filter/hilbert.h"
+#include "io.h"
 
 namespace {
 
@@ -53,6 +54,16 @@ void ref_ricker(float dt, float fr, int num_delay, int n, float* out) {
 void ref_hilbert_apply(int half, const float* in, float* out, int n) {
   Hilbert<float> h(half);
   h.apply(const_cast<float*>(in), out, n);
+}
+
+// writeSu (io.cpp:8-58): one trace of n1 samples (n2 == 0) or n2 traces.
+void ref_write_su(const char* path, int n1, int n2, const float* data) {
+  if (n2 == 0) {
+    writeSu(std::string(path), n1, const_cast<float*>(data));
+    return;
+  }
+  Rows r(data, n1, n2);
+  writeSu(std::string(path), n1, n2, r.p);
 }
 
 void ref_set_abs(int nz, int nx, int pml, float* w) {
