@@ -88,6 +88,24 @@ int seis_rtm(float dt, float dx, float dz, int nt, int ndel, int nx, int nz, int
 int seis_illum(float dt, float dx, float dz, int nt, int ndel, int nx, int nz, int pml, int sx, int sz,
                const float* wav, const float* vel, float* ss);
 
+/* Plane-wave engine (fd.h:6-7, 11).  The source is the line of row sz: every
+ * model column ix fires its own time function sou[ix][it-1] (sou is
+ * [nx][nt+ndel]); receivers sit at row gz of every column.
+ *   seis_born_plane  replaces forwordPlane (fd.cpp:178-274): rec[nx][nt];
+ *   seis_illum_plane replaces illumPlane   (fd.cpp:720-786): ss[nx][nz]
+ *                    (gz is not used, as in the reference);
+ *   seis_rtm_plane   replaces adjointPlane (fd.cpp:787-921): img[nx][nz].
+ *                    The reference's two `if (itp>=0);` guards there end at
+ *                    their semicolons, so with ndel > 0 it also injects
+ *                    samples from before the record (reading outside it);
+ *                    those samples are skipped here (identical for ndel = 0). */
+int seis_born_plane(float dt, float dx, float dz, int nt, int ndel, int nx, int nz, int pml, int sz, int gz,
+                    const float* vel, const float* sou, const float* dv, float* rec);
+int seis_illum_plane(float dt, float dx, float dz, int nt, int ndel, int nx, int nz, int pml, int sz, int gz,
+                     const float* vel, const float* sou, float* ss);
+int seis_rtm_plane(float dt, float dx, float dz, int nt, int ndel, int nx, int nz, int pml, int sz, int gz,
+                   const float* vel, const float* sou, const float* rec, float* img);
+
 /* Ricker wavelet of the reference (wavelet.cpp:7-18): out[i] for i < n,
  * peak frequency fr, peak at sample num_delay. */
 int seis_ricker(float dt, float fr, int num_delay, int n, float* out);
@@ -145,6 +163,17 @@ int seis_rtm_batch(const seis_grid* grid, int nshots, const seis_shot* shots, co
  * the shots' gz and rec offsets are not used. */
 int seis_illum_batch(const seis_grid* grid, int nshots, const seis_shot* shots, const float* vel,
                      const float* wav, float* ss);
+
+/* Plane-wave batches: shots[].wav is the offset of each shot's sou
+ * [nx][nt+ndel], shots[].sz its source row, shots[].gz the offset of its
+ * receiver depths gz[nx] (one row for the reference's plane-wave functions);
+ * shots[].sx is not used.  Outputs per shot as in the single-shot calls. */
+int seis_born_plane_batch(const seis_grid* grid, int nshots, const seis_shot* shots, const float* vel,
+                          const float* dv, const float* sou, const int* gz, float* rec);
+int seis_illum_plane_batch(const seis_grid* grid, int nshots, const seis_shot* shots, const float* vel,
+                           const float* sou, float* ss);
+int seis_rtm_plane_batch(const seis_grid* grid, int nshots, const seis_shot* shots, const float* vel,
+                         const float* sou, const int* gz, const float* rec, float* img);
 
 /* seis_rtm_batch with option bits; flags = 0 is seis_rtm_batch.
  * SEIS_RTM_ISOLATE_BAND: the reference reconstructs the source wavefield

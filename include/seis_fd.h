@@ -1,5 +1,6 @@
 // seis_fd.h -- C++ drop-in for the reference's engine header (bbsun/Seis
-// src/util/fd.h:3-5, 10).  Same names, argument order and meaning; the result is
+// src/util/fd.h:3-11; the up/down, rtm_true and boundary helpers are not
+// provided).  Same names, argument order and meaning; the result is
 // returned as a newly allocated 2D array laid out like MyAlloc<float>::alc
 // (memory.h:29-44: a table of n2 row pointers into one zeroed block of n1*n2
 // floats), so the reference's MyAlloc<float>::free (memory.h:103-110: delete[]
@@ -23,6 +24,18 @@ float** adjoint(float dt, float dx, float dz, int nt, int ndel, int nx, int nz, 
 // (fd.cpp:275-339)
 float** illum(float dt, float dx, float dz, int nt, int ndel, int nx, int nz, int pml, int sx, int sz, float* wav,
               float** vel);
+
+// Plane-wave engine (fd.cpp:178-274, 720-921): sou[nx][nt+ndel] fired at row
+// sz of every column, receivers at row gz.
+// rec[nx][nt] = Born record
+float** forwordPlane(float dt, float dx, float dz, int nt, int ndel, int nx, int nz, int pml, int sz, int gz,
+                     float** vel, float** sou, float** dv);
+// img[nx][nz] = plane-wave image of record rec
+float** adjointPlane(float dt, float dx, float dz, int nt, int ndel, int nx, int nz, int pml, int sz, int gz,
+                     float** vel, float** sou, float** rec);
+// ss[nx][nz] = illumination of the line source
+float** illumPlane(float dt, float dx, float dz, int nt, int ndel, int nx, int nz, int pml, int sz, int gz,
+                   float** vel, float** sou);
 
 // Releases a result of the functions above exactly as the reference's
 // MyAlloc<float>::free does (delete[] buf[0]; delete[] buf;), for callers that

@@ -6,7 +6,8 @@
 // It exists to generate golden vectors (tests/golden/make_golden.py) and to
 // serve as the timed "reference" CPU baseline.  It converts flat arrays
 // to the reference's row-pointer tables (memory.h:29-44), calls
-// modeling / forward / adjoint / illum (fd.h:3-5,10) and copies the newly allocated
+// modeling / forward / adjoint / illum and the plane-wave engine (fd.h:3-11)
+// and copies the newly allocated
 // result back, freeing it the reference's way (memory.h:103-110).
 
 #include <cstring>
@@ -87,6 +88,28 @@ int ref_illum(float dt, float dx, float dz, int nt, int ndel, int nx, int nz, in
               const float* wav, const float* vel, float* ss) {
   Rows v(vel, nz, nx);
   take(illum(dt, dx, dz, nt, ndel, nx, nz, pml, sx, sz, const_cast<float*>(wav), v.p), ss, (size_t)nx * nz);
+  return 0;
+}
+
+// Plane-wave engine (fd.h:6-7, 11): sou is [nx][nt+ndel].
+int ref_born_plane(float dt, float dx, float dz, int nt, int ndel, int nx, int nz, int pml, int sz, int gz,
+                   const float* vel, const float* sou, const float* dv, float* rec) {
+  Rows v(vel, nz, nx), s(sou, nt + ndel, nx), d(dv, nz, nx);
+  take(forwordPlane(dt, dx, dz, nt, ndel, nx, nz, pml, sz, gz, v.p, s.p, d.p), rec, (size_t)nt * nx);
+  return 0;
+}
+
+int ref_illum_plane(float dt, float dx, float dz, int nt, int ndel, int nx, int nz, int pml, int sz, int gz,
+                    const float* vel, const float* sou, float* ss) {
+  Rows v(vel, nz, nx), s(sou, nt + ndel, nx);
+  take(illumPlane(dt, dx, dz, nt, ndel, nx, nz, pml, sz, gz, v.p, s.p), ss, (size_t)nx * nz);
+  return 0;
+}
+
+int ref_adjoint_plane(float dt, float dx, float dz, int nt, int ndel, int nx, int nz, int pml, int sz, int gz,
+                      const float* vel, const float* sou, const float* rec, float* img) {
+  Rows v(vel, nz, nx), s(sou, nt + ndel, nx), r(rec, nt, nx);
+  take(adjointPlane(dt, dx, dz, nt, ndel, nx, nz, pml, sz, gz, v.p, s.p, r.p), img, (size_t)nx * nz);
   return 0;
 }
 

@@ -511,6 +511,161 @@ void rtm(T dt, T dx, T dz, int nt, int ndel, int nx, int nz, int pml, int sx, in
 
 
 // ---------------------------------------------------------------------------
+// Plane-wave engine (fd.h:6-7, 11).  The source is a line at row sz: every
+// model column ix gets vt2 * sou[ix][it-1] (sou is [nx][nt+ndel]); receivers
+// sit at row gz in every column.
+
+template <typename T>
+void add_line_source(const Geom& g, const Coefs<T>& k, T* u, int sz, const T* sou, int NT, int it) {
+  if (it - 1 < 0) return;
+  for (int ix = 0; ix < g.nx; ix++) {
+    const size_t c = g.at(ix + g.pml, sz + g.pml);
+    u[c] += k.vt2[c] * sou[(size_t)ix * NT + it - 1];
+  }
+}
+
+// forwordPlane, fd.cpp:178-274: Born modeling driven by the line source.
+template <typename T>
+void born_plane(T dt, T dx, T dz, int nt, int ndel, int nx, int nz, int pml, int sz, int gz, const T* v,
+                const T* sou, const T* dv, T* rec) {
+  Geom g(nx, nz, pml);
+  Coefs<T> k(g, dt, dx, dz, v, true);
+  std::vector<T> dvp(g.cells());
+  pad_perturbation(g, dv, dvp.data());
+  const T idt2 = T(1) / dt / dt;
+  std::vector<T> a(g.cells()), b(g.cells()), c(g.cells()), ow(g.cells());
+  std::vector<T> ab(g.cells()), bb(g.cells()), cb(g.cells()), owb(g.cells()), lap(g.cells());
+  T *u0 = a.data(), *u1 = b.data(), *u2 = c.data();
+  T *s0 = ab.data(), *s1 = bb.data(), *s2 = cb.data();
+  for (size_t i = 0; i < (size_t)nx * nt; i++) rec[i] = T(0);
+  const int NT = nt + ndel;
+  for (int it = 0; it < NT; it++) {
+    stencil_step(g, u0, u1, u2, k.vvzz.data(), k.vvxx.data());
+    add_line_source(g, k, u2, sz, sou, NT, it);
+    habc_update(g, dt, dx, dz, k.vel.data(), ow.data(), u2, u1, u0);
+    blend(g, u2, ow.data(), k.w.data());
+    for (size_t i = 0; i < g.cells(); i++) lap[i] = (u2[i] + u0[i] - T(2) * u1[i]) * idt2;
+    stencil_step(g, s0, s1, s2, k.vvzz.data(), k.vvxx.data());
+    for (size_t i = 0; i < g.cells(); i++) s2[i] += k.vt2[i] * lap[i] * k.v3[i] * dvp[i];
+    habc_update(g, dt, dx, dz, k.vel.data(), owb.data(), s2, s1, s0);
+    blend(g, s2, owb.data(), k.w.data());
+    const int itp = it - ndel;
+    if (itp >= 0)
+      for (int ix = 0; ix < nx; ix++) rec[(size_t)ix * nt + itp] = s2[g.at(ix + pml, gz + pml)];
+    T* t = u0; u0 = u1; u1 = u2; u2 = t;
+    t = s0; s0 = s1; s1 = s2; s2 = t;
+  }
+}
+
+// illumPlane, fd.cpp:720-786: illum with the line source.
+template <typename T>
+void illum_plane(T dt, T dx, T dz, int nt, int ndel, int nx, int nz, int pml, int sz, const T* v, const T* sou,
+                 T* ss) {
+  Geom g(nx, nz, pml);
+  Coefs<T> k(g, dt, dx, dz, v, false);
+  std::vector<T> a(g.cells()), b(g.cells()), c(g.cells()), ow(g.cells());
+  T *u0 = a.data(), *u1 = b.data(), *u2 = c.data();
+  for (size_t i = 0; i < (size_t)nx * nz; i++) ss[i] = T(0);
+  const int NT = nt + ndel;
+  for (int it = 0; it < NT; it++) {
+    stencil_step(g, u0, u1, u2, k.vvzz.data(), k.vvxx.data());
+    add_line_source(g, k, u2, sz, sou, NT, it);
+    habc_update(g, dt, dx, dz, k.vel.data(), ow.data(), u2, u1, u0);
+    blend(g, u2, ow.data(), k.w.data());
+    if (it > ndel)
+      for (int ix = 0; ix < nx; ix++)
+        for (int iz = 0; iz < nz; iz++) {
+          const T u = u2[g.at(ix + pml, iz + pml)];
+          ss[(size_t)ix * nz + iz] += u * u;
+        }
+    T* t = u0; u0 = u1; u1 = u2; u2 = t;
+  }
+}
+
+// adjointPlane, fd.cpp:787-921: the receiver field is propagated backward in
+// time first (record injected at row gz; the 4-layer strips of each step's
+// current level saved at that step), then rebuilt forward in time from its
+// last two levels (the record subtracted from the previous level at row gz,
+// the strips restored into each new current level) beside the line-source
+// field, imaging 1/2 (U R - H[U] H[R]) (FaQi alone) with U = lap 2/v^3.  The
+// reference's two `if (itp>=0);` guards end at their semicolons, so it also
+// injects samples before the record (reads outside rec) when ndel > 0; those
+// samples are skipped here, which is the same for ndel = 0.  img is [nx][nz].
+template <typename T>
+void adjoint_plane(T dt, T dx, T dz, int nt, int ndel, int nx, int nz, int pml, int sz, int gz, const T* v,
+                   const T* sou, const T* rec, T* img) {
+  const int layer = 4, half = 30;
+  Geom g(nx, nz, pml);
+  Coefs<T> k(g, dt, dx, dz, v, true);
+  const T idt2 = T(1) / dt / dt;
+  const int NT = nt + ndel;
+  Strips<T> st(layer, NT, nx, nz);
+  std::vector<T> a(g.cells()), b(g.cells()), c(g.cells()), ow(g.cells());
+  std::vector<T> ab(g.cells()), bb(g.cells()), cb(g.cells()), owb(g.cells()), lap(g.cells()),
+      imgp(g.cells());
+  T *u0 = a.data(), *u1 = b.data(), *u2 = c.data();
+  T *r0 = ab.data(), *r1 = bb.data(), *r2 = cb.data();
+  auto inject = [&](T* r, int itp, T sign) {
+    if (itp < 0) return;
+    for (int ix = 0; ix < nx; ix++) {
+      const size_t cc = g.at(ix + pml, gz + pml);
+      if (sign > 0)
+        r[cc] += rec[(size_t)ix * nt + itp] * k.vt2[cc];
+      else
+        r[cc] -= rec[(size_t)ix * nt + itp] * k.vt2[cc];
+    }
+  };
+  for (int it = NT - 1; it >= 0; it--) {
+    stencil_step(g, r2, r1, r0, k.vvzz.data(), k.vvxx.data());
+    inject(r0, it - ndel, T(1));
+    habc_update(g, dt, dx, dz, k.vel.data(), owb.data(), r0, r1, r2);
+    blend(g, r0, owb.data(), k.w.data());
+    st.move(g, r1, it, true);
+    T* t = r2; r2 = r1; r1 = r0; r0 = t;
+  }
+  {
+    T* t = r0; r0 = r1; r1 = r2; r2 = t;
+    std::fill(r2, r2 + g.cells(), T(0));
+  }
+  std::vector<T> h(2 * half + 1);
+  hilbert_taps(half, h.data());
+  const T scal = T(2) / T(4);
+  for (int it = 2; it < NT; it++) {
+    stencil_step(g, u0, u1, u2, k.vvzz.data(), k.vvxx.data());
+    add_line_source(g, k, u2, sz, sou, NT, it);
+    habc_update(g, dt, dx, dz, k.vel.data(), ow.data(), u2, u1, u0);
+    blend(g, u2, ow.data(), k.w.data());
+    for (size_t i = 0; i < g.cells(); i++) lap[i] = (u2[i] + u0[i] - T(2) * u1[i]) * idt2;
+    inject(r0, it - ndel - 1, T(-1));
+    stencil_step(g, r0, r1, r2, k.vvzz.data(), k.vvxx.data());
+    habc_update(g, dt, dx, dz, k.vel.data(), owb.data(), r2, r1, r0);
+    blend(g, r2, owb.data(), k.w.data());
+    if (it - ndel > 0) {
+      for (size_t i = 0; i < g.cells(); i++) lap[i] = lap[i] * k.v3[i];
+      set_omp();
+#pragma omp parallel
+      {
+        std::vector<T> uhl(g.NZ), rhl(g.NZ);
+#pragma omp for
+        for (int ix = pml; ix < g.NX - pml; ix++) {
+          const T* U = lap.data() + (size_t)ix * g.NZ;
+          const T* R = r2 + (size_t)ix * g.NZ;
+          hilbert_apply(h.data(), half, U, uhl.data(), g.NZ);
+          hilbert_apply(h.data(), half, R, rhl.data(), g.NZ);
+          T* out = imgp.data() + (size_t)ix * g.NZ;
+          for (int iz = pml; iz < g.NZ - pml; iz++) out[iz] += scal * (U[iz] * R[iz] - uhl[iz] * rhl[iz]);
+        }
+      }
+    }
+    T* t = u0; u0 = u1; u1 = u2; u2 = t;
+    t = r0; r0 = r1; r1 = r2; r2 = t;
+    st.move(g, r1, it, false);
+  }
+  for (int ix = 0; ix < nx; ix++)
+    for (int iz = 0; iz < nz; iz++) img[(size_t)ix * nz + iz] = imgp[g.at(ix + pml, iz + pml)];
+}
+
+// ---------------------------------------------------------------------------
 // Exact transpose of the Born operator.  The Born map of born() is linear in
 // dv: the scattered field s obeys s^{n+1} = Step(s^n, s^{n-1}) + injection of
 // q^n = vt2 * lap^n * 2/v^3 * dv before the absorbing boundary, and the record
@@ -767,6 +922,21 @@ int oracle_get_threads(void) { return g_threads; }
   int oracle_illum_##SUF(T dt, T dx, T dz, int nt, int ndel, int nx, int nz, int pml, int sx, int sz, \
                          const T* wav, const T* vel, T* ss) {                                         \
     illum<T>(dt, dx, dz, nt, ndel, nx, nz, pml, sx, sz, wav, vel, ss);                               \
+    return 0;                                                                                         \
+  }                                                                                                   \
+  int oracle_born_plane_##SUF(T dt, T dx, T dz, int nt, int ndel, int nx, int nz, int pml, int sz,     \
+                              int gz, const T* vel, const T* sou, const T* dv, T* rec) {              \
+    born_plane<T>(dt, dx, dz, nt, ndel, nx, nz, pml, sz, gz, vel, sou, dv, rec);                      \
+    return 0;                                                                                         \
+  }                                                                                                   \
+  int oracle_illum_plane_##SUF(T dt, T dx, T dz, int nt, int ndel, int nx, int nz, int pml, int sz,    \
+                               const T* vel, const T* sou, T* ss) {                                   \
+    illum_plane<T>(dt, dx, dz, nt, ndel, nx, nz, pml, sz, vel, sou, ss);                              \
+    return 0;                                                                                         \
+  }                                                                                                   \
+  int oracle_adjoint_plane_##SUF(T dt, T dx, T dz, int nt, int ndel, int nx, int nz, int pml, int sz,  \
+                                 int gz, const T* vel, const T* sou, const T* rec, T* img) {          \
+    adjoint_plane<T>(dt, dx, dz, nt, ndel, nx, nz, pml, sz, gz, vel, sou, rec, img);                  \
     return 0;                                                                                         \
   }                                                                                                   \
   int oracle_born_##SUF(T dt, T dx, T dz, int nt, int ndel, int nx, int nz, int pml, int sx, int sz,  \

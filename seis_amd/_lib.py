@@ -46,6 +46,12 @@ _SIGS = {
     "seis_born": (_I, [_F, _F, _F, _I, _I, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P]),
     "seis_rtm": (_I, [_F, _F, _F, _I, _I, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P]),
     "seis_illum": (_I, [_F, _F, _F, _I, _I, _I, _I, _I, _I, _I, _P, _P, _P]),
+    "seis_born_plane": (_I, [_F, _F, _F, _I, _I, _I, _I, _I, _I, _I, _P, _P, _P, _P]),
+    "seis_illum_plane": (_I, [_F, _F, _F, _I, _I, _I, _I, _I, _I, _I, _P, _P, _P]),
+    "seis_rtm_plane": (_I, [_F, _F, _F, _I, _I, _I, _I, _I, _I, _I, _P, _P, _P, _P]),
+    "seis_born_plane_batch": (_I, [ctypes.POINTER(seis_grid), _I, ctypes.POINTER(seis_shot), _P, _P, _P, _P, _P]),
+    "seis_illum_plane_batch": (_I, [ctypes.POINTER(seis_grid), _I, ctypes.POINTER(seis_shot), _P, _P, _P]),
+    "seis_rtm_plane_batch": (_I, [ctypes.POINTER(seis_grid), _I, ctypes.POINTER(seis_shot), _P, _P, _P, _P, _P]),
     "seis_illum_batch": (_I, [ctypes.POINTER(seis_grid), _I, ctypes.POINTER(seis_shot), _P, _P, _P]),
     "seis_modeling_batch": (_I, [ctypes.POINTER(seis_grid), _I, ctypes.POINTER(seis_shot), _P, _P, _P, _P]),
     "seis_born_batch": (_I, [ctypes.POINTER(seis_grid), _I, ctypes.POINTER(seis_shot), _P, _P, _P, _P, _P]),

@@ -134,3 +134,48 @@ def illum_batch(grid, shots, vel, wavs):
     g = grid.c()
     check(lib().seis_illum_batch(ctypes.byref(g), len(shots), arr, dvel.ptr, dwav.ptr, dss.ptr))
     return dss.download().reshape(len(shots), grid.nx, grid.nz)
+
+
+def _plane_batch(fn, grid, shots, vel, sous, gz_rows, extra=None, recs=None):
+    arr, nw, ng, nr = _layout(grid, shots)
+    sou = np.concatenate([np.ascontiguousarray(u, np.float32).ravel() for u in sous])
+    assert sou.shape[0] == nw * grid.nx
+    # the layout's wavelet offsets count samples; sou blocks hold nx of them
+    for j in range(len(shots)):
+        arr[j].wav *= grid.nx
+    gz = np.concatenate([np.full(grid.nx, r, np.int32) for r in gz_rows])
+    keep = []
+    dvel, k = _dev(vel, np.float32); keep.append(k)
+    dsou, k = _dev(sou, np.float32); keep.append(k)
+    dgz, k = _dev(gz, np.int32); keep.append(k)
+    g = grid.c()
+    n = len(shots)
+    if fn == "born":
+        ddv, k = _dev(extra, np.float32); keep.append(k)
+        drec = DeviceArray((nr,), np.float32, zero=True)
+        check(lib().seis_born_plane_batch(ctypes.byref(g), n, arr, dvel.ptr, ddv.ptr, dsou.ptr, dgz.ptr, drec.ptr))
+        return split_records(grid, shots, drec.download())
+    dout = DeviceArray((n * grid.nx * grid.nz,), np.float32, zero=True)
+    if fn == "illum":
+        check(lib().seis_illum_plane_batch(ctypes.byref(g), n, arr, dvel.ptr, dsou.ptr, dout.ptr))
+    else:
+        rec = np.concatenate([np.ascontiguousarray(r, np.float32).ravel() for r in recs])
+        drec, k = _dev(rec, np.float32); keep.append(k)
+        check(lib().seis_rtm_plane_batch(ctypes.byref(g), n, arr, dvel.ptr, dsou.ptr, dgz.ptr, drec.ptr, dout.ptr))
+    return dout.download().reshape(n, grid.nx, grid.nz)
+
+
+def born_plane_batch(grid, shots, vel, sous, gz_rows, dv):
+    """Plane-wave Born records of every shot (sous[j]: [nx][nt+ndel] fired at
+    row shots[j].sz; receivers at row gz_rows[j])."""
+    return _plane_batch("born", grid, shots, vel, sous, gz_rows, extra=dv)
+
+
+def illum_plane_batch(grid, shots, vel, sous):
+    """Plane-wave illumination of every shot: [nshots][nx][nz]."""
+    return _plane_batch("illum", grid, shots, vel, sous, [0] * len(shots))
+
+
+def rtm_plane_batch(grid, shots, vel, sous, gz_rows, recs):
+    """Plane-wave images of every shot (adjointPlane): [nshots][nx][nz]."""
+    return _plane_batch("rtm", grid, shots, vel, sous, gz_rows, recs=recs)

@@ -94,6 +94,41 @@ int single(int mode, float dt, float dx, float dz, int nt, int ndel, int nx, int
   return SEIS_OK;
 }
 
+// Plane-wave engine: one shot, host buffers.
+int plane_single(int kind, float dt, float dx, float dz, int nt, int ndel, int nx, int nz, int pml, int sz, int gz,
+                 const float* vel, const float* sou, const float* extra, float* out) {
+  if (!vel || !sou || !out || (kind != 1 && !extra)) return set_error("null array argument"), SEIS_ERR_ARG;
+  if (nt <= 0 || ndel < 0 || nx <= 0 || nz <= 0 || pml < 0)
+    return set_error("nt, nx, nz must be > 0; ndel, pml >= 0"), SEIS_ERR_ARG;
+  if (sz < 0 || sz >= nz) return set_error("source row outside the model"), SEIS_ERR_ARG;
+  if (kind != 1 && (gz < 0 || gz >= nz)) return set_error("receiver row outside the model"), SEIS_ERR_ARG;
+  const size_t nm = (size_t)nx * nz, nr = (size_t)nx * nt, ns = (size_t)nx * (nt + ndel);
+  const std::vector<int> gzs(nx, kind != 1 ? gz : 0);
+  DevBuf dgz, dsou, dvel, dextra, dout;
+  int rc;
+  if ((rc = upload(dgz, gzs.data(), sizeof(int) * nx))) return rc;
+  if ((rc = upload(dsou, sou, sizeof(float) * ns))) return rc;
+  if ((rc = upload(dvel, vel, sizeof(float) * nm))) return rc;
+  const size_t nout = kind == 0 ? nr : nm;
+  if (kind == 0 && (rc = upload(dextra, extra, sizeof(float) * nm))) return rc;  // dv
+  if (kind == 2 && (rc = upload(dextra, extra, sizeof(float) * nr))) return rc;  // rec
+  if ((rc = alloc(dout, sizeof(float) * nout))) return rc;
+  seis_grid g{nx, nz, pml, dt, dx, dz};
+  seis_shot s{0, sz, nt, ndel, 0, 0, 0, 0};
+  if (kind == 0)
+    rc = run_batch<float>(RUN_BORN, g, 1, &s, (const float*)dvel.p, (const float*)dextra.p, (const float*)dsou.p,
+                          (const int*)dgz.p, (float*)dout.p, nullptr, 0, 0, 1);
+  else if (kind == 1)
+    rc = run_batch<float>(RUN_ILLUM, g, 1, &s, (const float*)dvel.p, nullptr, (const float*)dsou.p, nullptr, nullptr,
+                          (float*)dout.p, 0, 0, 1);
+  else
+    rc = run_rtm_plane<float>(g, 1, &s, (const float*)dvel.p, (const float*)dsou.p, (const int*)dgz.p,
+                              (const float*)dextra.p, (float*)dout.p, 0);
+  if (rc) return rc;
+  hipError_t e = hipMemcpy(out, dout.p, sizeof(float) * nout, hipMemcpyDeviceToHost);
+  return e == hipSuccess ? SEIS_OK : hip_fail("hipMemcpy", e);
+}
+
 }  // namespace
 }  // namespace seis
 
@@ -211,6 +246,43 @@ int seis_illum_batch(const seis_grid* grid, int nshots, const seis_shot* shots, 
   int rc = check_batch(grid, nshots, shots);
   if (rc) return rc;
   return run_batch<float>(RUN_ILLUM, *grid, nshots, shots, vel, nullptr, wav, nullptr, nullptr, ss, 0);
+}
+
+
+int seis_born_plane(float dt, float dx, float dz, int nt, int ndel, int nx, int nz, int pml, int sz, int gz,
+                    const float* vel, const float* sou, const float* dv, float* rec) {
+  return plane_single(0, dt, dx, dz, nt, ndel, nx, nz, pml, sz, gz, vel, sou, dv, rec);
+}
+
+int seis_illum_plane(float dt, float dx, float dz, int nt, int ndel, int nx, int nz, int pml, int sz, int gz,
+                     const float* vel, const float* sou, float* ss) {
+  return plane_single(1, dt, dx, dz, nt, ndel, nx, nz, pml, sz, gz, vel, sou, nullptr, ss);
+}
+
+int seis_rtm_plane(float dt, float dx, float dz, int nt, int ndel, int nx, int nz, int pml, int sz, int gz,
+                   const float* vel, const float* sou, const float* rec, float* img) {
+  return plane_single(2, dt, dx, dz, nt, ndel, nx, nz, pml, sz, gz, vel, sou, rec, img);
+}
+
+int seis_born_plane_batch(const seis_grid* grid, int nshots, const seis_shot* shots, const float* vel,
+                          const float* dv, const float* sou, const int* gz, float* rec) {
+  int rc = check_batch(grid, nshots, shots);
+  if (rc) return rc;
+  return run_batch<float>(RUN_BORN, *grid, nshots, shots, vel, dv, sou, gz, rec, nullptr, 0, 0, 1);
+}
+
+int seis_illum_plane_batch(const seis_grid* grid, int nshots, const seis_shot* shots, const float* vel,
+                           const float* sou, float* ss) {
+  int rc = check_batch(grid, nshots, shots);
+  if (rc) return rc;
+  return run_batch<float>(RUN_ILLUM, *grid, nshots, shots, vel, nullptr, sou, nullptr, nullptr, ss, 0, 0, 1);
+}
+
+int seis_rtm_plane_batch(const seis_grid* grid, int nshots, const seis_shot* shots, const float* vel,
+                         const float* sou, const int* gz, const float* rec, float* img) {
+  int rc = check_batch(grid, nshots, shots);
+  if (rc) return rc;
+  return run_rtm_plane<float>(*grid, nshots, shots, vel, sou, gz, rec, img, 0);
 }
 
 int seis_modeling_batch(const seis_grid* grid, int nshots, const seis_shot* shots, const float* vel,

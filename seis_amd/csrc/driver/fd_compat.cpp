@@ -86,4 +86,28 @@ float** illum(float dt, float dx, float dz, int nt, int ndel, int nx, int nz, in
   return finish(seis_illum(dt, dx, dz, nt, ndel, nx, nz, pml, sx, sz, wav, v.p, ss[0]), ss);
 }
 
+float** forwordPlane(float dt, float dx, float dz, int nt, int ndel, int nx, int nz, int pml, int sz, int gz,
+                     float** vel, float** sou, float** dv) {
+  Flat v(vel, nz, nx), s(sou, nt + ndel, nx), d(dv, nz, nx);
+  float** rec = alloc2(nt, nx);
+  if (!rec) return nullptr;
+  return finish(seis_born_plane(dt, dx, dz, nt, ndel, nx, nz, pml, sz, gz, v.p, s.p, d.p, rec[0]), rec);
+}
+
+float** adjointPlane(float dt, float dx, float dz, int nt, int ndel, int nx, int nz, int pml, int sz, int gz,
+                     float** vel, float** sou, float** rec) {
+  Flat v(vel, nz, nx), s(sou, nt + ndel, nx), r(rec, nt, nx);
+  float** img = alloc2(nz, nx);
+  if (!img) return nullptr;
+  return finish(seis_rtm_plane(dt, dx, dz, nt, ndel, nx, nz, pml, sz, gz, v.p, s.p, r.p, img[0]), img);
+}
+
+float** illumPlane(float dt, float dx, float dz, int nt, int ndel, int nx, int nz, int pml, int sz, int gz,
+                   float** vel, float** sou) {
+  Flat v(vel, nz, nx), s(sou, nt + ndel, nx);
+  float** ss = alloc2(nz, nx);
+  if (!ss) return nullptr;
+  return finish(seis_illum_plane(dt, dx, dz, nt, ndel, nx, nz, pml, sz, gz, v.p, s.p, ss[0]), ss);
+}
+
 extern "C" void seis_fd_free(float** buf) { free2(buf); }

@@ -33,6 +33,8 @@ struct DevShot {
   int nx, NX, nt, ndel, sx, sz;
   int snap_kind;   // SNAP_NATIVE, SNAP_BF16 or SNAP_FP16
   int rtm_flags;   // SEIS_RTM_* bits (RTM reverse pass)
+  int line_src;    // 1: plane-wave line source at row sz of every model
+                   // column, wav = sou [nx][nt+ndel]; 0: point source
   T* raw;          // stencil values of the band cells of the new level, grid
                    // origin; second field at raw + fstride
 };
@@ -47,7 +49,42 @@ constexpr int NAUX = 4;
 // propagates the scattered field from those records; ILLUM is the source
 // illumination of illum (fd.cpp:275-339): modeling without receivers that
 // accumulates u^2 of the model cells into DevShot::img.
-enum Mode { MODEL = 0, BORN = 1, RTM_FWD = 2, RTM_BWD = 3, SNAP_MAX = 4, SNAP_STORE = 5, BORN_S = 6, ILLUM = 7 };
+// PW_BACK and PW_FWD are the two passes of the plane-wave imaging
+// adjointPlane (fd.cpp:787-921): the receiver field backward in time saving
+// its strips, then the line-source field forward beside the receiver field
+// rebuilt forward in time.
+enum Mode {
+  MODEL = 0,
+  BORN = 1,
+  RTM_FWD = 2,
+  RTM_BWD = 3,
+  SNAP_MAX = 4,
+  SNAP_STORE = 5,
+  BORN_S = 6,
+  ILLUM = 7,
+  PW_BACK = 8,
+  PW_FWD = 9
+};
+
+// Time index of launch step `step` of a shot in `mode`, or -1 when the shot
+// has no such step: forward loops run it = step; the RTM reverse pass
+// it = nt+ndel-1-step down to ndel; the plane-wave passes it = nt+ndel-1-step
+// down to 0, then it = step+2 up to nt+ndel-1 (fd.cpp:829, 853).
+template <typename T>
+SEIS_HD inline int step_it(int mode, const DevShot<T>& s, int step) {
+  const int NT = s.nt + s.ndel;
+  int it;
+  if (mode == RTM_BWD) {
+    it = NT - 1 - step;
+    return it >= s.ndel ? it : -1;
+  }
+  if (mode == PW_BACK) {
+    it = NT - 1 - step;
+    return it >= 0 ? it : -1;
+  }
+  it = mode == PW_FWD ? step + 2 : step;
+  return it < NT ? it : -1;
+}
 
 // Storage of the Born-source snapshots vt2 * lap * 2/v^3 (model cells only).
 // 16-bit forms store value / scale[step], with scale[step] the step's max |value|.

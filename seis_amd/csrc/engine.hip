@@ -48,7 +48,7 @@ struct BornInject {
     const T v3 = T(2) / (v * v * v);
     return val + vt2 * lap * v3 * d;
   }
-  __device__ __forceinline__ T prev(const T* P, long long i) const { return P[i]; }
+  __device__ __forceinline__ T prev(const T* P, long long i, int, int) const { return P[i]; }
 };
 
 // Born source read back from the snapshots: S(step) * p at model cells.
@@ -64,8 +64,27 @@ struct SnapInject {
     if (d == T(0)) return val;
     return val + snap_load(*s, nz, it, j) * d;
   }
-  __device__ __forceinline__ T prev(const T* P, long long i) const { return P[i]; }
+  __device__ __forceinline__ T prev(const T* P, long long i, int, int) const { return P[i]; }
 };
+
+// The source of step `it` of a shot: the point source, or the plane-wave
+// line source (DevShot::line_src); inactive at it = 0.
+template <typename T>
+__device__ __forceinline__ PointSource<T> step_source(const Consts<T>& k, const DevShot<T>& s, long long src, int it) {
+  PointSource<T> ps{-1, T(0)};
+  if (it - 1 < 0) return ps;
+  if (s.line_src) {
+    ps.line = s.wav + (it - 1);
+    ps.lstride = s.nt + s.ndel;
+    ps.lrow = s.sz + k.pml;
+    ps.lx0 = k.pml;
+    ps.lnx = s.nx;
+  } else {
+    ps.src = src;
+    ps.amp = s.wav[it - 1];
+  }
+  return ps;
+}
 
 // Generic update of one cell: any cell of the padded grid, including the
 // absorbing band and the outer ring, through the pure per-cell functions of
@@ -83,7 +102,7 @@ __device__ __forceinline__ void cell_update(const Consts<T>& k, const DevShot<T>
   const Model<T>& m = c.m;
   if (MODE == SNAP_MAX || MODE == SNAP_STORE) {
     // band cells lie outside the model (pml >= 4 is required): background only
-    PointSource<T> ps{it - 1 >= 0 ? c.src : -1, it - 1 >= 0 ? s.wav[it - 1] : T(0)};
+    const PointSource<T> ps = step_source(k, s, c.src, it);
     N[i] = final_value(k, m, NX, C, P, ps, x, z);
   } else if (MODE == BORN_S) {
     SnapInject<T> si{&s, k.nz, k.pml, it};
@@ -92,7 +111,7 @@ __device__ __forceinline__ void cell_update(const Consts<T>& k, const DevShot<T>
     const int ix = x - k.pml;
     if (itp >= 0 && ix >= 0 && ix < s.nx && z == s.gz[ix] + k.pml) s.rec[(long long)ix * s.nt + itp] = u;
   } else if (MODE == MODEL || MODE == RTM_FWD || MODE == ILLUM) {
-    PointSource<T> ps{it - 1 >= 0 ? c.src : -1, it - 1 >= 0 ? s.wav[it - 1] : T(0)};
+    const PointSource<T> ps = step_source(k, s, c.src, it);
     const T u = final_value(k, m, NX, C, P, ps, x, z);
     N[i] = u;
     if (MODE == ILLUM) {
@@ -107,8 +126,33 @@ __device__ __forceinline__ void cell_update(const Consts<T>& k, const DevShot<T>
       T* st = s.strips + (long long)itp * 8 * (s.nx + k.nz);
       for (int q = 0; q < n; q++) st[sl[q]] = u;
     }
+  } else if (MODE == PW_BACK) {
+    // receiver field of adjointPlane, backward in time: record sample
+    // it - ndel injected at the receivers; the strips of the new level are
+    // the ones saved at step it - 1 (fd.cpp:829-846)
+    const ReceiverInject<T> ri{s.gz, s.rec, s.nx, s.nt, it - s.ndel, k.pml};
+    const T u = final_value(k, m, NX, C, P, ri, x, z);
+    N[i] = u;
+    if (it >= 1) {
+      int sl[4];
+      const int n = strip_slots(x, z, s.nx, k.nz, k.pml, sl);
+      T* st = s.strips + (long long)(it - 1) * 8 * (s.nx + k.nz);
+      for (int q = 0; q < n; q++) st[sl[q]] = u;
+    }
+  } else if (MODE == PW_FWD) {
+    // line-source field and U = lap 2/v^3; the receiver field rebuilt
+    // forward with record sample it - ndel - 1 removed from its previous
+    // level (fd.cpp:851-867)
+    const PointSource<T> ps = step_source(k, s, c.src, it);
+    const T u = final_value(k, m, NX, C, P, ps, x, z);
+    const T lap = (u + P[i] - T(2) * C[i]) * k.idt2;
+    const T v = m.vel(x, z);
+    c.aux[i] = lap * (T(2) / (v * v * v));
+    N[i] = u;
+    const ReceiverRemoval<T> rr{s.gz, s.rec, s.nx, s.nt, it - s.ndel - 1, k.pml, k.dt, &m};
+    N[3 * s.fstride + i] = final_value(k, m, NX, C + 3 * s.fstride, P + 3 * s.fstride, rr, x, z);
   } else if (MODE == BORN) {
-    PointSource<T> ps{it - 1 >= 0 ? c.src : -1, it - 1 >= 0 ? s.wav[it - 1] : T(0)};
+    const PointSource<T> ps = step_source(k, s, c.src, it);
     T nb_raw;
     const T ub = final_value(k, m, NX, C, P, ps, x, z, &nb_raw);
     const T* Ps = P + 3 * s.fstride;
@@ -135,7 +179,7 @@ __device__ __forceinline__ void cell_update(const Consts<T>& k, const DevShot<T>
     const bool band = x < k.pml || x >= k.pml + s.nx || z < k.pml || z >= k.pml + k.nz;
     const bool iso = band && (s.rtm_flags & SEIS_RTM_ISOLATE_BAND);
     const T u = iso ? T(0) : final_value(k, m, NX, C, P, sr, x, z);
-    const T lap = (sr.prev(P, i) + u - T(2) * C[i]) * k.idt2;
+    const T lap = (sr.prev(P, i, x, z) + u - T(2) * C[i]) * k.idt2;
     const T* Pr = P + 3 * s.fstride;
     const T* Cr = C + 3 * s.fstride;
     T* Nr = N + 3 * s.fstride;
@@ -183,7 +227,8 @@ __device__ __forceinline__ void band_raw(const Consts<T>& k, const DevShot<T>& s
     RB[i] = raw_value(k, m, NX, c.C + 3 * s.fstride, c.P + 3 * s.fstride, NoInject{}, x, z);
   } else {
     RA[i] = raw_value(k, m, NX, c.C, c.P, NoInject{}, x, z);
-    if (MODE == BORN) RB[i] = raw_value(k, m, NX, c.C + 3 * s.fstride, c.P + 3 * s.fstride, NoInject{}, x, z);
+    if (MODE == BORN || MODE == PW_FWD)
+      RB[i] = raw_value(k, m, NX, c.C + 3 * s.fstride, c.P + 3 * s.fstride, NoInject{}, x, z);
   }
 }
 
@@ -210,14 +255,8 @@ template <typename T, int MODE>
 __global__ __launch_bounds__(WG) void band_kernel(Consts<T> k, const DevShot<T>* __restrict__ shots, Ring B,
                                                   int step) {
   const DevShot<T>& s = shots[blockIdx.y];
-  int it;
-  if (MODE == RTM_BWD) {
-    it = s.nt + s.ndel - 1 - step;
-    if (it < s.ndel) return;
-  } else {
-    it = step;
-    if (it >= s.nt + s.ndel) return;
-  }
+  const int it = step_it<T>(MODE, s, step);
+  if (it < 0) return;
   const long long j = (long long)blockIdx.x * WG + threadIdx.x;
   if (j >= B.n) return;
   const int NX = s.NX, NZ = k.NZ, pml = k.pml;
@@ -244,12 +283,20 @@ __global__ __launch_bounds__(WG) void band_kernel(Consts<T> k, const DevShot<T>*
     T u = T(0);
     if (!(s.rtm_flags & SEIS_RTM_ISOLATE_BAND))
       u = band_blend(k, m, NX, C, P, sr, x, z, RA[i], RawNeighbour<T>{RA, N, ld, NX, NZ, pml + 1});
-    const T lap = (sr.prev(P, i) + u - T(2) * C[i]) * k.idt2;
+    const T lap = (sr.prev(P, i, x, z) + u - T(2) * C[i]) * k.idt2;
     const T v = m.vel(x, z);
     s.aux[(step % NAUX) * s.fstride + i] = lap * (T(2) / (v * v * v));
     N[i] = u;
     N[f3 + i] = band_blend(k, m, NX, C + f3, P + f3, NoInject{}, x, z, RB[i],
                            RawNeighbour<T>{RB, N + f3, ld, NX, NZ, pml});
+  } else if (MODE == PW_FWD) {
+    const T u = band_blend(k, m, NX, C, P, NoInject{}, x, z, RA[i], RawNeighbour<T>{RA, N, ld, NX, NZ, pml});
+    const T lap = (u + P[i] - T(2) * C[i]) * k.idt2;
+    const T v = m.vel(x, z);
+    s.aux[(step % NAUX) * s.fstride + i] = lap * (T(2) / (v * v * v));
+    N[i] = u;
+    const ReceiverRemoval<T> rr{s.gz, s.rec, s.nx, s.nt, it - s.ndel - 1, pml, k.dt, &m};
+    N[f3 + i] = band_blend(k, m, NX, C + f3, P + f3, rr, x, z, RB[i], RawNeighbour<T>{RB, N + f3, ld, NX, NZ, pml});
   } else {
     N[i] = band_blend(k, m, NX, C, P, NoInject{}, x, z, RA[i], RawNeighbour<T>{RA, N, ld, NX, NZ, pml});
     if (MODE == BORN)
@@ -280,7 +327,7 @@ __device__ __forceinline__ T d2_z(const Consts<T>& k, const T* r) {
 template <typename T, int MODE>
 __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot<T>& s, const StepCtx<T>& c, int z0,
                                                int zend, int x0, int x1, T* lds) {
-  constexpr bool TWO = (MODE == BORN || MODE == RTM_BWD);
+  constexpr bool TWO = (MODE == BORN || MODE == RTM_BWD || MODE == PW_FWD);
   const int lane = threadIdx.x & 63;
   const int z = z0 + lane;
   const bool act = z < zend;
@@ -313,7 +360,12 @@ __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot
   // column of software pipelining; the last column re-reads itself).
   // The receiver depth of the column (and, in the reverse pass, its record
   // sample) is fetched the same way.
-  constexpr bool GZ = (MODE == MODEL || MODE == BORN || MODE == BORN_S || MODE == RTM_BWD);
+  constexpr bool GZ =
+      (MODE == MODEL || MODE == BORN || MODE == BORN_S || MODE == RTM_BWD || MODE == PW_BACK || MODE == PW_FWD);
+  // record sample read per column: injected (RTM_BWD, PW_BACK) or removed
+  // from the previous level (PW_FWD)
+  constexpr bool RECIN = (MODE == RTM_BWD || MODE == PW_BACK || MODE == PW_FWD);
+  const int rec_t = MODE == PW_FWD ? c.it - s.ndel - 1 : c.itp;
   T nca, ncb = T(0), npa, npb = T(0), nv, nha = T(0), nhb = T(0), nrec = T(0), nss = T(0);
   int ngz = -1;
   const bool acc = MODE == ILLUM && c.it > s.ndel && act;  // accumulate u^2 (interior cells are model cells)
@@ -331,14 +383,17 @@ __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot
       if (TWO) nhb = CB[r + hz];
     }
     if (GZ) ngz = GZA[xx - pml];
-    if (MODE == RTM_BWD) nrec = REC[(long long)(xx - pml) * s.nt + c.itp];
+    if (RECIN && rec_t >= 0) nrec = REC[(long long)(xx - pml) * s.nt + rec_t];
     if (acc) nss = SSA[r + zl];
   };
   fetch(x0);
   // per-step scalars
-  constexpr bool PSRC = (MODE != RTM_BWD && MODE != BORN_S);
+  constexpr bool PSRC = (MODE != RTM_BWD && MODE != BORN_S && MODE != PW_BACK);
   T amp = T(0);
-  if (PSRC && c.it - 1 >= 0) amp = s.wav[c.it - 1];
+  if (PSRC && c.it - 1 >= 0 && !s.line_src) amp = s.wav[c.it - 1];
+  // the plane-wave line source: this lane's row carries it
+  const bool lrow = PSRC && s.line_src && c.it - 1 >= 0 && z == szp;
+  const long long lstride = s.nt + s.ndel;
   T sub = T(0);
   if (MODE == RTM_BWD && c.it - 1 > 0) {
     const T as = k.dt * c.m.vel(sxp, szp);
@@ -378,7 +433,11 @@ __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot
     T pa = pa_cur;
     if (MODE == RTM_BWD && sub != T(0) && x == sxp && z == szp) pa = pa - sub;
     T ua = vvzz * uzz + vvxx * uxx + T(2) * qa[4] - pa;
-    if (PSRC && x == sxp && z == szp && c.it - 1 >= 0) ua = ua + vt2 * amp;
+    if (lrow) {
+      if (ix >= 0 && ix < s.nx) ua = ua + vt2 * SEIS_GL(s.wav)[(long long)ix * lstride + c.it - 1];
+    } else if (PSRC && !s.line_src && x == sxp && z == szp && c.it - 1 >= 0) {
+      ua = ua + vt2 * amp;
+    }
     if (MODE == SNAP_MAX || MODE == SNAP_STORE) {
       // Born source coefficient of this step (fd.cpp:134-139): vt2 * lap * 2/v^3
       const T lap = (ua + pa - T(2) * qa[4]) * k.idt2;
@@ -403,7 +462,9 @@ __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot
       uxxb = uxxb + qb[6] * k.c[2];
       uxxb = uxxb + qb[7] * k.c[3];
       uxxb = uxxb + qb[8] * k.c[4];
-      ub = vvzz * uzzb + vvxx * uxxb + T(2) * qb[4] - pb_cur;
+      T pb = pb_cur;
+      if (MODE == PW_FWD && rec_t >= 0 && z == gzr) pb = pb - rec_cur * vt2;
+      ub = vvzz * uzzb + vvxx * uxxb + T(2) * qb[4] - pb;
     }
     wave_sync();
     if (act) {
@@ -437,6 +498,21 @@ __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot
         NA[i] = ua;
         NB[i] = ub;
         if (c.itp >= 0 && z == gzr) REC[(long long)ix * s.nt + c.itp] = ub;
+      } else if (MODE == PW_BACK) {
+        if (rec_t >= 0 && z == gzr) ua = ua + rec_cur * vt2;
+        NA[i] = ua;
+        if (c.it >= 1) {
+          int sl[4];
+          const int n = strip_slots(x, z, s.nx, k.nz, pml, sl);
+          const auto st = SEIS_GL(s.strips + (long long)(c.it - 1) * 8 * (s.nx + k.nz));
+          for (int q = 0; q < n; q++) st[sl[q]] = ua;
+        }
+      } else if (MODE == PW_FWD) {
+        const T lap = (ua + pa - T(2) * qa[4]) * k.idt2;
+        const T v3 = T(2) / (v * v * v);
+        SEIS_GL(c.aux)[i] = lap * v3;
+        NA[i] = ua;
+        NB[i] = ub;
       } else {  // RTM_BWD: source field ua, receiver field ub
         if (z == gzr) ub = ub + rec_cur * vt2;
         const T lap = (pa + ua - T(2) * qa[4]) * k.idt2;
@@ -468,13 +544,8 @@ __global__ __launch_bounds__(WG) void step_kernel(Consts<T> k, const DevShot<T>*
   __shared__ T lds[WAVES][2 * LROW];
   const DevShot<T>& s = shots[blockIdx.y];
   StepCtx<T> c;
-  if (MODE == RTM_BWD) {
-    c.it = s.nt + s.ndel - 1 - step;
-    if (c.it < s.ndel) return;
-  } else {
-    c.it = step;
-    if (c.it >= s.nt + s.ndel) return;
-  }
+  c.it = step_it<T>(MODE, s, step);
+  if (c.it < 0) return;
   c.itp = c.it - s.ndel;
   const int r0 = step % 3, r1 = (step + 1) % 3, r2 = (step + 2) % 3;
   c.P = s.fld + r0 * s.fstride;
@@ -513,15 +584,17 @@ __global__ __launch_bounds__(WG) void step_kernel(Consts<T> k, const DevShot<T>*
 // U R over the whole padded grid but returns only the interior, which is all
 // that is accumulated here; the two additions stay separate roundings.
 // Even taps are zero and are skipped (adding 0 * x leaves the sum unchanged).
+constexpr int HILBERT_HALF = 30;  // FaQi's filter half length (fd.cpp:1276)
 struct HilbertTaps {
-  float h[61];
+  float h[2 * HILBERT_HALF + 1];
 };
 
 // One step's term at row z of a column held in LDS (sU, sR over NZ rows),
-// added to img in the reference's order.
+// added to img in the reference's order: U R first unless `faqi_only`
+// (adjointPlane images with FaQi alone, fd.cpp:868-870).
 template <int HALF>
 __device__ __forceinline__ float image_term(const HilbertTaps& taps, const float* sU, const float* sR, int z, int NZ,
-                                            float img) {
+                                            float img, bool faqi_only = false) {
   float uh = 0.f, rh = 0.f;
   if (z >= HALF && z + HALF <= NZ - 1) {
     // full window: inputs z-HALF+1 .. z+HALF-1 at odd offsets, tap index
@@ -543,7 +616,7 @@ __device__ __forceinline__ float image_term(const HilbertTaps& taps, const float
   }
   const float scal = 2.0f / 4.0f;
   const float ur = sU[z] * sR[z];
-  img += ur;
+  if (!faqi_only) img += ur;
   img += scal * (ur - uh * rh);
   return img;
 }
@@ -551,17 +624,19 @@ __device__ __forceinline__ float image_term(const HilbertTaps& taps, const float
 // Reverse steps `step` and, when `pair`, step + 1 applied to one interior
 // column (the image is read and written once for both).  U of step st is in
 // aux field st % NAUX; R of step st is in field slot 3 + (st + 2) % 3.
+// `plane`: step `step` of adjointPlane's forward pass instead (it = step+2,
+// imaged when it > ndel, FaQi term only; never paired).
 template <int HALF>
 __global__ __launch_bounds__(128) void image_kernel(Consts<float> k, const DevShot<float>* __restrict__ shots,
-                                                    int step, HilbertTaps taps, int pair) {
+                                                    int step, HilbertTaps taps, int pair, int plane) {
   extern __shared__ unsigned char smem_raw[];
   float* sU0 = reinterpret_cast<float*>(smem_raw);
   const DevShot<float>& s = shots[blockIdx.y];
   const int x = k.pml + blockIdx.x;
   if (x >= s.NX - k.pml) return;
-  const int it = s.nt + s.ndel - 1 - step;
-  if (it < s.ndel) return;
-  const bool two = pair && it - 1 >= s.ndel;
+  const int it = plane ? step + 2 : s.nt + s.ndel - 1 - step;
+  if (plane ? (it >= s.nt + s.ndel || it - s.ndel <= 0) : it < s.ndel) return;
+  const bool two = !plane && pair && it - 1 >= s.ndel;
   const int NZ = k.NZ;
   float* sR0 = sU0 + NZ;
   float* sU1 = sR0 + NZ;
@@ -583,7 +658,7 @@ __global__ __launch_bounds__(128) void image_kernel(Consts<float> k, const DevSh
   __syncthreads();
   for (int z = k.pml + threadIdx.x; z < NZ - k.pml; z += blockDim.x) {
     float img = IMG[z];
-    img = image_term<HALF>(taps, sU0, sR0, z, NZ, img);
+    img = image_term<HALF>(taps, sU0, sR0, z, NZ, img, plane != 0);
     if (two) img = image_term<HALF>(taps, sU1, sR1, z, NZ, img);
     IMG[z] = img;
   }
@@ -598,6 +673,28 @@ __global__ void restore_strips_kernel(Consts<T> k, const DevShot<T>* __restrict_
   const int itp = from_end ? s.nt - 1 : 0;
   T* F = s.fld + slot * s.fstride;
   const T* st = s.strips + (long long)itp * n;
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+    int x, z;
+    if (j < 4 * nx) { x = j / 4 + pml; z = pml + j % 4; }
+    else if (j < 8 * nx) { int q = j - 4 * nx; x = q / 4 + pml; z = nz + pml - 1 - q % 4; }
+    else if (j < 8 * nx + 4 * nz) { int q = j - 8 * nx; x = pml + q / nz; z = q % nz + pml; }
+    else { int q = j - 8 * nx - 4 * nz; x = nx + pml - 1 - q / nz; z = q % nz + pml; }
+    F[fidx(k.ld, x, z)] = st[j];
+  }
+}
+
+// adjointPlane's forward pass: the strips saved at time it = step + 2 go
+// into the receiver field's new level (after the imaging of the step,
+// fd.cpp:878).
+template <typename T>
+__global__ void restore_plane_strips_kernel(Consts<T> k, const DevShot<T>* __restrict__ shots, int step) {
+  const DevShot<T>& s = shots[blockIdx.y];
+  const int it = step + 2;
+  if (it >= s.nt + s.ndel) return;
+  const int nx = s.nx, nz = k.nz, pml = k.pml;
+  const int n = 8 * (nx + nz);
+  T* F = s.fld + (3 + (step + 2) % 3) * s.fstride;
+  const T* st = s.strips + (long long)it * n;
   for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
     int x, z;
     if (j < 4 * nx) { x = j / 4 + pml; z = pml + j % 4; }
@@ -642,6 +739,8 @@ void launch_step(int mode, const Consts<T>& k, const DevShot<T>* dev, const Layo
     SEIS_CASE(SNAP_STORE)
     SEIS_CASE(BORN_S)
     SEIS_CASE(ILLUM)
+    SEIS_CASE(PW_BACK)
+    SEIS_CASE(PW_FWD)
 #undef SEIS_CASE
   }
 }
@@ -709,7 +808,7 @@ int check_grid(const GridSpec& g) {
 
 template <typename T>
 int run_batch(int mode, const GridSpec& g, int nshots, const ShotSpec* shots, const T* vel, const T* dv,
-              const T* wav, const int* gz, T* rec, T* img, hipStream_t stream, int rtm_flags) {
+              const T* wav, const int* gz, T* rec, T* img, hipStream_t stream, int rtm_flags, int line_src) {
   int rc = check_grid(g);
   if (rc) return rc;
   if (nshots <= 0) return SEIS_OK;
@@ -727,7 +826,7 @@ int run_batch(int mode, const GridSpec& g, int nshots, const ShotSpec* shots, co
   for (int s = 0; s < nshots; s++) {
     const ShotSpec& q = shots[s];
     if (q.nt <= 0 || q.ndel < 0) return set_error("shot: nt must be > 0 and ndel >= 0"), SEIS_ERR_ARG;
-    if (q.sx < 0 || q.sx >= g.nx || q.sz < 0 || q.sz >= g.nz)
+    if ((!line_src && (q.sx < 0 || q.sx >= g.nx)) || q.sz < 0 || q.sz >= g.nz)
       return set_error("shot: source cell outside the model"), SEIS_ERR_ARG;
     ntmax = std::max(ntmax, q.nt);
     NTmax = std::max(NTmax, q.nt + q.ndel);
@@ -772,6 +871,7 @@ int run_batch(int mode, const GridSpec& g, int nshots, const ShotSpec* shots, co
     d.sx = q.sx;
     d.sz = q.sz;
     d.rtm_flags = rtm_flags;
+    d.line_src = line_src;
   }
   HIPCHECK(hipMalloc(&br.dev, sizeof(DevShot<T>) * nshots));
   HIPCHECK(hipMemcpyAsync(br.dev, br.host.data(), sizeof(DevShot<T>) * nshots, hipMemcpyHostToDevice, stream));
@@ -855,7 +955,7 @@ int run_batch(int mode, const GridSpec& g, int nshots, const ShotSpec* shots, co
     for (int s = 0; s < nshots; s++)
       HIPCHECK(hipMemsetAsync(p + per_shot * s, 0, sizeof(T) * (size_t)fe * nslots, stream));
     hipLaunchKernelGGL((restore_strips_kernel<T>), dim3(16, nshots), dim3(256), 0, stream, k, br.dev, 1, 1);
-    constexpr int half = 30;
+    constexpr int half = HILBERT_HALF;
     HilbertTaps taps{};
     hilbert_taps_host<float>(half, taps.h);
     const size_t smem1 = sizeof(T) * 2 * NZ;
@@ -888,7 +988,7 @@ int run_batch(int mode, const GridSpec& g, int nshots, const ShotSpec* shots, co
       timed_launch(is.s, "rtm_image", 2 * E * faqi_cells * (n0 + (pair ? n : 0)) + 2 * E * n0 * (double)g.nx * g.nz,
                    0, [&] {
                      hipLaunchKernelGGL((image_kernel<half>), dim3(g.nx, nshots), dim3(128), smem, is.s, k, br.dev,
-                                        first, taps, pair);
+                                        first, taps, pair, 0);
                    });
       HIPCHECK(hipEventRecord(is.done[q % ImageStream::NEV], is.s));
     }
@@ -903,6 +1003,144 @@ int run_batch(int mode, const GridSpec& g, int nshots, const ShotSpec* shots, co
 }
 
 template int run_batch<float>(int, const GridSpec&, int, const ShotSpec*, const float*, const float*, const float*,
-                              const int*, float*, float*, hipStream_t, int);
+                              const int*, float*, float*, hipStream_t, int, int);
+
+template <typename T>
+int run_rtm_plane(const GridSpec& g, int nshots, const ShotSpec* shots, const T* vel, const T* sou, const int* gz,
+                  const T* rec, T* img, hipStream_t stream) {
+  int rc = check_grid(g);
+  if (rc) return rc;
+  if (nshots <= 0) return SEIS_OK;
+  if (g.nx < 4 || g.nz < 4)
+    return set_error("rtm_plane: the 4-layer boundary strips need nx >= 4 and nz >= 4"), SEIS_ERR_ARG;
+  const int NZ = g.nz + 2 * g.pml;
+  const int NX = g.nx + 2 * g.pml;
+  const long long ld = field_ld(NZ);
+  const long long fe = field_elems(NX, ld);
+  int NTmax = 0;
+  size_t strip_elems = 0;
+  for (int s = 0; s < nshots; s++) {
+    const ShotSpec& q = shots[s];
+    if (q.nt <= 0 || q.ndel < 0) return set_error("shot: nt must be > 0 and ndel >= 0"), SEIS_ERR_ARG;
+    if (q.sz < 0 || q.sz >= g.nz) return set_error("shot: source row outside the model"), SEIS_ERR_ARG;
+    NTmax = std::max(NTmax, q.nt + q.ndel);
+    strip_elems += (size_t)(q.nt + q.ndel) * 8 * (g.nx + g.nz);
+  }
+  // per shot: source field and receiver field (3 levels each), their band
+  // stencil values, the image and the U buffers; then strips of every step
+  const int nslots = 6, nraw = 2;
+  const size_t per_shot = (size_t)fe * (nslots + nraw + 1 + NAUX);
+  const size_t total = per_shot * nshots + strip_elems;
+  BatchRun<T> br;
+  br.pool_bytes = total * sizeof(T);
+  HIPCHECK(hipMalloc(&br.pool, br.pool_bytes));
+  // all zero, strips included: the strips saved at the last step are those
+  // of the zero level the backward pass starts from
+  HIPCHECK(hipMemsetAsync(br.pool, 0, br.pool_bytes, stream));
+  T* p = reinterpret_cast<T*>(br.pool);
+  T* strip_p = p + per_shot * nshots;
+  const long long origin = (long long)MX * ld + MZ;
+  br.host.resize(nshots);
+  for (int s = 0; s < nshots; s++) {
+    const ShotSpec& q = shots[s];
+    DevShot<T>& d = br.host[s];
+    T* base = p + per_shot * s;
+    d = DevShot<T>{};
+    d.vel = vel + q.model;
+    d.wav = sou + q.wav;
+    d.gz = gz + q.gz;
+    d.rec = const_cast<T*>(rec) + q.rec;
+    d.fld = base + origin;
+    d.fstride = fe;
+    d.raw = base + (size_t)fe * nslots + origin;
+    d.img = base + (size_t)fe * (nslots + nraw) + origin;
+    d.aux = base + (size_t)fe * (nslots + nraw + 1) + origin;
+    d.strips = strip_p;
+    strip_p += (size_t)(q.nt + q.ndel) * 8 * (g.nx + g.nz);
+    d.nx = g.nx;
+    d.NX = NX;
+    d.nt = q.nt;
+    d.ndel = q.ndel;
+    d.sx = 0;
+    d.sz = q.sz;
+    d.line_src = 1;
+  }
+  HIPCHECK(hipMalloc(&br.dev, sizeof(DevShot<T>) * nshots));
+  HIPCHECK(hipMemcpyAsync(br.dev, br.host.data(), sizeof(DevShot<T>) * nshots, hipMemcpyHostToDevice, stream));
+  const Consts<T> k = make_consts<T>(g, ld);
+  Layout L = make_layout(NX, NZ, g.pml);
+  const Ring B = band_ring(NX, NZ, g.pml);
+  {
+    // strip length: a backward-pass step 0 is a trial that writes only what
+    // the real step 0 writes again (its new level, band values, strips)
+    struct Trial {
+      const Consts<T>* k;
+      const DevShot<T>* dev;
+      Layout L;
+      Ring B;
+      int nshots;
+      hipStream_t s;
+    };
+    Trial tr{&k, br.dev, L, B, nshots, stream};
+    auto fn = [](void* c, int bx) {
+      const Trial* t = static_cast<const Trial*>(c);
+      launch_step<T>(PW_BACK, *t->k, t->dev, with_strip(t->L, bx), t->B, t->nshots, 0, t->s);
+    };
+    L = with_strip(L, tuned_strip("rtm_plane", NX, NZ, nshots, stream, fn, &tr));
+  }
+  const double cells = (double)NX * NZ;
+  const double E = sizeof(T);
+  const double strip = 8.0 * (g.nx + g.nz) * E;
+  auto active = [&](int mode, int st) {
+    int n = 0;
+    for (int s = 0; s < nshots; s++) n += step_it<T>(mode, br.host[s], st) >= 0;
+    return n;
+  };
+  // pass 1: receiver field backward in time
+  for (int st = 0; st < NTmax; st++) {
+    const int n = active(PW_BACK, st);
+    timed_launch(stream, "step_plane_back", 4 * E * n * cells + strip * n, n * cells, [&] {
+      launch_step<T>(PW_BACK, k, br.dev, L, B, nshots, st, stream);
+    });
+  }
+  // its last two levels start the forward rebuild (fd.cpp:848-852): the
+  // level of time 0 as previous, of time 1 as current; the source field
+  // starts from zero
+  for (int s = 0; s < nshots; s++) {
+    const int NT = shots[s].nt + shots[s].ndel;
+    T* base = p + per_shot * s;
+    HIPCHECK(hipMemcpyAsync(base + (size_t)fe * 3, base + (size_t)fe * ((NT + 1) % 3), sizeof(T) * fe,
+                            hipMemcpyDeviceToDevice, stream));
+    HIPCHECK(hipMemcpyAsync(base + (size_t)fe * 4, base + (size_t)fe * (NT % 3), sizeof(T) * fe,
+                            hipMemcpyDeviceToDevice, stream));
+    HIPCHECK(hipMemsetAsync(base, 0, sizeof(T) * (size_t)fe * 3, stream));
+  }
+  // pass 2: line-source field forward, receiver field rebuilt forward,
+  // FaQi imaging, strips restored into the receiver field's new level
+  constexpr int half = HILBERT_HALF;
+  HilbertTaps taps{};
+  hilbert_taps_host<float>(half, taps.h);
+  const size_t smem = sizeof(T) * 2 * NZ;
+  if (smem > 64 * 1024) return set_error("rtm_plane: padded nz too large for the column Hilbert kernel"), SEIS_ERR_ARG;
+  const double faqi_cells = (double)g.nx * NZ;
+  for (int st = 0; st + 2 < NTmax; st++) {
+    const int n = active(PW_FWD, st);
+    timed_launch(stream, "step_plane_fwd", 8 * E * n * cells, 2 * n * cells, [&] {
+      launch_step<T>(PW_FWD, k, br.dev, L, B, nshots, st, stream);
+    });
+    timed_launch(stream, "plane_image", 2 * E * n * faqi_cells + 2 * E * n * (double)g.nx * g.nz, 0, [&] {
+      hipLaunchKernelGGL((image_kernel<half>), dim3(g.nx, nshots), dim3(128), smem, stream, k, br.dev, st, taps, 0,
+                         1);
+    });
+    hipLaunchKernelGGL((restore_plane_strips_kernel<T>), dim3(16, nshots), dim3(256), 0, stream, k, br.dev, st);
+  }
+  hipLaunchKernelGGL((crop_kernel<T>), dim3(64, nshots), dim3(256), 0, stream, k, br.dev, img);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipStreamSynchronize(stream));
+  return SEIS_OK;
+}
+
+template int run_rtm_plane<float>(const GridSpec&, int, const ShotSpec*, const float*, const float*, const int*,
+                                  const float*, float*, hipStream_t);
 
 }  // namespace seis

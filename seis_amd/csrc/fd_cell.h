@@ -98,19 +98,28 @@ struct NoInject {
   template <typename T>
   SEIS_HD inline T add(T val, int, int, T, long long) const { return val; }
   template <typename T>
-  SEIS_HD inline T prev(const T* P, long long i) const { return SEIS_GL(P)[i]; }
+  SEIS_HD inline T prev(const T* P, long long i, int, int) const { return SEIS_GL(P)[i]; }
 };
 
-// Point source u += vt2 * wav[it-1] at one cell (fd.cpp:49-51).
+// Source term u += vt2 * amp: at one cell with amp = wav[it-1]
+// (fd.cpp:49-51), or, for the plane-wave engine, at every model column ix of
+// one row with that column's own sample sou[ix][it-1] (fd.cpp:223-226).
 template <typename T>
 struct PointSource {
   long long src;  // field index of the source cell, -1 when inactive this step
   T amp;          // wav[it-1]
-  SEIS_HD inline T add(T val, int, int, T vt2, long long i) const {
+  const T* line = nullptr;  // line source: column ix's sample at line[ix * lstride]
+  long long lstride = 0;
+  int lrow = -1, lx0 = 0, lnx = 0;  // padded row, first model column, model columns
+  SEIS_HD inline T add(T val, int x, int z, T vt2, long long i) const {
+    if (line) {
+      const int ix = x - lx0;
+      return (z == lrow && ix >= 0 && ix < lnx) ? val + vt2 * SEIS_GL(line)[ix * lstride] : val;
+    }
     return i == src ? val + vt2 * amp : val;
   }
   template <typename U>
-  SEIS_HD inline U prev(const U* P, long long i) const { return SEIS_GL(P)[i]; }
+  SEIS_HD inline U prev(const U* P, long long i, int, int) const { return SEIS_GL(P)[i]; }
 };
 
 // Reverse-time source removal from the previous level (fd.cpp:659-661):
@@ -121,13 +130,13 @@ struct SourceRemoval {
   T sub;  // vt2[src] * wav[it-1], 0 when inactive
   bool on;
   SEIS_HD inline T add(T val, int, int, T, long long) const { return val; }
-  SEIS_HD inline T prev(const T* P, long long i) const {
+  SEIS_HD inline T prev(const T* P, long long i, int, int) const {
     return (on && i == src) ? SEIS_GL(P)[i] - sub : SEIS_GL(P)[i];
   }
 };
 
 // Receiver-side injection u += rec[ix][itp] * vt2 at (ix+pml, gz[ix]+pml)
-// for every model column (fd.cpp:673-677).
+// for every model column (fd.cpp:673-677); nothing for itp < 0.
 template <typename T>
 struct ReceiverInject {
   const int* gz;      // [nx] receiver depth per column (model cells)
@@ -135,11 +144,33 @@ struct ReceiverInject {
   int nx, nt, itp, pml;
   SEIS_HD inline T add(T val, int x, int z, T vt2, long long) const {
     int ix = x - pml;
-    if (ix >= 0 && ix < nx && z == SEIS_GL(gz)[ix] + pml) return val + SEIS_GL(rec)[(long long)ix * nt + itp] * vt2;
+    if (itp >= 0 && ix >= 0 && ix < nx && z == SEIS_GL(gz)[ix] + pml)
+      return val + SEIS_GL(rec)[(long long)ix * nt + itp] * vt2;
     return val;
   }
   template <typename U>
-  SEIS_HD inline U prev(const U* P, long long i) const { return SEIS_GL(P)[i]; }
+  SEIS_HD inline U prev(const U* P, long long i, int, int) const { return SEIS_GL(P)[i]; }
+};
+
+// Record removal from the previous level, P -= rec[ix][itp] * vt2 at the
+// receiver cells (adjointPlane's forward rebuild of the receiver field,
+// fd.cpp:862-866); nothing for itp < 0.
+template <typename T>
+struct ReceiverRemoval {
+  const int* gz;
+  const T* rec;
+  int nx, nt, itp, pml;
+  T dt;
+  const Model<T>* m;
+  SEIS_HD inline T add(T val, int, int, T, long long) const { return val; }
+  SEIS_HD inline T prev(const T* P, long long i, int x, int z) const {
+    const int ix = x - pml;
+    if (itp >= 0 && ix >= 0 && ix < nx && z == SEIS_GL(gz)[ix] + pml) {
+      const T a = dt * m->vel(x, z);
+      return SEIS_GL(P)[i] - SEIS_GL(rec)[(long long)ix * nt + itp] * (a * a);
+    }
+    return SEIS_GL(P)[i];
+  }
 };
 
 // ---------------------------------------------------------------------------
@@ -159,7 +190,7 @@ SEIS_HD inline T raw_value(const Consts<T>& k, const Model<T>& m, int NX, const 
     auto cu = [&](int xx, int zz) { return SEIS_GL(C)[fidx(ld, xx, zz)]; };
     T r;
     const T c0 = cu(x, z);
-    const T p0 = inj.prev(P, i);
+    const T p0 = inj.prev(P, i, x, z);
     if (xe && ze) {
       r = T(2) * c0 - p0;
     } else if (ze) {
@@ -219,7 +250,7 @@ SEIS_HD inline T raw_value(const Consts<T>& k, const Model<T>& m, int NX, const 
     uzz = SEIS_GL(C)[i - 1] + SEIS_GL(C)[i + 1] - T(2) * c0;
     uxx = SEIS_GL(C)[i + ld] + SEIS_GL(C)[i - ld] - T(2) * c0;
   }
-  T val = vvzz * uzz + vvxx * uxx + T(2) * c0 - inj.prev(P, i);
+  T val = vvzz * uzz + vvxx * uxx + T(2) * c0 - inj.prev(P, i, x, z);
   return inj.add(val, x, z, vt2, i);
 }
 
@@ -242,7 +273,7 @@ SEIS_HD inline T band_blend(const Consts<T>& k, const Model<T>& m, int NX, const
   const long long ld = k.ld;
   const long long i = fidx(ld, x, z);
   auto Cu = [&](int xx, int zz) { return SEIS_GL(C)[fidx(ld, xx, zz)]; };
-  auto Pv = [&](int xx, int zz) { return inj.prev(P, fidx(ld, xx, zz)); };
+  auto Pv = [&](int xx, int zz) { return inj.prev(P, fidx(ld, xx, zz), xx, zz); };
   const T v = m.vel(x, z);
   T ow;
   if ((dtp == mn || db == mn) && (dl == mn || dr == mn)) {

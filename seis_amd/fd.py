@@ -71,3 +71,43 @@ def illum(dt, dx, dz, nt, ndel, nx, nz, pml, sx, sz, wav, vel):
     ss = np.zeros((nx, nz), np.float32)
     check(lib().seis_illum(dt, dx, dz, nt, ndel, nx, nz, pml, sx, sz, _p(wav), _p(vel), _p(ss)))
     return ss
+
+
+def _plane_prep(nt, ndel, nx, nz, vel, sou):
+    vel = np.ascontiguousarray(vel, np.float32)
+    sou = np.ascontiguousarray(sou, np.float32)
+    if vel.shape != (nx, nz):
+        raise ValueError("vel must be [nx][nz]")
+    if sou.shape != (nx, nt + ndel):
+        raise ValueError("sou must be [nx][nt + ndel]")
+    return vel, sou
+
+
+def forward_plane(dt, dx, dz, nt, ndel, nx, nz, pml, sz, gz, vel, sou, dv):
+    """Born record rec[nx][nt] of the plane-wave line source sou[nx][nt+ndel]
+    fired at row sz, receivers at row gz (forwordPlane, fd.cpp:178-274)."""
+    vel, sou = _plane_prep(nt, ndel, nx, nz, vel, sou)
+    dv = np.ascontiguousarray(dv, np.float32)
+    rec = np.zeros((nx, nt), np.float32)
+    check(lib().seis_born_plane(dt, dx, dz, nt, ndel, nx, nz, pml, sz, gz, _p(vel), _p(sou), _p(dv), _p(rec)))
+    return rec
+
+
+def illum_plane(dt, dx, dz, nt, ndel, nx, nz, pml, sz, gz, vel, sou):
+    """Illumination ss[nx][nz] of the line source (illumPlane, fd.cpp:720-786)."""
+    vel, sou = _plane_prep(nt, ndel, nx, nz, vel, sou)
+    ss = np.zeros((nx, nz), np.float32)
+    check(lib().seis_illum_plane(dt, dx, dz, nt, ndel, nx, nz, pml, sz, gz, _p(vel), _p(sou), _p(ss)))
+    return ss
+
+
+def adjoint_plane(dt, dx, dz, nt, ndel, nx, nz, pml, sz, gz, vel, sou, rec):
+    """Plane-wave image img[nx][nz] of record rec[nx][nt] (adjointPlane,
+    fd.cpp:787-921)."""
+    vel, sou = _plane_prep(nt, ndel, nx, nz, vel, sou)
+    rec = np.ascontiguousarray(rec, np.float32)
+    if rec.shape != (nx, nt):
+        raise ValueError("rec must be [nx][nt]")
+    img = np.zeros((nx, nz), np.float32)
+    check(lib().seis_rtm_plane(dt, dx, dz, nt, ndel, nx, nz, pml, sz, gz, _p(vel), _p(sou), _p(rec), _p(img)))
+    return img

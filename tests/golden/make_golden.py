@@ -4,7 +4,8 @@ Run in the container that holds the reference tree, after `make -C oracle ref`:
 
     python tests/golden/make_golden.py
 
-It calls the reference's own modeling / forward / adjoint / illum (src/util/fd.h:3-5,10)
+It calls the reference's own modeling / forward / adjoint / illum and the
+plane-wave forwordPlane / adjointPlane / illumPlane (src/util/fd.h:3-11)
 through oracle/_ref/libseisref.so and stores inputs and outputs as .npz
 fixtures next to this script.  The fixtures are data only; they pin the CPU
 oracle (tests/test_oracle_golden.py) and, through it, the HIP engine.
@@ -60,8 +61,28 @@ def run_case(ref, c, what):
     return out
 
 
+def plane_case(ref, seed, nx, nz, pml, nt, sz, gz, dip, dt=0.001, dx=10.0, dz=10.0):
+    """Plane-wave case: the line source's column ix fires the reference
+    Ricker delayed by dip * ix samples (sou is [nx][nt], ndel = 0)."""
+    rng = np.random.default_rng(seed)
+    vel = layered(nx, nz, dz, [nz * dz / 2], [1800.0, 2600.0])
+    vel += rng.uniform(-60, 60, vel.shape).astype(np.float32)
+    fr, delay = 20.0, 25
+    wav = ref.ricker(dt, fr, delay, nt)
+    sou = np.zeros((nx, nt), np.float32)
+    for ix in range(nx):
+        sh = int(dip * ix)
+        sou[ix, sh:] = wav[: nt - sh]
+    dv = (rng.standard_normal((nx, nz)) * (rng.random((nx, nz)) < 0.05) * 50).astype(np.float32)
+    rec_in = rng.standard_normal((nx, nt)).astype(np.float32)
+    return dict(nx=nx, nz=nz, pml=pml, nt=nt, ndel=0, dt=dt, dx=dx, dz=dz, sx=0, sz=sz, gz_row=gz, fr=fr,
+                delay=delay, vel=vel, sou=sou, dv=dv, rec_in=rec_in)
+
+
 def save(name, c, out, note):
     meta = {k: c[k] for k in ("nx", "nz", "pml", "nt", "ndel", "dt", "dx", "dz", "sx", "sz", "fr", "delay")}
+    if "gz_row" in c:
+        meta["gz_row"] = c["gz_row"]
     meta["note"] = note
     arrays = {k: v for k, v in c.items() if isinstance(v, np.ndarray)}
     arrays.update(out)
@@ -82,6 +103,15 @@ def main():
         c["wav"] = ref.ricker(c["dt"], c["fr"], c["delay"] + c["ndel"], c["nt"] + c["ndel"])
         out = run_case(ref, c, ("modeling", "born", "rtm", "illum"))
         save(name, c, out, "seeded small grid; modeling, born, rtm and illum of the reference engine")
+
+    # Plane-wave engine (fd.h:6-7, 11): forwordPlane, illumPlane, adjointPlane.
+    for name, args in (("plane_a.npz", (21, 36, 30, 10, 120, 1, 2, 0.35)),
+                       ("plane_b.npz", (22, 50, 41, 14, 170, 0, 3, -0.0))):
+        c = plane_case(ref, *args)
+        a = (c["dt"], c["dx"], c["dz"], c["nt"], c["ndel"], c["nx"], c["nz"], c["pml"], c["sz"], c["gz_row"],
+             c["vel"], c["sou"])
+        out = {"born": ref.born_plane(*a, c["dv"]), "ss": ref.illum_plane(*a), "img": ref.adjoint_plane(*a, c["rec_in"])}
+        save(name, c, out, "plane-wave line source; forwordPlane, illumPlane and adjointPlane of the reference engine")
 
     # Plain forward modeling, 1 shot: 201 x 301 at 10 m, 3 layers 1500/2200/3000 m/s with
     # interfaces at 600 m and 1400 m, dt = 1 ms, nt = 1500, pml 30, Ricker 15 Hz (delay 67

@@ -99,6 +99,38 @@ class _Lib:
         return ss
 
 
+    # plane-wave engine (fd.h:6-7, 11): sou is [nx][nt+ndel], receivers at row gz
+    def _plane_args(self, dtype, dt, dx, dz, nt, ndel, nx, nz, pml, sz, vel, sou):
+        S = self._scalar(dtype)
+        vel = np.ascontiguousarray(vel, dtype)
+        sou = np.ascontiguousarray(sou, dtype)
+        assert vel.shape == (nx, nz) and sou.shape == (nx, nt + ndel)
+        head = [S(dt), S(dx), S(dz), _I(nt), _I(ndel), _I(nx), _I(nz), _I(pml), _I(sz)]
+        return head, vel, sou
+
+    def born_plane(self, dt, dx, dz, nt, ndel, nx, nz, pml, sz, gz, vel, sou, dv, dtype=np.float32):
+        head, vel, sou = self._plane_args(dtype, dt, dx, dz, nt, ndel, nx, nz, pml, sz, vel, sou)
+        dv = np.ascontiguousarray(dv, dtype)
+        rec = np.zeros((nx, nt), dtype)
+        self._fn("born_plane", dtype)(*head, _I(gz), _ptr(vel), _ptr(sou), _ptr(dv), _ptr(rec))
+        return rec
+
+    def illum_plane(self, dt, dx, dz, nt, ndel, nx, nz, pml, sz, gz, vel, sou, dtype=np.float32):
+        head, vel, sou = self._plane_args(dtype, dt, dx, dz, nt, ndel, nx, nz, pml, sz, vel, sou)
+        ss = np.zeros((nx, nz), dtype)
+        extra = [] if self.typed else [_I(gz)]  # the reference's illumPlane takes (and ignores) gz
+        self._fn("illum_plane", dtype)(*head, *extra, _ptr(vel), _ptr(sou), _ptr(ss))
+        return ss
+
+    def adjoint_plane(self, dt, dx, dz, nt, ndel, nx, nz, pml, sz, gz, vel, sou, rec, dtype=np.float32):
+        head, vel, sou = self._plane_args(dtype, dt, dx, dz, nt, ndel, nx, nz, pml, sz, vel, sou)
+        rec = np.ascontiguousarray(rec, dtype)
+        assert rec.shape == (nx, nt)
+        img = np.zeros((nx, nz), dtype)
+        self._fn("adjoint_plane", dtype)(*head, _I(gz), _ptr(vel), _ptr(sou), _ptr(rec), _ptr(img))
+        return img
+
+
 class Oracle(_Lib):
     def __init__(self, path=ORACLE_SO):
         super().__init__(path, "oracle_", True)

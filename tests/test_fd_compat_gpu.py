@@ -64,3 +64,34 @@ def test_fd_h_entry_points_match_reference(engine, name):
     assert np.array_equal(_result(ss, c["nx"], c["nz"]), c["ss"])
     lib.seis_fd_free(ss)
     del keep, dv_a, rin_a
+
+
+@pytest.mark.parametrize("name", ["plane_a.npz", "plane_b.npz"])
+def test_fd_h_plane_entry_points_match_reference(engine, name):
+    """forwordPlane / adjointPlane / illumPlane (fd.h:6-7, 11) through their
+    mangled symbols with float** tables."""
+    lib = engine.lib()
+    c = load_fixture(name)
+    print("case fd.h plane", name, flush=True)
+    fns = {}
+    for k, mangled in (("born", "_Z12forwordPlanefffiiiiiiiPPfS0_S0_"), ("rtm", "_Z12adjointPlanefffiiiiiiiPPfS0_S0_"),
+                       ("illum", "_Z10illumPlanefffiiiiiiiPPfS0_")):
+        f = getattr(lib, mangled)
+        f.restype = PP
+        fns[k] = f
+    head = (F(c["dt"]), F(c["dx"]), F(c["dz"]), I(c["nt"]), I(c["ndel"]), I(c["nx"]), I(c["nz"]), I(c["pml"]),
+            I(c["sz"]), I(c["gz_row"]))
+    vel_t, vel_a = _table(c["vel"])
+    sou_t, sou_a = _table(c["sou"])
+    dv_t, dv_a = _table(c["dv"])
+    rin_t, rin_a = _table(c["rec_in"])
+    out = fns["born"](*head, vel_t, sou_t, dv_t)
+    assert np.array_equal(_result(out, c["nx"], c["nt"]), c["born"])
+    lib.seis_fd_free(out)
+    out = fns["illum"](*head, vel_t, sou_t)
+    assert np.array_equal(_result(out, c["nx"], c["nz"]), c["ss"])
+    lib.seis_fd_free(out)
+    out = fns["rtm"](*head, vel_t, sou_t, rin_t)
+    assert np.array_equal(_result(out, c["nx"], c["nz"]), c["img"])
+    lib.seis_fd_free(out)
+    del vel_a, sou_a, dv_a, rin_a

@@ -174,3 +174,54 @@ def test_illum_batch_matches_oracle(engine, oracle, pml):
     for j, s in enumerate(shots):
         ref = oracle.illum(dt, 10.0, 10.0, s.nt, s.ndel, nx, nz, pml, s.sx, s.sz, wavs[j], v)
         assert np.array_equal(ss[j], ref), j
+
+
+PLANE = [n for n in fixture_names() if n.startswith("plane_")]
+
+
+@pytest.mark.parametrize("name", PLANE)
+def test_plane_wave_matches_reference(engine, name):
+    """forwordPlane / illumPlane / adjointPlane (fd.h:6-7, 11) against the
+    reference's golden vectors, bit for bit."""
+    c = load_fixture(name)
+    print("case plane", name, flush=True)
+    a = (c["dt"], c["dx"], c["dz"], c["nt"], c["ndel"], c["nx"], c["nz"], c["pml"], c["sz"], c["gz_row"], c["vel"],
+         c["sou"])
+    assert np.array_equal(engine.forward_plane(*a, c["dv"]), c["born"])
+    assert np.array_equal(engine.illum_plane(*a), c["ss"])
+    img = engine.adjoint_plane(*a, c["rec_in"])
+    assert rel_l2(img, c["img"]) <= IMAGE_TOL
+    assert np.array_equal(img, c["img"])
+
+
+@pytest.mark.parametrize("pml", [2, 10])
+def test_plane_wave_batch_matches_oracle(engine, oracle, pml):
+    """Plane-wave shots of uneven length and pre-roll in one batch, each as
+    the oracle gives it alone; pml 2 puts model cells in the ring path."""
+    from seis_amd.batch import Grid, Shot, born_plane_batch, illum_plane_batch, rtm_plane_batch
+    print("case plane_batch pml=%d" % pml, flush=True)
+    rng = np.random.default_rng(60 + pml)
+    nx, nz, dt = 42, 34, 0.001
+    g = Grid(nx, nz, pml, dt, 10.0, 10.0)
+    v = (1900.0 + rng.uniform(0, 700, (nx, nz))).astype(np.float32)
+    dv = (rng.standard_normal((nx, nz)) * (rng.random((nx, nz)) < 0.05) * 40).astype(np.float32)
+    specs = [(110, 0, 1, 2), (75, 4, 0, 3), (96, 9, 2, 1)]
+    shots = [Shot(sx=0, sz=sz, nt=nt, ndel=nd) for nt, nd, sz, _ in specs]
+    gzr = [r for _, _, _, r in specs]
+    sous = []
+    for s in shots:
+        w = oracle.ricker(dt, 25.0, 15, s.nt + s.ndel)
+        u = np.zeros((nx, s.nt + s.ndel), np.float32)
+        for ix in range(nx):
+            sh = int(0.4 * ix)
+            u[ix, sh:] = w[: s.nt + s.ndel - sh]
+        sous.append(u)
+    recs = [rng.standard_normal((nx, s.nt)).astype(np.float32) for s in shots]
+    born = born_plane_batch(g, shots, v, sous, gzr, dv)
+    ss = illum_plane_batch(g, shots, v, sous)
+    img = rtm_plane_batch(g, shots, v, sous, gzr, recs)
+    for j, s in enumerate(shots):
+        a = (dt, 10.0, 10.0, s.nt, s.ndel, nx, nz, pml, s.sz, gzr[j], v, sous[j])
+        assert np.array_equal(born[j], oracle.born_plane(*a, dv)), j
+        assert np.array_equal(ss[j], oracle.illum_plane(*a)), j
+        assert np.array_equal(img[j], oracle.adjoint_plane(*a, recs[j])), j

@@ -93,3 +93,38 @@ def test_fixture_wavelet_is_ricker(oracle):
     c = load_fixture("c1_modeling.npz")
     w = oracle.ricker(c["dt"], c["fr"], c["delay"] + c["ndel"], c["nt"] + c["ndel"])
     assert np.array_equal(w, c["wav"])
+
+
+PLANE = [n for n in fixture_names() if n.startswith("plane_")]
+
+
+def _plane_args(c):
+    return (c["dt"], c["dx"], c["dz"], c["nt"], c["ndel"], c["nx"], c["nz"], c["pml"], c["sz"], c["gz_row"],
+            c["vel"], c["sou"])
+
+
+@pytest.mark.parametrize("name", PLANE)
+def test_plane_wave_bitexact(oracle, name):
+    """forwordPlane / illumPlane / adjointPlane (fd.h:6-7, 11)."""
+    c = load_fixture(name)
+    a = _plane_args(c)
+    assert np.array_equal(oracle.born_plane(*a, c["dv"]), c["born"]), name
+    assert np.array_equal(oracle.illum_plane(*a), c["ss"]), name
+    assert np.array_equal(oracle.adjoint_plane(*a, c["rec_in"]), c["img"]), name
+
+
+def test_line_source_in_one_column_is_the_point_source(oracle):
+    """Independent of the plane-wave code: a line source that fires in one
+    column only is the point source there, so Born records and illumination
+    equal the (golden-pinned) point-source ones with all receivers at one
+    depth."""
+    c = load_fixture("small_c.npz")
+    nx, nt, ndel = c["nx"], c["nt"], c["ndel"]
+    sou = np.zeros((nx, nt + ndel), np.float32)
+    sou[c["sx"]] = c["wav"][: nt + ndel]
+    gzr = 2
+    gz = np.full(nx, gzr, np.int32)
+    pa = (c["dt"], c["dx"], c["dz"], nt, ndel, nx, c["nz"], c["pml"], c["sz"], gzr, c["vel"], sou)
+    pt = (c["dt"], c["dx"], c["dz"], nt, ndel, nx, c["nz"], c["pml"], c["sx"], c["sz"], gz, c["wav"], c["vel"])
+    assert np.array_equal(oracle.born_plane(*pa, c["dv"]), oracle.born(*pt, c["dv"]))
+    assert np.array_equal(oracle.illum_plane(*pa), oracle.illum(*_illum_args(c)))
