@@ -34,7 +34,7 @@ HBM_PEAK_GBS = 8000.0
 # rocprofv3 FETCH_SIZE / WRITE_SIZE (calibrated against a 1 GiB copy; the
 # commands and counter values are in profiles/rtm_batch_hbm_traffic.md).
 # Scaled by the number of shots in a launch.
-MEASURED_TRAFFIC_96 = {"step_rtm_forward": 4.819e+09, "step_rtm_reverse": 9.453e+09, "rtm_image": 2.821e+09}
+MEASURED_TRAFFIC_96 = {"step_rtm_forward": 4.615e+09, "step_rtm_reverse": 9.107e+09, "rtm_image": 4.291e+09}
 
 
 def rtm_workload(nshots, nt=5000):
@@ -136,6 +136,164 @@ def c1_single_shot(calls, with_cpu, threads):
     return out
 
 
+def c3_born_adjoint(nshots, pair_shots, seed=1003):
+    """Batched Born modeling with uneven shots: 500 x 1600 cells @ 5 m,
+    pml 30, dt = 0.5 ms, record lengths drawn in 2000..4000 and spreads of
+    200..1200 receivers, a sparse spike perturbation (2 % density, N(0,1)),
+    every shot in one batch.  Then the BF16-snapshot Born / exact-adjoint pair
+    on the first `pair_shots` shots with its FP64 dot-product test."""
+    import ctypes
+    import seis_amd
+    from seis_amd import DeviceArray, models
+    from seis_amd.batch import _layout, Grid, Shot
+    from seis_amd._lib import check
+    L = seis_amd.lib()
+    nx, nz, pml, dx, dt, fr = 1600, 500, 30, 5.0, 0.0005, 15.0
+    rng = np.random.default_rng(seed)
+    # 1500-4500 m/s keeps v dt / dx at 0.45, inside the 8th-order scheme's
+    # stability limit (0.55) with a margin
+    v = models.layered_faulted(nx, nz, dx, dx, vmax=4500.0, seed=seed)
+    dv = models.sparse_spikes(nx, nz, 0.02, seed)
+    delay = int(round(1.0 / (fr * dt)))
+    nts = rng.integers(2000, 4001, nshots)
+    ngs = rng.integers(200, 1201, nshots)
+    sxs = np.linspace(5, nx - 6, nshots).astype(int)
+    shots = [Shot(sx=int(sx), sz=2, nt=int(n)) for sx, n in zip(sxs, nts)]
+    grid = Grid(nx, nz, pml, dt, dx, dx)
+    g = grid.c()
+    arr, nw, ngz, nr = _layout(grid, shots)
+    wav = np.concatenate([seis_amd.ricker(dt, fr, delay, int(n)) for n in nts])
+    d_v, d_dv = DeviceArray.from_host(v), DeviceArray.from_host(dv)
+    d_wav = DeviceArray.from_host(wav)
+    d_gz = DeviceArray.from_host(np.full(nx * nshots, 2, np.int32))
+    d_rec = DeviceArray((nr,), np.float32, zero=True)
+    cells = (nx + 2 * pml) * (nz + 2 * pml)
+    out = {"workload": "born_batch: %d shots in one batch, 500x1600 cells @ 5 m, pml 30, dt=0.5 ms, nt in "
+                       "2000..4000, 200..1200 receivers, 2%% sparse spike perturbation" % nshots,
+           "nt_range": [int(nts.min()), int(nts.max())]}
+    # (the strip-length trial of this configuration runs inside: 12 launches)
+    t0 = time.perf_counter()
+    check(L.seis_born_batch(ctypes.byref(g), nshots, arr, d_v.ptr, d_dv.ptr, d_wav.ptr, d_gz.ptr, d_rec.ptr))
+    seis_amd.synchronize()
+    el = time.perf_counter() - t0
+    out["born_s"] = round(el, 3)
+    out["born_gcell_updates_per_s"] = round(2.0 * cells * float(nts.sum()) / el / 1e9, 1)
+    log("C3: Born batch of %d shots in %.1f s" % (nshots, el))
+    del d_rec
+    # the Born / adjoint pair with BF16 snapshots on a subset
+    sub = shots[:pair_shots]
+    sarr, snw, sng, snr = _layout(grid, sub)
+    mask = np.zeros(snr, np.float32)
+    off = 0
+    for s, ng in zip(sub, ngs[:pair_shots]):
+        lo, hi = max(0, s.sx - int(ng) // 2), min(nx, s.sx + int(ng) // 2)
+        mask[off:off + nx * s.nt].reshape(nx, s.nt)[lo:hi] = 1.0
+        off += nx * s.nt
+    sw = DeviceArray.from_host(np.concatenate([seis_amd.ricker(dt, fr, delay, s.nt) for s in sub]))
+    sgz = DeviceArray.from_host(np.full(nx * pair_shots, 2, np.int32))
+    plan = ctypes.c_void_p()
+    t0 = time.perf_counter()
+    check(L.seis_lsrtm_create(ctypes.byref(plan), ctypes.byref(g), pair_shots, sarr, d_v.ptr, sw.ptr, sgz.ptr, 1))
+    dmask = DeviceArray.from_host(mask)
+    check(L.seis_lsrtm_set_mask(plan, dmask.ptr))
+    seis_amd.synchronize()
+    out["pair_snapshot_s"] = round(time.perf_counter() - t0, 3)
+    r = DeviceArray.from_host((rng.standard_normal(snr) * mask).astype(np.float32))
+    lm = DeviceArray((snr,), np.float32, zero=True)
+    ltr = DeviceArray((nx, nz), np.float32, zero=True)
+    t0 = time.perf_counter()
+    check(L.seis_lsrtm_born(plan, d_dv.ptr, lm.ptr))
+    seis_amd.synchronize()
+    t1 = time.perf_counter()
+    check(L.seis_lsrtm_adjoint(plan, r.ptr, ltr.ptr))
+    seis_amd.synchronize()
+    t2 = time.perf_counter()
+    def dot(pl, x, y, n):
+        o = ctypes.c_double()
+        check(L.seis_lsrtm_dot(pl, x.ptr, y.ptr, n, ctypes.byref(o)))
+        return o.value
+
+    def dot_test(pl, m, lm, r, ltr, nrec):
+        a, b = dot(pl, lm, r, nrec), dot(pl, m, ltr, nx * nz)
+        norms = np.sqrt(dot(pl, lm, lm, nrec) * dot(pl, r, r, nrec))
+        return abs(a - b) / max(abs(a), abs(b), 1e-300), abs(a - b) / max(norms, 1e-300)
+
+    rel, reln = dot_test(plan, d_dv, lm, r, ltr, snr)
+    check(L.seis_lsrtm_destroy(plan))
+    pair_cells = cells * float(sum(s.nt for s in sub))
+    out["pair_shots"] = pair_shots
+    out["pair_born_gcell_updates_per_s"] = round(pair_cells / (t1 - t0) / 1e9, 1)
+    out["pair_adjoint_gcell_updates_per_s"] = round(pair_cells / (t2 - t1) / 1e9, 1)
+    # |<L m, r> - <m, L^T r>| relative to the value and to ||L m|| ||r||: a
+    # random r makes the two products cancel to ~1/sqrt(n) of the norms, so
+    # FP32 rounding shows up far larger against the value
+    out["dot_test_fp32_rel"] = float("%.3e" % rel)
+    out["dot_test_fp32_rel_to_norms"] = float("%.3e" % reln)
+    # the same test in FP64 (propagation, snapshots and products) on 4 shots
+    n64 = min(4, pair_shots)
+    sub64 = sub[:n64]
+    a64, nw64, ng64, nr64 = _layout(grid, sub64)
+    v64 = DeviceArray.from_host(v.astype(np.float64))
+    w64 = DeviceArray.from_host(np.concatenate([seis_amd.ricker(dt, fr, delay, s.nt) for s in sub64]).astype(np.float64))
+    p64 = ctypes.c_void_p()
+    check(L.seis_lsrtm_create_f64(ctypes.byref(p64), ctypes.byref(g), n64, a64, v64.ptr, w64.ptr, sgz.ptr))
+    m64 = DeviceArray.from_host(dv.astype(np.float64))
+    r64 = DeviceArray.from_host((rng.standard_normal(nr64) * mask[:nr64]).astype(np.float64))
+    lm64 = DeviceArray((nr64,), np.float64, zero=True)
+    ltr64 = DeviceArray((nx, nz), np.float64, zero=True)
+    check(L.seis_lsrtm_born(p64, m64.ptr, lm64.ptr))
+    check(L.seis_lsrtm_adjoint(p64, r64.ptr, ltr64.ptr))
+    rel64, reln64 = dot_test(p64, m64, lm64, r64, ltr64, nr64)
+    check(L.seis_lsrtm_destroy(p64))
+    out["dot_test_fp64_rel"] = float("%.3e" % rel64)
+    out["dot_test_fp64_shots"] = n64
+    out["dot_test_note"] = ("FP32: FP32 propagation, BF16 snapshots, FP64 inner products; FP64: every step in FP64 "
+                            "(the stated bar, <= 1e-6)")
+    log("C3: pair on %d shots; dot test FP32 %.2e (%.2e of the norms), FP64 %.2e" % (pair_shots, rel, reln, rel64))
+    return out
+
+
+def c5_sample(nshots, nt, seed=1005):
+    """RTM throughput on the large grid: 2000 x 8000 cells @ 5 m, pml 30,
+    layered-faulted model with a 4500 m/s salt body, dt = 0.5 ms, a short
+    record of the workload's shots."""
+    import ctypes
+    import seis_amd
+    from seis_amd import DeviceArray, models
+    from seis_amd.batch import _layout, Grid, Shot, RTM_ISOLATE_BAND
+    from seis_amd._lib import check
+    L = seis_amd.lib()
+    nx, nz, pml, dx, dt, fr = 8000, 2000, 30, 5.0, 0.0005, 12.0
+    v = models.salt_body(models.layered_faulted(nx, nz, dx, dx, seed=seed), dx, dx, seed=seed)
+    vm = models.smooth_model(v, 20.0)
+    delay = int(round(1.0 / (fr * dt)))
+    sxs = np.linspace(500, nx - 500, nshots).astype(int)
+    shots = [Shot(sx=int(sx), sz=2, nt=nt) for sx in sxs]
+    grid = Grid(nx, nz, pml, dt, dx, dx)
+    g = grid.c()
+    arr, nw, ngz, nr = _layout(grid, shots)
+    d_v, d_vm = DeviceArray.from_host(v), DeviceArray.from_host(vm)
+    d_wav = DeviceArray.from_host(np.tile(seis_amd.ricker(dt, fr, delay, nt), nshots))
+    d_gz = DeviceArray.from_host(np.full(nx * nshots, 2, np.int32))
+    d_rec = DeviceArray((nr,), np.float32, zero=True)
+    d_img = DeviceArray((nshots * nx * nz,), np.float32, zero=True)
+    check(L.seis_modeling_batch(ctypes.byref(g), nshots, arr, d_v.ptr, d_wav.ptr, d_gz.ptr, d_rec.ptr))
+    seis_amd.synchronize()
+    t0 = time.perf_counter()
+    check(L.seis_rtm_batch_ex(ctypes.byref(g), nshots, arr, d_vm.ptr, d_wav.ptr, d_gz.ptr, d_rec.ptr, d_img.ptr,
+                              RTM_ISOLATE_BAND))
+    seis_amd.synchronize()
+    el = time.perf_counter() - t0
+    cells = (nx + 2 * pml) * (nz + 2 * pml)
+    img = d_img.download()
+    log("C5 sample: RTM of %d shots x %d steps in %.1f s" % (nshots, nt, el))
+    return {"workload": "rtm_batch sample: %d shots, 2000x8000 cells @ 5 m, pml 30, nt=%d of the workload's 6000, "
+                        "dt=0.5 ms, layered-faulted model with a salt body, isolated source band" % (nshots, nt),
+            "s": round(el, 3), "s_per_shot_per_1000_steps": round(el / nshots * 1000.0 / nt, 3),
+            "gcell_updates_per_s": round(3.0 * nt * cells * nshots / el / 1e9, 1),
+            "image_finite": bool(np.isfinite(img).all())}
+
+
 def lsrtm_bench(nshots, nt, iters, err_shots, err_iters):
     """Seconds per CG iteration of least-squares RTM on the 376 x 1151 @ 8 m
     workload (the RTM model at half resolution, nt = 3000, dt = 0.8 ms, 400
@@ -229,6 +387,7 @@ def lsrtm_bench(nshots, nt, iters, err_shots, err_iters):
     seis_amd.synchronize()
     plain = time.perf_counter() - t0
     out["s_per_iter_plain_fp32"] = round(plain, 3)
+    out["cells_per_propagation"] = float((nx + 2 * pml) * (nz + 2 * pml)) * nt * ns
     out["speedup_vs_plain_fp32"] = round(plain / out["s_per_cg_iter_bf16"], 2)
     if "s_per_cg_iter_bf16_marginal" in out:
         out["speedup_vs_plain_fp32_marginal"] = round(plain / out["s_per_cg_iter_bf16_marginal"], 2)
@@ -284,6 +443,10 @@ def main():
     ap.add_argument("--lsrtm-nt", type=int, default=3000)
     ap.add_argument("--lsrtm-iters", type=int, default=3)
     ap.add_argument("--c1-calls", type=int, default=10, help="timed calls of the C1 single-shot case (0 skips it)")
+    ap.add_argument("--c3-shots", type=int, default=256, help="shots of the C3 Born batch (0 skips it)")
+    ap.add_argument("--c3-pair-shots", type=int, default=16)
+    ap.add_argument("--c5-shots", type=int, default=2, help="shots of the C5 RTM sample (0 skips it)")
+    ap.add_argument("--c5-nt", type=int, default=500)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 8 device-to-device copies of 1 GiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -396,17 +559,28 @@ def main():
         "image_finite": bool(np.isfinite(img).all()),
     }
     threads = min(16, len(os.sched_getaffinity(0)))
+    if args.c3_shots > 0:
+        del d_rec, d_img
+        out["c3_born_adjoint"] = c3_born_adjoint(args.c3_shots, args.c3_pair_shots)
+    if args.c5_shots > 0:
+        out["c5_sample"] = c5_sample(args.c5_shots, args.c5_nt)
     if args.c1_calls > 0:
         log("C1 single-shot modeling, %d calls per mode" % args.c1_calls)
         out["c1_single_shot"] = c1_single_shot(args.c1_calls, not args.no_cpu, threads)
     if args.lsrtm_shots > 0:
-        del d_rec, d_img
         out["lsrtm"] = lsrtm_bench(args.lsrtm_shots, args.lsrtm_nt, args.lsrtm_iters, 2, 3)
     if not args.no_cpu:
         log("timing the CPU baseline on %d threads" % threads)
         cb = cpu_baseline(w, args.cpu_nt, threads)
         cb["gpu_over_cpu"] = round(value / cb["value"], 1)
         out["cpu_baseline"] = cb
+        ls = out.get("lsrtm")
+        if ls and "s_per_cg_iter_bf16_marginal" in ls:
+            # the reference's iteration is one Born (2 propagations) and one
+            # RTM (3) per shot; priced at the measured reference rate
+            cpu_iter = 5.0 * ls["cells_per_propagation"] / (cb["value"] * 1e9)
+            ls["s_per_iter_cpu_reference_est"] = round(cpu_iter, 1)
+            ls["speedup_vs_cpu_reference"] = round(cpu_iter / ls["s_per_cg_iter_bf16_marginal"], 1)
     print(json.dumps(out))
 
 

@@ -68,3 +68,23 @@ def three_layer(nx, nz, dz, interfaces=(600.0, 1400.0), speeds=(1500.0, 2200.0, 
     for zi, v in zip(interfaces, speeds[1:]):
         col[zz >= zi] = v
     return np.repeat(col[None, :], nx, axis=0)
+
+
+def salt_body(v, dx, dz, speed=4500.0, seed=0):
+    """Adds a salt-like body: the inside of a random smooth closed contour
+    (a radius 1 + sum of 4 low harmonics around a centre in the middle third
+    of the section) set to `speed`."""
+    rng = np.random.default_rng(seed)
+    nx, nz = v.shape
+    width, depth = nx * dx, nz * dz
+    cx, cz = rng.uniform(0.35, 0.65) * width, rng.uniform(0.35, 0.6) * depth
+    r0 = rng.uniform(0.12, 0.18) * min(width, depth)
+    amp = rng.uniform(0.05, 0.2, 4) / np.arange(1, 5)
+    ph = rng.uniform(0, 2 * np.pi, 4)
+    x = np.arange(nx)[:, None] * dx - cx
+    z = np.arange(nz)[None, :] * dz - cz
+    th = np.arctan2(z, x)
+    r = r0 * (1.0 + sum(a * np.cos((k + 1) * th + p) for k, (a, p) in enumerate(zip(amp, ph))))
+    out = np.array(v, np.float32, copy=True)
+    out[np.hypot(x, z) < r] = speed
+    return out
