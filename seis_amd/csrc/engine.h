@@ -117,14 +117,37 @@ SEIS_HD inline float f16_to_f32(unsigned short u) {
   return __half2float(h);
 }
 
+// Stored snapshot word of model cell j = ix*nz + iz at step it: the value
+// (native storage) or its 16-bit pattern, loaded apart from its decoding so
+// that a sweep can issue the load a column ahead.
+template <typename T>
+struct SnapWord {
+  T v;
+  unsigned short h;
+};
+
+template <typename T>
+__device__ __forceinline__ SnapWord<T> snap_fetch(const DevShot<T>& s, int nz, int it, long long j) {
+  const long long o = (long long)it * s.nx * nz + j;
+  SnapWord<T> w{T(0), 0};
+  if (s.snap_kind == SNAP_NATIVE)
+    w.v = SEIS_GL(reinterpret_cast<const T*>(s.snap))[o];
+  else
+    w.h = SEIS_GL(reinterpret_cast<const unsigned short*>(s.snap))[o];
+  return w;
+}
+
+template <typename T>
+__device__ __forceinline__ T snap_decode(const DevShot<T>& s, int it, SnapWord<T> w) {
+  if (s.snap_kind == SNAP_NATIVE) return w.v;
+  const float f = s.snap_kind == SNAP_BF16 ? bf16_to_f32(w.h) : f16_to_f32(w.h);
+  return (T)(f * SEIS_GL(s.scale)[it]);
+}
+
 // Snapshot value of model cell j = ix*nz + iz at step it.
 template <typename T>
 __device__ __forceinline__ T snap_load(const DevShot<T>& s, int nz, int it, long long j) {
-  const long long o = (long long)it * s.nx * nz + j;
-  if (s.snap_kind == SNAP_NATIVE) return SEIS_GL(reinterpret_cast<const T*>(s.snap))[o];
-  const unsigned short h = SEIS_GL(reinterpret_cast<const unsigned short*>(s.snap))[o];
-  const float f = s.snap_kind == SNAP_BF16 ? bf16_to_f32(h) : f16_to_f32(h);
-  return (T)(f * SEIS_GL(s.scale)[it]);
+  return snap_decode(s, it, snap_fetch(s, nz, it, j));
 }
 
 template <typename T>

@@ -366,7 +366,8 @@ __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot
   // from the previous level (PW_FWD)
   constexpr bool RECIN = (MODE == RTM_BWD || MODE == PW_BACK || MODE == PW_FWD);
   const int rec_t = MODE == PW_FWD ? c.it - s.ndel - 1 : c.itp;
-  T nca, ncb = T(0), npa, npb = T(0), nv, nha = T(0), nhb = T(0), nrec = T(0), nss = T(0);
+  T nca, ncb = T(0), npa, npb = T(0), nv, nha = T(0), nhb = T(0), nrec = T(0), nss = T(0), ndv = T(0);
+  SnapWord<T> nsn{T(0), 0};
   int ngz = -1;
   const bool acc = MODE == ILLUM && c.it > s.ndel && act;  // accumulate u^2 (interior cells are model cells)
   const auto SSA = SEIS_GL(s.img);
@@ -385,6 +386,12 @@ __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot
     if (GZ) ngz = GZA[xx - pml];
     if (RECIN && rec_t >= 0) nrec = REC[(long long)(xx - pml) * s.nt + rec_t];
     if (acc) nss = SSA[r + zl];
+    if (MODE == BORN_S && act) {
+      // perturbation and Born-source snapshot of this model cell
+      const long long j = (long long)(xx - pml) * k.nz + iz;
+      ndv = SEIS_GL(s.dv)[j];
+      nsn = snap_fetch(s, k.nz, c.it, j);
+    }
   };
   fetch(x0);
   // per-step scalars
@@ -411,7 +418,8 @@ __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot
     if (TWO) lb[lane + 4] = qb[4];
     const int ix = x - pml;
     const T v = nv;
-    const T pa_cur = npa, pb_cur = npb, rec_cur = nrec, ss_cur = nss;
+    const T pa_cur = npa, pb_cur = npb, rec_cur = nrec, ss_cur = nss, dv_cur = ndv;
+    const SnapWord<T> sn_cur = nsn;
     const int gzr = ngz + pml;  // receiver row of this column
     fetch(x + 1);
     const T av = k.dt * v;
@@ -471,9 +479,7 @@ __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot
       if (MODE == SNAP_MAX || MODE == SNAP_STORE) {
         NA[i] = ua;
       } else if (MODE == BORN_S) {
-        const long long j = (long long)ix * k.nz + iz;
-        const T d = SEIS_GL(s.dv)[j];
-        if (d != T(0)) ua = ua + snap_load(s, k.nz, c.it, j) * d;
+        if (dv_cur != T(0)) ua = ua + snap_decode(s, c.it, sn_cur) * dv_cur;
         NA[i] = ua;
         if (c.itp >= 0 && z == gzr) REC[(long long)ix * s.nt + c.itp] = ua;
       } else if (MODE == MODEL || MODE == RTM_FWD || MODE == ILLUM) {
