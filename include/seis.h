@@ -120,8 +120,18 @@ int seis_ricker(float dt, float fr, int num_delay, int n, float* out);
  *   "lsrtm"     least-squares RTM, maxiter CG iterations
  *   "illum"     source illumination of every shot -> <wdir>ILLUM<is>.dat,
  *               and their sum (illum_MPI)
+ * and, with planeTag != 0, per ray parameter p_k = pmin + k dp (k < np):
+ *   "plane_prepare"  plane-wave sources and records of the <precsg> gathers
+ *                    -> <wdir>PLANE_S_NEW<k>.dat, <wdir>PLANE_R_NEW<k>.dat
+ *                    (planeWavePrepare); writes no image
+ *   "rtm_plane"      image of each PLANE_R_NEW<k> -> <wdir>IMAGE<k>.dat, and
+ *                    their sum (adjointPlane_MPI, RTM_IMG)
+ *   "rtm_plane_born" the same of the <wdir>BORN<k> records (LSRTM_STEP)
+ *   "illum_plane"    plane-wave illumination -> <wdir>ILLUM_PLANE<k>.dat, and
+ *                    their sum (illumPlane_MPI)
  * Images are written to out_file (default <wdir>RTM_IMAGE.dat,
- * <wdir>LSRTM_IMAGE.dat or <wdir>ILLUM.dat) as float32 [nx][nz]. */
+ * <wdir>LSRTM_IMAGE.dat, <wdir>ILLUM.dat, <wdir>PLANE_IMAGE.dat or
+ * <wdir>ILLUM_PLANE.dat) as float32 [nx][nz]. */
 int seis_run(const char* param_file, const char* mode, const char* model_file, const char* out_file);
 
 /* SU interop.  seis_write_su writes n2 traces of n1 float samples
@@ -174,6 +184,31 @@ int seis_illum_plane_batch(const seis_grid* grid, int nshots, const seis_shot* s
                            const float* sou, float* ss);
 int seis_rtm_plane_batch(const seis_grid* grid, int nshots, const seis_shot* shots, const float* vel,
                          const float* sou, const int* gz, const float* rec, float* img);
+
+/* Plane-wave phase shift of one trace, harmonic (InitInversion.cpp:141-151,
+ * declared InitInversion.h:8): out[t] = cos(kx x) in[t] - sin(kx x) H[in][t],
+ * H the 201-tap Hamming-tapered Hilbert filter (hilbert.h, half length 100),
+ * cos and sin taken in float of the float product kx*x.  Host buffers of n
+ * samples. */
+int seis_harmonic(const float* in, float* out, int n, float kx, float x);
+
+/* Plane-wave synthesis of shot gathers, planeWavePrepare (inversion.cpp:
+ * 808-912), for all ray parameters p_k = pmin + k*dp (k < np) in one call.
+ * Host arrays:   sx[ns]   source x of each shot (metres);
+ *                isx[ns]  source column, in [0, nx);
+ *                ng[ns]   traces of each shot;
+ *                col[sum ng] model column of each trace, -1 for none (a
+ *                         shot's later trace wins a shared column, as in
+ *                         swapReord, inversion.cpp:626-643).
+ * Device arrays: wav[ntw] the source wavelet; data[sum ng][nt] the gathers,
+ *                shot after shot; sou[np][nx][ntw] and rec[np][nx][nt] out:
+ *   sou_k[isx_s] = sum over shots s of harmonic(wav, p_k, sx_s),
+ *   rec_k[col]   = sum over shots of harmonic(trace, p_k, sx_s),
+ * the shots summed in order, bit-identical to the reference's PLANE_S_NEW /
+ * PLANE_R_NEW files. */
+int seis_plane_prepare(int nx, int nt, int ntw, int ns, int np, float pmin, float dp, const float* sx,
+                       const int* isx, const int* ng, const int* col, const float* wav, const float* data,
+                       float* sou, float* rec);
 
 /* seis_rtm_batch with option bits; flags = 0 is seis_rtm_batch.
  * SEIS_RTM_ISOLATE_BAND: the reference reconstructs the source wavefield

@@ -57,6 +57,12 @@ void ref_hilbert_apply(int half, const float* in, float* out, int n) {
   h.apply(const_cast<float*>(in), out, n);
 }
 
+// harmonic (InitInversion.cpp:141-151): cos(kx x) in - sin(kx x) H[in] with
+// the 201-tap Hilbert filter, the plane-wave phase shift of planeWavePrepare.
+void ref_harmonic(const float* in, float* out, int n, float kx, float x) {
+  harmonic(const_cast<float*>(in), out, n, kx, x);
+}
+
 // writeSu (io.cpp:8-58): one trace of n1 samples (n2 == 0) or n2 traces.
 void ref_write_su(const char* path, int n1, int n2, const float* data) {
   if (n2 == 0) {

@@ -21,6 +21,11 @@
 //   sponge_weights, pad_velocity, pad_perturbation : brute force on tiny grids
 //                                       and the golden vectors of the compiled reference
 //   ricker, hilbert_taps              : closed forms + golden vectors
+//   harmonic                          : golden vectors of the compiled reference;
+//                                       kx*x = 0 and pi/2 reduce it to in and -H[in]
+//   plane_prepare                     : ray parameter 0 reduces it to stacking the
+//                                       wavelet and the traces by column (closed form);
+//                                       its harmonic is pinned as above
 //   stencil_step                      : polynomial exactness + golden vectors
 //   habc_update, modeling, born, rtm  : golden vectors of the compiled reference
 //   born_adjoint (exact transpose)    : dot-product test (an invariant), and
@@ -140,6 +145,54 @@ void hilbert_apply(const T* h, int half, const T* in, T* out, int n) {
       int j = i + m - half;
       if (j >= 0 && j < n) out[j] += in[i] * h[m];
     }
+}
+
+// Plane-wave phase shift, InitInversion.cpp:141-151: H[in] with the 201-tap
+// filter (Hilbert<float> hl(100)), then out = cos(kx x) in - sin(kx x) H[in],
+// cos and sin of the T product kx*x in T (cosf / sinf for float).
+template <typename T>
+void harmonic(const T* in, T* out, int n, T kx, T x) {
+  const T ang = kx * x;
+  const T cn = std::cos(ang), sn = std::sin(ang);
+  std::vector<T> h(201);
+  hilbert_taps<T>(100, h.data());
+  hilbert_apply<T>(h.data(), 100, in, out, n);
+  for (int it = 0; it < n; it++) out[it] = cn * in[it] - sn * out[it];
+}
+
+// planeWavePrepare, inversion.cpp:860-900, for ray parameters
+// p_k = pmin + k*dp (k < np): per shot s in order, the source term
+// harmonic(wav, p_k, sx[s]) is added into sou_k[isx[s]] (:878-881) and the
+// phase-shifted traces are placed by column (swapReord: zeroed, later traces
+// overwrite earlier ones, :626-643) and added into rec_k (:887-893).
+// data holds the gathers [sum ng][nt] shot after shot; col[] the column of
+// every trace or -1.  sou [np][nx][ntw], rec [np][nx][nt].
+template <typename T>
+void plane_prepare(int nx, int nt, int ntw, int ns, int np, T pmin, T dp, const T* sx, const int* isx,
+                   const int* ng, const int* col, const T* wav, const T* data, T* sou, T* rec) {
+  std::vector<T> tw(ntw), tr((size_t)nx * nt), csgn;
+  for (int k = 0; k < np; k++) {
+    T* so = sou + (size_t)k * nx * ntw;
+    T* re = rec + (size_t)k * nx * nt;
+    std::fill(so, so + (size_t)nx * ntw, T(0));
+    std::fill(re, re + (size_t)nx * nt, T(0));
+    const T p = pmin + T(k) * dp;
+    size_t r0 = 0;
+    for (int s = 0; s < ns; s++) {
+      harmonic<T>(wav, tw.data(), ntw, p, sx[s]);
+      T* dst = so + (size_t)isx[s] * ntw;
+      for (int it = 0; it < ntw; it++) dst[it] = dst[it] + tw[it];
+      csgn.assign((size_t)ng[s] * nt, T(0));
+      for (int g = 0; g < ng[s]; g++)
+        harmonic<T>(data + (r0 + g) * nt, csgn.data() + (size_t)g * nt, nt, p, sx[s]);
+      std::fill(tr.begin(), tr.end(), T(0));
+      for (int g = 0; g < ng[s]; g++)
+        if (col[r0 + g] >= 0)
+          std::memcpy(tr.data() + (size_t)col[r0 + g] * nt, csgn.data() + (size_t)g * nt, sizeof(T) * nt);
+      for (size_t i = 0; i < tr.size(); i++) re[i] = tr[i] + re[i];
+      r0 += ng[s];
+    }
+  }
 }
 
 // One leapfrog step of the interior stencil, fd.cpp:1226-1271.
@@ -896,6 +949,12 @@ int oracle_get_threads(void) { return g_threads; }
     std::vector<T> h(2 * half + 1);                                                                   \
     hilbert_taps<T>(half, h.data());                                                                  \
     hilbert_apply<T>(h.data(), half, in, out, n);                                                     \
+  }                                                                                                   \
+  void oracle_harmonic_##SUF(const T* in, T* out, int n, T kx, T x) { harmonic<T>(in, out, n, kx, x); }  \
+  void oracle_plane_prepare_##SUF(int nx, int nt, int ntw, int ns, int np, T pmin, T dp, const T* sx,        \
+                                  const int* isx, const int* ng, const int* col, const T* wav,              \
+                                  const T* data, T* sou, T* rec) {                                          \
+    plane_prepare<T>(nx, nt, ntw, ns, np, pmin, dp, sx, isx, ng, col, wav, data, sou, rec);                \
   }                                                                                                   \
   void oracle_sponge_weights_##SUF(int nz, int nx, int pml, T* w) { sponge_weights<T>(Geom(nx, nz, pml), w); } \
   void oracle_pad_velocity_##SUF(int nz, int nx, int pml, const T* v, T* vp) {                        \

@@ -68,6 +68,8 @@ _SIGS = {
     "seis_lsrtm_set_mask": (_I, [_P, _P]),
     "seis_lsrtm_cg": (_I, [_P, _P, _I, ctypes.c_double, _P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I)]),
     "seis_ricker": (_I, [_F, _F, _I, _I, _P]),
+    "seis_harmonic": (_I, [_P, _P, _I, _F, _F]),
+    "seis_plane_prepare": (_I, [_I, _I, _I, _I, _I, _F, _F, _P, _P, _P, _P, _P, _P, _P, _P]),
     "seis_write_su": (_I, [ctypes.c_char_p, _I, _I, _P]),
     "seis_su_split": (_I, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p]),
     "seis_fd_free": (None, [_P]),

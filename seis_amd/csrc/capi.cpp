@@ -315,4 +315,36 @@ int seis_rtm_batch_ex(const seis_grid* grid, int nshots, const seis_shot* shots,
                           flags);
 }
 
+int seis_harmonic(const float* in, float* out, int n, float kx, float x) {
+  if (!in || !out) return set_error("null array argument"), SEIS_ERR_ARG;
+  if (n <= 0) return set_error("harmonic: n must be > 0"), SEIS_ERR_ARG;
+  DevBuf din, dout;
+  int rc;
+  if ((rc = upload(din, in, sizeof(float) * n))) return rc;
+  if ((rc = alloc(dout, sizeof(float) * n))) return rc;
+  if ((rc = run_harmonic((const float*)din.p, (float*)dout.p, n, kx, x, 0))) return rc;
+  hipError_t e = hipMemcpy(out, dout.p, sizeof(float) * n, hipMemcpyDeviceToHost);
+  return e == hipSuccess ? SEIS_OK : hip_fail("hipMemcpy", e);
+}
+
+int seis_plane_prepare(int nx, int nt, int ntw, int ns, int np, float pmin, float dp, const float* sx,
+                       const int* isx, const int* ng, const int* col, const float* wav, const float* data,
+                       float* sou, float* rec) {
+  if (nx <= 0 || nt <= 0 || ntw <= 0 || ns < 0 || np < 0)
+    return set_error("plane prepare: nx, nt, ntw must be > 0; ns, np >= 0"), SEIS_ERR_ARG;
+  if (ns > 0 && (!sx || !isx || !ng || !col)) return set_error("null array argument"), SEIS_ERR_ARG;
+  long long ntr = 0;
+  for (int s = 0; s < ns; s++) {
+    if (isx[s] < 0 || isx[s] >= nx) return set_error("plane prepare: source column outside [0, nx)"), SEIS_ERR_ARG;
+    if (ng[s] < 0) return set_error("plane prepare: negative trace count"), SEIS_ERR_ARG;
+    ntr += ng[s];
+  }
+  if (ntr > 0x7fffffffLL) return set_error("plane prepare: too many traces"), SEIS_ERR_ARG;
+  for (long long i = 0; i < ntr; i++)
+    if (col[i] < -1 || col[i] >= nx) return set_error("plane prepare: trace column outside [-1, nx)"), SEIS_ERR_ARG;
+  if (np == 0) return SEIS_OK;
+  if ((ntr > 0 && !data) || !wav || !sou || !rec) return set_error("null array argument"), SEIS_ERR_ARG;
+  return run_plane_prepare(nx, nt, ntw, ns, np, pmin, dp, sx, isx, ng, col, wav, data, sou, rec, 0);
+}
+
 }  // extern "C"

@@ -50,6 +50,10 @@ int seis_run(const char* param_file, const char* mode, const char* model_file, c
     if (!ok) return set_error(inv.error()), SEIS_ERR_HIP;
     return SEIS_OK;
   }
+  if (m == "plane_prepare") {
+    if (!inv.planeWavePrepare()) return set_error(inv.error()), SEIS_ERR_HIP;
+    return SEIS_OK;
+  }
   std::vector<float> img(n);
   if (m == "rtm" || m == "rtm_born") {
     if (!inv.adjoint(img.data(), m == "rtm" ? RTM_IMG : LSRTM_STEP)) return set_error(inv.error()), SEIS_ERR_HIP;
@@ -57,12 +61,24 @@ int seis_run(const char* param_file, const char* mode, const char* model_file, c
     if (!inv.cgsolver(p.maxiter.val, 0.f, img.data())) return set_error(inv.error()), SEIS_ERR_HIP;
   } else if (m == "illum") {
     if (!inv.illum(img.data())) return set_error(inv.error()), SEIS_ERR_HIP;
+  } else if (m == "rtm_plane" || m == "rtm_plane_born") {
+    if (!p.planeTag.val) return set_error("seis_run: plane-wave modes need planeTag != 0"), SEIS_ERR_ARG;
+    if (!inv.adjointPlane(img.data(), m == "rtm_plane" ? RTM_IMG : LSRTM_STEP))
+      return set_error(inv.error()), SEIS_ERR_HIP;
+  } else if (m == "illum_plane") {
+    if (!p.planeTag.val) return set_error("seis_run: plane-wave modes need planeTag != 0"), SEIS_ERR_ARG;
+    if (!inv.illumPlane(img.data())) return set_error(inv.error()), SEIS_ERR_HIP;
   } else {
-    return set_error("seis_run: mode must be modeling, born, rtm, rtm_born, lsrtm or illum"), SEIS_ERR_ARG;
+    return set_error("seis_run: mode must be modeling, born, rtm, rtm_born, lsrtm, illum, plane_prepare, "
+                     "rtm_plane, rtm_plane_born or illum_plane"),
+           SEIS_ERR_ARG;
   }
-  const std::string f = out_file ? out_file
-                                 : p.wdir.val + (m == "lsrtm" ? "LSRTM_IMAGE.dat" : m == "illum" ? "ILLUM.dat"
-                                                                                               : "RTM_IMAGE.dat");
+  const char* name = m == "lsrtm"         ? "LSRTM_IMAGE.dat"
+                     : m == "illum"       ? "ILLUM.dat"
+                     : m == "illum_plane" ? "ILLUM_PLANE.dat"
+                     : m == "rtm_plane" || m == "rtm_plane_born" ? "PLANE_IMAGE.dat"
+                                                                 : "RTM_IMAGE.dat";
+  const std::string f = out_file ? out_file : p.wdir.val + name;
   if (!write_floats(f, n, img.data(), err)) return set_error(err), SEIS_ERR_ARG;
   return SEIS_OK;
 }

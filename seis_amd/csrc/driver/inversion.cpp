@@ -236,6 +236,142 @@ bool Inversion::illum(float* ss) {
   return true;
 }
 
+bool Inversion::planeWavePrepare() {
+  if (!param.planeTag.val) return err_ = "in param planeTag must be true(none zero) to use subroutine planeWavePrepare", false;
+  const int nx = param.nx.val, nt = param.nt.val, np = param.np.val, ns = survey.ns();
+  const int NT = nt + param.delaycal.val;
+  const float dx = param.dx.val, velfx = param.velfx.val;
+  if (np <= 0) return true;
+  std::vector<float> wav(NT);
+  ricker_host(param.dt.val, param.fr.val, param.delay.val + param.delaycal.val, NT, wav.data());
+  // source columns: every shot must lie in [velfx - 0.0001 dx, max sx]
+  // (inversion.cpp:845-850, 872-880)
+  float sxmax = ns > 0 ? survey.sx[0] : 0.f;
+  for (int is = 1; is < ns; is++) sxmax = sxmax < survey.sx[is] ? survey.sx[is] : sxmax;
+  const float velmin = velfx - 0.0001 * dx, velmax = sxmax;
+  std::vector<int> isx(ns), ng(ns), col;
+  size_t ntr = 0;
+  for (int is = 0; is < ns; is++) {
+    const float sx = survey.sx[is];
+    if (!((sx - velmin) >= 0 && (velmax - sx) >= 0)) return err_ = "sx must be in the range [velmin,velmax]", false;
+    isx[is] = (int)((sx - velfx) / dx + 0.001f);
+    if (isx[is] < 0 || isx[is] >= nx) return err_ = "isx must be in the range [0 nx) in planeWavePrepare", false;
+    std::vector<int> c;
+    if (!receiver_columns(param, survey, is, velfx, nx, c, err_)) return false;
+    ng[is] = (int)c.size();
+    col.insert(col.end(), c.begin(), c.end());
+    ntr += c.size();
+  }
+  std::vector<float> data(ntr * nt);
+  for (int is = 0, r = 0; is < ns; r += ng[is], is++)
+    if (!read_floats(csg_name(param, is), (size_t)ng[is] * nt, data.data() + (size_t)r * nt, err_)) return false;
+  // All ray parameters in one call when their outputs fit in 2 GiB of device
+  // memory, else one call per parameter: p_k is pmin + k dp in float either
+  // way (inversion.cpp:866), so the files are the same.
+  const size_t per = (size_t)nx * (NT + nt) * sizeof(float);
+  const int chunk = per * np <= ((size_t)2 << 30) ? np : 1;
+  Dev dd, dw, dsou, drec;
+  if (!dd.put(data.data(), sizeof(float) * std::max<size_t>(data.size(), 1)) ||
+      !dw.put(wav.data(), sizeof(float) * wav.size()) || !dsou.alloc((size_t)chunk * nx * NT * sizeof(float)) ||
+      !drec.alloc((size_t)chunk * nx * nt * sizeof(float)))
+    return err_ = gpu_error("device buffers"), false;
+  std::vector<float> sou((size_t)chunk * nx * NT), rec((size_t)chunk * nx * nt);
+  const std::vector<float> sxs(survey.sx.begin(), survey.sx.end());
+  for (int k0 = 0; k0 < np; k0 += chunk) {
+    const float pk = param.pmin.val + (float)k0 * param.dp.val;
+    if (seis_plane_prepare(nx, nt, NT, ns, chunk, pk, param.dp.val, sxs.data(), isx.data(), ng.data(), col.data(),
+                           (const float*)dw.p, (const float*)dd.p, (float*)dsou.p, (float*)drec.p))
+      return err_ = gpu_error("plane prepare"), false;
+    if (!dsou.get(sou.data(), sizeof(float) * sou.size()) || !drec.get(rec.data(), sizeof(float) * rec.size()))
+      return err_ = gpu_error("download"), false;
+    for (int k = 0; k < chunk; k++)
+      if (!write_floats(plane_source_name(param, k0 + k), (size_t)nx * NT, sou.data() + (size_t)k * nx * NT, err_) ||
+          !write_floats(plane_record_name(param, k0 + k), (size_t)nx * nt, rec.data() + (size_t)k * nx * nt, err_))
+        return false;
+  }
+  return true;
+}
+
+// kind 0: illumination, 1: migration.  Reads the per-parameter files, runs
+// every ray parameter in one batch on the whole model, returns the outputs
+// [np][nx][nz].
+bool Inversion::plane_batch(int kind, std::vector<float>& out, int migTag) {
+  std::vector<float> v0;
+  if (!read_model(param.v0file.val, v0)) return false;
+  const int nx = param.nx.val, nz = param.nz.val, nt = param.nt.val, np = param.np.val;
+  const int ndel = param.delaycal.val, NT = nt + ndel;
+  if (survey.ns() < 1 || survey.ng(0) < 1) return err_ = "plane-wave modes need shot 0 and its first receiver", false;
+  // source and receiver rows of shot 0 (inversion.cpp:331-332, 1064-1065)
+  const int sz = (int)((survey.sz[0] - 0.0f) / param.dz.val + 0.0001f);
+  const int gz = (int)((survey.gz[0][0] - 0.0f) / param.dz.val + 0.0001f);
+  if (sz < 0 || sz >= nz || gz < 0 || gz >= nz) return err_ = "plane-wave source or receiver row outside the model", false;
+  out.assign((size_t)np * nx * nz, 0.f);
+  if (np <= 0) return true;
+  std::vector<float> sou((size_t)np * nx * NT), rec(kind == 1 ? (size_t)np * nx * nt : 0);
+  std::vector<seis_shot> sh(np);
+  for (int k = 0; k < np; k++) {
+    if (!read_floats(plane_source_name(param, k), (size_t)nx * NT, sou.data() + (size_t)k * nx * NT, err_)) return false;
+    if (kind == 1) {
+      const std::string f = migTag == RTM_IMG ? plane_record_name(param, k) : born_name(param, k);
+      if (!read_floats(f, (size_t)nx * nt, rec.data() + (size_t)k * nx * nt, err_)) return false;
+    }
+    sh[k] = seis_shot{0, sz, nt, ndel, 0, (long long)k * nx * NT, 0, (long long)k * nx * nt};
+  }
+  const std::vector<int> gzs(nx, gz);
+  seis_grid g{nx, nz, param.npml.val, param.dt.val, param.dx.val, param.dz.val};
+  Dev dv, ds, dg, dr, di;
+  if (!dv.put(v0.data(), sizeof(float) * v0.size()) || !ds.put(sou.data(), sizeof(float) * sou.size()) ||
+      !dg.put(gzs.data(), sizeof(int) * nx) || (kind == 1 && !dr.put(rec.data(), sizeof(float) * rec.size())) ||
+      !di.alloc(sizeof(float) * out.size()))
+    return err_ = gpu_error("device buffers"), false;
+  const int rc = kind == 0 ? seis_illum_plane_batch(&g, np, sh.data(), (float*)dv.p, (float*)ds.p, (float*)di.p)
+                           : seis_rtm_plane_batch(&g, np, sh.data(), (float*)dv.p, (float*)ds.p, (int*)dg.p,
+                                                  (float*)dr.p, (float*)di.p);
+  if (rc) return err_ = gpu_error("plane-wave batch"), false;
+  if (!di.get(out.data(), sizeof(float) * out.size())) return err_ = gpu_error("download"), false;
+  return true;
+}
+
+bool Inversion::illumPlane(float* ss) {
+  std::vector<float> all;
+  if (!plane_batch(0, all, RTM_IMG)) return false;
+  const size_t nm = (size_t)param.nx.val * param.nz.val;
+  for (int k = 0; k < param.np.val; k++)
+    if (!write_floats(plane_illum_name(param, k), nm, all.data() + k * nm, err_)) return false;
+  // the sum reads every parameter's file back (inversion.cpp:352-360)
+  std::fill(ss, ss + nm, 0.f);
+  std::vector<float> t(nm);
+  for (int k = 0; k < param.np.val; k++) {
+    if (!read_floats(plane_illum_name(param, k), nm, t.data(), err_)) return false;
+    for (size_t i = 0; i < nm; i++) ss[i] = t[i] + ss[i];
+  }
+  return true;
+}
+
+bool Inversion::adjointPlane(float* img, int migTag) {
+  std::vector<float> all;
+  if (!plane_batch(1, all, migTag)) return false;
+  const size_t nm = (size_t)param.nx.val * param.nz.val;
+  std::vector<float> mask;
+  if (param.mask.val) {
+    mask.resize(nm);
+    if (!read_floats(param.maskfile.val, nm, mask.data(), err_)) return false;
+  }
+  for (int k = 0; k < param.np.val; k++) {
+    float* m = all.data() + k * nm;
+    if (param.mask.val)
+      for (size_t i = 0; i < nm; i++) m[i] = m[i] * mask[i];
+    if (!write_floats(plane_image_name(param, k), nm, m, err_)) return false;
+  }
+  std::fill(img, img + nm, 0.f);
+  std::vector<float> t(nm);
+  for (int k = 0; k < param.np.val; k++) {
+    if (!read_floats(plane_image_name(param, k), nm, t.data(), err_)) return false;
+    for (size_t i = 0; i < nm; i++) img[i] = t[i] + img[i];
+  }
+  return true;
+}
+
 bool Inversion::cgsolver(int maxIter, float reError, float* img, std::vector<double>* rnorm) {
   std::vector<float> v0;
   if (!read_model(param.v0file.val, v0)) return false;

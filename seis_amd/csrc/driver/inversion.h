@@ -37,6 +37,21 @@ class Inversion {
   // background -> <wdir>ILLUM<is>.dat, summed over all ns shots in shot order
   // (illum_MPI, inversion.cpp:207-290).
   bool illum(float* ss);
+  // Plane-wave synthesis (planeWavePrepare, inversion.cpp:808-912): for each
+  // ray parameter p_k = pmin + k dp (k < np) the phase-shifted sum over all
+  // shots of the source wavelet at the source columns and of the CSG<is>
+  // gathers by receiver column -> <wdir>PLANE_S_NEW<k>.dat [nx][nt+delaycal]
+  // and <wdir>PLANE_R_NEW<k>.dat [nx][nt].  Needs planeTag != 0.
+  bool planeWavePrepare();
+  // Plane-wave illumination (illumPlane_MPI, inversion.cpp:291-364): per ray
+  // parameter <wdir>ILLUM_PLANE<k>.dat [nx][nz] of the v0file background,
+  // summed over k in order into ss [nx][nz].
+  bool illumPlane(float* ss);
+  // Plane-wave migration (adjointPlane_MPI, inversion.cpp:1019-1103): per ray
+  // parameter the image of PLANE_S_NEW<k> / PLANE_R_NEW<k> (RTM_IMG) or of
+  // BORN<k> (LSRTM_STEP), times the mask when mask != 0 ->
+  // <wdir>IMAGE<k>.dat, summed over k in order into img [nx][nz].
+  bool adjointPlane(float* img, int migTag);
   // Least-squares RTM of the CSG records by CG on the exact Born operator
   // (the declared conope_MPI / cgsolver_MPI, inversion.h:81,87): maxIter
   // iterations or until ||L^T r|| / ||L^T d|| < reError.  img [nx][nz].
@@ -48,6 +63,7 @@ class Inversion {
 
  private:
   bool read_model(const std::string& file, std::vector<float>& v);
+  bool plane_batch(int kind, std::vector<float>& out, int migTag);
   int rank_, nprocs_;
   std::string err_;
 };

@@ -166,11 +166,10 @@ void source_cell(const Params& p, const Survey& s, int is, float fx, int* sx, in
   *sz = (int)((s.sz[is] - 0.0) / p.dz.val + 0.0001f);
 }
 
-bool reorder(const Params& p, const Survey& s, int is, int nt, float* csg, float* rec, float fx, int nx,
-             bool to_gather, std::string& err) {
+bool receiver_columns(const Params& p, const Survey& s, int is, float fx, int nx, std::vector<int>& col,
+                      std::string& err) {
   const int ng = s.ng(is);
-  if (to_gather) std::fill(csg, csg + (size_t)ng * nt, 0.f);
-  else std::fill(rec, rec + (size_t)nx * nt, 0.f);
+  col.assign(ng, -1);
   const float dx = p.dx.val;
   const float x_min = fx - 0.001 * dx;
   const float x_max = fx + (nx - 1) * dx + 0.001 * dx;
@@ -182,9 +181,23 @@ bool reorder(const Params& p, const Survey& s, int is, int nt, float* csg, float
         err = "receiver column outside the window";
         return false;
       }
-      if (to_gather) std::memcpy(csg + (size_t)ig * nt, rec + (size_t)xloc * nt, sizeof(float) * nt);
-      else std::memcpy(rec + (size_t)xloc * nt, csg + (size_t)ig * nt, sizeof(float) * nt);
+      col[ig] = xloc;
     }
+  }
+  return true;
+}
+
+bool reorder(const Params& p, const Survey& s, int is, int nt, float* csg, float* rec, float fx, int nx,
+             bool to_gather, std::string& err) {
+  const int ng = s.ng(is);
+  std::vector<int> col;
+  if (!receiver_columns(p, s, is, fx, nx, col, err)) return false;
+  if (to_gather) std::fill(csg, csg + (size_t)ng * nt, 0.f);
+  else std::fill(rec, rec + (size_t)nx * nt, 0.f);
+  for (int ig = 0; ig < ng; ig++) {
+    if (col[ig] < 0) continue;
+    if (to_gather) std::memcpy(csg + (size_t)ig * nt, rec + (size_t)col[ig] * nt, sizeof(float) * nt);
+    else std::memcpy(rec + (size_t)col[ig] * nt, csg + (size_t)ig * nt, sizeof(float) * nt);
   }
   return true;
 }
@@ -258,5 +271,9 @@ std::string csg_name(const Params& p, int is) { return join(p.precsg.val, "", is
 std::string born_name(const Params& p, int is) { return join(p.wdir.val, "BORN", is); }
 std::string shot_image_name(const Params& p, int is) { return join(p.wdir.val, "CSG_IMAGE", is); }
 std::string illum_name(const Params& p, int is) { return join(p.wdir.val, "ILLUM", is); }
+std::string plane_source_name(const Params& p, int ip) { return join(p.wdir.val, "PLANE_S_NEW", ip); }
+std::string plane_record_name(const Params& p, int ip) { return join(p.wdir.val, "PLANE_R_NEW", ip); }
+std::string plane_image_name(const Params& p, int ip) { return join(p.wdir.val, "IMAGE", ip); }
+std::string plane_illum_name(const Params& p, int ip) { return join(p.wdir.val, "ILLUM_PLANE", ip); }
 
 }  // namespace seis

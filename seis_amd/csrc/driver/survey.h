@@ -47,6 +47,12 @@ bool receiver_depths(const Params& p, const Survey& s, int is, float fx, int nx,
 // Source cell of a shot in its window (inversion.cpp:186-187).
 void source_cell(const Params& p, const Survey& s, int is, float fx, int* sx, int* sz);
 
+// Window column of every receiver of shot is, as swapReord places them
+// (inversion.cpp:626-643): -1 outside [fx, fx + (nx-1) dx] (with a 0.001 dx
+// margin); false when a column falls outside [0, nx).
+bool receiver_columns(const Params& p, const Survey& s, int is, float fx, int nx, std::vector<int>& col,
+                      std::string& err);
+
 // Gather <-> per-column record (swapReord, inversion.cpp:626-643).
 //   to_gather: csg[ig][t] = rec[column of receiver ig][t]
 //   otherwise: rec[column][t] = csg[ig][t] (zero for columns without a receiver)
@@ -68,5 +74,10 @@ std::string csg_name(const Params& p, int is);         // <precsg><is>.dat
 std::string born_name(const Params& p, int is);        // <wdir>BORN<is>.dat
 std::string shot_image_name(const Params& p, int is);  // <wdir>CSG_IMAGE<is>.dat
 std::string illum_name(const Params& p, int is);       // <wdir>ILLUM<is>.dat
+// Plane-wave files, per ray parameter (inversion.cpp:335, 896-897, 1068).
+std::string plane_source_name(const Params& p, int ip);  // <wdir>PLANE_S_NEW<ip>.dat
+std::string plane_record_name(const Params& p, int ip);  // <wdir>PLANE_R_NEW<ip>.dat
+std::string plane_image_name(const Params& p, int ip);   // <wdir>IMAGE<ip>.dat
+std::string plane_illum_name(const Params& p, int ip);   // <wdir>ILLUM_PLANE<ip>.dat
 
 }  // namespace seis

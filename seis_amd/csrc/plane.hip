@@ -1,0 +1,227 @@
+// Plane-wave synthesis of shot gathers (planeWavePrepare,
+// lsrtm/inversion.cpp:808-912): for every ray parameter p_k = pmin + k dp,
+//
+//   sou_k[isx_s] += harmonic(wav, p_k, sx_s)          per shot s, in shot order
+//   rec_k[col]   += harmonic(trace, p_k, sx_s)        per shot, one trace per column
+//
+// with harmonic(in, p, x) = cos(p x) in - sin(p x) H[in] (InitInversion.cpp:
+// 141-151) and H the 201-tap Hamming-tapered Hilbert filter (hilbert.h).
+//
+// The Hilbert transform of a trace does not depend on p, so the GPU filters
+// every trace and the wavelet once (hilbert_kernel), then one stacking pass
+// (plane_stack_kernel) forms cos*d - sin*H[d] for a block of ray parameters
+// per thread and sums the shots.  The cosines and sines are taken on the host
+// with the C library's cosf / sinf of the float product p*sx, as the
+// reference does, so every output is bit-identical to the reference's.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "engine.h"
+#include "prof.h"
+#include "seis_internal.h"
+
+namespace seis {
+
+namespace {
+
+constexpr int HARM_HALF = 100;             // Hilbert<float> hl(100), InitInversion.cpp:145
+constexpr int HARM_ODD = HARM_HALF / 2;    // nonzero taps on each side (odd offsets)
+constexpr int HWG = 256;                   // hilbert_kernel work-group
+constexpr int HOUT = 4;                    // outputs per thread (strided by HWG)
+constexpr int HTILE = HWG * HOUT;          // outputs per work-group
+constexpr int HSPAN = HTILE + 2 * HARM_HALF;
+
+// Taps at odd offsets d = 1, 3, ..., 99: h[L + d]; h[L - d] = -h[L + d].
+struct HarmTaps {
+  float h[HARM_ODD];
+};
+
+// H[in] of each row of in [rows][n] -> out [rows][n].  out[j] is the
+// reference's sum of in[i] * h[j - i + L] over i in increasing order
+// (hilbert.h:42-54).  Terms whose tap is zero (even j - i) and terms outside
+// the trace are left out: each would add a zero product to a partial sum that
+// is never -0, which leaves it unchanged (for finite samples).
+__global__ __launch_bounds__(HWG) void hilbert_kernel(HarmTaps taps, const float* __restrict__ in,
+                                                      float* __restrict__ out, int n) {
+  __shared__ float s[HSPAN];
+  const long long row = blockIdx.y;
+  const int j0 = blockIdx.x * HTILE;
+  const float* src = in + row * n;
+  for (int i = threadIdx.x; i < HSPAN; i += HWG) {
+    const int t = j0 - HARM_HALF + i;
+    s[i] = (t >= 0 && t < n) ? src[t] : 0.f;
+  }
+  __syncthreads();
+  for (int r = 0; r < HOUT; r++) {
+    const int jl = threadIdx.x + r * HWG;
+    const int j = j0 + jl;
+    if (j >= n) break;
+    // i = j - d for d = 99, 97, ..., 1, then -1, ..., -99: increasing i
+    const float* c = s + jl + HARM_HALF;
+    float acc = 0.f;
+#pragma unroll
+    for (int k = HARM_ODD - 1; k >= 0; k--) acc = acc + c[-(2 * k + 1)] * taps.h[k];
+#pragma unroll
+    for (int k = 0; k < HARM_ODD; k++) acc = acc + c[2 * k + 1] * (-taps.h[k]);
+    out[row * n + j] = acc;
+  }
+}
+
+constexpr int SWG = 256;  // plane_stack_kernel work-group
+constexpr int KB = 8;     // ray parameters per thread
+
+// out[k][x][t] for k in [k0, k0 + KB): the sum over shots s (in order) of
+// cs[k][s].cos * d[t] - cs[k][s].sin * H[d][t], d the row map[s][x] of in (no
+// term where map[s][x] < 0).  One work-group per (t tile, column x, k block).
+// With `single` the output is the one shot's term itself (harmonic), not a
+// sum started from zero (which turns a -0 term into +0).
+__global__ __launch_bounds__(SWG) void plane_stack_kernel(const float* __restrict__ in, const float* __restrict__ hin,
+                                                          int n, const int* __restrict__ map, int ns, int nx,
+                                                          const float2* __restrict__ cs, int np, int single,
+                                                          float* __restrict__ out) {
+  const int t = blockIdx.x * SWG + threadIdx.x;
+  const int x = blockIdx.y;
+  const int k0 = blockIdx.z * KB;
+  const int kn = min(KB, np - k0);
+  float acc[KB];
+#pragma unroll
+  for (int b = 0; b < KB; b++) acc[b] = 0.f;
+  if (t < n) {
+    for (int s = 0; s < ns; s++) {
+      const int r = map[(long long)s * nx + x];
+      if (r < 0) continue;
+      const float d = in[(long long)r * n + t];
+      const float hd = hin[(long long)r * n + t];
+#pragma unroll
+      for (int b = 0; b < KB; b++) {
+        if (b < kn) {
+          const float2 c = cs[(long long)(k0 + b) * ns + s];
+          const float v = c.x * d - c.y * hd;
+          acc[b] = single ? v : v + acc[b];
+        }
+      }
+    }
+#pragma unroll
+    for (int b = 0; b < KB; b++)
+      if (b < kn) out[((long long)(k0 + b) * nx + x) * n + t] = acc[b];
+  }
+}
+
+HarmTaps harm_taps() {
+  std::vector<float> h(2 * HARM_HALF + 1);
+  hilbert_taps_host<float>(HARM_HALF, h.data());
+  HarmTaps t{};
+  for (int k = 0; k < HARM_ODD; k++) t.h[k] = h[HARM_HALF + 2 * k + 1];
+  return t;
+}
+
+int hip_err(const char* what, hipError_t e) {
+  set_error(std::string(what) + ": " + hipGetErrorString(e));
+  return e == hipErrorNoDevice ? SEIS_ERR_NODEV : SEIS_ERR_HIP;
+}
+
+struct Scratch {
+  void* p = nullptr;
+  ~Scratch() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+int launch_hilbert(const float* in, float* out, int n, long long rows, hipStream_t st) {
+  if (rows <= 0 || n <= 0) return SEIS_OK;
+  const HarmTaps taps = harm_taps();
+  for (long long r0 = 0; r0 < rows; r0 += 65535) {
+    const int nr = (int)std::min<long long>(65535, rows - r0);
+    dim3 grid((n + HTILE - 1) / HTILE, nr);
+    timed_launch(st, "plane_hilbert", 8.0 * n * nr, 0, [&] {
+      hilbert_kernel<<<grid, HWG, 0, st>>>(taps, in + r0 * n, out + r0 * n, n);
+    });
+  }
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? SEIS_OK : hip_err("hilbert_kernel", e);
+}
+
+int launch_stack(const float* in, const float* hin, int n, const int* map, int ns, int nx, const float2* cs, int np,
+                 float* out, hipStream_t st, int single = 0) {
+  if (np <= 0 || nx <= 0 || n <= 0) return SEIS_OK;
+  dim3 grid((n + SWG - 1) / SWG, nx, (np + KB - 1) / KB);
+  timed_launch(st, "plane_stack", 4.0 * n * nx * np, 0, [&] {
+    plane_stack_kernel<<<grid, SWG, 0, st>>>(in, hin, n, map, ns, nx, cs, np, single, out);
+  });
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? SEIS_OK : hip_err("plane_stack_kernel", e);
+}
+
+}  // namespace
+
+int run_harmonic(const float* in, float* out, int n, float kx, float x, hipStream_t st) {
+  Scratch h, cs, map;
+  hipError_t e = hipMalloc(&h.p, sizeof(float) * n);
+  if (e != hipSuccess) return hip_err("hipMalloc", e);
+  const float ang = kx * x;
+  const float2 c{cosf(ang), sinf(ang)};
+  const int zero = 0;
+  if ((e = hipMalloc(&cs.p, sizeof(float2))) != hipSuccess || (e = hipMalloc(&map.p, sizeof(int))) != hipSuccess)
+    return hip_err("hipMalloc", e);
+  if ((e = hipMemcpyAsync(cs.p, &c, sizeof(c), hipMemcpyHostToDevice, st)) != hipSuccess ||
+      (e = hipMemcpyAsync(map.p, &zero, sizeof(int), hipMemcpyHostToDevice, st)) != hipSuccess)
+    return hip_err("hipMemcpyAsync", e);
+  int rc = launch_hilbert(in, (float*)h.p, n, 1, st);
+  if (!rc) rc = launch_stack(in, (const float*)h.p, n, (const int*)map.p, 1, 1, (const float2*)cs.p, 1, out, st, 1);
+  if (rc) return rc;
+  e = hipStreamSynchronize(st);
+  return e == hipSuccess ? SEIS_OK : hip_err("plane harmonic", e);
+}
+
+int run_plane_prepare(int nx, int nt, int ntw, int ns, int np, float pmin, float dp, const float* sx,
+                      const int* isx, const int* ng, const int* col, const float* wav, const float* data,
+                      float* sou, float* rec, hipStream_t st) {
+  long long ntr = 0;
+  for (int s = 0; s < ns; s++) ntr += ng[s];
+  // per-shot row maps: the source wavelet (row 0) at column isx, and for the
+  // record the last trace of the shot that lands in each column (swapReord
+  // copies the traces in order, inversion.cpp:626-643)
+  std::vector<int> smap((size_t)ns * nx, -1), rmap((size_t)ns * nx, -1);
+  long long r0 = 0;
+  for (int s = 0; s < ns; s++) {
+    smap[(size_t)s * nx + isx[s]] = 0;
+    for (int g = 0; g < ng[s]; g++)
+      if (col[r0 + g] >= 0) rmap[(size_t)s * nx + col[r0 + g]] = (int)(r0 + g);
+    r0 += ng[s];
+  }
+  // cos / sin of p_k * sx_s; p_k = pmin + k * dp in float (inversion.cpp:866)
+  std::vector<float2> cs((size_t)np * ns);
+  for (int k = 0; k < np; k++) {
+    const float p = pmin + (float)k * dp;
+    for (int s = 0; s < ns; s++) {
+      const float ang = p * sx[s];
+      cs[(size_t)k * ns + s] = float2{cosf(ang), sinf(ang)};
+    }
+  }
+  Scratch hw, hd, dcs, dsm, drm;
+  hipError_t e;
+  if ((e = hipMalloc(&hw.p, sizeof(float) * std::max(ntw, 1))) != hipSuccess ||
+      (e = hipMalloc(&hd.p, sizeof(float) * std::max<long long>(ntr * nt, 1))) != hipSuccess ||
+      (e = hipMalloc(&dcs.p, sizeof(float2) * std::max<size_t>(cs.size(), 1))) != hipSuccess ||
+      (e = hipMalloc(&dsm.p, sizeof(int) * smap.size())) != hipSuccess ||
+      (e = hipMalloc(&drm.p, sizeof(int) * rmap.size())) != hipSuccess)
+    return hip_err("hipMalloc", e);
+  if ((e = hipMemcpyAsync(dcs.p, cs.data(), sizeof(float2) * cs.size(), hipMemcpyHostToDevice, st)) != hipSuccess ||
+      (e = hipMemcpyAsync(dsm.p, smap.data(), sizeof(int) * smap.size(), hipMemcpyHostToDevice, st)) != hipSuccess ||
+      (e = hipMemcpyAsync(drm.p, rmap.data(), sizeof(int) * rmap.size(), hipMemcpyHostToDevice, st)) != hipSuccess)
+    return hip_err("hipMemcpyAsync", e);
+  int rc = launch_hilbert(wav, (float*)hw.p, ntw, 1, st);
+  if (!rc) rc = launch_hilbert(data, (float*)hd.p, nt, ntr, st);
+  if (!rc)
+    rc = launch_stack(wav, (const float*)hw.p, ntw, (const int*)dsm.p, ns, nx, (const float2*)dcs.p, np, sou, st);
+  if (!rc)
+    rc = launch_stack(data, (const float*)hd.p, nt, (const int*)drm.p, ns, nx, (const float2*)dcs.p, np, rec, st);
+  if (rc) return rc;
+  e = hipStreamSynchronize(st);
+  return e == hipSuccess ? SEIS_OK : hip_err("plane prepare", e);
+}
+
+}  // namespace seis

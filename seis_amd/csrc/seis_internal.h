@@ -31,4 +31,12 @@ template <typename T>
 int run_rtm_plane(const GridSpec& g, int nshots, const ShotSpec* shots, const T* vel, const T* sou, const int* gz,
                   const T* rec, T* img, hipStream_t stream);
 
+// harmonic of one trace of n samples (device buffers).
+int run_harmonic(const float* in, float* out, int n, float kx, float x, hipStream_t stream);
+
+// planeWavePrepare for np ray parameters; arguments as seis_plane_prepare.
+int run_plane_prepare(int nx, int nt, int ntw, int ns, int np, float pmin, float dp, const float* sx,
+                      const int* isx, const int* ng, const int* col, const float* wav, const float* data,
+                      float* sou, float* rec, hipStream_t stream);
+
 }  // namespace seis

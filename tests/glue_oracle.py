@@ -8,6 +8,8 @@ Follows lsrtm/inversion.cpp of the reference with its float/double promotions
   to_global     swapModel    inversion.cpp:644-665
   remove_direct removeDirect inversion.cpp:699-752
   source_cell                inversion.cpp:186-187
+  plane_geometry planeWavePrepare's columns  inversion.cpp:845-893
+  plane_rows     the plane-wave source / receiver rows  inversion.cpp:331-332
 The reference's inversion.cpp needs MPI (absent here), so it cannot be run;
 these are pinned by the hand-computed cases in tests/test_driver_glue.py.
 """
@@ -124,3 +126,23 @@ def remove_direct(p, csg, v, sx, sz, gx, gz):
         it = max(it, 1)
         csg[ig, :it] = 0
     return csg
+
+
+def plane_geometry(p, sxs, gxs):
+    """Source column of every shot and model column of every trace (-1
+    outside the model) of planeWavePrepare, over the whole model."""
+    dx, velfx = f32(p["dx"]), f32(p["velfx"])
+    isx = [int(f32(f32(f32(f32(sx) - velfx) / dx) + f32(0.001))) for sx in sxs]
+    x_min, x_max = _edges(velfx, p["nx"], dx)
+    cols = []
+    for gx in gxs:
+        c = []
+        for g in np.asarray(gx, np.float32):
+            c.append(int(f32(f32(f32(g - x_min) / dx) + f32(0.1))) if x_min <= g <= x_max else -1)
+        cols.append(np.array(c, np.int32))
+    return np.array(isx, np.int32), cols
+
+
+def plane_rows(p, sz0, gz0):
+    dz = float(f32(p["dz"]))
+    return int((float(f32(sz0)) - 0.0) / dz + float(f32(0.0001))), int((float(f32(gz0)) - 0.0) / dz + float(f32(0.0001)))

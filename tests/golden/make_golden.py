@@ -2,7 +2,8 @@
 
 Run in the container that holds the reference tree, after `make -C oracle ref`:
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py            # every fixture
+    python tests/golden/make_golden.py harmonic   # only harmonic.npz
 
 It calls the reference's own modeling / forward / adjoint / illum and the
 plane-wave forwordPlane / adjointPlane / illumPlane (src/util/fd.h:3-11)
@@ -90,9 +91,39 @@ def save(name, c, out, note):
     print(name, {k: v.shape for k, v in arrays.items()})
 
 
+def harmonic_cases(ref):
+    """harmonic (InitInversion.cpp:141-151) of seeded traces: lengths around
+    the filter's half length (100) and past 1024 samples, phase angles kx*x of
+    zero, of both signs and beyond 2 pi, and a trace of signed zeros."""
+    rng = np.random.default_rng(31)
+    lengths = [1, 5, 99, 100, 101, 201, 250, 1030, 2500, 64]
+    angles = [(0.0, 1500.0), (2.5e-4, 3210.0), (-4.0e-4, 1999.0), (1.1e-3, 7525.0), (3.3e-4, -600.0),
+              (-1.0e-3, 40.0), (5.0e-4, 0.0), (2.0e-3, 4711.0), (7.7e-5, 12345.0), (6.0e-4, 2100.0)]
+    xs, out, n, kx, x0 = [], [], [], [], []
+    for i, (m, (k, x)) in enumerate(zip(lengths, angles)):
+        tr = rng.standard_normal(m).astype(np.float32)
+        if i == len(lengths) - 1:
+            tr = np.where(np.arange(m) % 2 == 0, np.float32(-0.0), np.float32(0.0)).astype(np.float32)
+        xs.append(tr)
+        out.append(ref.harmonic(tr, k, x))
+        n.append(m)
+        kx.append(k)
+        x0.append(x)
+    arrays = dict(x=np.concatenate(xs), out=np.concatenate(out), n=np.array(n, np.int32),
+                  kx=np.array(kx, np.float32), xs=np.array(x0, np.float32))
+    meta = {"note": "harmonic of the reference (cos(kx x) in - sin(kx x) H[in], 201-tap Hilbert); "
+                    "traces concatenated, lengths in n"}
+    np.savez_compressed(os.path.join(HERE, "harmonic.npz"), meta=json.dumps(meta), **arrays)
+    print("harmonic.npz", {k: v.shape for k, v in arrays.items()})
+
+
 def main():
     ref = Reference()
     ref.set_threads(os.cpu_count() or 1)
+    if sys.argv[1:] == ["harmonic"]:
+        harmonic_cases(ref)
+        return
+    harmonic_cases(ref)
     cases = [
         ("small_a.npz", small_case(11, 24, 20, 6, 80, 5)),
         ("small_b.npz", small_case(12, 40, 33, 10, 150, 0)),

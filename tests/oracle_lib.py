@@ -130,6 +130,14 @@ class _Lib:
         self._fn("adjoint_plane", dtype)(*head, _I(gz), _ptr(vel), _ptr(sou), _ptr(rec), _ptr(img))
         return img
 
+    def harmonic(self, x, kx, xs, dtype=np.float32):
+        """cos(kx xs) x - sin(kx xs) H[x] with the 201-tap Hilbert filter."""
+        S = self._scalar(dtype)
+        x = np.ascontiguousarray(x, dtype)
+        out = np.zeros_like(x)
+        self._fn("harmonic", dtype)(_ptr(x), _ptr(out), _I(x.shape[0]), S(kx), S(xs))
+        return out
+
 
 class Oracle(_Lib):
     def __init__(self, path=ORACLE_SO):
@@ -145,6 +153,25 @@ class Oracle(_Lib):
         grad = np.zeros((nx, nz), dtype)
         f(*args, _ptr(wav), _ptr(vel), _ptr(rec), _ptr(grad))
         return grad
+
+    def plane_prepare(self, nx, nt, ntw, np_, pmin, dp, sx, isx, ng, col, wav, data, dtype=np.float32):
+        """planeWavePrepare: sou [np][nx][ntw], rec [np][nx][nt]."""
+        S = self._scalar(dtype)
+        sx = np.ascontiguousarray(sx, dtype)
+        isx = np.ascontiguousarray(isx, np.int32)
+        ng = np.ascontiguousarray(ng, np.int32)
+        col = np.ascontiguousarray(col, np.int32)
+        wav = np.ascontiguousarray(wav, dtype)
+        data = np.ascontiguousarray(data, dtype)
+        ns = sx.shape[0]
+        assert isx.shape == (ns,) and ng.shape == (ns,) and col.shape == (int(ng.sum()),)
+        assert wav.shape == (ntw,) and data.size == int(ng.sum()) * nt
+        sou = np.zeros((np_, nx, ntw), dtype)
+        rec = np.zeros((np_, nx, nt), dtype)
+        self._fn("plane_prepare", dtype)(_I(nx), _I(nt), _I(ntw), _I(ns), _I(np_), S(pmin), S(dp), _ptr(sx),
+                                         _ptr(isx), _ptr(ng), _ptr(col), _ptr(wav), _ptr(data), _ptr(sou),
+                                         _ptr(rec))
+        return sou, rec
 
     def hilbert_taps(self, half, dtype=np.float32):
         h = np.zeros(2 * half + 1, dtype)

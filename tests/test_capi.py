@@ -55,3 +55,25 @@ def test_argument_errors_without_gpu():
                           np.ones((4, 4), np.float32))  # source outside the model
     msg = seis_amd.lib().seis_last_error().decode()
     assert "source" in msg
+
+
+def test_plane_prepare_argument_errors_without_gpu():
+    """planeWavePrepare's geometry is checked before any device work."""
+    import ctypes
+    import numpy as np
+    import seis_amd
+    L = seis_amd.lib()
+    P = ctypes.c_void_p
+
+    def call(isx, col):
+        sx = np.array([100.0], np.float32)
+        isx = np.array(isx, np.int32)
+        ng = np.array([len(col)], np.int32)
+        col = np.array(col, np.int32)
+        return L.seis_plane_prepare(8, 10, 12, 1, 2, 0.0, 1e-4, sx.ctypes.data_as(P), isx.ctypes.data_as(P),
+                                    ng.ctypes.data_as(P), col.ctypes.data_as(P), None, None, None, None)
+
+    assert call([8], [0, 1]) == 1 and "source column" in L.seis_last_error().decode()
+    assert call([3], [0, 9]) == 1 and "trace column" in L.seis_last_error().decode()
+    assert call([3], [0, -1]) == 1 and "null array" in L.seis_last_error().decode()
+    assert L.seis_harmonic(None, None, 4, 1e-3, 10.0) == 1

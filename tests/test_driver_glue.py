@@ -60,3 +60,12 @@ def test_remove_direct_mute_time():
     assert out[0, :145].sum() == 0 and out[0, 145] == 1
     # receiver 2 ten cells away: t = 10 * 10 / 2000 = 0.05 s = 50 samples
     assert out[1, :195].sum() == 0 and out[1, 195] == 1
+
+
+def test_plane_geometry_columns_and_rows():
+    # whole model [0, 290] m with a 0.01 m margin; a column is
+    # floor((gx + 0.01) / 10 + 0.1), so 4.99 m is column 0 and 9 m column 1
+    isx, cols = G.plane_geometry(P, [45.0, 290.0], [[-5.0, 0.0, 4.99, 9.0], [289.0, 290.01, 290.02]])
+    assert list(isx) == [4, 29]
+    assert list(cols[0]) == [-1, 0, 0, 1] and list(cols[1]) == [29, 29, -1]
+    assert G.plane_rows(P, 20.0, 10.0) == (2, 1)

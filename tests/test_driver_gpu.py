@@ -130,3 +130,57 @@ def test_illum_files(engine, oracle, tmp_path):
         assert np.array_equal(got, part), is_
         ref = part + ref
     assert np.array_equal(total, ref)
+
+
+def test_plane_wave_files(engine, oracle, tmp_path):
+    """seis_run plane_prepare / rtm_plane / illum_plane (planeWavePrepare,
+    adjointPlane_MPI, illumPlane_MPI) with a mask: the PLANE_S_NEW /
+    PLANE_R_NEW files bit for bit against the oracle's planeWavePrepare, the
+    per-parameter images and illuminations against the oracle's adjointPlane /
+    illumPlane, and their sums in parameter order."""
+    from seis_amd import driver
+    print("case driver_plane", flush=True)
+    P, shots, v, v0 = _survey(tmp_path)
+    param = tmp_path / "param.txt"
+    n_p, dp, pmin = 3, 1.0e-4, -1.0e-4
+    mask = np.ones((NX, NZ), np.float32)
+    mask[:, :6] = 0.0
+    mask.tofile(tmp_path / "mask.bin")
+    text = param.read_text().replace("maxiter 3\n", "maxiter 3\nmask 1\nplaneTag 1\n")
+    text += "maskfile %s/mask.bin\nnp %d\ndp %g\npmin %g\n" % (tmp_path, n_p, dp, pmin)
+    param.write_text(text)
+    rng = np.random.default_rng(5)
+    gathers = []
+    for is_, shot in enumerate(shots):
+        g = rng.standard_normal((len(shot[2]), NT)).astype(np.float32)
+        g.tofile(tmp_path / "CSG" / ("CSG%d.dat" % is_))
+        gathers.append(g)
+    driver.run(param, "plane_prepare")
+    wav = oracle.ricker(P["dt"], P["fr"], P["delay"], NT)
+    isx, cols = G.plane_geometry(P, [s[0] for s in shots], [s[2] for s in shots])
+    sou, rec = oracle.plane_prepare(NX, NT, NT, n_p, np.float32(pmin), np.float32(dp),
+                                    np.array([s[0] for s in shots], np.float32), isx, [len(c) for c in cols],
+                                    np.concatenate(cols), wav, np.concatenate(gathers))
+    for k in range(n_p):
+        gs = np.fromfile(tmp_path / "WDIR" / ("PLANE_S_NEW%d.dat" % k), np.float32).reshape(NX, NT)
+        gr = np.fromfile(tmp_path / "WDIR" / ("PLANE_R_NEW%d.dat" % k), np.float32).reshape(NX, NT)
+        assert np.array_equal(gs, sou[k]) and np.array_equal(gr, rec[k]), k
+    sz, gz = G.plane_rows(P, shots[0][1], shots[0][3][0])
+    driver.run(param, "rtm_plane")
+    driver.run(param, "illum_plane")
+    img = np.fromfile(tmp_path / "WDIR" / "PLANE_IMAGE.dat", np.float32).reshape(NX, NZ)
+    ss = np.fromfile(tmp_path / "WDIR" / "ILLUM_PLANE.dat", np.float32).reshape(NX, NZ)
+    img_sum = np.zeros((NX, NZ), np.float32)
+    ss_sum = np.zeros((NX, NZ), np.float32)
+    for k in range(n_p):
+        a = (P["dt"], P["dx"], P["dz"], NT, 0, NX, NZ, P["npml"], sz, gz, v0, sou[k])
+        want_img = oracle.adjoint_plane(*a, rec[k]) * mask
+        want_ss = oracle.illum_plane(*a)
+        got_img = np.fromfile(tmp_path / "WDIR" / ("IMAGE%d.dat" % k), np.float32).reshape(NX, NZ)
+        got_ss = np.fromfile(tmp_path / "WDIR" / ("ILLUM_PLANE%d.dat" % k), np.float32).reshape(NX, NZ)
+        assert rel_l2(got_img, want_img) <= 1e-4, k
+        assert rel_l2(got_ss, want_ss) <= 1e-5, k
+        assert not got_img[:, :6].any()
+        img_sum = got_img + img_sum
+        ss_sum = got_ss + ss_sum
+    assert np.array_equal(img, img_sum) and np.array_equal(ss, ss_sum)
