@@ -294,6 +294,78 @@ def c5_sample(nshots, nt, seed=1005):
             "image_finite": bool(np.isfinite(img).all())}
 
 
+def plane_prepare_leg(nshots, ng, nt, n_p, with_cpu, seed=1006):
+    """Plane-wave synthesis (planeWavePrepare): the workload's 96 shots of 480
+    traces (every 3rd column of 2301), nt = 5000, phase-shifted and stacked
+    for 8 ray parameters.  Gathers resident on the GPU before timing; the CPU
+    baseline is the oracle's restatement of the reference loop on one shot and
+    one ray parameter, scaled by shots x parameters."""
+    import ctypes
+    import seis_amd
+    from seis_amd import DeviceArray
+    from seis_amd._lib import check
+    L = seis_amd.lib()
+    nx, dx = 2301, 4.0
+    rng = np.random.default_rng(seed)
+    isx = np.linspace(300, nx - 300, nshots).astype(np.int32)
+    sx = (isx.astype(np.float32) * np.float32(dx)).astype(np.float32)
+    cols = []
+    for s in range(nshots):
+        c = (isx[s] - 3 * (ng // 2) + 3 * np.arange(ng)).astype(np.int32)
+        c[(c < 0) | (c >= nx)] = -1
+        cols.append(c)
+    col = np.concatenate(cols)
+    ngs = np.full(nshots, ng, np.int32)
+    ntw = nt + 500
+    wav = seis_amd.ricker(0.0004, 12.0, 208 + 500, ntw)
+    d_data = DeviceArray((nshots * ng, nt), np.float32)
+    chunk = 4 * ng
+    first = None
+    for r0 in range(0, nshots * ng, chunk):  # fill in pieces (bounded host memory)
+        n = min(chunk, nshots * ng - r0)
+        piece = rng.standard_normal((n, nt), dtype=np.float32)
+        if first is None:
+            first = piece[:ng].copy()
+        check(L.seis_memcpy_htod(ctypes.c_void_p(d_data.ptr.value + r0 * nt * 4),
+                                 piece.ctypes.data_as(ctypes.c_void_p), piece.nbytes))
+    d_wav = DeviceArray.from_host(wav)
+    d_sou = DeviceArray((n_p, nx, ntw), np.float32)
+    d_rec = DeviceArray((n_p, nx, nt), np.float32)
+    P = ctypes.c_void_p
+    pmin, dp = np.float32(-2.0e-4), np.float32(5.0e-5)
+    args = (nx, nt, ntw, nshots, n_p, float(pmin), float(dp), sx.ctypes.data_as(P), isx.ctypes.data_as(P),
+            ngs.ctypes.data_as(P), col.ctypes.data_as(P), d_wav.ptr, d_data.ptr, d_sou.ptr, d_rec.ptr)
+    check(L.seis_plane_prepare(*args))  # warm-up
+    seis_amd.synchronize()
+    seis_amd.profile_start()
+    t0 = time.perf_counter()
+    check(L.seis_plane_prepare(*args))
+    seis_amd.synchronize()
+    el = time.perf_counter() - t0
+    st = seis_amd.profile_stop()
+    out = {"workload": "planeWavePrepare: %d shots x %d traces, nt=%d, %d ray parameters, 2301 columns" %
+                       (nshots, ng, nt, n_p),
+           "s": round(el, 4), "trace_samples_per_s_per_parameter": round(nshots * ng * nt * n_p / el / 1e9, 2),
+           "unit": "G trace-samples/s (x parameters)",
+           "kernels": {k: {"launches": v["launches"], "ms": round(v["total_ms"], 3),
+                           "GB/s": round(v["alg_bytes"] / max(v["total_ms"], 1e-9) / 1e6, 1)}
+                       for k, v in st.items() if k.startswith("plane_")}}
+    rec = d_rec.download()
+    out["output_finite"] = bool(np.isfinite(rec).all())
+    if with_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from oracle_lib import Oracle
+        o = Oracle()
+        t0 = time.perf_counter()
+        o.plane_prepare(nx, nt, ntw, 1, pmin, dp, sx[:1], isx[:1], ngs[:1], cols[0], wav, first)
+        cpu1 = time.perf_counter() - t0
+        est = cpu1 * nshots * n_p
+        out["cpu"] = {"kind": "port", "cores": 1, "sample": "1 shot x %d traces x nt=%d, 1 ray parameter" % (ng, nt),
+                      "sample_s": round(cpu1, 2), "est_s": round(est, 1), "gpu_speedup": round(est / el, 1)}
+    log("plane prepare: %.3f s on the GPU" % el)
+    return out
+
+
 def lsrtm_bench(nshots, nt, iters, err_shots, err_iters):
     """Seconds per CG iteration of least-squares RTM on the 376 x 1151 @ 8 m
     workload (the RTM model at half resolution, nt = 3000, dt = 0.8 ms, 400
@@ -447,6 +519,7 @@ def main():
     ap.add_argument("--c3-pair-shots", type=int, default=16)
     ap.add_argument("--c5-shots", type=int, default=2, help="shots of the C5 RTM sample (0 skips it)")
     ap.add_argument("--c5-nt", type=int, default=500)
+    ap.add_argument("--plane-shots", type=int, default=96, help="shots of the plane-wave synthesis leg (0 skips it)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 8 device-to-device copies of 1 GiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -567,6 +640,8 @@ def main():
     if args.c1_calls > 0:
         log("C1 single-shot modeling, %d calls per mode" % args.c1_calls)
         out["c1_single_shot"] = c1_single_shot(args.c1_calls, not args.no_cpu, threads)
+    if args.plane_shots > 0:
+        out["plane_prepare"] = plane_prepare_leg(args.plane_shots, 480, 5000, 8, not args.no_cpu)
     if args.lsrtm_shots > 0:
         out["lsrtm"] = lsrtm_bench(args.lsrtm_shots, args.lsrtm_nt, args.lsrtm_iters, 2, 3)
     if not args.no_cpu:

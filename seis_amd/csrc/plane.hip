@@ -75,12 +75,15 @@ constexpr int KB = 8;     // ray parameters per thread
 
 // out[k][x][t] for k in [k0, k0 + KB): the sum over shots s (in order) of
 // cs[k][s].cos * d[t] - cs[k][s].sin * H[d][t], d the row map[s][x] of in (no
-// term where map[s][x] < 0).  One work-group per (t tile, column x, k block).
+// term where map[s][x] < 0).  span[x] bounds the shots that hold a row in
+// column x ([first, last + 1), empty for none), so the loop visits only those,
+// still in shot order.  One work-group per (t tile, column x, k block).
 // With `single` the output is the one shot's term itself (harmonic), not a
 // sum started from zero (which turns a -0 term into +0).
 __global__ __launch_bounds__(SWG) void plane_stack_kernel(const float* __restrict__ in, const float* __restrict__ hin,
-                                                          int n, const int* __restrict__ map, int ns, int nx,
-                                                          const float2* __restrict__ cs, int np, int single,
+                                                          int n, const int* __restrict__ map,
+                                                          const int2* __restrict__ span, int nx,
+                                                          const float2* __restrict__ cs, int ns, int np, int single,
                                                           float* __restrict__ out) {
   const int t = blockIdx.x * SWG + threadIdx.x;
   const int x = blockIdx.y;
@@ -90,7 +93,8 @@ __global__ __launch_bounds__(SWG) void plane_stack_kernel(const float* __restric
 #pragma unroll
   for (int b = 0; b < KB; b++) acc[b] = 0.f;
   if (t < n) {
-    for (int s = 0; s < ns; s++) {
+    const int2 sp = span[x];
+    for (int s = sp.x; s < sp.y; s++) {
       const int r = map[(long long)s * nx + x];
       if (r < 0) continue;
       const float d = in[(long long)r * n + t];
@@ -144,33 +148,54 @@ int launch_hilbert(const float* in, float* out, int n, long long rows, hipStream
   return e == hipSuccess ? SEIS_OK : hip_err("hilbert_kernel", e);
 }
 
-int launch_stack(const float* in, const float* hin, int n, const int* map, int ns, int nx, const float2* cs, int np,
-                 float* out, hipStream_t st, int single = 0) {
+// `placed`: (shot, column) pairs of the map that hold a row.  Algorithmic
+// bytes: every placed row's samples and their H read once per block of KB
+// ray parameters, every output sample written once.
+int launch_stack(const float* in, const float* hin, int n, const int* map, const int2* span, int ns, int nx,
+                 const float2* cs, int np, float* out, hipStream_t st, double placed, int single = 0) {
   if (np <= 0 || nx <= 0 || n <= 0) return SEIS_OK;
-  dim3 grid((n + SWG - 1) / SWG, nx, (np + KB - 1) / KB);
-  timed_launch(st, "plane_stack", 4.0 * n * nx * np, 0, [&] {
-    plane_stack_kernel<<<grid, SWG, 0, st>>>(in, hin, n, map, ns, nx, cs, np, single, out);
+  const int kblocks = (np + KB - 1) / KB;
+  dim3 grid((n + SWG - 1) / SWG, nx, kblocks);
+  timed_launch(st, "plane_stack", 8.0 * n * placed * kblocks + 4.0 * n * nx * np, 0, [&] {
+    plane_stack_kernel<<<grid, SWG, 0, st>>>(in, hin, n, map, span, nx, cs, ns, np, single, out);
   });
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? SEIS_OK : hip_err("plane_stack_kernel", e);
 }
 
+// [first, last + 1) of the shots s with map[s][x] >= 0, per column x.
+std::vector<int2> column_spans(const std::vector<int>& map, int ns, int nx) {
+  std::vector<int2> sp(nx, int2{0, 0});
+  for (int x = 0; x < nx; x++)
+    for (int s = 0; s < ns; s++)
+      if (map[(size_t)s * nx + x] >= 0) {
+        if (sp[x].y == 0) sp[x].x = s;
+        sp[x].y = s + 1;
+      }
+  return sp;
+}
+
 }  // namespace
 
 int run_harmonic(const float* in, float* out, int n, float kx, float x, hipStream_t st) {
-  Scratch h, cs, map;
+  Scratch h, cs, map, span;
   hipError_t e = hipMalloc(&h.p, sizeof(float) * n);
   if (e != hipSuccess) return hip_err("hipMalloc", e);
   const float ang = kx * x;
   const float2 c{cosf(ang), sinf(ang)};
   const int zero = 0;
-  if ((e = hipMalloc(&cs.p, sizeof(float2))) != hipSuccess || (e = hipMalloc(&map.p, sizeof(int))) != hipSuccess)
+  const int2 one{0, 1};
+  if ((e = hipMalloc(&cs.p, sizeof(float2))) != hipSuccess || (e = hipMalloc(&map.p, sizeof(int))) != hipSuccess ||
+      (e = hipMalloc(&span.p, sizeof(int2))) != hipSuccess)
     return hip_err("hipMalloc", e);
   if ((e = hipMemcpyAsync(cs.p, &c, sizeof(c), hipMemcpyHostToDevice, st)) != hipSuccess ||
-      (e = hipMemcpyAsync(map.p, &zero, sizeof(int), hipMemcpyHostToDevice, st)) != hipSuccess)
+      (e = hipMemcpyAsync(map.p, &zero, sizeof(int), hipMemcpyHostToDevice, st)) != hipSuccess ||
+      (e = hipMemcpyAsync(span.p, &one, sizeof(int2), hipMemcpyHostToDevice, st)) != hipSuccess)
     return hip_err("hipMemcpyAsync", e);
   int rc = launch_hilbert(in, (float*)h.p, n, 1, st);
-  if (!rc) rc = launch_stack(in, (const float*)h.p, n, (const int*)map.p, 1, 1, (const float2*)cs.p, 1, out, st, 1);
+  if (!rc)
+    rc = launch_stack(in, (const float*)h.p, n, (const int*)map.p, (const int2*)span.p, 1, 1, (const float2*)cs.p, 1,
+                      out, st, 1.0, 1);
   if (rc) return rc;
   e = hipStreamSynchronize(st);
   return e == hipSuccess ? SEIS_OK : hip_err("plane harmonic", e);
@@ -192,6 +217,9 @@ int run_plane_prepare(int nx, int nt, int ntw, int ns, int np, float pmin, float
       if (col[r0 + g] >= 0) rmap[(size_t)s * nx + col[r0 + g]] = (int)(r0 + g);
     r0 += ng[s];
   }
+  double placed = 0;
+  for (int v : rmap) placed += v >= 0;
+  const std::vector<int2> sspan = column_spans(smap, ns, nx), rspan = column_spans(rmap, ns, nx);
   // cos / sin of p_k * sx_s; p_k = pmin + k * dp in float (inversion.cpp:866)
   std::vector<float2> cs((size_t)np * ns);
   for (int k = 0; k < np; k++) {
@@ -201,24 +229,29 @@ int run_plane_prepare(int nx, int nt, int ntw, int ns, int np, float pmin, float
       cs[(size_t)k * ns + s] = float2{cosf(ang), sinf(ang)};
     }
   }
-  Scratch hw, hd, dcs, dsm, drm;
+  Scratch hw, hd, dcs, dsm, drm, dss, drs;
   hipError_t e;
   if ((e = hipMalloc(&hw.p, sizeof(float) * std::max(ntw, 1))) != hipSuccess ||
       (e = hipMalloc(&hd.p, sizeof(float) * std::max<long long>(ntr * nt, 1))) != hipSuccess ||
       (e = hipMalloc(&dcs.p, sizeof(float2) * std::max<size_t>(cs.size(), 1))) != hipSuccess ||
       (e = hipMalloc(&dsm.p, sizeof(int) * smap.size())) != hipSuccess ||
-      (e = hipMalloc(&drm.p, sizeof(int) * rmap.size())) != hipSuccess)
+      (e = hipMalloc(&drm.p, sizeof(int) * rmap.size())) != hipSuccess ||
+      (e = hipMalloc(&dss.p, sizeof(int2) * nx)) != hipSuccess || (e = hipMalloc(&drs.p, sizeof(int2) * nx)) != hipSuccess)
     return hip_err("hipMalloc", e);
   if ((e = hipMemcpyAsync(dcs.p, cs.data(), sizeof(float2) * cs.size(), hipMemcpyHostToDevice, st)) != hipSuccess ||
       (e = hipMemcpyAsync(dsm.p, smap.data(), sizeof(int) * smap.size(), hipMemcpyHostToDevice, st)) != hipSuccess ||
-      (e = hipMemcpyAsync(drm.p, rmap.data(), sizeof(int) * rmap.size(), hipMemcpyHostToDevice, st)) != hipSuccess)
+      (e = hipMemcpyAsync(drm.p, rmap.data(), sizeof(int) * rmap.size(), hipMemcpyHostToDevice, st)) != hipSuccess ||
+      (e = hipMemcpyAsync(dss.p, sspan.data(), sizeof(int2) * nx, hipMemcpyHostToDevice, st)) != hipSuccess ||
+      (e = hipMemcpyAsync(drs.p, rspan.data(), sizeof(int2) * nx, hipMemcpyHostToDevice, st)) != hipSuccess)
     return hip_err("hipMemcpyAsync", e);
   int rc = launch_hilbert(wav, (float*)hw.p, ntw, 1, st);
   if (!rc) rc = launch_hilbert(data, (float*)hd.p, nt, ntr, st);
   if (!rc)
-    rc = launch_stack(wav, (const float*)hw.p, ntw, (const int*)dsm.p, ns, nx, (const float2*)dcs.p, np, sou, st);
+    rc = launch_stack(wav, (const float*)hw.p, ntw, (const int*)dsm.p, (const int2*)dss.p, ns, nx,
+                      (const float2*)dcs.p, np, sou, st, ns);
   if (!rc)
-    rc = launch_stack(data, (const float*)hd.p, nt, (const int*)drm.p, ns, nx, (const float2*)dcs.p, np, rec, st);
+    rc = launch_stack(data, (const float*)hd.p, nt, (const int*)drm.p, (const int2*)drs.p, ns, nx,
+                      (const float2*)dcs.p, np, rec, st, placed);
   if (rc) return rc;
   e = hipStreamSynchronize(st);
   return e == hipSuccess ? SEIS_OK : hip_err("plane prepare", e);
