@@ -366,6 +366,77 @@ def plane_prepare_leg(nshots, ng, nt, n_p, with_cpu, seed=1006):
     return out
 
 
+def c2_snapshot_vs_recompute(nshots, nt, seed=1007):
+    """The RTM workload's grid (751 x 2301 @ 4 m, pml 30, dt = 0.4 ms) imaged two
+    ways for the same shots.
+    * Recomputation: the reference's method. The forward pass keeps 4-layer
+      boundary strips, and the source field is rebuilt backwards beside the
+      receiver field (3 propagations).
+    * Full snapshot storage: the forward pass stores the Born source of every
+      step and model cell in BF16 with a per-step scale, and the reverse pass
+      propagates the receiver field only and images against the snapshots
+      (the exact adjoint of Born modeling, 2 propagations plus the snapshots'
+      max-scale pass).
+    The records are synthetic; the two images are different operators (FaQi
+    cross-correlation and exact Born adjoint), so time and memory are compared,
+    not images."""
+    import ctypes
+    import seis_amd
+    from seis_amd import DeviceArray, models
+    from seis_amd.batch import _layout, Grid, Shot, RTM_ISOLATE_BAND
+    from seis_amd.lsrtm import LsrtmPlan, SNAP_BF16
+    from seis_amd._lib import check
+    L = seis_amd.lib()
+    nx, nz, pml, dx, dt, fr = 2301, 751, 30, 4.0, 0.0004, 12.0
+    vm = models.smooth_model(models.layered_faulted(nx, nz, dx, dx, seed=seed), 20.0)
+    delay = int(round(1.0 / (fr * dt)))
+    shots = [Shot(sx=int(sx), sz=2, nt=nt) for sx in np.linspace(300, nx - 300, nshots)]
+    grid = Grid(nx, nz, pml, dt, dx, dx)
+    g = grid.c()
+    arr, nw, ngz, nr = _layout(grid, shots)
+    wav = seis_amd.ricker(dt, fr, delay, nt)
+    gzs = [np.full(nx, 2, np.int32)] * nshots
+    d_vm = DeviceArray.from_host(vm)
+    d_wav = DeviceArray.from_host(np.tile(wav, nshots))
+    d_gz = DeviceArray.from_host(np.full(nx * nshots, 2, np.int32))
+    rng = np.random.default_rng(seed)
+    d_rec = DeviceArray.from_host(rng.standard_normal(nr, dtype=np.float32))
+    d_img = DeviceArray((nshots * nx * nz,), np.float32, zero=True)
+    seis_amd.synchronize()
+    t0 = time.perf_counter()
+    check(L.seis_rtm_batch_ex(ctypes.byref(g), nshots, arr, d_vm.ptr, d_wav.ptr, d_gz.ptr, d_rec.ptr, d_img.ptr,
+                              RTM_ISOLATE_BAND))
+    seis_amd.synchronize()
+    t_rec = time.perf_counter() - t0
+    del d_img
+    seis_amd.profile_start()
+    t0 = time.perf_counter()
+    plan = LsrtmPlan(grid, shots, vm, [wav] * nshots, gzs, snapshot=SNAP_BF16)
+    seis_amd.synchronize()
+    t_snap = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    img = plan.adjoint(d_rec)
+    t_adj = time.perf_counter() - t0
+    st = seis_amd.profile_stop()
+    plan.close()
+    cells = (nx + 2 * pml) * (nz + 2 * pml)
+    strips_gb = 32.0 * nt * (nx + nz) / 1e9
+    snap_gb = 2.0 * nx * nz * nt / 1e9
+    log("C2 snapshot vs recompute: %.2f s vs %.2f + %.2f s for %d shots" % (t_rec, t_snap, t_adj, nshots))
+    return {"workload": "rtm imaging, %d shots, 751x2301 cells @ 4 m, pml 30, nt=%d, dt=0.4 ms" % (nshots, nt),
+            "recompute": {"method": "boundary strips + backward source reconstruction (reference; isolated band)",
+                          "s": round(t_rec, 3), "s_per_shot": round(t_rec / nshots, 3),
+                          "propagations": 3, "stored_gb_per_shot": round(strips_gb, 2)},
+            "snapshots": {"method": "BF16 Born-source snapshot of every step, per-step max scale",
+                          "forward_s": round(t_snap, 3), "reverse_s": round(t_adj, 3),
+                          "s_per_shot": round((t_snap + t_adj) / nshots, 3), "propagations": 2,
+                          "stored_gb_per_shot": round(snap_gb, 2),
+                          "shots_per_288_gb": int(288.0 // snap_gb),
+                          "kernels_ms": {k: round(v["total_ms"], 1) for k, v in st.items()}},
+            "recompute_cells_per_s": round(3.0 * cells * nt * nshots / t_rec / 1e9, 1),
+            "image_finite": bool(np.isfinite(img).all())}
+
+
 def lsrtm_bench(nshots, nt, iters, err_shots, err_iters):
     """Seconds per CG iteration of least-squares RTM on the 376 x 1151 @ 8 m
     workload (the RTM model at half resolution, nt = 3000, dt = 0.8 ms, 400
@@ -519,6 +590,8 @@ def main():
     ap.add_argument("--c3-pair-shots", type=int, default=16)
     ap.add_argument("--c5-shots", type=int, default=2, help="shots of the C5 RTM sample (0 skips it)")
     ap.add_argument("--c5-nt", type=int, default=500)
+    ap.add_argument("--c2-snap-shots", type=int, default=4,
+                    help="shots of the snapshot-storage vs recomputation comparison (0 skips it)")
     ap.add_argument("--plane-shots", type=int, default=96, help="shots of the plane-wave synthesis leg (0 skips it)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 8 device-to-device copies of 1 GiB (HBM counter calibration)")
@@ -640,6 +713,8 @@ def main():
     if args.c1_calls > 0:
         log("C1 single-shot modeling, %d calls per mode" % args.c1_calls)
         out["c1_single_shot"] = c1_single_shot(args.c1_calls, not args.no_cpu, threads)
+    if args.c2_snap_shots > 0:
+        out["c2_snapshot_vs_recompute"] = c2_snapshot_vs_recompute(args.c2_snap_shots, args.nt)
     if args.plane_shots > 0:
         out["plane_prepare"] = plane_prepare_leg(args.plane_shots, 480, 5000, 8, not args.no_cpu)
     if args.lsrtm_shots > 0:

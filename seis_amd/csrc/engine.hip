@@ -406,6 +406,9 @@ __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot
     const T as = k.dt * c.m.vel(sxp, szp);
     sub = (as * as) * s.wav[c.it - 1];
   }
+  // SNAP_MAX: this lane's max |S| over the sweep; one atomic per wavefront
+  // after it (non-negative floats order like their bit patterns)
+  float smax = 0.f;
   for (int x = x0; x < x1; x++) {
     const long long row = (long long)x * ld;
     qa[8] = nca;
@@ -451,9 +454,7 @@ __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot
       const T lap = (ua + pa - T(2) * qa[4]) * k.idt2;
       const T S = vt2 * lap * (T(2) / (v * v * v));
       if (MODE == SNAP_MAX) {
-        float a = act ? fabsf((float)S) : 0.f;
-        for (int o = 32; o >= 1; o >>= 1) a = fmaxf(a, __shfl_xor(a, o, 64));
-        if (lane == 0) atomicMax(reinterpret_cast<unsigned int*>(s.scale) + c.it, __float_as_uint(a));
+        smax = fmaxf(smax, act ? fabsf((float)S) : 0.f);
       } else if (act) {
         snap_store(s, k.nz, c.it, (long long)ix * k.nz + iz, S);
       }
@@ -541,6 +542,10 @@ __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot
       qa[j] = qa[j + 1];
       if (TWO) qb[j] = qb[j + 1];
     }
+  }
+  if (MODE == SNAP_MAX) {
+    for (int o = 32; o >= 1; o >>= 1) smax = fmaxf(smax, __shfl_xor(smax, o, 64));
+    if (lane == 0) atomicMax(reinterpret_cast<unsigned int*>(s.scale) + c.it, __float_as_uint(smax));
   }
 }
 
