@@ -508,7 +508,25 @@ def lsrtm_bench(nshots, nt, iters, err_shots, err_iters):
     st = seis_amd.profile_stop()
     out["kernels"] = {k: {"launches": v["launches"], "ms": round(v["total_ms"], 2),
                           "GB/s": round(v["alg_bytes"] / max(v["total_ms"], 1e-9) / 1e6, 1)} for k, v in st.items()}
+    # the same CG with the cross-correlation adjoint (the reference's imaging
+    # condition on the snapshots, SEIS_LSRTM_ADJ_XCORR)
+    check(L.seis_lsrtm_set_adjoint(plan, 1))
+    img_x = DeviceArray((nx, nz), np.float32, zero=True)
+    done_x = ctypes.c_int(0)
+    seis_amd.profile_start()
+    t0 = time.perf_counter()
+    check(L.seis_lsrtm_cg(plan, dobs.ptr, iters, 0.0, img_x.ptr, None, ctypes.byref(done_x)))
+    seis_amd.synchronize()
+    el_x = time.perf_counter() - t0
+    st_x = seis_amd.profile_stop()
     check(L.seis_lsrtm_destroy(plan))
+    out["kernels_xcorr"] = {k: {"launches": v["launches"], "ms": round(v["total_ms"], 2)} for k, v in st_x.items()}
+    out["s_per_cg_iter_bf16_xcorr"] = round(el_x / max(done_x.value, 1), 3)
+    if done_x.value > 0 and "lsrtm_born" in st_x and "lsrtm_adjoint_xcorr" in st_x:
+        out["s_per_cg_iter_bf16_xcorr_marginal"] = round((st_x["lsrtm_born"]["total_ms"] / done_x.value
+                                                          + st_x["lsrtm_adjoint_xcorr"]["total_ms"]
+                                                          / (done_x.value + 1)) / 1e3, 3)
+    log("lsrtm: %d CG iterations with the cross-correlation adjoint in %.1f s" % (done_x.value, el_x))
     h = np.array(hist[:]).reshape(-1, 2)[: done.value + 1]
     out["cg_iterations"] = done.value
     out["s_per_cg_iter_bf16"] = round(el / max(done.value, 1), 3)
@@ -534,6 +552,9 @@ def lsrtm_bench(nshots, nt, iters, err_shots, err_iters):
     out["speedup_vs_plain_fp32"] = round(plain / out["s_per_cg_iter_bf16"], 2)
     if "s_per_cg_iter_bf16_marginal" in out:
         out["speedup_vs_plain_fp32_marginal"] = round(plain / out["s_per_cg_iter_bf16_marginal"], 2)
+    out["speedup_vs_plain_fp32_xcorr"] = round(plain / out["s_per_cg_iter_bf16_xcorr"], 2)
+    if "s_per_cg_iter_bf16_xcorr_marginal" in out:
+        out["speedup_vs_plain_fp32_xcorr_marginal"] = round(plain / out["s_per_cg_iter_bf16_xcorr_marginal"], 2)
     log("lsrtm: plain FP32 path, one iteration's Born + RTM in %.1f s" % plain)
     del d_img, d_rec, d_pert, dobs
     # --- image error of 16/32-bit snapshots against the FP64 plan (small batch)
@@ -544,7 +565,8 @@ def lsrtm_bench(nshots, nt, iters, err_shots, err_iters):
         egz = DeviceArray.from_host(np.full(nx * len(eshots), 2, np.int32))
         ed = born_data(eshots, earr, enr, emask, d_v0, ew, egz, 12)
         imgs = {}
-        for kind, name in ((0, "fp32"), (1, "bf16"), (2, "fp16"), (-1, "fp64")):
+        for kind, name, adj in ((0, "fp32", 0), (1, "bf16", 0), (2, "fp16", 0), (-1, "fp64", 0),
+                                (1, "bf16_xcorr", 1), (-1, "fp64_xcorr", 1)):
             pl = ctypes.c_void_p()
             if kind < 0:
                 dt64 = np.float64
@@ -559,6 +581,7 @@ def lsrtm_bench(nshots, nt, iters, err_shots, err_iters):
                                           egz.ptr, kind))
             m = DeviceArray.from_host(emask.astype(dt64))
             check(L.seis_lsrtm_set_mask(pl, m.ptr))
+            check(L.seis_lsrtm_set_adjoint(pl, adj))
             dd = DeviceArray.from_host(ed.astype(dt64))
             im = DeviceArray((nx, nz), dt64, zero=True)
             check(L.seis_lsrtm_cg(pl, dd.ptr, err_iters, 0.0, im.ptr, None, None))
@@ -567,6 +590,10 @@ def lsrtm_bench(nshots, nt, iters, err_shots, err_iters):
         ref = imgs["fp64"]
         out["image_rel_l2_vs_fp64"] = {k: float("%.3e" % (np.linalg.norm(imgs[k] - ref) / np.linalg.norm(ref)))
                                        for k in ("fp32", "bf16", "fp16")}
+        refx = imgs["fp64_xcorr"]
+        out["image_rel_l2_vs_fp64"]["bf16_xcorr"] = float("%.3e" % (np.linalg.norm(imgs["bf16_xcorr"] - refx)
+                                                                    / np.linalg.norm(refx)))
+        out["xcorr_vs_exact_fp64_image_rel"] = float("%.3e" % (np.linalg.norm(refx - ref) / np.linalg.norm(ref)))
         out["error_case"] = "%d shots, %d CG iterations, same grid and nt" % (len(eshots), err_iters)
         log("lsrtm: image errors vs FP64 %s" % out["image_rel_l2_vs_fp64"])
     return out

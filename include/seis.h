@@ -247,8 +247,20 @@ int seis_lsrtm_create_f64(seis_lsrtm** plan, const seis_grid* grid, int nshots, 
 int seis_lsrtm_destroy(seis_lsrtm* plan);
 /* rec = L pert  (float, or double for an FP64 plan) */
 int seis_lsrtm_born(seis_lsrtm* plan, const void* pert, void* rec);
-/* grad = L^T rec */
+/* grad = L^T rec (or the cross-correlation adjoint, see seis_lsrtm_set_adjoint) */
 int seis_lsrtm_adjoint(seis_lsrtm* plan, const void* rec, void* grad);
+/* Which adjoint seis_lsrtm_adjoint and seis_lsrtm_cg apply:
+ *   SEIS_LSRTM_ADJ_EXACT (default): the exact transpose of the Born operator
+ *     (the dot test holds to rounding);
+ *   SEIS_LSRTM_ADJ_XCORR: the reference's imaging condition on the stored
+ *     snapshots, as its intended normal operator pairs forward_MPI with the
+ *     RTM adjoint_MPI (inversion.cpp:420-423). The receiver field is run
+ *     backwards with the forward operator and the record injected times vt2,
+ *     and grad += u S^n / vt2. It equals L^T in the interior and differs in
+ *     the absorbing band; it costs about one Born step per step. */
+#define SEIS_LSRTM_ADJ_EXACT 0
+#define SEIS_LSRTM_ADJ_XCORR 1
+int seis_lsrtm_set_adjoint(seis_lsrtm* plan, int kind);
 /* *out = <a, b> over n elements, accumulated in FP64 on the GPU */
 int seis_lsrtm_dot(seis_lsrtm* plan, const void* a, const void* b, long long n, double* out);
 /* Optional record mask (device, packed like the records; NULL clears): L

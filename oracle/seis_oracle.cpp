@@ -31,6 +31,8 @@
 //   born_adjoint (exact transpose)    : dot-product test (an invariant), and
 //                                       linearization of modeling
 //   cg_solve                          : dense FP64 solve on small SPD systems
+//   born_adjoint_xcorr                : equals born_adjoint where no wave reaches
+//                                       the band (FP64), and the dot test with born there
 
 #include <cmath>
 #include <cstddef>
@@ -935,6 +937,44 @@ void born_adjoint(T dt, T dx, T dz, int nt, int ndel, int nx, int nz, int pml, i
     for (int iz = 0; iz < nz; iz++) grad[(size_t)ix * nz + iz] = gp[g.at(ix + pml, iz + pml)];
 }
 
+// Cross-correlation adjoint of born(): the receiver field run backwards in
+// time with the forward step (stencil, record injected as rec * vt2 after the
+// stencil as fd.cpp:673-677 injects it, HABC, blend) and grad += u S^n / vt2
+// over the model cells, S^n the Born source coefficient of step n.  In the
+// interior u is vt2 times the field of born_adjoint, so the two agree where
+// no wave reaches the absorbing band.
+template <typename T>
+void born_adjoint_xcorr(T dt, T dx, T dz, int nt, int ndel, int nx, int nz, int pml, int sx, int sz, const int* gz,
+                        const T* wav, const T* v, const T* rec, T* grad) {
+  Geom g(nx, nz, pml);
+  Coefs<T> k(g, dt, dx, dz, v, true);
+  const size_t C = g.cells();
+  const int NT = nt + ndel;
+  std::vector<T> bs;
+  born_source_history(g, dt, dx, dz, k, ndel, nt, sx, sz, wav, bs);
+  std::vector<T> a(C), b(C), c(C), ow(C), gp(C);
+  T *u0 = a.data(), *u1 = b.data(), *u2 = c.data();
+  for (int n = NT - 1; n >= 0; n--) {
+    stencil_step(g, u0, u1, u2, k.vvzz.data(), k.vvxx.data());
+    if (n - ndel >= 0)
+      for (int ix = 0; ix < nx; ix++) {
+        const size_t r = g.at(ix + pml, gz[ix] + pml);
+        u2[r] = u2[r] + rec[(size_t)ix * nt + n - ndel] * k.vt2[r];
+      }
+    habc_update(g, dt, dx, dz, k.vel.data(), ow.data(), u2, u1, u0);
+    blend(g, u2, ow.data(), k.w.data());
+    const T* q = bs.data() + (size_t)n * C;
+    for (int ix = pml; ix < pml + nx; ix++)
+      for (int iz = pml; iz < pml + nz; iz++) {
+        const size_t i = g.at(ix, iz);
+        gp[i] = gp[i] + u2[i] * (q[i] / k.vt2[i]);
+      }
+    T* t = u0; u0 = u1; u1 = u2; u2 = t;
+  }
+  for (int ix = 0; ix < nx; ix++)
+    for (int iz = 0; iz < nz; iz++) grad[(size_t)ix * nz + iz] = gp[g.at(ix + pml, iz + pml)];
+}
+
 }  // namespace
 
 extern "C" {
@@ -1006,6 +1046,12 @@ int oracle_get_threads(void) { return g_threads; }
   int oracle_rtm_##SUF(T dt, T dx, T dz, int nt, int ndel, int nx, int nz, int pml, int sx, int sz,   \
                        const int* gz, const T* wav, const T* vel, const T* rec, T* img) {            \
     rtm<T>(dt, dx, dz, nt, ndel, nx, nz, pml, sx, sz, gz, wav, vel, rec, img);                        \
+    return 0;                                                                                         \
+  }                                                                                                   \
+  int oracle_born_adjoint_xcorr_##SUF(T dt, T dx, T dz, int nt, int ndel, int nx, int nz, int pml, int sx, \
+                                      int sz, const int* gz, const T* wav, const T* vel, const T* rec,     \
+                                      T* grad) {                                                        \
+    born_adjoint_xcorr<T>(dt, dx, dz, nt, ndel, nx, nz, pml, sx, sz, gz, wav, vel, rec, grad);          \
     return 0;                                                                                         \
   }                                                                                                   \
   int oracle_born_adjoint_##SUF(T dt, T dx, T dz, int nt, int ndel, int nx, int nz, int pml, int sx,  \

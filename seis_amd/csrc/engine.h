@@ -63,7 +63,8 @@ enum Mode {
   BORN_S = 6,
   ILLUM = 7,
   PW_BACK = 8,
-  PW_FWD = 9
+  PW_FWD = 9,
+  ADJ_X = 10  // LSRTM cross-correlation adjoint: receiver field backward, imaged against the snapshots
 };
 
 // Time index of launch step `step` of a shot in `mode`, or -1 when the shot
@@ -78,7 +79,7 @@ SEIS_HD inline int step_it(int mode, const DevShot<T>& s, int step) {
     it = NT - 1 - step;
     return it >= s.ndel ? it : -1;
   }
-  if (mode == PW_BACK) {
+  if (mode == PW_BACK || mode == ADJ_X) {
     it = NT - 1 - step;
     return it >= 0 ? it : -1;
   }

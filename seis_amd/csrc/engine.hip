@@ -139,6 +139,19 @@ __device__ __forceinline__ void cell_update(const Consts<T>& k, const DevShot<T>
       T* st = s.strips + (long long)(it - 1) * 8 * (s.nx + k.nz);
       for (int q = 0; q < n; q++) st[sl[q]] = u;
     }
+  } else if (MODE == ADJ_X) {
+    // cross-correlation adjoint: record sample it - ndel injected (times
+    // vt2), then grad += u S^it / vt2 at model cells
+    const ReceiverInject<T> ri{s.gz, s.rec, s.nx, s.nt, it - s.ndel, k.pml};
+    const T u = final_value(k, m, NX, C, P, ri, x, z);
+    N[i] = u;
+    const int ix = x - k.pml, iz = z - k.pml;
+    if (ix >= 0 && ix < s.nx && iz >= 0 && iz < k.nz) {
+      const long long j = (long long)ix * k.nz + iz;
+      const T a = k.dt * m.vel(x, z);
+      const T vt2 = a * a;
+      s.grad[j] = s.grad[j] + u * (snap_load(s, k.nz, it, j) / vt2);
+    }
   } else if (MODE == PW_FWD) {
     // line-source field and U = lap 2/v^3; the receiver field rebuilt
     // forward with record sample it - ndel - 1 removed from its previous
@@ -360,11 +373,11 @@ __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot
   // column of software pipelining; the last column re-reads itself).
   // The receiver depth of the column (and, in the reverse pass, its record
   // sample) is fetched the same way.
-  constexpr bool GZ =
-      (MODE == MODEL || MODE == BORN || MODE == BORN_S || MODE == RTM_BWD || MODE == PW_BACK || MODE == PW_FWD);
+  constexpr bool GZ = (MODE == MODEL || MODE == BORN || MODE == BORN_S || MODE == RTM_BWD || MODE == PW_BACK ||
+                       MODE == PW_FWD || MODE == ADJ_X);
   // record sample read per column: injected (RTM_BWD, PW_BACK) or removed
   // from the previous level (PW_FWD)
-  constexpr bool RECIN = (MODE == RTM_BWD || MODE == PW_BACK || MODE == PW_FWD);
+  constexpr bool RECIN = (MODE == RTM_BWD || MODE == PW_BACK || MODE == PW_FWD || MODE == ADJ_X);
   const int rec_t = MODE == PW_FWD ? c.it - s.ndel - 1 : c.itp;
   T nca, ncb = T(0), npa, npb = T(0), nv, nha = T(0), nhb = T(0), nrec = T(0), nss = T(0), ndv = T(0);
   SnapWord<T> nsn{T(0), 0};
@@ -392,10 +405,16 @@ __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot
       ndv = SEIS_GL(s.dv)[j];
       nsn = snap_fetch(s, k.nz, c.it, j);
     }
+    if (MODE == ADJ_X && act) {
+      // gradient and Born-source snapshot of this model cell
+      const long long j = (long long)(xx - pml) * k.nz + iz;
+      nss = SEIS_GL(s.grad)[j];
+      nsn = snap_fetch(s, k.nz, c.it, j);
+    }
   };
   fetch(x0);
   // per-step scalars
-  constexpr bool PSRC = (MODE != RTM_BWD && MODE != BORN_S && MODE != PW_BACK);
+  constexpr bool PSRC = (MODE != RTM_BWD && MODE != BORN_S && MODE != PW_BACK && MODE != ADJ_X);
   T amp = T(0);
   if (PSRC && c.it - 1 >= 0 && !s.line_src) amp = s.wav[c.it - 1];
   // the plane-wave line source: this lane's row carries it
@@ -520,6 +539,10 @@ __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot
         SEIS_GL(c.aux)[i] = lap * v3;
         NA[i] = ua;
         NB[i] = ub;
+      } else if (MODE == ADJ_X) {
+        if (rec_t >= 0 && z == gzr) ua = ua + rec_cur * vt2;
+        NA[i] = ua;
+        SEIS_GL(s.grad)[(long long)ix * k.nz + iz] = ss_cur + ua * (snap_decode(s, c.it, sn_cur) / vt2);
       } else {  // RTM_BWD: source field ua, receiver field ub
         if (z == gzr) ub = ub + rec_cur * vt2;
         const T lap = (pa + ua - T(2) * qa[4]) * k.idt2;
@@ -752,6 +775,7 @@ void launch_step(int mode, const Consts<T>& k, const DevShot<T>* dev, const Layo
     SEIS_CASE(ILLUM)
     SEIS_CASE(PW_BACK)
     SEIS_CASE(PW_FWD)
+    SEIS_CASE(ADJ_X)
 #undef SEIS_CASE
   }
 }

@@ -369,6 +369,7 @@ struct LsrtmPlan {
   Ring P{};     // cells whose raw adjoint is precomputed
   Ring B{};     // absorbing band of the forward steps
   bool adj_tuned = false;
+  int adj_kind = SEIS_LSRTM_ADJ_EXACT;  // seis_lsrtm_set_adjoint
   hipStream_t stream = 0;
 
   ~LsrtmPlan() {
@@ -536,8 +537,44 @@ struct LsrtmPlan {
     return SEIS_OK;
   }
 
-  // grad (device [nx][nz]) = L^T rec, summed over shots
+  // grad (device [nx][nz]) = the adjoint of the plan's kind applied to rec
   int adjoint(const T* rec, T* grad) {
+    return adj_kind == SEIS_LSRTM_ADJ_XCORR ? adjoint_xcorr(rec, grad) : adjoint_exact(rec, grad);
+  }
+
+  // Cross-correlation adjoint: the receiver field run backwards in time with
+  // the forward operator (stencil, HABC, sponge), the record injected times
+  // vt2 at the receivers, and grad += u S^n / vt2 at every model cell and
+  // step.  In the interior this field is vt2 times the exact adjoint field,
+  // so it images like L^T there; in the absorbing band it is the forward
+  // operator instead of its transpose (the reference's RTM imaging, fd.cpp:
+  // 592-719, with stored snapshots in place of the rebuilt source field).
+  int adjoint_xcorr(const T* rec, T* grad) {
+    if (mask) {
+      if (!masked) LS_CHECK(hipMalloc(&masked, sizeof(T) * (size_t)nrec));
+      hipLaunchKernelGGL((mul_kernel<T>), dim3(1024), dim3(256), 0, stream, masked, rec, mask, nrec);
+      rec = masked;
+    }
+    for (int s = 0; s < nshots; s++) host[s].rec = const_cast<T*>(rec) + spec[s].rec;
+    LS_CHECK(hipMemcpyAsync(dev, host.data(), sizeof(DevShot<T>) * nshots, hipMemcpyHostToDevice, stream));
+    int rc = zero_fields(LS_FIELD, 3);
+    if (rc) return rc;
+    LS_CHECK(hipMemsetAsync(grads, 0, sizeof(T) * (size_t)model * nshots, stream));
+    const double E = sizeof(T);
+    for (int st = 0; st < NTmax; st++) {
+      const int n = active_fwd(st);  // shot s runs backward steps st < nt + ndel
+      const double by = 4 * E * n * cells() + (snap_elem() + 2 * E) * n * (double)model;
+      timed_launch(stream, "lsrtm_adjoint_xcorr", by, n * cells(),
+                   [&] { launch_step<T>(ADJ_X, k, dev, F, B, nshots, st, stream); });
+    }
+    hipLaunchKernelGGL((sum_grads_kernel<T>), dim3(256), dim3(256), 0, stream, dev, nshots, model, grad);
+    LS_CHECK(hipGetLastError());
+    LS_CHECK(hipStreamSynchronize(stream));
+    return SEIS_OK;
+  }
+
+  // grad (device [nx][nz]) = L^T rec, summed over shots
+  int adjoint_exact(const T* rec, T* grad) {
     if (mask) {
       if (!masked) LS_CHECK(hipMalloc(&masked, sizeof(T) * (size_t)nrec));
       hipLaunchKernelGGL((mul_kernel<T>), dim3(1024), dim3(256), 0, stream, masked, rec, mask, nrec);
@@ -701,6 +738,17 @@ int seis_lsrtm_dot(seis_lsrtm* plan, const void* a, const void* b, long long n, 
   if (!plan || !a || !b || !out) return set_error("lsrtm: null argument"), SEIS_ERR_ARG;
   return plan->fp64 ? plan->d->dot((const double*)a, (const double*)b, n, out)
                     : plan->f->dot((const float*)a, (const float*)b, n, out);
+}
+
+int seis_lsrtm_set_adjoint(seis_lsrtm* plan, int kind) {
+  if (!plan) return set_error("null plan"), SEIS_ERR_ARG;
+  if (kind != SEIS_LSRTM_ADJ_EXACT && kind != SEIS_LSRTM_ADJ_XCORR)
+    return set_error("lsrtm: adjoint kind must be SEIS_LSRTM_ADJ_EXACT or SEIS_LSRTM_ADJ_XCORR"), SEIS_ERR_ARG;
+  if (plan->fp64)
+    plan->d->adj_kind = kind;
+  else
+    plan->f->adj_kind = kind;
+  return SEIS_OK;
 }
 
 int seis_lsrtm_set_mask(seis_lsrtm* plan, const void* mask) {

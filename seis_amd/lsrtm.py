@@ -15,6 +15,7 @@ from .batch import _layout
 from .device import DeviceArray
 
 SNAP_FP32, SNAP_BF16, SNAP_FP16 = 0, 1, 2
+ADJ_EXACT, ADJ_XCORR = 0, 1  # exact transpose / the reference's cross-correlation imaging
 
 
 class LsrtmPlan:
@@ -75,6 +76,11 @@ class LsrtmPlan:
         out = ctypes.c_double()
         check(lib().seis_lsrtm_dot(self.handle, da.ptr, db.ptr, int(np.size(a)), ctypes.byref(out)))
         return out.value
+
+    def set_adjoint(self, kind):
+        """ADJ_EXACT (the exact transpose, default) or ADJ_XCORR (the
+        reference's imaging condition on the snapshots) for adjoint() and cg()."""
+        check(lib().seis_lsrtm_set_adjoint(self.handle, int(kind)))
 
     def set_mask(self, mask):
         """Record mask (packed like the records) applied in L and L^T; None clears."""

@@ -173,6 +173,18 @@ class Oracle(_Lib):
                                          _ptr(rec))
         return sou, rec
 
+    def born_adjoint_xcorr(self, dt, dx, dz, nt, ndel, nx, nz, pml, sx, sz, gz, wav, vel, rec, dtype=np.float32):
+        """grad[nx][nz]: the cross-correlation adjoint of born() (forward operator
+        run backwards, grad += u S / vt2)."""
+        f, args, keep = self._geom_call("born_adjoint_xcorr", dtype, dt, dx, dz, nt, ndel, nx, nz, pml, sx, sz, gz)
+        wav = np.ascontiguousarray(wav, dtype)
+        vel = np.ascontiguousarray(vel, dtype)
+        rec = np.ascontiguousarray(rec, dtype)
+        assert vel.shape == (nx, nz) and rec.shape == (nx, nt)
+        grad = np.zeros((nx, nz), dtype)
+        f(*args, _ptr(wav), _ptr(vel), _ptr(rec), _ptr(grad))
+        return grad
+
     def hilbert_taps(self, half, dtype=np.float32):
         h = np.zeros(2 * half + 1, dtype)
         self._fn("hilbert_taps", dtype)(_I(half), _ptr(h))

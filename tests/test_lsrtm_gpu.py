@@ -145,3 +145,52 @@ def test_cgls_matches_textbook_cgls_on_explicit_operator(engine, oracle):
     assert rmin * (1 - 1e-10) <= hist_n[-1, 0] <= rmin * (1 + 1e-4)
     assert np.all(np.diff(hist_n[:, 0]) <= 1e-12 * hist_n[0, 0])  # CGLS residuals never increase
     plan.close()
+
+
+def test_xcorr_adjoint_matches_oracle(engine, oracle):
+    """The cross-correlation adjoint (the reference's imaging condition on the
+    stored snapshots) against the oracle's restatement: the FP32 plan with FP32
+    snapshots bit for bit (summed over shots in shot order), the FP64 plan to
+    1e-12."""
+    from seis_amd.lsrtm import LsrtmPlan, SNAP_FP32, ADJ_XCORR
+    print("case lsrtm_adjoint_xcorr", flush=True)
+    rng, g, shots, vel, wavs, gzs = _setup(engine, oracle, seed=6)
+    r = rng.standard_normal(sum(g.nx * s.nt for s in shots))
+    rs = _split(g, shots, r)
+    a = lambda j, s, dt: (g.dt, g.dx, g.dz, s.nt, s.ndel, g.nx, g.nz, g.pml, s.sx, s.sz, gzs[j])  # noqa: E731
+    ref32 = sum(oracle.born_adjoint_xcorr(*a(j, s, None), wavs[j], vel, rs[j].astype(np.float32))
+                for j, s in enumerate(shots))
+    p32 = LsrtmPlan(g, shots, vel, wavs, gzs, snapshot=SNAP_FP32)
+    p32.set_adjoint(ADJ_XCORR)
+    got32 = p32.adjoint(r.astype(np.float32))
+    print("xcorr fp32 rel %.3e bitexact %s" % (rel_l2(got32, ref32), np.array_equal(got32, ref32)))
+    assert rel_l2(got32, ref32) <= 1e-6
+    assert np.array_equal(got32, ref32)
+    ref64 = sum(oracle.born_adjoint_xcorr(*a(j, s, None), wavs[j].astype(np.float64), vel.astype(np.float64), rs[j],
+                                          dtype=np.float64) for j, s in enumerate(shots))
+    p64 = LsrtmPlan(g, shots, vel.astype(np.float64), [w.astype(np.float64) for w in wavs], gzs, fp64=True)
+    p64.set_adjoint(ADJ_XCORR)
+    assert rel_l2(p64.adjoint(r), ref64) <= 1e-12
+
+
+def test_xcorr_cg_with_bf16_snapshots(engine, oracle):
+    """CGLS with the cross-correlation adjoint and BF16 snapshots: the data
+    residual falls and the image stays within 1e-3 of the FP64 plan run the same
+    way; the exact adjoint stays the default."""
+    from seis_amd.lsrtm import LsrtmPlan, SNAP_BF16, ADJ_XCORR
+    print("case lsrtm_xcorr_cg", flush=True)
+    rng, g, shots, vel, wavs, gzs = _setup(engine, oracle, seed=7)
+    dv = np.zeros((g.nx, g.nz), np.float32)
+    dv[10:30, 12:24] = rng.standard_normal((20, 12)).astype(np.float32) * 40
+    p64 = LsrtmPlan(g, shots, vel.astype(np.float64), [w.astype(np.float64) for w in wavs], gzs, fp64=True)
+    d = p64.born(dv.astype(np.float64))
+    p64.set_adjoint(ADJ_XCORR)
+    x64, h64 = p64.cg(d, niter=4)
+    pb = LsrtmPlan(g, shots, vel, wavs, gzs, snapshot=SNAP_BF16)
+    pb.set_adjoint(ADJ_XCORR)
+    xb, hb = pb.cg(d.astype(np.float32), niter=4)
+    assert hb.shape == (5, 2)
+    assert hb[-1, 0] < 0.5 * hb[0, 0]
+    err = rel_l2(xb, x64)
+    print("xcorr cg bf16 vs fp64 image rel %.3e, residual %.3e -> %.3e" % (err, hb[0, 0], hb[-1, 0]))
+    assert err <= 1e-3

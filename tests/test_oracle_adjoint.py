@@ -72,3 +72,60 @@ def test_born_is_derivative_of_modeling(oracle):
         errs.append(rel_l2(fd, born))
     # second-order convergence of the central difference to the Born record
     assert errs[1] < 1e-6 and errs[1] < errs[0], errs
+
+
+def _interior_case():
+    """Source, perturbation and live receivers well inside the model, pml 20,
+    and a record short enough (70 steps, at most 14 cells of travel at
+    2000 m/s) that no wave reaches the absorbing band."""
+    nx = nz = 64
+    pml, nt, ndel = 20, 70, 0
+    rng = np.random.default_rng(21)
+    vel = (1900 + 200 * rng.random((nx, nz))).astype(np.float64)
+    gz = np.full(nx, 32, np.int32)
+    dv = np.zeros((nx, nz))
+    dv[26:38, 26:38] = rng.standard_normal((12, 12))
+    r = np.zeros((nx, nt))
+    r[24:40] = rng.standard_normal((16, nt))
+    c = dict(dt=0.001, dx=10.0, dz=10.0, nt=nt, ndel=ndel, nx=nx, nz=nz, pml=pml, sx=32, sz=30, gz=gz, vel=vel)
+    return c, dv, r
+
+
+def test_xcorr_adjoint_is_the_transpose_away_from_the_band(oracle):
+    """Where no wave reaches the band, the cross-correlation adjoint (forward
+    operator run backwards, rec * vt2 injected, grad += u S / vt2) is the
+    exact transpose: same image as born_adjoint, and the dot test with born
+    holds, in FP64.  A one-step shift of the imaging time would break both."""
+    c, dv, r = _interior_case()
+    wav = oracle.ricker(c["dt"], 25.0, 40, c["nt"] + c["ndel"], dtype=np.float64)
+    gx = oracle.born_adjoint_xcorr(*_args(c, wav), r, dtype=np.float64)
+    ge = oracle.born_adjoint(*_args(c, wav), r, dtype=np.float64)
+    assert np.abs(gx).max() > 0
+    assert np.linalg.norm(gx - ge) <= 1e-10 * np.linalg.norm(ge)
+    Ld = oracle.born(*_args(c, wav), dv, dtype=np.float64)
+    lhs, rhs = float(np.sum(Ld * r)), float(np.sum(dv * gx))
+    assert abs(lhs - rhs) <= 1e-10 * max(abs(lhs), abs(rhs)), (lhs, rhs)
+
+
+def test_xcorr_adjoint_detects_a_time_shift(oracle):
+    """The same comparison with the record delayed by one sample fails by far:
+    the agreement above pins the time alignment, not just the amplitude."""
+    c, dv, r = _interior_case()
+    wav = oracle.ricker(c["dt"], 25.0, 40, c["nt"] + c["ndel"], dtype=np.float64)
+    ge = oracle.born_adjoint(*_args(c, wav), r, dtype=np.float64)
+    shifted = np.zeros_like(r)
+    shifted[:, 1:] = r[:, :-1]
+    gs = oracle.born_adjoint_xcorr(*_args(c, wav), shifted, dtype=np.float64)
+    assert np.linalg.norm(gs - ge) > 1e-2 * np.linalg.norm(ge)
+
+
+def test_xcorr_adjoint_differs_only_through_the_band(oracle):
+    """With waves in the band the two adjoints differ, but stay close."""
+    c = _case(5)
+    rng = c["rng"]
+    wav = oracle.ricker(c["dt"], 30.0, 15 + c["ndel"], c["nt"] + c["ndel"], dtype=np.float64)
+    r = rng.standard_normal((c["nx"], c["nt"]))
+    gx = oracle.born_adjoint_xcorr(*_args(c, wav), r, dtype=np.float64)
+    ge = oracle.born_adjoint(*_args(c, wav), r, dtype=np.float64)
+    err = np.linalg.norm(gx - ge) / np.linalg.norm(ge)
+    assert 1e-8 < err < 0.5, err
