@@ -118,6 +118,8 @@ int seis_ricker(float dt, float fr, int num_delay, int n, float* out);
  *   "rtm"       image of the <precsg> records       (adjoint_MPI, RTM_IMG)
  *   "rtm_born"  image of the <wdir>BORN records     (adjoint_MPI, LSRTM_STEP)
  *   "lsrtm"     least-squares RTM, maxiter CG iterations
+ *   "lsrtm_xcorr" the same with the cross-correlation adjoint
+ *               (SEIS_LSRTM_ADJ_XCORR)
  *   "illum"     source illumination of every shot -> <wdir>ILLUM<is>.dat,
  *               and their sum (illum_MPI)
  * and, with planeTag != 0, per ray parameter p_k = pmin + k dp (k < np):

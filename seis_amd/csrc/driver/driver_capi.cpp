@@ -57,8 +57,9 @@ int seis_run(const char* param_file, const char* mode, const char* model_file, c
   std::vector<float> img(n);
   if (m == "rtm" || m == "rtm_born") {
     if (!inv.adjoint(img.data(), m == "rtm" ? RTM_IMG : LSRTM_STEP)) return set_error(inv.error()), SEIS_ERR_HIP;
-  } else if (m == "lsrtm") {
-    if (!inv.cgsolver(p.maxiter.val, 0.f, img.data())) return set_error(inv.error()), SEIS_ERR_HIP;
+  } else if (m == "lsrtm" || m == "lsrtm_xcorr") {
+    const int kind = m == "lsrtm" ? SEIS_LSRTM_ADJ_EXACT : SEIS_LSRTM_ADJ_XCORR;
+    if (!inv.cgsolver(p.maxiter.val, 0.f, img.data(), nullptr, kind)) return set_error(inv.error()), SEIS_ERR_HIP;
   } else if (m == "illum") {
     if (!inv.illum(img.data())) return set_error(inv.error()), SEIS_ERR_HIP;
   } else if (m == "rtm_plane" || m == "rtm_plane_born") {
@@ -69,11 +70,11 @@ int seis_run(const char* param_file, const char* mode, const char* model_file, c
     if (!p.planeTag.val) return set_error("seis_run: plane-wave modes need planeTag != 0"), SEIS_ERR_ARG;
     if (!inv.illumPlane(img.data())) return set_error(inv.error()), SEIS_ERR_HIP;
   } else {
-    return set_error("seis_run: mode must be modeling, born, rtm, rtm_born, lsrtm, illum, plane_prepare, "
-                     "rtm_plane, rtm_plane_born or illum_plane"),
+    return set_error("seis_run: mode must be modeling, born, rtm, rtm_born, lsrtm, lsrtm_xcorr, illum, "
+                     "plane_prepare, rtm_plane, rtm_plane_born or illum_plane"),
            SEIS_ERR_ARG;
   }
-  const char* name = m == "lsrtm"         ? "LSRTM_IMAGE.dat"
+  const char* name = m == "lsrtm" || m == "lsrtm_xcorr" ? "LSRTM_IMAGE.dat"
                      : m == "illum"       ? "ILLUM.dat"
                      : m == "illum_plane" ? "ILLUM_PLANE.dat"
                      : m == "rtm_plane" || m == "rtm_plane_born" ? "PLANE_IMAGE.dat"

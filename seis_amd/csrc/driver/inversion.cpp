@@ -372,7 +372,7 @@ bool Inversion::adjointPlane(float* img, int migTag) {
   return true;
 }
 
-bool Inversion::cgsolver(int maxIter, float reError, float* img, std::vector<double>* rnorm) {
+bool Inversion::cgsolver(int maxIter, float reError, float* img, std::vector<double>* rnorm, int adjoint_kind) {
   std::vector<float> v0;
   if (!read_model(param.v0file.val, v0)) return false;
   const int nx = param.nx.val, nz = param.nz.val, nt = param.nt.val, ns = survey.ns();
@@ -412,6 +412,7 @@ bool Inversion::cgsolver(int maxIter, float reError, float* img, std::vector<dou
   std::vector<double> hist(2 * (maxIter + 1), 0.0);
   int done = 0;
   int rc = seis_lsrtm_set_mask(plan, dm.p);
+  if (!rc) rc = seis_lsrtm_set_adjoint(plan, adjoint_kind);
   if (!rc) rc = seis_lsrtm_cg(plan, dd.p, maxIter, (double)reError, di.p, hist.data(), &done);
   seis_lsrtm_destroy(plan);
   if (rc) return err_ = gpu_error("lsrtm cg"), false;

@@ -55,7 +55,10 @@ class Inversion {
   // Least-squares RTM of the CSG records by CG on the exact Born operator
   // (the declared conope_MPI / cgsolver_MPI, inversion.h:81,87): maxIter
   // iterations or until ||L^T r|| / ||L^T d|| < reError.  img [nx][nz].
-  bool cgsolver(int maxIter, float reError, float* img, std::vector<double>* rnorm = nullptr);
+  // adjoint_kind: SEIS_LSRTM_ADJ_EXACT (the transpose) or
+  // SEIS_LSRTM_ADJ_XCORR (the reference's cross-correlation imaging).
+  bool cgsolver(int maxIter, float reError, float* img, std::vector<double>* rnorm = nullptr,
+                int adjoint_kind = 0);
 
   const std::string& error() const { return err_; }
   Params param;

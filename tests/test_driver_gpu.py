@@ -108,6 +108,10 @@ def test_modeling_rtm_lsrtm_files(engine, oracle, tmp_path):
     driver.run(param, "lsrtm")
     ls = np.fromfile(tmp_path / "WDIR" / "LSRTM_IMAGE.dat", np.float32).reshape(NX, NZ)
     assert np.isfinite(ls).all() and np.abs(ls).max() > 0
+    # the same CG with the cross-correlation adjoint: a nearby image
+    driver.run(param, "lsrtm_xcorr", out_file=tmp_path / "WDIR" / "LSRTM_XCORR.dat")
+    lx = np.fromfile(tmp_path / "WDIR" / "LSRTM_XCORR.dat", np.float32).reshape(NX, NZ)
+    assert np.isfinite(lx).all() and rel_l2(lx, ls) < 0.5
 
 
 def test_illum_files(engine, oracle, tmp_path):
