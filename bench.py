@@ -366,7 +366,7 @@ def plane_prepare_leg(nshots, ng, nt, n_p, with_cpu, seed=1006):
     return out
 
 
-def c2_snapshot_vs_recompute(nshots, nt, seed=1007):
+def snapshot_vs_recompute(nshots, nt, seed=1007):
     """The RTM workload's grid (751 x 2301 @ 4 m, pml 30, dt = 0.4 ms) imaged two
     ways for the same shots.
     * Recomputation: the reference's method. The forward pass keeps 4-layer
@@ -617,7 +617,7 @@ def main():
     ap.add_argument("--c3-pair-shots", type=int, default=16)
     ap.add_argument("--c5-shots", type=int, default=2, help="shots of the C5 RTM sample (0 skips it)")
     ap.add_argument("--c5-nt", type=int, default=500)
-    ap.add_argument("--c2-snap-shots", type=int, default=4,
+    ap.add_argument("--snap-shots", type=int, default=4,
                     help="shots of the snapshot-storage vs recomputation comparison (0 skips it)")
     ap.add_argument("--plane-shots", type=int, default=96, help="shots of the plane-wave synthesis leg (0 skips it)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -740,8 +740,8 @@ def main():
     if args.c1_calls > 0:
         log("C1 single-shot modeling, %d calls per mode" % args.c1_calls)
         out["c1_single_shot"] = c1_single_shot(args.c1_calls, not args.no_cpu, threads)
-    if args.c2_snap_shots > 0:
-        out["c2_snapshot_vs_recompute"] = c2_snapshot_vs_recompute(args.c2_snap_shots, args.nt)
+    if args.snap_shots > 0:
+        out["snapshot_vs_recompute"] = snapshot_vs_recompute(args.snap_shots, args.nt)
     if args.plane_shots > 0:
         out["plane_prepare"] = plane_prepare_leg(args.plane_shots, 480, 5000, 8, not args.no_cpu)
     if args.lsrtm_shots > 0:
