@@ -4,6 +4,7 @@ Run in the container that holds the reference tree, after `make -C oracle ref`:
 
     python tests/golden/make_golden.py            # every fixture
     python tests/golden/make_golden.py harmonic   # only harmonic.npz
+    python tests/golden/make_golden.py c1_rtm     # only c1_rtm.npz (after c1_modeling.npz)
 
 It calls the reference's own modeling / forward / adjoint / illum and the
 plane-wave forwordPlane / adjointPlane / illumPlane (src/util/fd.h:3-11)
@@ -117,13 +118,40 @@ def harmonic_cases(ref):
     print("harmonic.npz", {k: v.shape for k, v in arrays.items()})
 
 
+def c1_rtm(ref):
+    """The reference's adjoint (RTM, fd.cpp:592-719) on the one-shot 201 x 301
+    grid, nt = 1500: the record of c1_modeling.npz imaged with a smoothed
+    model (only the image is stored; the inputs are rebuilt from that fixture)."""
+    with np.load(os.path.join(HERE, "c1_modeling.npz"), allow_pickle=False) as f:
+        c = {k: f[k] for k in f.files}
+    meta = json.loads(str(c.pop("meta")))
+    vel = c["vel"].astype(np.float64)
+    k = np.array([1, 4, 6, 4, 1], np.float64) / 16.0
+    for _ in range(3):  # a smooth migration model: separable binomial passes with edge replication
+        vel = np.apply_along_axis(lambda a: np.convolve(np.pad(a, 2, mode="edge"), k, mode="valid"), 0, vel)
+        vel = np.apply_along_axis(lambda a: np.convolve(np.pad(a, 2, mode="edge"), k, mode="valid"), 1, vel)
+    vmig = vel.astype(np.float32)
+    t0 = time.time()
+    img = ref.rtm(meta["dt"], meta["dx"], meta["dz"], meta["nt"], meta["ndel"], meta["nx"], meta["nz"], meta["pml"],
+                  meta["sx"], meta["sz"], c["gz"], c["wav"], vmig, c["rec"])
+    print("C1 RTM %.2f s" % (time.time() - t0))
+    np.savez_compressed(os.path.join(HERE, "c1_rtm.npz"),
+                        meta=json.dumps({"note": "reference adjoint (RTM) of the c1_modeling record with a smoothed "
+                                                 "model; inputs in c1_modeling.npz"}),
+                        vmig=vmig, img=img)
+
+
 def main():
     ref = Reference()
     ref.set_threads(os.cpu_count() or 1)
     if sys.argv[1:] == ["harmonic"]:
         harmonic_cases(ref)
         return
+    if sys.argv[1:] == ["c1_rtm"]:
+        c1_rtm(ref)
+        return
     harmonic_cases(ref)
+    c1_rtm(ref)
     cases = [
         ("small_a.npz", small_case(11, 24, 20, 6, 80, 5)),
         ("small_b.npz", small_case(12, 40, 33, 10, 150, 0)),

@@ -225,3 +225,17 @@ def test_plane_wave_batch_matches_oracle(engine, oracle, pml):
         assert np.array_equal(born[j], oracle.born_plane(*a, dv)), j
         assert np.array_equal(ss[j], oracle.illum_plane(*a)), j
         assert np.array_equal(img[j], oracle.adjoint_plane(*a, recs[j])), j
+
+
+def test_c1_rtm_matches_reference(engine):
+    """fd.h adjoint (RTM) on the one-shot 201 x 301 grid, nt = 1500, against the
+    compiled reference's image (tests/golden/c1_rtm.npz)."""
+    c = load_fixture("c1_modeling.npz")
+    g = load_fixture("c1_rtm.npz")
+    print("case c1_rtm 201x301 nt=1500", flush=True)
+    img = engine.adjoint(c["dt"], c["dx"], c["dz"], c["nt"], c["ndel"], c["nx"], c["nz"], c["pml"], c["sx"], c["sz"],
+                         c["gz"], c["wav"], g["vmig"], c["rec"])
+    err = rel_l2(img, g["img"])
+    print("c1 rtm rel_l2 %.3e bitexact %s" % (err, np.array_equal(img, g["img"])))
+    assert err <= IMAGE_TOL
+    assert np.array_equal(img, g["img"])

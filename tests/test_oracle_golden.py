@@ -245,3 +245,13 @@ def test_plane_prepare_sums_each_shots_phase_shift(oracle):
                 placed[c] = oracle.harmonic(traces[s][g], p, sx[s])
         rs = placed + rs
     assert np.array_equal(sou[0], ws) and np.array_equal(rec[0], rs)
+
+
+def test_c1_rtm_bitexact(oracle):
+    """The reference's adjoint (RTM) on the one-shot 201 x 301 grid, nt = 1500,
+    imaging the modeled record with a smoothed model."""
+    c = load_fixture("c1_modeling.npz")
+    g = load_fixture("c1_rtm.npz")
+    img = oracle.rtm(c["dt"], c["dx"], c["dz"], c["nt"], c["ndel"], c["nx"], c["nz"], c["pml"], c["sx"], c["sz"],
+                     c["gz"], c["wav"], g["vmig"], c["rec"])
+    assert np.array_equal(img, g["img"])
