@@ -5,6 +5,7 @@ Run in the container that holds the reference tree, after `make -C oracle ref`:
     python tests/golden/make_golden.py            # every fixture
     python tests/golden/make_golden.py harmonic   # only harmonic.npz
     python tests/golden/make_golden.py c1_rtm     # only c1_rtm.npz (after c1_modeling.npz)
+    python tests/golden/make_golden.py c1_born_illum  # only c1_born_illum.npz (after c1_rtm.npz)
 
 It calls the reference's own modeling / forward / adjoint / illum and the
 plane-wave forwordPlane / adjointPlane / illumPlane (src/util/fd.h:3-11)
@@ -141,6 +142,32 @@ def c1_rtm(ref):
                         vmig=vmig, img=img)
 
 
+def c1_born_illum(ref):
+    """Born records (fd.cpp:81-177) and source illumination (fd.cpp:275-339) of
+    the reference on the one-shot 201 x 301 grid, nt = 1500.  The perturbation
+    is the true model minus the smoothed one of c1_rtm.npz.  The Born record
+    (1.8 MB) is kept as the SHA-256 of its float32 bytes plus its norm; the
+    illumination image is kept whole."""
+    import hashlib
+    with np.load(os.path.join(HERE, "c1_modeling.npz"), allow_pickle=False) as f:
+        c = {k: f[k] for k in f.files}
+    meta = json.loads(str(c.pop("meta")))
+    with np.load(os.path.join(HERE, "c1_rtm.npz"), allow_pickle=False) as f:
+        vmig = f["vmig"]
+    dv = (c["vel"] - vmig).astype(np.float32)
+    a = (meta["dt"], meta["dx"], meta["dz"], meta["nt"], meta["ndel"], meta["nx"], meta["nz"], meta["pml"],
+         meta["sx"], meta["sz"])
+    born = ref.born(*a, c["gz"], c["wav"], vmig, dv)
+    ss = ref.illum(*a, c["wav"], vmig)
+    np.savez_compressed(os.path.join(HERE, "c1_born_illum.npz"),
+                        meta=json.dumps({"note": "reference Born record (sha256 of float32 bytes, norm) and "
+                                                 "illumination of the one-shot grid; dv = vel - vmig of c1_rtm.npz"}),
+                        born_sha256=np.frombuffer(hashlib.sha256(np.ascontiguousarray(born).tobytes()).digest(),
+                                                  np.uint8),
+                        born_norm=np.array(np.linalg.norm(born.astype(np.float64))), ss=ss)
+    print("c1_born_illum.npz written")
+
+
 def main():
     ref = Reference()
     ref.set_threads(os.cpu_count() or 1)
@@ -150,8 +177,12 @@ def main():
     if sys.argv[1:] == ["c1_rtm"]:
         c1_rtm(ref)
         return
+    if sys.argv[1:] == ["c1_born_illum"]:
+        c1_born_illum(ref)
+        return
     harmonic_cases(ref)
     c1_rtm(ref)
+    c1_born_illum(ref)
     cases = [
         ("small_a.npz", small_case(11, 24, 20, 6, 80, 5)),
         ("small_b.npz", small_case(12, 40, 33, 10, 150, 0)),

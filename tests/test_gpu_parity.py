@@ -239,3 +239,20 @@ def test_c1_rtm_matches_reference(engine):
     print("c1 rtm rel_l2 %.3e bitexact %s" % (err, np.array_equal(img, g["img"])))
     assert err <= IMAGE_TOL
     assert np.array_equal(img, g["img"])
+
+
+def test_c1_born_and_illum_match_reference(engine):
+    """fd.h forward (Born) and illum on the one-shot 201 x 301 grid, nt = 1500,
+    against the compiled reference (Born by the SHA-256 of its record)."""
+    import hashlib
+    c = load_fixture("c1_modeling.npz")
+    vmig = load_fixture("c1_rtm.npz")["vmig"]
+    g = load_fixture("c1_born_illum.npz")
+    a = (c["dt"], c["dx"], c["dz"], c["nt"], c["ndel"], c["nx"], c["nz"], c["pml"], c["sx"], c["sz"])
+    print("case c1_born_illum 201x301 nt=1500", flush=True)
+    born = engine.forward(*a, c["gz"], c["wav"], vmig, (c["vel"] - vmig).astype(np.float32))
+    assert abs(np.linalg.norm(born.astype(np.float64)) - float(g["born_norm"])) <= GATHER_TOL * float(g["born_norm"])
+    assert hashlib.sha256(np.ascontiguousarray(born).tobytes()).digest() == g["born_sha256"].tobytes()
+    ss = engine.illum(*a, c["wav"], vmig)
+    assert rel_l2(ss, g["ss"]) <= IMAGE_TOL
+    assert np.array_equal(ss, g["ss"])

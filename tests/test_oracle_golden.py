@@ -255,3 +255,21 @@ def test_c1_rtm_bitexact(oracle):
     img = oracle.rtm(c["dt"], c["dx"], c["dz"], c["nt"], c["ndel"], c["nx"], c["nz"], c["pml"], c["sx"], c["sz"],
                      c["gz"], c["wav"], g["vmig"], c["rec"])
     assert np.array_equal(img, g["img"])
+
+
+def _c1_born_illum_inputs():
+    c = load_fixture("c1_modeling.npz")
+    vmig = load_fixture("c1_rtm.npz")["vmig"]
+    a = (c["dt"], c["dx"], c["dz"], c["nt"], c["ndel"], c["nx"], c["nz"], c["pml"], c["sx"], c["sz"])
+    return c, vmig, (c["vel"] - vmig).astype(np.float32), a
+
+
+def test_c1_born_and_illum_bitexact(oracle):
+    """The reference's Born record (by its SHA-256) and illumination on the
+    one-shot 201 x 301 grid, nt = 1500."""
+    import hashlib
+    c, vmig, dv, a = _c1_born_illum_inputs()
+    g = load_fixture("c1_born_illum.npz")
+    born = oracle.born(*a, c["gz"], c["wav"], vmig, dv)
+    assert hashlib.sha256(np.ascontiguousarray(born).tobytes()).digest() == g["born_sha256"].tobytes()
+    assert np.array_equal(oracle.illum(*a, c["wav"], vmig), g["ss"])
