@@ -30,9 +30,8 @@ namespace {
 constexpr int HARM_HALF = 100;             // Hilbert<float> hl(100), InitInversion.cpp:145
 constexpr int HARM_ODD = HARM_HALF / 2;    // nonzero taps on each side (odd offsets)
 constexpr int HWG = 256;                   // hilbert_kernel work-group
-constexpr int HOUT = 4;                    // outputs per thread (strided by HWG)
-constexpr int HTILE = HWG * HOUT;          // outputs per work-group
-constexpr int HSPAN = HTILE + 2 * HARM_HALF;
+constexpr int HTILE = 1024;                // outputs per work-group (512 of each parity)
+constexpr int HPAR = HTILE / 2 + 104;      // inputs per parity array (multiple of 4)
 
 // Taps at odd offsets d = 1, 3, ..., 99: h[L + d]; h[L - d] = -h[L + d].
 struct HarmTaps {
@@ -44,29 +43,57 @@ struct HarmTaps {
 // (hilbert.h:42-54).  Terms whose tap is zero (even j - i) and terms outside
 // the trace are left out: each would add a zero product to a partial sum that
 // is never -0, which leaves it unchanged (for finite samples).
+//
+// Output j of a tile at j0 uses the inputs j - 99, j - 97, ..., j + 99: all of
+// one parity.  The tile's inputs are split by parity into two LDS arrays so
+// that output n of parity p (j = j0 + p + 2n) reads A_p[n .. n + 99] with
+//   A_0[k] = in[j0 - 99 + 2k],  A_1[k] = in[j0 - 98 + 2k].
+// A thread computes the outputs n = 4t .. 4t + 3 of one parity from
+// A_p[4t .. 4t + 102], read as 26 aligned float4 (conflict-free), each input
+// added into every accumulator whose window holds it, in increasing order.
 __global__ __launch_bounds__(HWG) void hilbert_kernel(HarmTaps taps, const float* __restrict__ in,
                                                       float* __restrict__ out, int n) {
-  __shared__ float s[HSPAN];
+  __shared__ __align__(16) float A[2][HPAR];
+  __shared__ float O[HTILE];
   const long long row = blockIdx.y;
   const int j0 = blockIdx.x * HTILE;
   const float* src = in + row * n;
-  for (int i = threadIdx.x; i < HSPAN; i += HWG) {
-    const int t = j0 - HARM_HALF + i;
-    s[i] = (t >= 0 && t < n) ? src[t] : 0.f;
+  for (int k = threadIdx.x; k < 2 * HPAR; k += HWG) {
+    const int p = k / HPAR, kk = k - p * HPAR;
+    const int t = j0 - 99 + p + 2 * kk;
+    A[p][kk] = (t >= 0 && t < n) ? src[t] : 0.f;
   }
   __syncthreads();
-  for (int r = 0; r < HOUT; r++) {
-    const int jl = threadIdx.x + r * HWG;
-    const int j = j0 + jl;
-    if (j >= n) break;
-    // i = j - d for d = 99, 97, ..., 1, then -1, ..., -99: increasing i
-    const float* c = s + jl + HARM_HALF;
-    float acc = 0.f;
+  const int p = threadIdx.x >> 7;   // parity of this thread's outputs
+  const int t = threadIdx.x & 127;  // outputs n = 4t .. 4t + 3
+  const float4* a4 = reinterpret_cast<const float4*>(&A[p][4 * t]);
+  float acc[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int k = HARM_ODD - 1; k >= 0; k--) acc = acc + c[-(2 * k + 1)] * taps.h[k];
+  for (int v = 0; v < 26; v++) {
+    const float4 q = a4[v];
+    const float x[4] = {q.x, q.y, q.z, q.w};
 #pragma unroll
-    for (int k = 0; k < HARM_ODD; k++) acc = acc + c[2 * k + 1] * (-taps.h[k]);
-    out[row * n + j] = acc;
+    for (int e = 0; e < 4; e++) {
+      const int w = 4 * v + e;  // input A_p[4t + w]
+#pragma unroll
+      for (int m = 0; m < 4; m++) {
+        const int u = w - m;  // its index in output (4t + m)'s window
+        if (u >= 0 && u < 100) {
+          // h[L + 99 - 2u]: taps.h[49 - u] for u < 50, -taps.h[u - 50] after
+          if (u < 50)
+            acc[m] = acc[m] + x[e] * taps.h[49 - u];
+          else
+            acc[m] = acc[m] + x[e] * (-taps.h[u - 50]);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < 4; m++) O[p + 2 * (4 * t + m)] = acc[m];
+  __syncthreads();
+  for (int k = threadIdx.x; k < HTILE; k += HWG) {
+    const int j = j0 + k;
+    if (j < n) out[row * n + j] = O[k];
   }
 }
 
