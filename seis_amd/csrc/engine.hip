@@ -856,6 +856,9 @@ int run_batch(int mode, const GridSpec& g, int nshots, const ShotSpec* shots, co
   const bool rtm = (mode == RUN_RTM);
   if (rtm && (g.nx < 4 || g.nz < 4))
     return set_error("rtm: the 4-layer boundary strips need nx >= 4 and nz >= 4"), SEIS_ERR_ARG;
+  // the imaging pass holds U and R of a whole padded column in LDS
+  if (rtm && sizeof(T) * 2 * (size_t)NZ > 64 * 1024)
+    return set_error("rtm: padded nz too large for the column Hilbert kernel"), SEIS_ERR_ARG;
   int ntmax = 0, NTmax = 0;
   size_t strip_elems = 0;
   for (int s = 0; s < nshots; s++) {
@@ -993,8 +996,7 @@ int run_batch(int mode, const GridSpec& g, int nshots, const ShotSpec* shots, co
     constexpr int half = HILBERT_HALF;
     HilbertTaps taps{};
     hilbert_taps_host<float>(half, taps.h);
-    const size_t smem1 = sizeof(T) * 2 * NZ;
-    if (smem1 > 64 * 1024) return set_error("rtm: padded nz too large for the column Hilbert kernel"), SEIS_ERR_ARG;
+    const size_t smem1 = sizeof(T) * 2 * NZ;  // at most 64 KiB (checked with the arguments)
     // two reverse steps per imaging pass where four columns fit in LDS
     const int per = 2 * smem1 <= 64 * 1024 ? 2 : 1;
     const size_t smem = per * smem1;
@@ -1050,6 +1052,8 @@ int run_rtm_plane(const GridSpec& g, int nshots, const ShotSpec* shots, const T*
     return set_error("rtm_plane: the 4-layer boundary strips need nx >= 4 and nz >= 4"), SEIS_ERR_ARG;
   const int NZ = g.nz + 2 * g.pml;
   const int NX = g.nx + 2 * g.pml;
+  if (sizeof(T) * 2 * (size_t)NZ > 64 * 1024)
+    return set_error("rtm_plane: padded nz too large for the column Hilbert kernel"), SEIS_ERR_ARG;
   const long long ld = field_ld(NZ);
   const long long fe = field_elems(NX, ld);
   int NTmax = 0;
@@ -1155,8 +1159,7 @@ int run_rtm_plane(const GridSpec& g, int nshots, const ShotSpec* shots, const T*
   constexpr int half = HILBERT_HALF;
   HilbertTaps taps{};
   hilbert_taps_host<float>(half, taps.h);
-  const size_t smem = sizeof(T) * 2 * NZ;
-  if (smem > 64 * 1024) return set_error("rtm_plane: padded nz too large for the column Hilbert kernel"), SEIS_ERR_ARG;
+  const size_t smem = sizeof(T) * 2 * NZ;  // at most 64 KiB (checked with the arguments)
   const double faqi_cells = (double)g.nx * NZ;
   for (int st = 0; st + 2 < NTmax; st++) {
     const int n = active(PW_FWD, st);

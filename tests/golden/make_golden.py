@@ -6,6 +6,7 @@ Run in the container that holds the reference tree, after `make -C oracle ref`:
     python tests/golden/make_golden.py harmonic   # only harmonic.npz
     python tests/golden/make_golden.py c1_rtm     # only c1_rtm.npz (after c1_modeling.npz)
     python tests/golden/make_golden.py c1_born_illum  # only c1_born_illum.npz (after c1_rtm.npz)
+    python tests/golden/make_golden.py edge_grids     # only edge_grids.npz
 
 It calls the reference's own modeling / forward / adjoint / illum and the
 plane-wave forwordPlane / adjointPlane / illumPlane (src/util/fd.h:3-11)
@@ -168,6 +169,30 @@ def c1_born_illum(ref):
     print("c1_born_illum.npz written")
 
 
+def edge_cases(ref):
+    """Grids at the edges of the engine's layout: no absorbing band (pml 0),
+    and padded grids too small for an interior rectangle, where every cell
+    takes the per-cell path.  Modeling, Born, RTM and illumination of each."""
+    rng = np.random.default_rng(41)
+    arrays = {}
+    cases = [(20, 16, 0), (4, 5, 2), (6, 4, 3), (9, 8, 0)]
+    for n, (nx, nz, pml) in enumerate(cases):
+        vel = (2000 + 500 * rng.random((nx, nz))).astype(np.float32)
+        gz = rng.integers(0, nz, nx).astype(np.int32)
+        dv = (rng.standard_normal((nx, nz)) * 20).astype(np.float32)
+        nt, ndel = 60, 2
+        wav = ref.ricker(0.001, 30.0, 10 + ndel, nt + ndel)
+        a = (0.001, 10.0, 10.0, nt, ndel, nx, nz, pml, nx // 2, nz // 2, gz, wav, vel)
+        rec = ref.modeling(*a)
+        arrays.update({"vel%d" % n: vel, "gz%d" % n: gz, "dv%d" % n: dv, "wav%d" % n: wav, "rec%d" % n: rec,
+                       "born%d" % n: ref.born(*a, dv), "img%d" % n: ref.rtm(*a, rec),
+                       "ss%d" % n: ref.illum(*a[:10], wav, vel)})
+    meta = {"note": "edge grids (pml 0; no interior rectangle): modeling, born, rtm (of the modeled record), illum",
+            "cases": cases, "dt": 0.001, "dx": 10.0, "dz": 10.0, "nt": 60, "ndel": 2}
+    np.savez_compressed(os.path.join(HERE, "edge_grids.npz"), meta=json.dumps(meta), **arrays)
+    print("edge_grids.npz", len(arrays), "arrays")
+
+
 def main():
     ref = Reference()
     ref.set_threads(os.cpu_count() or 1)
@@ -180,9 +205,13 @@ def main():
     if sys.argv[1:] == ["c1_born_illum"]:
         c1_born_illum(ref)
         return
+    if sys.argv[1:] == ["edge_grids"]:
+        edge_cases(ref)
+        return
     harmonic_cases(ref)
     c1_rtm(ref)
     c1_born_illum(ref)
+    edge_cases(ref)
     cases = [
         ("small_a.npz", small_case(11, 24, 20, 6, 80, 5)),
         ("small_b.npz", small_case(12, 40, 33, 10, 150, 0)),

@@ -77,3 +77,17 @@ def test_plane_prepare_argument_errors_without_gpu():
     assert call([3], [0, 9]) == 1 and "trace column" in L.seis_last_error().decode()
     assert call([3], [0, -1]) == 1 and "null array" in L.seis_last_error().decode()
     assert L.seis_harmonic(None, None, 4, 1e-3, 10.0) == 1
+
+
+def test_rtm_column_limit_without_gpu():
+    """The imaging pass holds a whole padded column (U and R) in LDS: a padded
+    depth beyond 8192 cells is rejected with the arguments, before any device
+    work (so also here, without a GPU)."""
+    import ctypes
+    import seis_amd
+    from seis_amd._lib import seis_grid, seis_shot
+    L = seis_amd.lib()
+    g = seis_grid(8, 8200, 0, 0.001, 10.0, 10.0)
+    shots = (seis_shot * 1)(seis_shot(2, 2, 10, 0, 0, 0, 0, 0))
+    assert L.seis_rtm_batch(ctypes.byref(g), 1, shots, None, None, None, None, None) == 1
+    assert "too large" in L.seis_last_error().decode()
