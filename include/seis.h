@@ -120,6 +120,9 @@ int seis_ricker(float dt, float fr, int num_delay, int n, float* out);
  *   "lsrtm"     least-squares RTM, maxiter CG iterations
  *   "lsrtm_xcorr" the same with the cross-correlation adjoint
  *               (SEIS_LSRTM_ADJ_XCORR)
+ *   "lsrtm_illum" lsrtm preconditioned by the source illumination of all
+ *               shots, W = sqrt(max / (ss + 1e-3 max)) (seis_lsrtm_set_weight);
+ *               the per-shot ILLUM<is>.dat are written on the way
  *   "illum"     source illumination of every shot -> <wdir>ILLUM<is>.dat,
  *               and their sum (illum_MPI)
  * and, with planeTag != 0, per ray parameter p_k = pmin + k dp (k < np):
@@ -270,6 +273,13 @@ int seis_lsrtm_dot(seis_lsrtm* plan, const void* a, const void* b, long long n, 
  * the direct-arrival mute of removeDirect (inversion.cpp:699-752).  The
  * caller keeps the mask alive while the plan uses it. */
 int seis_lsrtm_set_mask(seis_lsrtm* plan, const void* mask);
+/* Optional image-space preconditioner of seis_lsrtm_cg (device [nx][nz] in
+ * the plan's precision; NULL clears): CGLS runs on L W with W = diag(weight)
+ * and returns the image m = W x.  Weighting by the inverse square root of
+ * the source illumination (illum_MPI, inversion.cpp:207-290; fd.cpp:275-339)
+ * is the illumination compensation for which the reference computes it.
+ * The caller keeps the weight alive while the plan uses it. */
+int seis_lsrtm_set_weight(seis_lsrtm* plan, const void* weight);
 /* CGLS for min ||L x - dobs||^2 from x = 0: at most niter iterations, stops
  * when ||L^T r|| <= rtol ||L^T dobs|| (rtol <= 0: never); image = x.
  * hist (host, 2*(niter+1) doubles, may be NULL): ||r_i||^2, ||L^T r_i||^2;

@@ -1,5 +1,6 @@
 // C ABI of the parameter-file driver (include/seis.h: seis_run).
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -60,6 +61,17 @@ int seis_run(const char* param_file, const char* mode, const char* model_file, c
   } else if (m == "lsrtm" || m == "lsrtm_xcorr") {
     const int kind = m == "lsrtm" ? SEIS_LSRTM_ADJ_EXACT : SEIS_LSRTM_ADJ_XCORR;
     if (!inv.cgsolver(p.maxiter.val, 0.f, img.data(), nullptr, kind)) return set_error(inv.error()), SEIS_ERR_HIP;
+  } else if (m == "lsrtm_illum") {
+    // illumination-compensated LSRTM: the source illumination of every shot
+    // (illum_MPI) as the preconditioner W = sqrt(max / (ss + 1e-3 max))
+    std::vector<float> ss(n), w(n);
+    if (!inv.illum(ss.data())) return set_error(inv.error()), SEIS_ERR_HIP;
+    float mx = 0.f;
+    for (float v : ss) mx = std::max(mx, v);
+    if (!(mx > 0.f)) return set_error("seis_run: lsrtm_illum: the survey has no illumination"), SEIS_ERR_ARG;
+    for (size_t i = 0; i < n; i++) w[i] = std::sqrt(mx / (ss[i] + 1e-3f * mx));
+    if (!inv.cgsolver(p.maxiter.val, 0.f, img.data(), nullptr, SEIS_LSRTM_ADJ_EXACT, w.data()))
+      return set_error(inv.error()), SEIS_ERR_HIP;
   } else if (m == "illum") {
     if (!inv.illum(img.data())) return set_error(inv.error()), SEIS_ERR_HIP;
   } else if (m == "rtm_plane" || m == "rtm_plane_born") {
@@ -70,11 +82,11 @@ int seis_run(const char* param_file, const char* mode, const char* model_file, c
     if (!p.planeTag.val) return set_error("seis_run: plane-wave modes need planeTag != 0"), SEIS_ERR_ARG;
     if (!inv.illumPlane(img.data())) return set_error(inv.error()), SEIS_ERR_HIP;
   } else {
-    return set_error("seis_run: mode must be modeling, born, rtm, rtm_born, lsrtm, lsrtm_xcorr, illum, "
-                     "plane_prepare, rtm_plane, rtm_plane_born or illum_plane"),
+    return set_error("seis_run: mode must be modeling, born, rtm, rtm_born, lsrtm, lsrtm_xcorr, lsrtm_illum, "
+                     "illum, plane_prepare, rtm_plane, rtm_plane_born or illum_plane"),
            SEIS_ERR_ARG;
   }
-  const char* name = m == "lsrtm" || m == "lsrtm_xcorr" ? "LSRTM_IMAGE.dat"
+  const char* name = m == "lsrtm" || m == "lsrtm_xcorr" || m == "lsrtm_illum" ? "LSRTM_IMAGE.dat"
                      : m == "illum"       ? "ILLUM.dat"
                      : m == "illum_plane" ? "ILLUM_PLANE.dat"
                      : m == "rtm_plane" || m == "rtm_plane_born" ? "PLANE_IMAGE.dat"

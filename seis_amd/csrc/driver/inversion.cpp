@@ -372,7 +372,8 @@ bool Inversion::adjointPlane(float* img, int migTag) {
   return true;
 }
 
-bool Inversion::cgsolver(int maxIter, float reError, float* img, std::vector<double>* rnorm, int adjoint_kind) {
+bool Inversion::cgsolver(int maxIter, float reError, float* img, std::vector<double>* rnorm, int adjoint_kind,
+                         const float* weight) {
   std::vector<float> v0;
   if (!read_model(param.v0file.val, v0)) return false;
   const int nx = param.nx.val, nz = param.nz.val, nt = param.nt.val, ns = survey.ns();
@@ -400,7 +401,8 @@ bool Inversion::cgsolver(int maxIter, float reError, float* img, std::vector<dou
     std::copy(igz.begin(), igz.end(), gz.begin() + (size_t)nx * is);
     sh[is] = seis_shot{sx, sz, nt, ndel, 0, (long long)NT * is, (long long)nx * is, (long long)nx * nt * is};
   }
-  Dev dv, dw, dg, dd, dm, di;
+  Dev dv, dw, dg, dd, dm, di, dwt;
+  if (weight && !dwt.put(weight, sizeof(float) * (size_t)nx * nz)) return err_ = gpu_error("device buffers"), false;
   if (!dv.put(v0.data(), sizeof(float) * v0.size()) || !dw.put(wv.data(), sizeof(float) * wv.size()) ||
       !dg.put(gz.data(), sizeof(int) * gz.size()) || !dd.put(d.data(), sizeof(float) * d.size()) ||
       !dm.put(mask.data(), sizeof(float) * mask.size()) || !di.alloc(sizeof(float) * (size_t)nx * nz))
@@ -413,6 +415,7 @@ bool Inversion::cgsolver(int maxIter, float reError, float* img, std::vector<dou
   int done = 0;
   int rc = seis_lsrtm_set_mask(plan, dm.p);
   if (!rc) rc = seis_lsrtm_set_adjoint(plan, adjoint_kind);
+  if (!rc && weight) rc = seis_lsrtm_set_weight(plan, dwt.p);
   if (!rc) rc = seis_lsrtm_cg(plan, dd.p, maxIter, (double)reError, di.p, hist.data(), &done);
   seis_lsrtm_destroy(plan);
   if (rc) return err_ = gpu_error("lsrtm cg"), false;

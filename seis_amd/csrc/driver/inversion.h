@@ -57,8 +57,10 @@ class Inversion {
   // iterations or until ||L^T r|| / ||L^T d|| < reError.  img [nx][nz].
   // adjoint_kind: SEIS_LSRTM_ADJ_EXACT (the transpose) or
   // SEIS_LSRTM_ADJ_XCORR (the reference's cross-correlation imaging).
+  // weight: optional image-space preconditioner [nx][nz] (host), see
+  // seis_lsrtm_set_weight.
   bool cgsolver(int maxIter, float reError, float* img, std::vector<double>* rnorm = nullptr,
-                int adjoint_kind = 0);
+                int adjoint_kind = 0, const float* weight = nullptr);
 
   const std::string& error() const { return err_; }
   Params param;

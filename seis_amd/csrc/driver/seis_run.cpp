@@ -5,6 +5,7 @@
 //   seis_run <param file> rtm_born [image file]   RTM image of the Born records
 //   seis_run <param file> lsrtm [image file]      least-squares RTM (maxiter CG steps)
 //   seis_run <param file> lsrtm_xcorr [file]      the same with the cross-correlation adjoint
+//   seis_run <param file> lsrtm_illum [file]      the same preconditioned by the source illumination
 //   seis_run <param file> illum [image file]      source illumination (per shot <wdir>ILLUM<is>.dat, summed)
 //   seis_run <param file> plane_prepare           plane-wave sources / records <wdir>PLANE_S_NEW<k>.dat, PLANE_R_NEW<k>.dat
 //   seis_run <param file> rtm_plane [image file]  plane-wave image (per ray parameter <wdir>IMAGE<k>.dat, summed)
@@ -19,7 +20,7 @@
 
 int main(int argc, char** argv) {
   if (argc < 3) {
-    std::fprintf(stderr, "usage: %s <param file> <modeling|born|rtm|rtm_born|lsrtm|lsrtm_xcorr|illum|plane_prepare|rtm_plane|rtm_plane_born|illum_plane> [file]\n", argv[0]);
+    std::fprintf(stderr, "usage: %s <param file> <modeling|born|rtm|rtm_born|lsrtm|lsrtm_xcorr|lsrtm_illum|illum|plane_prepare|rtm_plane|rtm_plane_born|illum_plane> [file]\n", argv[0]);
     return 1;
   }
   const std::string mode = argv[2];

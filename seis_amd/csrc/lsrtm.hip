@@ -361,6 +361,7 @@ struct LsrtmPlan {
   double* part = nullptr;  // reduction scratch (RGRID partials + results)
   T* cg = nullptr;         // CG vectors: x, s, p (model) and r, q (records)
   const T* mask = nullptr; // optional record mask (caller-owned, device)
+  const T* weight = nullptr;  // optional image-space preconditioner W of CG (caller-owned, device)
   T* masked = nullptr;     // masked copy of an adjoint input
   unsigned* pmask = nullptr;  // band_terms of the ring cells P
   Consts<T> k{};
@@ -627,13 +628,22 @@ struct LsrtmPlan {
   // CGLS on min ||L x - d||^2 from x = 0 (with the record mask, if set, in
   // L).  hist[2*i] = ||r_i||^2, hist[2*i+1] = ||L^T r_i||^2 for i = 0..done;
   // stops after niter iterations or when ||L^T r|| <= rtol ||L^T d||.
+  // With a weight W set, CGLS runs on the operator L W (right
+  // preconditioning: x is the preconditioned image) and returns m = W x.
   int cgls(const T* d, int niter, double rtol, T* x_out, double* hist, int* done) {
-    if (!cg) LS_CHECK(hipMalloc(&cg, sizeof(T) * (size_t)(3 * model + 2 * nrec)));
+    if (!cg) LS_CHECK(hipMalloc(&cg, sizeof(T) * (size_t)(4 * model + 2 * nrec)));
     T *x = x_out, *s = cg, *p = cg + model, *r = cg + 2 * model, *q = cg + 2 * model + nrec;
+    T* wp = cg + 2 * model + 2 * nrec;  // W p
     const int G = 1024, B = 256;
+    // the operator's adjoint side: s = W (L^T r)
+    auto adjoint_w = [&](const T* rr, T* ss) {
+      int e = adjoint(rr, ss);
+      if (!e && weight) hipLaunchKernelGGL((mul_kernel<T>), dim3(G), dim3(B), 0, stream, ss, ss, weight, model);
+      return e;
+    };
     hipLaunchKernelGGL((fill_kernel<T>), dim3(G), dim3(B), 0, stream, x, T(0), model);
     LS_CHECK(hipMemcpyAsync(r, d, sizeof(T) * (size_t)nrec, hipMemcpyDeviceToDevice, stream));
-    int rc = adjoint(r, s);
+    int rc = adjoint_w(r, s);
     if (rc) return rc;
     LS_CHECK(hipMemcpyAsync(p, s, sizeof(T) * (size_t)model, hipMemcpyDeviceToDevice, stream));
     double gamma, rr;
@@ -646,14 +656,19 @@ struct LsrtmPlan {
     const double gamma0 = gamma;
     for (int it = 0; it < niter; it++) {
       if (!(gamma > 0) || (rtol > 0 && gamma <= rtol * rtol * gamma0)) break;
-      if ((rc = born(p, q))) return rc;
+      if (weight) {
+        hipLaunchKernelGGL((mul_kernel<T>), dim3(G), dim3(B), 0, stream, wp, p, weight, model);
+        if ((rc = born(wp, q))) return rc;
+      } else if ((rc = born(p, q))) {
+        return rc;
+      }
       double delta;
       if ((rc = dot(q, q, nrec, &delta))) return rc;
       if (!(delta > 0)) break;
       const double alpha = gamma / delta;
       hipLaunchKernelGGL((axpy_kernel<T>), dim3(G), dim3(B), 0, stream, x, p, alpha, model);
       hipLaunchKernelGGL((axpy_kernel<T>), dim3(G), dim3(B), 0, stream, r, q, -alpha, nrec);
-      if ((rc = adjoint(r, s))) return rc;
+      if ((rc = adjoint_w(r, s))) return rc;
       double gnew;
       if ((rc = dot(s, s, model, &gnew)) || (rc = dot(r, r, nrec, &rr))) return rc;
       const double beta = gnew / gamma;
@@ -665,6 +680,7 @@ struct LsrtmPlan {
       }
       if (done) *done = it + 1;
     }
+    if (weight) hipLaunchKernelGGL((mul_kernel<T>), dim3(G), dim3(B), 0, stream, x, x, weight, model);
     LS_CHECK(hipGetLastError());
     LS_CHECK(hipStreamSynchronize(stream));
     return SEIS_OK;
@@ -748,6 +764,15 @@ int seis_lsrtm_set_adjoint(seis_lsrtm* plan, int kind) {
     plan->d->adj_kind = kind;
   else
     plan->f->adj_kind = kind;
+  return SEIS_OK;
+}
+
+int seis_lsrtm_set_weight(seis_lsrtm* plan, const void* weight) {
+  if (!plan) return set_error("null plan"), SEIS_ERR_ARG;
+  if (plan->fp64)
+    plan->d->weight = static_cast<const double*>(weight);
+  else
+    plan->f->weight = static_cast<const float*>(weight);
   return SEIS_OK;
 }
 

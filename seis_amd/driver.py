@@ -17,7 +17,7 @@ def read_params(path):
 
 
 def run(param_file, mode, model_file=None, out_file=None):
-    """Runs one driver mode: modeling, born, rtm, rtm_born, lsrtm, lsrtm_xcorr, illum,
-    plane_prepare, rtm_plane, rtm_plane_born or illum_plane."""
+    """Runs one driver mode: modeling, born, rtm, rtm_born, lsrtm, lsrtm_xcorr, lsrtm_illum,
+    illum, plane_prepare, rtm_plane, rtm_plane_born or illum_plane."""
     enc = lambda s: None if s is None else str(s).encode()  # noqa: E731
     check(lib().seis_run(enc(param_file), enc(mode), enc(model_file), enc(out_file)))

@@ -82,6 +82,16 @@ class LsrtmPlan:
         reference's imaging condition on the snapshots) for adjoint() and cg()."""
         check(lib().seis_lsrtm_set_adjoint(self.handle, int(kind)))
 
+    def set_weight(self, weight):
+        """Image-space preconditioner W ([nx][nz]) of cg(): CGLS on L W, image
+        W x; None clears."""
+        if weight is None:
+            self._weight = None
+            check(lib().seis_lsrtm_set_weight(self.handle, None))
+            return
+        self._weight = self._dev(weight, self.model)
+        check(lib().seis_lsrtm_set_weight(self.handle, self._weight.ptr))
+
     def set_mask(self, mask):
         """Record mask (packed like the records) applied in L and L^T; None clears."""
         if mask is None:

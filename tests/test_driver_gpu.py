@@ -112,6 +112,10 @@ def test_modeling_rtm_lsrtm_files(engine, oracle, tmp_path):
     driver.run(param, "lsrtm_xcorr", out_file=tmp_path / "WDIR" / "LSRTM_XCORR.dat")
     lx = np.fromfile(tmp_path / "WDIR" / "LSRTM_XCORR.dat", np.float32).reshape(NX, NZ)
     assert np.isfinite(lx).all() and rel_l2(lx, ls) < 0.5
+    # illumination-preconditioned: a different, finite image
+    driver.run(param, "lsrtm_illum", out_file=tmp_path / "WDIR" / "LSRTM_ILLUM.dat")
+    li = np.fromfile(tmp_path / "WDIR" / "LSRTM_ILLUM.dat", np.float32).reshape(NX, NZ)
+    assert np.isfinite(li).all() and np.abs(li).max() > 0 and not np.array_equal(li, ls)
 
 
 def test_illum_files(engine, oracle, tmp_path):
