@@ -102,8 +102,8 @@ def stdout_to_stderr():
 def c1_single_shot(calls, with_cpu, threads):
     """The reference's CPU-runnable case: one-shot forward modeling of the
     3-layer model, 201 x 301 cells @ 10 m, pml 30, nt = 1500, through the
-    single-shot host-buffer entry point (fd.h modeling), with the time loop
-    replayed as a HIP graph and launched step by step."""
+    single-shot host-buffer entry point (fd.h modeling), its time loop
+    launched step by step."""
     import seis_amd
     nx, nz, pml, nt, dt, dx = 301, 201, 30, 1500, 0.001, 10.0
     vel = np.empty((nx, nz), np.float32)

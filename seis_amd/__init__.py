@@ -8,7 +8,8 @@ array layouts, plus batched, device-resident variants.  There is no CPU
 fallback: if the library is missing or no GPU is present, calls raise.
 """
 from ._lib import SeisError, lib, library_path  # noqa: F401
-from .device import DeviceArray, device_count, profile_start, profile_stop, synchronize  # noqa: F401
+from .device import (DeviceArray, device_count, profile_start, profile_stop, set_tuning, synchronize,  # noqa: F401
+                     tuned_sweep)
 from .fd import adjoint, adjoint_plane, forward, forward_plane, illum, illum_plane, modeling  # noqa: F401
 from .batch import Grid, Shot, born_batch, illum_batch, modeling_batch, rtm_batch  # noqa: F401
 from .plane import harmonic, plane_prepare  # noqa: F401

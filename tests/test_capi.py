@@ -91,3 +91,15 @@ def test_rtm_column_limit_without_gpu():
     shots = (seis_shot * 1)(seis_shot(2, 2, 10, 0, 0, 0, 0, 0))
     assert L.seis_rtm_batch(ctypes.byref(g), 1, shots, None, None, None, None, None) == 1
     assert "too large" in L.seis_last_error().decode()
+
+
+def test_tuning_modes_without_gpu():
+    """seis_set_tuning takes modes 0, 1 and 2 only; the choice table is host
+    state, so this needs no GPU."""
+    import seis_amd
+    from seis_amd._lib import SeisError
+    with pytest.raises(SeisError):
+        seis_amd.set_tuning(3)
+    assert "mode" in seis_amd.lib().seis_last_error().decode()
+    for m in (0, 2, 1):
+        seis_amd.set_tuning(m)
