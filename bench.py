@@ -77,7 +77,20 @@ def cpu_baseline(w, nt_sample, threads):
     cells = (nx + 2 * pml) * (nz + 2 * pml)
     updates = 3.0 * nt_sample * cells
     return dict(value=updates / el / 1e9, unit="Gcell-updates/s", cores=threads, kind=kind,
-                sample="one shot of the same RTM workload with nt=%d (%.1f s of CPU work)" % (nt_sample, el))
+                sample="one shot of the same RTM workload with nt=%d (%.1f s of CPU work)" % (nt_sample, el),
+                cpu_model=host_cpu_model())
+
+
+def host_cpu_model():
+    """Model name of the host CPU (first processor of /proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def log(msg):
