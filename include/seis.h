@@ -287,23 +287,18 @@ int seis_lsrtm_set_weight(seis_lsrtm* plan, const void* weight);
 int seis_lsrtm_cg(seis_lsrtm* plan, const void* dobs, int niter, double rtol, void* image, double* hist,
                   int* iters);
 
-/* Sweep autotuning of the step launches (no counterpart in the reference).
- * mode 1 (the default): the first run of a configuration (kind of pass,
- * padded grid, shots per launch, device) times a few trial launches of each
- * candidate on its zeroed fields and keeps the fastest for the process.  A
- * candidate is a strip length of the interior sweep (8, 16, 32 columns; 1, 2,
- * 4, 8, 16 when the padded grid times the shots is under 4 Mi cells) and, on
- * such small grids, also one of 1, 2, 4, 8 with the band cells finished in
- * the step launch itself (each recomputes its neighbours' stencil values;
- * one launch per step instead of two).  Results are identical either way.
- * mode 0: no tuning, strip length 16, band in its own launch.  mode 2: no
- * tuning, strip length 16, band in the step launch (the exact LSRTM adjoint
- * keeps its own band launch).  Modes 0 and 2 forget the table.
- * seis_tuned_strip reports the choice for a configuration: the strip length,
- * plus 256 when the band is in the step launch (0 when none was made).
- * kind: "forward", "born", "rtm", "illum", "rtm_plane", "lsrtm_forward",
- * "lsrtm_adjoint".  SEIS_ERR_ARG for another mode. */
-int seis_set_tuning(int mode);
+/* Strip-length autotuning of the interior sweeps (no counterpart in the
+ * reference).  On by default: the first run of a configuration (kind of
+ * pass, padded grid, shots per launch, device) times a few trial launches of
+ * each candidate strip length (8, 16, 32 columns; 1, 2, 4, 8, 16 when the
+ * padded grid times the shots is under 4 Mi cells) on its zeroed fields and
+ * keeps the fastest for the process.  Results are identical for every strip
+ * length.  seis_set_tuning(0) turns it off (strip length 16) and forgets the
+ * table; another value than 0 or 1 is SEIS_ERR_ARG.  seis_tuned_strip
+ * reports the choice for a configuration (0 when none was made).  kind:
+ * "forward", "born", "rtm", "illum", "rtm_plane", "lsrtm_forward",
+ * "lsrtm_adjoint". */
+int seis_set_tuning(int on);
 int seis_tuned_strip(const char* kind, int nx_padded, int nz_padded, int nshots, int* bx);
 
 /* Kernel timing: instrumentation with no counterpart in the reference.

@@ -9,7 +9,7 @@ fallback: if the library is missing or no GPU is present, calls raise.
 """
 from ._lib import SeisError, lib, library_path  # noqa: F401
 from .device import (DeviceArray, device_count, profile_start, profile_stop, set_tuning, synchronize,  # noqa: F401
-                     tuned_sweep)
+                     tuned_strip)
 from .fd import adjoint, adjoint_plane, forward, forward_plane, illum, illum_plane, modeling  # noqa: F401
 from .batch import Grid, Shot, born_batch, illum_batch, modeling_batch, rtm_batch  # noqa: F401
 from .plane import harmonic, plane_prepare  # noqa: F401

@@ -223,7 +223,6 @@ struct Layout {
   int nfwg;           // workgroups per shot for the interior sweep
   long long nband;    // band cells per shot
   int nbwg;           // workgroups per shot for the band cells
-  int onepass;        // band cells finished in the step launch (neighbours' stencils recomputed)
 };
 
 template <typename T>
@@ -285,18 +284,10 @@ constexpr int NXCD = 8;
 // which read 8 columns in common, meet in the same L2.
 __device__ __forceinline__ int xcd_tile(int b, int nfwg) { return (b % NXCD) * (nfwg / NXCD) + b / NXCD; }
 
-// A sweep choice is a strip length (1..255), plus SWEEP_ONEPASS when the
-// band cells are finished in the step launch itself: each band cell then
-// recomputes its neighbours' stencil values instead of reading them after a
-// second launch.  That costs more per band cell and saves one launch per
-// step, which pays on small grids, where a step is mostly launch latency.
-constexpr int SWEEP_ONEPASS = 1 << 8;
-
-// Sweep choice of the interior sweep: the derived counts (the sweep's
+// Strip length bx of the interior sweep: the derived counts (the sweep's
 // workgroup count is padded to a multiple of NXCD; the padding blocks exit).
-inline Layout with_strip(Layout L, int sweep) {
-  L.bx = sweep % SWEEP_ONEPASS;
-  L.onepass = (sweep & SWEEP_ONEPASS) ? 1 : 0;
+inline Layout with_strip(Layout L, int bx) {
+  L.bx = bx;
   L.nzc = (L.zb - L.za + 63) / 64;
   L.nxs = (L.xb - L.xa + L.bx - 1) / L.bx;
   if (L.xb == L.xa) L.nzc = L.nxs = 0;
@@ -321,15 +312,12 @@ inline Layout make_layout(int NX, int NZ, int pml, int bx = 16) {
   return with_strip(L, bx);
 }
 
-// Sweep autotuning (tune.cpp).  `trial(sweep)` enqueues a few launches with
-// that sweep choice on `stream`; the first time a configuration is seen each
-// candidate is timed and the fastest kept for the process.  One-pass band
-// candidates are offered only when `onepass_ok` (the launches go through
-// launch_step).  With tuning off (seis_set_tuning) the default strip length
-// is used, with the band in its own launch (mode 0) or in the step launch
-// (mode 2, when onepass_ok).
+// Strip-length autotuning (tune.cpp).  `trial(bx)` enqueues a few launches
+// with strip length bx on `stream`; the first time a configuration `key` is
+// seen each candidate is timed and the fastest kept for the process.  With
+// tuning off (seis_set_tuning) the default strip length is used.
 int tuned_strip(const char* kind, int NX, int NZ, int nshots, hipStream_t stream, void (*trial)(void*, int),
-                void* ctx, bool onepass_ok);
+                void* ctx);
 
 // Enumeration of the cells outside an inner rectangle [xa, xb) x [za, zb) of
 // the padded grid: the rows above and below it over the full width, then the

@@ -604,7 +604,7 @@ __global__ __launch_bounds__(WG) void step_kernel(Consts<T> k, const DevShot<T>*
   const int NX = s.NX, NZ = k.NZ;
   int x, z;
   ring_cell(Ring{L.xa, L.xb, L.za, L.zb, L.nband}, NX, NZ, j, &x, &z);
-  if (!L.onepass && edge_dist(x, z, NX, NZ) < k.pml)
+  if (edge_dist(x, z, NX, NZ) < k.pml)
     band_raw<T, MODE>(k, s, c, x, z);
   else
     cell_update<T, MODE>(k, s, c, x, z);
@@ -763,8 +763,7 @@ void launch_step(int mode, const Consts<T>& k, const DevShot<T>* dev, const Layo
 #define SEIS_CASE(M)                                                                            \
   case M:                                                                                       \
     hipLaunchKernelGGL((step_kernel<T, M>), grid, block, 0, stream, k, dev, L, step);           \
-    if (B.n > 0 && !L.onepass)                                                                  \
-      hipLaunchKernelGGL((band_kernel<T, M>), bgrid, block, 0, stream, k, dev, B, step);        \
+    if (B.n > 0) hipLaunchKernelGGL((band_kernel<T, M>), bgrid, block, 0, stream, k, dev, B, step); \
     break;
     SEIS_CASE(MODEL)
     SEIS_CASE(BORN)
@@ -938,7 +937,7 @@ int run_batch(int mode, const GridSpec& g, int nshots, const ShotSpec* shots, co
       launch_step<T>(t->mode, *t->k, t->dev, with_strip(t->L, bx), t->B, t->nshots, 0, t->s);
     };
     const char* kind = mode == RUN_MODEL ? "forward" : mode == RUN_BORN ? "born" : ilm ? "illum" : "rtm";
-    L = with_strip(L, tuned_strip(kind, NX, NZ, nshots, stream, fn, &tr, true));
+    L = with_strip(L, tuned_strip(kind, NX, NZ, nshots, stream, fn, &tr));
   }
   const double cells = (double)NX * NZ;
   const double E = sizeof(T);
@@ -1126,7 +1125,7 @@ int run_rtm_plane(const GridSpec& g, int nshots, const ShotSpec* shots, const T*
       const Trial* t = static_cast<const Trial*>(c);
       launch_step<T>(PW_BACK, *t->k, t->dev, with_strip(t->L, bx), t->B, t->nshots, 0, t->s);
     };
-    L = with_strip(L, tuned_strip("rtm_plane", NX, NZ, nshots, stream, fn, &tr, true));
+    L = with_strip(L, tuned_strip("rtm_plane", NX, NZ, nshots, stream, fn, &tr));
   }
   const double cells = (double)NX * NZ;
   const double E = sizeof(T);

@@ -479,7 +479,7 @@ struct LsrtmPlan {
         LsrtmPlan* p = static_cast<LsrtmPlan*>(c);
         launch_step<T>(SNAP_MAX, p->k, p->dev, with_strip(p->F, bx), p->B, p->nshots, 0, p->stream);
       };
-      F = with_strip(F, tuned_strip("lsrtm_forward", NX, NZ, nshots, stream, fn, this, true));
+      F = with_strip(F, tuned_strip("lsrtm_forward", NX, NZ, nshots, stream, fn, this));
     }
     return make_snapshots();
   }
@@ -592,7 +592,7 @@ struct LsrtmPlan {
         hipLaunchKernelGGL((adj_step_kernel<T>), dim3(a.nfwg + a.nbwg, p->nshots), dim3(AWG), 0, p->stream, p->k,
                            p->dev, a, 0);
       };
-      A = with_strip(A, tuned_strip("lsrtm_adjoint", NX, NZ, nshots, stream, fn, this, false));
+      A = with_strip(A, tuned_strip("lsrtm_adjoint", NX, NZ, nshots, stream, fn, this));
       adj_tuned = true;
     }
     int rc = zero_fields(0, LS_NSLOT);

@@ -1,6 +1,4 @@
-// Sweep autotuner of the step launches (engine.h: tuned_strip): strip length
-// of the interior sweep, and whether the band cells are finished in the same
-// launch.
+// Strip-length autotuner of the interior sweeps (engine.h: tuned_strip).
 #include <hip/hip_runtime.h>
 
 #include <map>
@@ -20,38 +18,33 @@ constexpr int kCandidates[] = {8, 16, 32};
 // Small launches (fewer padded cells over all shots than kSmallCells): at a
 // strip of 8 columns they hold too few wavefronts to fill 256 CUs, and each
 // wavefront's serial sweep of its strip sets the kernel's duration, so
-// shorter strips are tried as well, and the band finished in the step launch.
+// shorter strips are tried as well.
 constexpr long long kSmallCells = 4ll << 20;
 constexpr int kSmallCandidates[] = {1, 2, 4, 8, 16};
-constexpr int kOnepassCandidates[] = {1, 2, 4, 8};
 constexpr int kReps = 3;
 
 std::mutex g_mu;
-int g_mode = 1;  // 0 fixed default, 1 tuned, 2 fixed default with the band in the step launch
-std::map<std::tuple<std::string, int, int, int, int>, int> g_table;  // (kind, device, NX, NZ, shots) -> sweep
+bool g_on = true;
+std::map<std::tuple<std::string, int, int, int, int>, int> g_table;  // (kind, device, NX, NZ, shots) -> bx
 
 }  // namespace
 
 int tuned_strip(const char* kind, int NX, int NZ, int nshots, hipStream_t stream, void (*trial)(void*, int),
-                void* ctx, bool onepass_ok) {
+                void* ctx) {
   int dev = 0;
   (void)hipGetDevice(&dev);
   const auto key = std::make_tuple(std::string(kind), dev, NX, NZ, nshots);
   {
     std::lock_guard<std::mutex> lk(g_mu);
-    if (g_mode == 0) return kDefaultStrip;
-    if (g_mode == 2) return onepass_ok ? kDefaultStrip | SWEEP_ONEPASS : kDefaultStrip;
+    if (!g_on) return kDefaultStrip;
     auto it = g_table.find(key);
     if (it != g_table.end()) return it->second;
   }
   std::vector<int> cand;
-  if ((long long)NX * NZ * nshots < kSmallCells) {
+  if ((long long)NX * NZ * nshots < kSmallCells)
     cand.assign(std::begin(kSmallCandidates), std::end(kSmallCandidates));
-    if (onepass_ok)
-      for (int bx : kOnepassCandidates) cand.push_back(bx | SWEEP_ONEPASS);
-  } else {
+  else
     cand.assign(std::begin(kCandidates), std::end(kCandidates));
-  }
   hipEvent_t a, b;
   if (hipEventCreate(&a) != hipSuccess) return kDefaultStrip;
   if (hipEventCreate(&b) != hipSuccess) {
@@ -61,10 +54,10 @@ int tuned_strip(const char* kind, int NX, int NZ, int nshots, hipStream_t stream
   int best = kDefaultStrip;
   float best_ms = 0.f;
   for (size_t ci = 0; ci < cand.size(); ci++) {
-    const int sweep = cand[ci];
-    trial(ctx, sweep);  // warm-up launch of this shape
+    const int bx = cand[ci];
+    trial(ctx, bx);  // warm-up launch of this shape
     (void)hipEventRecord(a, stream);
-    for (int r = 0; r < kReps; r++) trial(ctx, sweep);
+    for (int r = 0; r < kReps; r++) trial(ctx, bx);
     (void)hipEventRecord(b, stream);
     float ms = 0.f;
     if (hipEventSynchronize(b) != hipSuccess || hipEventElapsedTime(&ms, a, b) != hipSuccess) {
@@ -72,7 +65,7 @@ int tuned_strip(const char* kind, int NX, int NZ, int nshots, hipStream_t stream
       break;
     }
     if (ci == 0 || ms < best_ms) {
-      best = sweep;
+      best = bx;
       best_ms = ms;
     }
   }
@@ -87,11 +80,11 @@ int tuned_strip(const char* kind, int NX, int NZ, int nshots, hipStream_t stream
 
 using namespace seis;
 
-extern "C" int seis_set_tuning(int mode) {
-  if (mode < 0 || mode > 2) return set_error("seis_set_tuning: mode must be 0, 1 or 2"), SEIS_ERR_ARG;
+extern "C" int seis_set_tuning(int on) {
+  if (on != 0 && on != 1) return set_error("seis_set_tuning: on must be 0 or 1"), SEIS_ERR_ARG;
   std::lock_guard<std::mutex> lk(g_mu);
-  g_mode = mode;
-  if (g_mode != 1) g_table.clear();
+  g_on = on != 0;
+  if (!g_on) g_table.clear();
   return SEIS_OK;
 }
 

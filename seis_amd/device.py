@@ -16,19 +16,18 @@ def synchronize():
     check(lib().seis_synchronize())
 
 
-def set_tuning(mode):
-    """Sweep autotuning of the step launches: 1 tuned (default), 0 fixed
-    strip of 16 with the band in its own launch, 2 fixed strip of 16 with the
-    band finished in the step launch.  Results are identical in every mode."""
-    check(lib().seis_set_tuning(int(mode)))
+def set_tuning(on):
+    """Strip-length autotuning of the interior sweeps: on (the default) or
+    off (strip length 16).  Results are identical either way."""
+    check(lib().seis_set_tuning(int(on)))
 
 
-def tuned_sweep(kind, nx_padded, nz_padded, nshots):
-    """The tuner's choice for a configuration: (strip length, band in the step
-    launch), or None when none was made."""
+def tuned_strip(kind, nx_padded, nz_padded, nshots):
+    """The tuner's strip length for a configuration, or None when none was
+    made."""
     v = ctypes.c_int(0)
     check(lib().seis_tuned_strip(kind.encode(), int(nx_padded), int(nz_padded), int(nshots), ctypes.byref(v)))
-    return None if v.value == 0 else (v.value % 256, v.value >= 256)
+    return v.value or None
 
 
 def profile_start():

@@ -93,13 +93,13 @@ def test_rtm_column_limit_without_gpu():
     assert "too large" in L.seis_last_error().decode()
 
 
-def test_tuning_modes_without_gpu():
-    """seis_set_tuning takes modes 0, 1 and 2 only; the choice table is host
-    state, so this needs no GPU."""
+def test_tuning_switch_without_gpu():
+    """seis_set_tuning takes 0 or 1 only; the choice table is host state, so
+    this needs no GPU."""
     import seis_amd
     from seis_amd._lib import SeisError
     with pytest.raises(SeisError):
-        seis_amd.set_tuning(3)
-    assert "mode" in seis_amd.lib().seis_last_error().decode()
-    for m in (0, 2, 1):
+        seis_amd.set_tuning(2)
+    assert "0 or 1" in seis_amd.lib().seis_last_error().decode()
+    for m in (0, 1):
         seis_amd.set_tuning(m)
