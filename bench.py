@@ -34,7 +34,7 @@ HBM_PEAK_GBS = 8000.0
 # rocprofv3 FETCH_SIZE / WRITE_SIZE (calibrated against a 1 GiB copy; the
 # commands and counter values are in profiles/rtm_batch_hbm_traffic.md).
 # Scaled by the number of shots in a launch.
-MEASURED_TRAFFIC_96 = {"step_rtm_forward": 4.615e+09, "step_rtm_reverse": 9.107e+09, "rtm_image": 4.291e+09}
+MEASURED_TRAFFIC_96 = {"step_rtm_forward": 4.474e+09, "step_rtm_reverse": 8.846e+09, "rtm_image": 4.291e+09}
 
 
 def rtm_workload(nshots, nt=5000):
