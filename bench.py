@@ -132,6 +132,7 @@ def c1_single_shot(calls, with_cpu, threads):
     for _ in range(calls):
         seis_amd.modeling(*args)
     out["ms_per_call"] = round((time.perf_counter() - t0) * 1e3 / calls, 3)
+    out["strip_length"] = seis_amd.tuned_strip("forward", nx + 2 * pml, nz + 2 * pml, 1)
     cells = (nx + 2 * pml) * (nz + 2 * pml) * nt
     out["gcell_updates_per_s"] = round(cells / (out["ms_per_call"] * 1e-3) / 1e9, 2)
     if with_cpu:
@@ -738,7 +739,8 @@ def main():
         "config": {"workload": "rtm_batch: %d shots, 751x2301 cells @ 4 m, pml 30, nt=%d, dt=0.4 ms, "
                                "Ricker 12 Hz, 480-channel split spread @ 12 m, reference imaging condition" % (ns, nt),
                    "shots": ns, "grid": [nz, nx], "padded_cells": cells, "nt": nt,
-                   "propagations_per_shot": 3, "source_band": "isolated (SEIS_RTM_ISOLATE_BAND)"},
+                   "propagations_per_shot": 3, "source_band": "isolated (SEIS_RTM_ISOLATE_BAND)",
+                   "strip_length": seis_amd.tuned_strip("rtm", nx + 2 * w["pml"], nz + 2 * w["pml"], ns)},
         "roofline": {"bound": "hbm", "kernel": dom[0], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": (MEASURED_TRAFFIC_96[dom[0]] * ns / 96 if dom[0] in MEASURED_TRAFFIC_96 else None),
