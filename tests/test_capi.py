@@ -103,3 +103,17 @@ def test_tuning_switch_without_gpu():
     assert "0 or 1" in seis_amd.lib().seis_last_error().decode()
     for m in (0, 1):
         seis_amd.set_tuning(m)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU's share; asked for more GPUs it exits with a
+    message before touching any device."""
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120, cwd=root)
+    assert r.returncode != 0
+    assert "one GPU" in r.stderr + r.stdout
+    assert r.stdout.strip() == ""  # no JSON line
