@@ -39,13 +39,22 @@ typedef struct seis_grid {
 
 /* One shot of a batch.  sx, sz: source cell (fd.h:3 sx, sz); nt: record
  * length; ndel: pre-roll steps before recording starts (fd.h:3 ndel).
+ * nx: columns of this shot's own grid, its model window (the per-shot
+ * window of getVel, inversion.cpp:552-598); 0 means the grid's nx.  Shots
+ * of different widths share one launch; all share the grid's nz, pml, dt,
+ * dx, dz.  Below, nx is the shot's width.
  * model / wav / gz / rec: element offsets of this shot's [nx][nz] model
  * (and perturbation), wavelet (nt + ndel samples), receiver-depth row (nx
  * entries, one receiver per column as in fd.cpp:58-63) and [nx][nt] record
- * inside the batch arrays. */
+ * inside the batch arrays.
+ * x0: LSRTM plans only (seis_lsrtm_create): the shot's window is columns
+ * [x0, x0 + nx) of the plan's image grid (grid->nx columns); its
+ * perturbation is read from and its gradient added to those columns.  The
+ * other calls ignore it. */
 typedef struct seis_shot {
   int sx, sz, nt, ndel;
   long long model, wav, gz, rec;
+  int nx, x0;
 } seis_shot;
 
 /* Message of the calling thread's last error ("" if none). */
@@ -168,21 +177,44 @@ int seis_modeling_batch(const seis_grid* grid, int nshots, const seis_shot* shot
                         const float* wav, const int* gz, float* rec);
 int seis_born_batch(const seis_grid* grid, int nshots, const seis_shot* shots, const float* vel,
                     const float* dv, const float* wav, const int* gz, float* rec);
-/* img: nshots consecutive [nx][nz] images, one per shot (the per-shot
- * CSG_IMAGE<is> of inversion.cpp:434-439). */
+/* img: nshots consecutive [nx][nz] images, one per shot in shot order, each
+ * of its shot's width (the per-shot CSG_IMAGE<is> of inversion.cpp:434-439). */
 int seis_rtm_batch(const seis_grid* grid, int nshots, const seis_shot* shots, const float* vel,
                    const float* wav, const int* gz, const float* rec, float* img);
 
-/* ss: nshots consecutive [nx][nz] illuminations, one per shot (the
+/* ss: nshots consecutive [nx][nz] illuminations, one per shot of its width (the
  * per-shot ILLUM<is> of illum_MPI, inversion.cpp:207-290, which sums them);
  * the shots' gz and rec offsets are not used. */
 int seis_illum_batch(const seis_grid* grid, int nshots, const seis_shot* shots, const float* vel,
                      const float* wav, float* ss);
 
+/* Batch memory.  Every shot of a launch holds its wavefields (and, for RTM,
+ * its boundary strips: nt * 8 * (nx + nz) floats) for the whole run, so each
+ * batch call cuts its shot list into consecutive chunks whose working set
+ * fits a budget and runs them one after another; every shot's output is the
+ * same for any cut.  The budget is the free device memory less a reserve of
+ * max(1 GiB, 1/32 of it), capped by seis_set_batch_limits(bytes, shots):
+ * bytes > 0 caps the working set of a chunk, shots > 0 the shots per launch
+ * (0, 0 restores the defaults).  A shot larger than the budget is
+ * SEIS_ERR_ARG.  The reference images one shot per MPI rank at a time
+ * (masterRun, inversion.cpp:753-807); no counterpart of these two calls.
+ * seis_batch_plan computes the cut a batch call would make for `budget`
+ * bytes and `max_shots` (0: no cap), without a GPU: chunk c holds shots
+ * [starts[c], starts[c+1]) for c < *nchunks (starts: nshots + 1 ints). */
+#define SEIS_PLAN_MODEL 0      /* seis_modeling_batch */
+#define SEIS_PLAN_BORN 1       /* seis_born_batch, seis_born_plane_batch */
+#define SEIS_PLAN_RTM 2        /* seis_rtm_batch(_ex) */
+#define SEIS_PLAN_ILLUM 3      /* seis_illum_batch, seis_illum_plane_batch */
+#define SEIS_PLAN_RTM_PLANE 4  /* seis_rtm_plane_batch */
+int seis_set_batch_limits(size_t bytes, int shots);
+int seis_batch_plan(int mode, const seis_grid* grid, int nshots, const seis_shot* shots, size_t budget,
+                    int max_shots, int* starts, int* nchunks);
+
 /* Plane-wave batches: shots[].wav is the offset of each shot's sou
  * [nx][nt+ndel], shots[].sz its source row, shots[].gz the offset of its
  * receiver depths gz[nx] (one row for the reference's plane-wave functions);
- * shots[].sx is not used.  Outputs per shot as in the single-shot calls. */
+ * shots[].sx is not used, and shots[].nx must be 0 or the grid's nx (a plane
+ * wave spans the model).  Outputs per shot as in the single-shot calls. */
 int seis_born_plane_batch(const seis_grid* grid, int nshots, const seis_shot* shots, const float* vel,
                           const float* dv, const float* sou, const int* gz, float* rec);
 int seis_illum_plane_batch(const seis_grid* grid, int nshots, const seis_shot* shots, const float* vel,

@@ -21,7 +21,8 @@ class seis_grid(ctypes.Structure):
 
 class seis_shot(ctypes.Structure):
     _fields_ = [("sx", _I), ("sz", _I), ("nt", _I), ("ndel", _I), ("model", ctypes.c_longlong),
-                ("wav", ctypes.c_longlong), ("gz", ctypes.c_longlong), ("rec", ctypes.c_longlong)]
+                ("wav", ctypes.c_longlong), ("gz", ctypes.c_longlong), ("rec", ctypes.c_longlong),
+                ("nx", _I), ("x0", _I)]
 
 
 class seis_kernel_stat(ctypes.Structure):
@@ -77,6 +78,9 @@ _SIGS = {
     "seis_fd_free": (None, [_P]),
     "seis_run": (_I, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p]),
     "seis_read_params": (_I, [ctypes.c_char_p, ctypes.c_char_p, _I]),
+    "seis_set_batch_limits": (_I, [_SZ, _I]),
+    "seis_batch_plan": (_I, [_I, ctypes.POINTER(seis_grid), _I, ctypes.POINTER(seis_shot), _SZ, _I,
+                             ctypes.POINTER(_I), ctypes.POINTER(_I)]),
     "seis_profile_start": (_I, []),
     "seis_profile_stop": (_I, [ctypes.POINTER(seis_kernel_stat), _I, ctypes.POINTER(_I)]),
 }

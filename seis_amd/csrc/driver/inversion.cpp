@@ -66,42 +66,63 @@ bool Inversion::read_model(const std::string& file, std::vector<float>& v) {
 
 namespace {
 
-// Runs one of the engine's batched calls for shots grouped by window width.
-// `kind`: 0 modeling, 1 Born, 2 RTM, 3 illumination.  models / perts are
-// global [nx][nz]; records are per shot [nxw][nt] on input (RTM) or output
-// (modeling, Born); images per shot [nxw][nz] (RTM, illumination).
+// Shots staged on the host per engine call: their windowed models and
+// records are copied in and out per group, so host memory stays bounded on
+// survey-scale runs (the engine cuts each call further by device memory).
+constexpr size_t kStageBytes = size_t(4) << 30;
+
+// Runs one of the engine's batched calls over the shots' own windows (shots
+// of every width share the launches).  `kind`: 0 modeling, 1 Born, 2 RTM,
+// 3 illumination.  models / perts are global [nx][nz]; records are per shot
+// [nxw][nt] on input (RTM) or output (modeling, Born); images per shot
+// [nxw][nz] (RTM, illumination).
 bool run_windows(const Params& p, int kind, std::vector<ShotJob>& jobs, const float* model, const float* pert,
                  std::map<int, std::vector<float>>& recs, std::map<int, std::vector<float>>& imgs, std::string& err) {
   const int nz = p.nz.val, nt = p.nt.val, ndel = p.delaycal.val;
   const int NT = nt + ndel;
   std::vector<float> wav(NT);
   ricker_host(p.dt.val, p.fr.val, p.delay.val + ndel, NT, wav.data());
-  std::map<int, std::vector<int>> groups;
-  for (size_t j = 0; j < jobs.size(); j++) groups[jobs[j].w.nx].push_back((int)j);
-  for (auto& gr : groups) {
-    const int nxw = gr.first;
-    const std::vector<int>& idx = gr.second;
-    const int n = (int)idx.size();
-    const size_t M = (size_t)nxw * nz, R = (size_t)nxw * nt;
-    std::vector<float> vm(M * n), dm(kind == 1 ? M * n : 0), wv((size_t)NT * n), rc(R * n, 0.f);
-    std::vector<int> gz((size_t)nxw * n);
+  size_t j0 = 0;
+  while (j0 < jobs.size()) {
+    // next group: as many shots as fit the staging bound (at least one)
+    size_t j1 = j0, stage = 0;
+    while (j1 < jobs.size()) {
+      const size_t nxw = (size_t)jobs[j1].w.nx;
+      const size_t b = sizeof(float) * nxw * ((kind == 1 ? 2 : 1) * (size_t)nz + nt + (kind >= 2 ? nz : 0));
+      if (j1 > j0 && stage + b > kStageBytes) break;
+      stage += b;
+      j1++;
+    }
+    const int n = (int)(j1 - j0);
+    size_t M = 0, R = 0;
+    std::vector<size_t> moff(n), roff(n), goff(n);
+    for (int j = 0; j < n; j++) {
+      const size_t nxw = (size_t)jobs[j0 + j].w.nx;
+      moff[j] = M;
+      roff[j] = R;
+      goff[j] = M / nz;  // gz rows: one entry per column, like the models
+      M += nxw * nz;
+      R += nxw * nt;
+    }
+    std::vector<float> vm(M), dm(kind == 1 ? M : 0), wv((size_t)NT * n), rc(R, 0.f);
+    std::vector<int> gz(M / nz);
     std::vector<seis_shot> sh(n);
     for (int j = 0; j < n; j++) {
-      const ShotJob& q = jobs[idx[j]];
-      window_model(q.w, nz, model, vm.data() + M * j);
-      if (kind == 1) window_model(q.w, nz, pert, dm.data() + M * j);
+      const ShotJob& q = jobs[j0 + j];
+      window_model(q.w, nz, model, vm.data() + moff[j]);
+      if (kind == 1) window_model(q.w, nz, pert, dm.data() + moff[j]);
       std::copy(wav.begin(), wav.end(), wv.begin() + (size_t)NT * j);
-      std::copy(q.igz.begin(), q.igz.end(), gz.begin() + (size_t)nxw * j);
-      if (kind == 2) std::copy(recs[q.is].begin(), recs[q.is].end(), rc.begin() + R * j);
-      sh[j] = seis_shot{q.sx, q.sz, nt, ndel, (long long)(M * j), (long long)NT * j, (long long)nxw * j,
-                        (long long)(R * j)};
+      std::copy(q.igz.begin(), q.igz.end(), gz.begin() + goff[j]);
+      if (kind == 2) std::copy(recs[q.is].begin(), recs[q.is].end(), rc.begin() + roff[j]);
+      sh[j] = seis_shot{q.sx, q.sz, nt, ndel, (long long)moff[j], (long long)NT * j, (long long)goff[j],
+                        (long long)roff[j], q.w.nx, 0};
     }
-    seis_grid g{nxw, nz, p.npml.val, p.dt.val, p.dx.val, p.dz.val};
+    seis_grid g{p.nx.val, nz, p.npml.val, p.dt.val, p.dx.val, p.dz.val};
     Dev dv, dd, dw, dg, dr, di;
     if (!dv.put(vm.data(), sizeof(float) * vm.size()) || !dw.put(wv.data(), sizeof(float) * wv.size()) ||
         !dg.put(gz.data(), sizeof(int) * gz.size()) || !dr.put(rc.data(), sizeof(float) * rc.size()) ||
         (kind == 1 && !dd.put(dm.data(), sizeof(float) * dm.size())) ||
-        (kind >= 2 && !di.alloc(sizeof(float) * M * n))) {
+        (kind >= 2 && !di.alloc(sizeof(float) * M))) {
       err = gpu_error("device buffers");
       return false;
     }
@@ -119,13 +140,40 @@ bool run_windows(const Params& p, int kind, std::vector<ShotJob>& jobs, const fl
       return false;
     }
     if (kind >= 2) {
-      std::vector<float> im(M * n);
+      std::vector<float> im(M);
       if (!di.get(im.data(), sizeof(float) * im.size())) return err = gpu_error("download"), false;
-      for (int j = 0; j < n; j++) imgs[jobs[idx[j]].is].assign(im.begin() + M * j, im.begin() + M * (j + 1));
+      for (int j = 0; j < n; j++)
+        imgs[jobs[j0 + j].is].assign(im.begin() + moff[j], im.begin() + moff[j] + (size_t)jobs[j0 + j].w.nx * nz);
     } else {
       if (!dr.get(rc.data(), sizeof(float) * rc.size())) return err = gpu_error("download"), false;
-      for (int j = 0; j < n; j++) recs[jobs[idx[j]].is].assign(rc.begin() + R * j, rc.begin() + R * (j + 1));
+      for (int j = 0; j < n; j++)
+        recs[jobs[j0 + j].is].assign(rc.begin() + roff[j], rc.begin() + roff[j] + (size_t)jobs[j0 + j].w.nx * nt);
     }
+    if (kind == 2)
+      for (int j = 0; j < n; j++) recs.erase(jobs[j0 + j].is);  // records imaged: release them
+    j0 = j1;
+  }
+  return true;
+}
+
+// Calls fn(group) for consecutive groups of jobs whose host staging fits
+// kStageBytes (at least one shot each), so that records, windowed models and
+// images are read, run and written group by group.
+template <class F>
+bool for_stage_groups(const Params& p, int kind, std::vector<ShotJob>& jobs, F&& fn) {
+  const size_t nz = (size_t)p.nz.val, nt = (size_t)p.nt.val;
+  size_t j0 = 0;
+  while (j0 < jobs.size()) {
+    size_t j1 = j0, stage = 0;
+    while (j1 < jobs.size()) {
+      const size_t b = sizeof(float) * (size_t)jobs[j1].w.nx * ((kind == 1 ? 2 : 1) * nz + nt + (kind >= 2 ? nz : 0));
+      if (j1 > j0 && stage + b > kStageBytes) break;
+      stage += b;
+      j1++;
+    }
+    std::vector<ShotJob> group(jobs.begin() + j0, jobs.begin() + j1);
+    if (!fn(group)) return false;
+    j0 = j1;
   }
   return true;
 }
@@ -160,9 +208,11 @@ bool Inversion::modeling(const float* v) {
   std::vector<ShotJob> jobs(survey.ns());
   for (int is = 0; is < survey.ns(); is++)
     if (!make_job(param, survey, is, jobs[is], err_)) return false;
-  std::map<int, std::vector<float>> recs, imgs;
-  if (!run_windows(param, 0, jobs, v, nullptr, recs, imgs, err_)) return false;
-  return write_gathers(param, survey, jobs, recs, v, false, err_);
+  return for_stage_groups(param, 0, jobs, [&](std::vector<ShotJob>& group) {
+    std::map<int, std::vector<float>> recs, imgs;
+    return run_windows(param, 0, group, v, nullptr, recs, imgs, err_) &&
+           write_gathers(param, survey, group, recs, v, false, err_);
+  });
 }
 
 bool Inversion::forward(const float* dv) {
@@ -171,9 +221,11 @@ bool Inversion::forward(const float* dv) {
   std::vector<ShotJob> jobs(survey.ns());
   for (int is = 0; is < survey.ns(); is++)
     if (!make_job(param, survey, is, jobs[is], err_)) return false;
-  std::map<int, std::vector<float>> recs, imgs;
-  if (!run_windows(param, 1, jobs, v0.data(), dv, recs, imgs, err_)) return false;
-  return write_gathers(param, survey, jobs, recs, v0.data(), true, err_);
+  return for_stage_groups(param, 1, jobs, [&](std::vector<ShotJob>& group) {
+    std::map<int, std::vector<float>> recs, imgs;
+    return run_windows(param, 1, group, v0.data(), dv, recs, imgs, err_) &&
+           write_gathers(param, survey, group, recs, v0.data(), true, err_);
+  });
 }
 
 bool Inversion::adjoint(float* img, int migTag) {
@@ -181,32 +233,38 @@ bool Inversion::adjoint(float* img, int migTag) {
   if (!read_model(param.v0file.val, v0)) return false;
   const int nx = param.nx.val, nz = param.nz.val, nt = param.nt.val, ns = survey.ns();
   std::vector<ShotJob> jobs;
-  std::map<int, std::vector<float>> recs, imgs;
   for (int is = std::max(0, param.sbegin.val); is <= std::min(ns - 1, param.send.val); is++) {
     ShotJob q;
     if (!make_job(param, survey, is, q, err_)) return false;
-    std::vector<float> csg((size_t)survey.ng(is) * nt);
-    const std::string f = migTag == RTM_IMG ? csg_name(param, is) : born_name(param, is);
-    if (!read_floats(f, csg.size(), csg.data(), err_)) return false;
-    if (!remove_direct(param, survey, is, csg.data(), v0.data(), err_)) return false;
-    std::vector<float>& r = recs[is];
-    r.assign((size_t)q.w.nx * nt, 0.f);
-    if (!reorder(param, survey, is, nt, csg.data(), r.data(), q.w.fx, q.w.nx, false, err_)) return false;
     jobs.push_back(q);
   }
-  if (!jobs.empty() && !run_windows(param, 2, jobs, v0.data(), nullptr, recs, imgs, err_)) return false;
   std::vector<float> mask;
   if (param.mask.val) {
     mask.resize((size_t)nx * nz);
     if (!read_floats(param.maskfile.val, mask.size(), mask.data(), err_)) return false;
   }
   std::vector<float> g((size_t)nx * nz);
-  for (ShotJob& q : jobs) {
-    window_to_global(param, imgs[q.is].data(), g.data(), q.w.fx, param.velfx.val, q.w.nx, nx, nz, false);
-    if (param.mask.val)
-      for (size_t i = 0; i < g.size(); i++) g[i] = g[i] * mask[i];
-    if (!write_floats(shot_image_name(param, q.is), g.size(), g.data(), err_)) return false;
-  }
+  const bool ok = for_stage_groups(param, 2, jobs, [&](std::vector<ShotJob>& group) {
+    std::map<int, std::vector<float>> recs, imgs;
+    for (const ShotJob& q : group) {
+      std::vector<float> csg((size_t)survey.ng(q.is) * nt);
+      const std::string f = migTag == RTM_IMG ? csg_name(param, q.is) : born_name(param, q.is);
+      if (!read_floats(f, csg.size(), csg.data(), err_)) return false;
+      if (!remove_direct(param, survey, q.is, csg.data(), v0.data(), err_)) return false;
+      std::vector<float>& r = recs[q.is];
+      r.assign((size_t)q.w.nx * nt, 0.f);
+      if (!reorder(param, survey, q.is, nt, csg.data(), r.data(), q.w.fx, q.w.nx, false, err_)) return false;
+    }
+    if (!run_windows(param, 2, group, v0.data(), nullptr, recs, imgs, err_)) return false;
+    for (ShotJob& q : group) {
+      window_to_global(param, imgs[q.is].data(), g.data(), q.w.fx, param.velfx.val, q.w.nx, nx, nz, false);
+      if (param.mask.val)
+        for (size_t i = 0; i < g.size(); i++) g[i] = g[i] * mask[i];
+      if (!write_floats(shot_image_name(param, q.is), g.size(), g.data(), err_)) return false;
+    }
+    return true;
+  });
+  if (!ok) return false;
   // the image is the sum of every shot's file, including shots imaged by
   // earlier runs (inversion.cpp:460-471)
   std::fill(img, img + (size_t)nx * nz, 0.f);
@@ -224,16 +282,18 @@ bool Inversion::illum(float* ss) {
   std::vector<ShotJob> jobs(ns);
   for (int is = 0; is < ns; is++)
     if (!make_job(param, survey, is, jobs[is], err_)) return false;
-  std::map<int, std::vector<float>> recs, imgs;
-  if (ns > 0 && !run_windows(param, 3, jobs, v0.data(), nullptr, recs, imgs, err_)) return false;
   std::fill(ss, ss + (size_t)nx * nz, 0.f);
   std::vector<float> g((size_t)nx * nz);
-  for (ShotJob& q : jobs) {
-    window_to_global(param, imgs[q.is].data(), g.data(), q.w.fx, param.velfx.val, q.w.nx, nx, nz, false);
-    if (!write_floats(illum_name(param, q.is), g.size(), g.data(), err_)) return false;
-    for (size_t i = 0; i < g.size(); i++) ss[i] = g[i] + ss[i];
-  }
-  return true;
+  return for_stage_groups(param, 3, jobs, [&](std::vector<ShotJob>& group) {
+    std::map<int, std::vector<float>> recs, imgs;
+    if (!run_windows(param, 3, group, v0.data(), nullptr, recs, imgs, err_)) return false;
+    for (ShotJob& q : group) {
+      window_to_global(param, imgs[q.is].data(), g.data(), q.w.fx, param.velfx.val, q.w.nx, nx, nz, false);
+      if (!write_floats(illum_name(param, q.is), g.size(), g.data(), err_)) return false;
+      for (size_t i = 0; i < g.size(); i++) ss[i] = g[i] + ss[i];
+    }
+    return true;
+  });
 }
 
 bool Inversion::planeWavePrepare() {
@@ -378,28 +438,42 @@ bool Inversion::cgsolver(int maxIter, float reError, float* img, std::vector<dou
   if (!read_model(param.v0file.val, v0)) return false;
   const int nx = param.nx.val, nz = param.nz.val, nt = param.nt.val, ns = survey.ns();
   const int ndel = param.delaycal.val, NT = nt + ndel;
-  // every shot on the whole model; data and mask per model column
-  std::vector<float> wav(NT), d((size_t)ns * nx * nt, 0.f), mask((size_t)ns * nx * nt, 0.f);
+  // every shot on its window (getVel's aperture, inversion.cpp:552-598,
+  // clipped to the model's columns: the plan's shots are views of the one
+  // background model); data and mask per window column
+  const float dx = param.dx.val, velfx = param.velfx.val;
+  std::vector<float> wav(NT), d, mask;
   ricker_host(param.dt.val, param.fr.val, param.delay.val + ndel, NT, wav.data());
   std::vector<float> wv((size_t)NT * ns);
-  std::vector<int> gz((size_t)nx * ns);
+  std::vector<int> gz;
   std::vector<seis_shot> sh(ns);
   for (int is = 0; is < ns; is++) {
+    const Window w = shot_window(param, survey, is);
+    if (w.nx <= 0) return err_ = "empty shot window", false;
+    const int a = (int)std::floor((w.fx - velfx) / dx + 0.5f);
+    const int c0 = std::max(0, a), c1 = std::min(nx - 1, a + w.nx - 1);
+    if (c1 < c0) return err_ = "shot window outside the model", false;
+    const int nxw = c1 - c0 + 1;
+    const float fx = velfx + c0 * dx;
     std::vector<int> igz;
-    if (!receiver_depths(param, survey, is, param.velfx.val, nx, igz, err_)) return false;
+    if (!receiver_depths(param, survey, is, fx, nxw, igz, err_)) return false;
     int sx, sz;
-    source_cell(param, survey, is, param.velfx.val, &sx, &sz);
+    source_cell(param, survey, is, fx, &sx, &sz);
+    if (sx < 0 || sx >= nxw || sz < 0 || sz >= nz) return err_ = "source outside its window", false;
     std::vector<float> csg((size_t)survey.ng(is) * nt), ones(csg.size(), 1.f);
     if (!read_floats(csg_name(param, is), csg.size(), csg.data(), err_)) return false;
     if (!remove_direct(param, survey, is, csg.data(), v0.data(), err_)) return false;
     if (!remove_direct(param, survey, is, ones.data(), v0.data(), err_)) return false;
-    float* dr = d.data() + (size_t)is * nx * nt;
-    float* mr = mask.data() + (size_t)is * nx * nt;
-    if (!reorder(param, survey, is, nt, csg.data(), dr, param.velfx.val, nx, false, err_)) return false;
-    if (!reorder(param, survey, is, nt, ones.data(), mr, param.velfx.val, nx, false, err_)) return false;
+    const size_t r0 = d.size();
+    d.resize(r0 + (size_t)nxw * nt, 0.f);
+    mask.resize(r0 + (size_t)nxw * nt, 0.f);
+    if (!reorder(param, survey, is, nt, csg.data(), d.data() + r0, fx, nxw, false, err_)) return false;
+    if (!reorder(param, survey, is, nt, ones.data(), mask.data() + r0, fx, nxw, false, err_)) return false;
     std::copy(wav.begin(), wav.end(), wv.begin() + (size_t)NT * is);
-    std::copy(igz.begin(), igz.end(), gz.begin() + (size_t)nx * is);
-    sh[is] = seis_shot{sx, sz, nt, ndel, 0, (long long)NT * is, (long long)nx * is, (long long)nx * nt * is};
+    const size_t g0 = gz.size();
+    gz.insert(gz.end(), igz.begin(), igz.end());
+    sh[is] = seis_shot{sx, sz, nt, ndel, (long long)c0 * nz, (long long)NT * is, (long long)g0, (long long)r0, nxw,
+                       c0};
   }
   Dev dv, dw, dg, dd, dm, di, dwt;
   if (weight && !dwt.put(weight, sizeof(float) * (size_t)nx * nz)) return err_ = gpu_error("device buffers"), false;

@@ -37,6 +37,9 @@ struct DevShot {
                    // column, wav = sou [nx][nt+ndel]; 0: point source
   T* raw;          // stencil values of the band cells of the new level, grid
                    // origin; second field at raw + fstride
+  T* out;          // [nx][nz] destination of the cropped image (RTM, ILLUM)
+  const unsigned* pmask;  // LSRTM: band_terms of the cells of ring P of this shot's width
+  int x0;          // LSRTM: first column of the shot's window in the image grid
 };
 
 // U buffers of the RTM reverse pass: the imaging pass applies two
@@ -213,17 +216,44 @@ constexpr int WAVES = 4;     // wavefronts per workgroup
 constexpr int WG = 64 * WAVES;
 constexpr int LROW = 72;     // one LDS row: 64 cells + 4 halo cells on each side
 
-// Batch-uniform split of the padded grid: the interior rectangle
-// [xa, xb) x [za, zb) holds only cells with w = 1 and the 9-point stencil;
-// it is swept by wavefronts (64 z cells each) streaming along x.  Every
-// other cell is a "band" cell updated by cell_update.
-struct Layout {
+// Split of one shot's padded grid NX x NZ for a step kernel: the interior
+// rectangle [xa, xb) x [za, zb), at least `m` cells from every edge, holds
+// only cells with w = 1 and the 9-point stencil and is swept by wavefronts
+// (64 z cells each) streaming along x; every other cell is a "band" cell
+// updated by cell_update.  Shots of one launch share NZ and m, their widths
+// NX may differ (per-shot windows), so each kernel forms its shot's split.
+struct Split {
   int xa, xb, za, zb;
-  int nzc, nxs, bx;   // z chunks, x strips, strip length of the interior sweep
-  int nfwg;           // workgroups per shot for the interior sweep
-  long long nband;    // band cells per shot
-  int nbwg;           // workgroups per shot for the band cells
+  long long n;       // band cells (outside the rectangle)
 };
+
+SEIS_HD inline Split shot_split(int m, int NX, int NZ) {
+  Split q{m, NX - m, m, NZ - m, 0};
+  if (q.xb <= q.xa || q.zb <= q.za) {  // no interior rectangle: every cell is a band cell
+    q.xa = q.xb = 0;
+    q.za = 0;
+    q.zb = NZ;
+  }
+  q.n = (long long)NX * NZ - (long long)(q.xb - q.xa) * (q.zb - q.za);
+  return q;
+}
+
+// Launch shape of one step over a batch: the sweep part (blockIdx.x <
+// nfwg) covers the widest shot's interior, the band part (nbwg workgroups)
+// the shot with the most band cells; workgroups beyond a narrower shot's
+// extent exit at once.
+struct Layout {
+  int m;              // interior margin (shot_split)
+  int NZ;             // padded depth shared by the shots
+  int nzc, bx;        // z chunks of an interior, strip length of the sweep
+  int NXmax;          // widest padded grid of the launch
+  int nfwg;           // sweep workgroups per shot
+  long long nbandmax; // most band cells of a shot
+  int nbwg;           // band workgroups per shot
+};
+
+// x strips of a shot's interior at strip length bx.
+SEIS_HD inline int shot_strips(const Split& q, int bx) { return q.xb > q.xa ? (q.xb - q.xa + bx - 1) / bx : 0; }
 
 template <typename T>
 Consts<T> make_consts(const GridSpec& g, long long ld) {
@@ -288,28 +318,33 @@ __device__ __forceinline__ int xcd_tile(int b, int nfwg) { return (b % NXCD) * (
 // workgroup count is padded to a multiple of NXCD; the padding blocks exit).
 inline Layout with_strip(Layout L, int bx) {
   L.bx = bx;
-  L.nzc = (L.zb - L.za + 63) / 64;
-  L.nxs = (L.xb - L.xa + L.bx - 1) / L.bx;
-  if (L.xb == L.xa) L.nzc = L.nxs = 0;
-  L.nfwg = (L.nzc * L.nxs + WAVES - 1) / WAVES;
+  const Split q = shot_split(L.m, L.NXmax, L.NZ);
+  L.nzc = q.xb > q.xa ? (q.zb - q.za + 63) / 64 : 0;
+  const int nxs = shot_strips(q, bx);
+  L.nfwg = (L.nzc * nxs + WAVES - 1) / WAVES;
   L.nfwg = (L.nfwg + NXCD - 1) / NXCD * NXCD;
   return L;
 }
 
-inline Layout make_layout(int NX, int NZ, int pml, int bx = 16) {
-  Layout L;
-  L.xa = std::max(pml, 4);
-  L.xb = std::min(NX - pml, NX - 4);
-  L.za = std::max(pml, 4);
-  L.zb = std::min(NZ - pml, NZ - 4);
-  if (L.xb <= L.xa || L.zb <= L.za) {  // no interior rectangle: every cell is a band cell
-    L.xa = L.xb = 0;
-    L.za = 0;
-    L.zb = NZ;
+// Layout of a launch over shots of padded widths NXs[0..n) and depth NZ with
+// interior margin m; wg threads per band workgroup.
+inline Layout make_layout(const int* NXs, int n, int NZ, int m, int wg = WG, int bx = 16) {
+  Layout L{};
+  L.m = m;
+  L.NZ = NZ;
+  L.NXmax = 0;
+  L.nbandmax = 0;
+  for (int i = 0; i < n; i++) {
+    L.NXmax = std::max(L.NXmax, NXs[i]);
+    L.nbandmax = std::max(L.nbandmax, shot_split(m, NXs[i], NZ).n);
   }
-  L.nband = (long long)NX * NZ - (long long)(L.xb - L.xa) * (L.zb - L.za);
-  L.nbwg = (int)((L.nband + WG - 1) / WG);
+  L.nbwg = (int)((L.nbandmax + wg - 1) / wg);
   return with_strip(L, bx);
+}
+
+// Single-width form.
+inline Layout make_layout(int NX, int NZ, int pml, int bx = 16) {
+  return make_layout(&NX, 1, NZ, std::max(pml, 4), WG, bx);
 }
 
 // Strip-length autotuning (tune.cpp).  `trial(bx)` enqueues a few launches
@@ -319,16 +354,17 @@ inline Layout make_layout(int NX, int NZ, int pml, int bx = 16) {
 int tuned_strip(const char* kind, int NX, int NZ, int nshots, hipStream_t stream, void (*trial)(void*, int),
                 void* ctx);
 
-// Enumeration of the cells outside an inner rectangle [xa, xb) x [za, zb) of
-// the padded grid: the rows above and below it over the full width, then the
-// columns left and right of it (z fastest within each part).
-struct Ring {
-  int xa, xb, za, zb;
-  long long n;
-};
+// The strip length the tuner chose for a configuration, or the default one,
+// without timing anything.
+int strip_lookup(const char* kind, int NX, int NZ, int nshots);
 
-// j < R.n < 2^31 (checked when the layout is made): 32-bit unsigned
-// divisions, far cheaper on the GPU than 64-bit ones.
+// Enumeration of the cells outside the interior rectangle of a Split: the
+// rows above and below it over the full width, then the columns left and
+// right of it (z fastest within each part).
+using Ring = Split;
+
+// j < R.n < 2^31 (the padded grid is checked to hold fewer cells): 32-bit
+// unsigned divisions, far cheaper on the GPU than 64-bit ones.
 __device__ __forceinline__ void ring_cell(const Ring& R, int NX, int NZ, long long jl, int* x, int* z) {
   const unsigned j = (unsigned)jl;
   const unsigned top = (unsigned)NX * (unsigned)R.za, bot = (unsigned)NX * (unsigned)(NZ - R.zb);
@@ -352,31 +388,16 @@ __device__ __forceinline__ void ring_cell(const Ring& R, int NX, int NZ, long lo
   }
 }
 
-inline Ring make_ring(int NX, int NZ, int xa, int xb, int za, int zb) {
-  Ring R;
-  if (xb <= xa || zb <= za) {
-    R.xa = R.xb = 0;
-    R.za = 0;
-    R.zb = NZ;
-  } else {
-    R.xa = xa;
-    R.xb = xb;
-    R.za = za;
-    R.zb = zb;
-  }
-  R.n = (long long)NX * NZ - (long long)(R.xb - R.xa) * (R.zb - R.za);
-  return R;
-}
-
-// The absorbing band: cells closer than pml to the outer edge.
-inline Ring band_ring(int NX, int NZ, int pml) { return make_ring(NX, NZ, pml, NX - pml, pml, NZ - pml); }
-
 // Launches one step of `mode` for every shot of a batch (engine.hip): the
 // step kernel (every cell; band cells leave their stencil value in
 // DevShot::raw) followed by the band kernel (one-way update and blend of the
 // band cells from those values).
 template <typename T>
-void launch_step(int mode, const Consts<T>& k, const DevShot<T>* dev, const Layout& L, const Ring& B, int nshots,
+void launch_step(int mode, const Consts<T>& k, const DevShot<T>* dev, const Layout& L, const Layout& B, int nshots,
                  int step, hipStream_t stream);
+
+// Band shape of a launch: the cells closer than pml to an edge of each shot
+// (shot_split with margin pml), one thread each.
+inline Layout band_layout(const int* NXs, int n, int NZ, int pml) { return make_layout(NXs, n, NZ, pml, WG); }
 
 }  // namespace seis

@@ -265,14 +265,14 @@ struct RawNeighbour {
 // One-way update and blend of every band cell of every shot (after the step
 // kernel of the same step).
 template <typename T, int MODE>
-__global__ __launch_bounds__(WG) void band_kernel(Consts<T> k, const DevShot<T>* __restrict__ shots, Ring B,
-                                                  int step) {
+__global__ __launch_bounds__(WG) void band_kernel(Consts<T> k, const DevShot<T>* __restrict__ shots, int step) {
   const DevShot<T>& s = shots[blockIdx.y];
   const int it = step_it<T>(MODE, s, step);
   if (it < 0) return;
+  const int NX = s.NX, NZ = k.NZ, pml = k.pml;
+  const Ring B = shot_split(pml, NX, NZ);  // this shot's absorbing band
   const long long j = (long long)blockIdx.x * WG + threadIdx.x;
   if (j >= B.n) return;
-  const int NX = s.NX, NZ = k.NZ, pml = k.pml;
   int x, z;
   ring_cell(B, NX, NZ, j, &x, &z);
   const long long ld = k.ld;
@@ -588,22 +588,23 @@ __global__ __launch_bounds__(WG) void step_kernel(Consts<T> k, const DevShot<T>*
   c.m = Model<T>{s.vel, s.nx, k.nz, k.pml};
   c.src = fidx(k.ld, s.sx + k.pml, s.sz + k.pml);
   c.aux = s.aux ? s.aux + (step % NAUX) * s.fstride : nullptr;
+  const int NX = s.NX, NZ = k.NZ;
+  const Split q = shot_split(L.m, NX, NZ);  // this shot's interior (widths differ within a batch)
   if ((int)blockIdx.x < L.nfwg) {
     const int wave = threadIdx.x >> 6;
     const int item = xcd_tile(blockIdx.x, L.nfwg) * WAVES + wave;
-    if (item >= L.nzc * L.nxs) return;  // whole wavefront leaves; no workgroup barrier follows
+    if (item >= L.nzc * shot_strips(q, L.bx)) return;  // whole wavefront leaves; no workgroup barrier follows
     const int zc = item % L.nzc, xs = item / L.nzc;
-    const int z0 = L.za + zc * 64;
-    const int x0 = L.xa + xs * L.bx;
-    const int x1 = min(x0 + L.bx, L.xb);
-    interior_sweep<T, MODE>(k, s, c, z0, L.zb, x0, x1, lds[wave]);
+    const int z0 = q.za + zc * 64;
+    const int x0 = q.xa + xs * L.bx;
+    const int x1 = min(x0 + L.bx, q.xb);
+    interior_sweep<T, MODE>(k, s, c, z0, q.zb, x0, x1, lds[wave]);
     return;
   }
   const long long j = (long long)(blockIdx.x - L.nfwg) * WG + threadIdx.x;
-  if (j >= L.nband) return;
-  const int NX = s.NX, NZ = k.NZ;
+  if (j >= q.n) return;
   int x, z;
-  ring_cell(Ring{L.xa, L.xb, L.za, L.zb, L.nband}, NX, NZ, j, &x, &z);
+  ring_cell(q, NX, NZ, j, &x, &z);
   if (edge_dist(x, z, NX, NZ) < k.pml)
     band_raw<T, MODE>(k, s, c, x, z);
   else
@@ -739,12 +740,13 @@ __global__ void restore_plane_strips_kernel(Consts<T> k, const DevShot<T>* __res
   }
 }
 
-// Interior crop of a padded image: out[ix][iz] = img[ix+pml][iz+pml].
+// Interior crop of a padded image: out[ix][iz] = img[ix+pml][iz+pml] into
+// the shot's own destination.
 template <typename T>
-__global__ void crop_kernel(Consts<T> k, const DevShot<T>* __restrict__ shots, T* out) {
+__global__ void crop_kernel(Consts<T> k, const DevShot<T>* __restrict__ shots) {
   const DevShot<T>& s = shots[blockIdx.y];
   const long long n = (long long)s.nx * k.nz;
-  T* o = out + (long long)blockIdx.y * n;  // caller guarantees equal nx in a batch
+  T* o = s.out;
   for (long long j = blockIdx.x * (long long)blockDim.x + threadIdx.x; j < n; j += (long long)gridDim.x * blockDim.x) {
     const int ix = (int)(j / k.nz), iz = (int)(j % k.nz);
     o[j] = s.img[fidx(k.ld, ix + k.pml, iz + k.pml)];
@@ -755,15 +757,15 @@ __global__ void crop_kernel(Consts<T> k, const DevShot<T>* __restrict__ shots, T
 // Host side.
 
 template <typename T>
-void launch_step(int mode, const Consts<T>& k, const DevShot<T>* dev, const Layout& L, const Ring& B, int nshots,
+void launch_step(int mode, const Consts<T>& k, const DevShot<T>* dev, const Layout& L, const Layout& B, int nshots,
                  int step, hipStream_t stream) {
   dim3 grid(L.nfwg + L.nbwg, nshots), block(WG);
-  dim3 bgrid((unsigned)((B.n + WG - 1) / WG), nshots);
+  dim3 bgrid((unsigned)B.nbwg, nshots);
   switch (mode) {
 #define SEIS_CASE(M)                                                                            \
   case M:                                                                                       \
     hipLaunchKernelGGL((step_kernel<T, M>), grid, block, 0, stream, k, dev, L, step);           \
-    if (B.n > 0) hipLaunchKernelGGL((band_kernel<T, M>), bgrid, block, 0, stream, k, dev, B, step); \
+    if (B.nbwg > 0) hipLaunchKernelGGL((band_kernel<T, M>), bgrid, block, 0, stream, k, dev, step); \
     break;
     SEIS_CASE(MODEL)
     SEIS_CASE(BORN)
@@ -780,9 +782,9 @@ void launch_step(int mode, const Consts<T>& k, const DevShot<T>* dev, const Layo
   }
 }
 
-template void launch_step<float>(int, const Consts<float>&, const DevShot<float>*, const Layout&, const Ring&, int,
-                                 int, hipStream_t);
-template void launch_step<double>(int, const Consts<double>&, const DevShot<double>*, const Layout&, const Ring&,
+template void launch_step<float>(int, const Consts<float>&, const DevShot<float>*, const Layout&, const Layout&,
+                                 int, int, hipStream_t);
+template void launch_step<double>(int, const Consts<double>&, const DevShot<double>*, const Layout&, const Layout&,
                                   int, int, hipStream_t);
 
 // Second stream of the RTM reverse pass and its ordering events (default
@@ -810,17 +812,16 @@ struct ImageStream {
   }
 };
 
+// Zeroes the first `nfields` field slots of every shot (a fill kernel, so
+// that a caller may capture the time loop in a graph).
 template <typename T>
-struct BatchRun {
-  std::vector<DevShot<T>> host;
-  DevShot<T>* dev = nullptr;
-  void* pool = nullptr;
-  size_t pool_bytes = 0;
-  ~BatchRun() {
-    if (dev) (void)hipFree(dev);
-    if (pool) (void)hipFree(pool);
-  }
-};
+__global__ void zero_fields_kernel(const DevShot<T>* __restrict__ shots, long long origin, int nfields) {
+  const DevShot<T>& s = shots[blockIdx.y];
+  T* base = s.fld - origin;
+  const long long n = s.fstride * nfields;
+  for (long long j = blockIdx.x * (long long)blockDim.x + threadIdx.x; j < n; j += (long long)gridDim.x * blockDim.x)
+    base[j] = T(0);
+}
 
 #define HIPCHECK(x)                                                            \
   do {                                                                         \
@@ -841,55 +842,156 @@ int check_grid(const GridSpec& g) {
   return SEIS_OK;
 }
 
-template <typename T>
-int run_batch(int mode, const GridSpec& g, int nshots, const ShotSpec* shots, const T* vel, const T* dv,
-              const T* wav, const int* gz, T* rec, T* img, hipStream_t stream, int rtm_flags, int line_src) {
-  int rc = check_grid(g);
-  if (rc) return rc;
-  if (nshots <= 0) return SEIS_OK;
-  const int NZ = g.nz + 2 * g.pml;
-  const int NX = g.nx + 2 * g.pml;
-  const long long ld = field_ld(NZ);
-  const long long fe = field_elems(NX, ld);
-  const bool ilm = (mode == RUN_ILLUM);
+// ---------------------------------------------------------------------------
+// Batch memory: every shot of a launch needs its field slots (and, for RTM,
+// its boundary strips) for the whole run, so a batch is cut into consecutive
+// chunks whose working set fits a budget; the chunks run one after another.
+// Results are per shot and do not depend on the cut.
+
+namespace {
+std::mutex g_limit_mu;
+size_t g_limit_bytes = 0;  // seis_set_batch_limits: 0 = free memory less a reserve
+int g_limit_shots = 0;     // 0 = no cap on shots per launch
+}  // namespace
+
+int shot_width(const GridSpec& g, const ShotSpec& q) { return q.nx > 0 ? q.nx : g.nx; }
+
+// Field slots per shot of a run mode (see run_chunk).
+int fields_per_shot(int mode) {
+  const bool ilm = mode == RUN_ILLUM, rtm = mode == RUN_RTM;
   const int nslots = (mode == RUN_MODEL || ilm) ? 3 : 6;
-  const bool rtm = (mode == RUN_RTM);
-  if (rtm && (g.nx < 4 || g.nz < 4))
-    return set_error("rtm: the 4-layer boundary strips need nx >= 4 and nz >= 4"), SEIS_ERR_ARG;
-  // the imaging pass holds U and R of a whole padded column in LDS
-  if (rtm && sizeof(T) * 2 * (size_t)NZ > 64 * 1024)
-    return set_error("rtm: padded nz too large for the column Hilbert kernel"), SEIS_ERR_ARG;
-  int ntmax = 0, NTmax = 0;
-  size_t strip_elems = 0;
+  const int nraw = (mode == RUN_MODEL || ilm) ? 1 : 2;
+  if (mode == RUN_RTM_PLANE) return 6 + 2 + 1 + NAUX;
+  return nslots + nraw + (rtm ? 1 + NAUX : ilm ? 1 : 0);
+}
+
+// Device bytes one shot holds during a run of `mode` (fields, strips, and
+// its descriptor), each part a multiple of 256 bytes.
+size_t shot_bytes(int mode, const GridSpec& g, const ShotSpec& q, size_t elem, size_t desc) {
+  const int NX = shot_width(g, q) + 2 * g.pml, NZ = g.nz + 2 * g.pml;
+  const long long ld = field_ld(NZ);
+  size_t b = (size_t)field_elems(NX, ld) * fields_per_shot(mode) * elem;
+  long long strip_t = mode == RUN_RTM ? q.nt : mode == RUN_RTM_PLANE ? q.nt + q.ndel : 0;
+  size_t st = (size_t)strip_t * 8 * (shot_width(g, q) + g.nz) * elem;
+  return b + (st + 255) / 256 * 256 + desc;
+}
+
+int plan_chunks(const std::vector<size_t>& bytes, size_t budget, int max_shots, std::vector<int>& starts) {
+  starts.clear();
+  const size_t overhead = 256;  // alignment of the descriptor array
+  size_t used = 0;
+  int count = 0;
+  for (size_t i = 0; i < bytes.size(); i++) {
+    if (bytes[i] + overhead > budget) {
+      set_error("batch: one shot needs " + std::to_string(bytes[i] + overhead) + " bytes of device memory, more "
+                "than the " + std::to_string(budget) + " bytes available to a batch");
+      return SEIS_ERR_ARG;
+    }
+    if (starts.empty() || used + bytes[i] > budget || (max_shots > 0 && count >= max_shots)) {
+      starts.push_back((int)i);
+      used = overhead;
+      count = 0;
+    }
+    used += bytes[i];
+    count++;
+  }
+  starts.push_back((int)bytes.size());
+  return SEIS_OK;
+}
+
+// Working-set budget of a batch call: the caller's cap, or the free device
+// memory less a reserve of max(1 GiB, 1/32 of it).
+int batch_budget(size_t* out) {
+  size_t fr = 0, tot = 0;
+  HIPCHECK(hipMemGetInfo(&fr, &tot));
+  const size_t reserve = std::max<size_t>(1ull << 30, fr / 32);
+  size_t b = fr > reserve ? fr - reserve : 0;
+  std::lock_guard<std::mutex> lk(g_limit_mu);
+  if (g_limit_bytes) b = std::min(b, g_limit_bytes);
+  *out = b;
+  return SEIS_OK;
+}
+
+int batch_max_shots() {
+  std::lock_guard<std::mutex> lk(g_limit_mu);
+  return g_limit_shots;
+}
+
+// Per-shot checks shared by the batch runners.
+int check_shots(int mode, const GridSpec& g, int nshots, const ShotSpec* shots, int line_src) {
   for (int s = 0; s < nshots; s++) {
     const ShotSpec& q = shots[s];
     if (q.nt <= 0 || q.ndel < 0) return set_error("shot: nt must be > 0 and ndel >= 0"), SEIS_ERR_ARG;
-    if ((!line_src && (q.sx < 0 || q.sx >= g.nx)) || q.sz < 0 || q.sz >= g.nz)
+    if (q.nx < 0) return set_error("shot: nx must be >= 0 (0: the grid's nx)"), SEIS_ERR_ARG;
+    const int nx = shot_width(g, q);
+    GridSpec gs = g;
+    gs.nx = nx;
+    int rc = check_grid(gs);
+    if (rc) return rc;
+    if (line_src && nx != g.nx) return set_error("plane-wave shots span the grid's nx"), SEIS_ERR_ARG;
+    if ((!line_src && (q.sx < 0 || q.sx >= nx)) || q.sz < 0 || q.sz >= g.nz)
       return set_error("shot: source cell outside the model"), SEIS_ERR_ARG;
-    ntmax = std::max(ntmax, q.nt);
-    NTmax = std::max(NTmax, q.nt + q.ndel);
-    if (rtm) strip_elems += (size_t)q.nt * 8 * (g.nx + g.nz);
+    if ((mode == RUN_RTM || mode == RUN_RTM_PLANE) && (nx < 4 || g.nz < 4))
+      return set_error("rtm: the 4-layer boundary strips need nx >= 4 and nz >= 4"), SEIS_ERR_ARG;
   }
-  // per shot: nslots fields, the band stencil values of one or two fields
-  // (+ image and NAUX U buffers for RTM, + the illumination for ILLUM), then
-  // strips
+  return SEIS_OK;
+}
+
+// Device memory of one chunk: allocated here, or a region of a caller's
+// workspace.
+struct ChunkMem {
+  void* p = nullptr;
+  bool owned = false;
+  ~ChunkMem() {
+    if (owned && p) (void)hipFree(p);
+  }
+};
+
+// Launch with per-kernel timing, or plainly (the workspace path allocates
+// nothing, including profiler events).
+template <class F>
+void launch(bool quiet, hipStream_t s, const char* name, double bytes, double cells, F&& fn) {
+  if (quiet)
+    fn();
+  else
+    timed_launch(s, name, bytes, cells, fn);
+}
+
+// One chunk of a batch: shots [0, n) of `shots`, their image destinations in
+// `outs`, everything in `mem` (chunk_bytes).
+template <typename T>
+int run_chunk(int mode, const GridSpec& g, int n, const ShotSpec* shots, T* const* outs, const T* vel, const T* dv,
+              const T* wav, const int* gz, T* rec, void* mem, hipStream_t stream, int rtm_flags, int line_src,
+              bool quiet) {
+  const int NZ = g.nz + 2 * g.pml;
+  const long long ld = field_ld(NZ);
+  const bool ilm = (mode == RUN_ILLUM);
+  const int nslots = (mode == RUN_MODEL || ilm) ? 3 : 6;
+  const bool rtm = (mode == RUN_RTM);
   const int nraw = (mode == RUN_MODEL || ilm) ? 1 : 2;
-  const size_t per_shot = (size_t)fe * (nslots + nraw + (rtm ? 1 + NAUX : ilm ? 1 : 0));
-  const size_t total = per_shot * nshots + strip_elems;
-  BatchRun<T> br;
-  br.pool_bytes = total * sizeof(T);
-  HIPCHECK(hipMalloc(&br.pool, br.pool_bytes));
-  // fields and images start at zero; every strip sample is written by the
-  // forward pass before the reverse pass reads it
-  HIPCHECK(hipMemsetAsync(br.pool, 0, per_shot * nshots * sizeof(T), stream));
-  T* p = reinterpret_cast<T*>(br.pool);
-  T* strip_p = p + per_shot * nshots;
+  const int nf = fields_per_shot(mode);
+  int NTmax = 0, maxbwd = 0, nxmax = 0;
+  std::vector<int> NXs(n);
+  for (int s = 0; s < n; s++) {
+    NTmax = std::max(NTmax, shots[s].nt + shots[s].ndel);
+    maxbwd = std::max(maxbwd, shots[s].nt);
+    NXs[s] = shot_width(g, shots[s]) + 2 * g.pml;
+    nxmax = std::max(nxmax, shot_width(g, shots[s]));
+  }
+  // descriptors first, then per shot: nf fields (nslots levels, the band
+  // stencil values of one or two fields, then the image and NAUX U buffers
+  // for RTM or the illumination for ILLUM), then the strips
+  std::vector<DevShot<T>> host(n);
+  DevShot<T>* dev = reinterpret_cast<DevShot<T>*>(mem);
+  char* p = reinterpret_cast<char*>(mem) + (sizeof(DevShot<T>) * n + 255) / 256 * 256;
   const long long origin = (long long)MX * ld + MZ;
-  br.host.resize(nshots);
-  for (int s = 0; s < nshots; s++) {
+  for (int s = 0; s < n; s++) {
     const ShotSpec& q = shots[s];
-    DevShot<T>& d = br.host[s];
-    T* base = p + per_shot * s;
+    const int nx = shot_width(g, q);
+    const long long fe = field_elems(NXs[s], ld);
+    T* base = reinterpret_cast<T*>(p);
+    DevShot<T>& d = host[s];
+    d = DevShot<T>{};
     d.vel = vel + q.model;
     d.dv = dv ? dv + q.model : nullptr;
     d.wav = wav + q.wav;
@@ -900,10 +1002,12 @@ int run_batch(int mode, const GridSpec& g, int nshots, const ShotSpec* shots, co
     d.raw = base + (size_t)fe * nslots + origin;
     d.img = (rtm || ilm) ? base + (size_t)fe * (nslots + nraw) + origin : nullptr;
     d.aux = rtm ? base + (size_t)fe * (nslots + nraw + 1) + origin : nullptr;
-    d.strips = rtm ? strip_p : nullptr;
-    if (rtm) strip_p += (size_t)q.nt * 8 * (g.nx + g.nz);
-    d.nx = g.nx;
-    d.NX = NX;
+    d.out = outs ? outs[s] : nullptr;
+    const size_t st = rtm ? (size_t)q.nt * 8 * (nx + g.nz) * sizeof(T) : 0;
+    d.strips = rtm ? reinterpret_cast<T*>(p + (size_t)fe * nf * sizeof(T)) : nullptr;
+    p += (size_t)fe * nf * sizeof(T) + (st + 255) / 256 * 256;
+    d.nx = nx;
+    d.NX = NXs[s];
     d.nt = q.nt;
     d.ndel = q.ndel;
     d.sx = q.sx;
@@ -911,11 +1015,14 @@ int run_batch(int mode, const GridSpec& g, int nshots, const ShotSpec* shots, co
     d.rtm_flags = rtm_flags;
     d.line_src = line_src;
   }
-  HIPCHECK(hipMalloc(&br.dev, sizeof(DevShot<T>) * nshots));
-  HIPCHECK(hipMemcpyAsync(br.dev, br.host.data(), sizeof(DevShot<T>) * nshots, hipMemcpyHostToDevice, stream));
+  HIPCHECK(hipMemcpyAsync(dev, host.data(), sizeof(DevShot<T>) * n, hipMemcpyHostToDevice, stream));
+  // fields and images start at zero; every strip sample is written by the
+  // forward pass before the reverse pass reads it
+  hipLaunchKernelGGL((zero_fields_kernel<T>), dim3(512, n), dim3(256), 0, stream, dev, origin, nf);
   const Consts<T> k = make_consts<T>(g, ld);
-  Layout L = make_layout(NX, NZ, g.pml);
-  const Ring B = band_ring(NX, NZ, g.pml);
+  Layout L = make_layout(NXs.data(), n, NZ, std::max(g.pml, 4));
+  const Layout B = band_layout(NXs.data(), n, NZ, g.pml);
+  const int NXmax = L.NXmax;
   {
     // strip length: timed on the zeroed fields with this pass's step kernel
     // at step 0 (no injection yet, so the fields stay zero; records and strip
@@ -924,33 +1031,42 @@ int run_batch(int mode, const GridSpec& g, int nshots, const ShotSpec* shots, co
       int mode;
       const Consts<T>* k;
       const DevShot<T>* dev;
-      Layout L;
-      Ring B;
+      Layout L, B;
       int nshots;
       hipStream_t s;
     };
     // (an ILLUM trial at step 0 accumulates nothing: step 0 never exceeds ndel)
     const int proxy = mode == RUN_MODEL ? MODEL : mode == RUN_BORN ? BORN : ilm ? ILLUM : RTM_FWD;
-    Trial tr{proxy, &k, br.dev, L, B, nshots, stream};
+    Trial tr{proxy, &k, dev, L, B, n, stream};
     auto fn = [](void* c, int bx) {
       const Trial* t = static_cast<const Trial*>(c);
       launch_step<T>(t->mode, *t->k, t->dev, with_strip(t->L, bx), t->B, t->nshots, 0, t->s);
     };
     const char* kind = mode == RUN_MODEL ? "forward" : mode == RUN_BORN ? "born" : ilm ? "illum" : "rtm";
-    L = with_strip(L, tuned_strip(kind, NX, NZ, nshots, stream, fn, &tr));
+    L = with_strip(L, quiet ? strip_lookup(kind, NXmax, NZ, n) : tuned_strip(kind, NXmax, NZ, n, stream, fn, &tr));
   }
-  const double cells = (double)NX * NZ;
   const double E = sizeof(T);
-  // shots still running at forward step st / reverse step st
-  auto active_fwd = [&](int st) {
-    int n = 0;
-    for (int s = 0; s < nshots; s++) n += st < shots[s].nt + shots[s].ndel;
-    return n;
+  // cells of the shots still running at forward step st / reverse step st
+  auto cells_fwd = [&](int st) {
+    double c = 0;
+    for (int s = 0; s < n; s++)
+      if (st < shots[s].nt + shots[s].ndel) c += (double)NXs[s] * NZ;
+    return c;
   };
-  auto active_bwd = [&](int st) {
-    int n = 0;
-    for (int s = 0; s < nshots; s++) n += st < shots[s].nt;
-    return n;
+  auto cells_bwd = [&](int st, double* strip_bytes, double* faqi, double* model_cells) {
+    double c = 0, sb = 0, fq = 0, mc = 0;
+    for (int s = 0; s < n; s++)
+      if (st < shots[s].nt) {
+        const int nx = NXs[s] - 2 * g.pml;
+        c += (double)NXs[s] * NZ;
+        sb += 8.0 * (nx + g.nz) * E;
+        fq += (double)nx * NZ;
+        mc += (double)nx * g.nz;
+      }
+    if (strip_bytes) *strip_bytes = sb;
+    if (faqi) *faqi = fq;
+    if (model_cells) *model_cells = mc;
+    return c;
   };
   // Algorithmic HBM bytes per cell: MODEL reads vel, C, P and writes N (16 B
   // in FP32); BORN adds the scattered C, P, N and dv (32 B for 2 updates);
@@ -959,40 +1075,41 @@ int run_batch(int mode, const GridSpec& g, int nshots, const ShotSpec* shots, co
   // columns and read-modify-writes the interior image.
   if (mode == RUN_MODEL) {
     for (int st = 0; st < NTmax; st++) {
-      const double a = active_fwd(st) * cells;
-      timed_launch(stream, "step_model", 4 * E * a, a, [&] {
-        launch_step<T>(MODEL, k, br.dev, L, B, nshots, st, stream);
-      });
+      const double a = cells_fwd(st);
+      launch(quiet, stream, "step_model", 4 * E * a, a, [&] { launch_step<T>(MODEL, k, dev, L, B, n, st, stream); });
     }
   } else if (ilm) {
     // MODEL's bytes plus the read-modify-write of u^2 at the model cells
-    const double model_cells = (double)g.nx * g.nz;
     for (int st = 0; st < NTmax; st++) {
-      const double a = active_fwd(st);
-      timed_launch(stream, "step_illum", 4 * E * a * cells + 2 * E * a * model_cells, a * cells, [&] {
-        launch_step<T>(ILLUM, k, br.dev, L, B, nshots, st, stream);
-      });
+      double a = 0, mc = 0;
+      for (int s = 0; s < n; s++)
+        if (st < shots[s].nt + shots[s].ndel) {
+          a += (double)NXs[s] * NZ;
+          mc += (double)(NXs[s] - 2 * g.pml) * g.nz;
+        }
+      launch(quiet, stream, "step_illum", 4 * E * a + 2 * E * mc, a,
+             [&] { launch_step<T>(ILLUM, k, dev, L, B, n, st, stream); });
     }
-    hipLaunchKernelGGL((crop_kernel<T>), dim3(64, nshots), dim3(256), 0, stream, k, br.dev, img);
+    hipLaunchKernelGGL((crop_kernel<T>), dim3(64, n), dim3(256), 0, stream, k, dev);
   } else if (mode == RUN_BORN) {
     for (int st = 0; st < NTmax; st++) {
-      const double a = active_fwd(st) * cells;
-      timed_launch(stream, "step_born", 8 * E * a, 2 * a, [&] {
-        launch_step<T>(BORN, k, br.dev, L, B, nshots, st, stream);
-      });
+      const double a = cells_fwd(st);
+      launch(quiet, stream, "step_born", 8 * E * a, 2 * a, [&] { launch_step<T>(BORN, k, dev, L, B, n, st, stream); });
     }
   } else {
-    const double strip = 8.0 * (g.nx + g.nz) * E;
     for (int st = 0; st < NTmax; st++) {
-      const int n = active_fwd(st);
-      timed_launch(stream, "step_rtm_forward", 4 * E * n * cells + strip * n, n * cells, [&] {
-        launch_step<T>(RTM_FWD, k, br.dev, L, B, nshots, st, stream);
-      });
+      double a = 0, sb = 0;
+      for (int s = 0; s < n; s++)
+        if (st < shots[s].nt + shots[s].ndel) {
+          a += (double)NXs[s] * NZ;
+          sb += 8.0 * (NXs[s] - 2 * g.pml + g.nz) * E;
+        }
+      launch(quiet, stream, "step_rtm_forward", 4 * E * a + sb, a,
+             [&] { launch_step<T>(RTM_FWD, k, dev, L, B, n, st, stream); });
     }
     // reset the six wavefields, then restore the last strips into the current level
-    for (int s = 0; s < nshots; s++)
-      HIPCHECK(hipMemsetAsync(p + per_shot * s, 0, sizeof(T) * (size_t)fe * nslots, stream));
-    hipLaunchKernelGGL((restore_strips_kernel<T>), dim3(16, nshots), dim3(256), 0, stream, k, br.dev, 1, 1);
+    hipLaunchKernelGGL((zero_fields_kernel<T>), dim3(512, n), dim3(256), 0, stream, dev, origin, nslots);
+    hipLaunchKernelGGL((restore_strips_kernel<T>), dim3(16, n), dim3(256), 0, stream, k, dev, 1, 1);
     constexpr int half = HILBERT_HALF;
     HilbertTaps taps{};
     hilbert_taps_host<float>(half, taps.h);
@@ -1000,38 +1117,36 @@ int run_batch(int mode, const GridSpec& g, int nshots, const ShotSpec* shots, co
     // two reverse steps per imaging pass where four columns fit in LDS
     const int per = 2 * smem1 <= 64 * 1024 ? 2 : 1;
     const size_t smem = per * smem1;
-    int maxbwd = 0;
-    for (int s = 0; s < nshots; s++) maxbwd = std::max(maxbwd, shots[s].nt);
-    const double faqi_cells = (double)g.nx * NZ;
     // Imaging pass q covers reverse steps [per q, per q + per) and runs on a
     // second stream after the last of them, beside the next reverse step.
     // Step st rewrites the receiver slot of step st-3 (and the U buffer of
     // step st-4), so it first waits for the pass that covers step st-3.
     ImageStream is;
+    int rc;
     if ((rc = is.init())) return rc;
     for (int st = 0; st < maxbwd; st++) {
-      const int n = active_bwd(st);
+      double sb;
+      const double a = cells_bwd(st, &sb, nullptr, nullptr);
       if (st >= 3) HIPCHECK(hipStreamWaitEvent(stream, is.done[((st - 3) / per) % ImageStream::NEV], 0));
-      timed_launch(stream, "step_rtm_reverse", 8 * E * n * cells + strip * n, 2 * n * cells, [&] {
-        launch_step<T>(RTM_BWD, k, br.dev, L, B, nshots, st, stream);
-      });
+      launch(quiet, stream, "step_rtm_reverse", 8 * E * a + sb, 2 * a,
+             [&] { launch_step<T>(RTM_BWD, k, dev, L, B, n, st, stream); });
       if ((st + 1) % per != 0 && st != maxbwd - 1) continue;
       const int q = st / per, first = q * per;
       const int pair = st > first;
-      const int n0 = active_bwd(first);
+      double fq0, mc0, fq1 = 0;
+      cells_bwd(first, nullptr, &fq0, &mc0);
+      if (pair) cells_bwd(st, nullptr, &fq1, nullptr);
       HIPCHECK(hipEventRecord(is.ready[q % ImageStream::NEV], stream));
       HIPCHECK(hipStreamWaitEvent(is.s, is.ready[q % ImageStream::NEV], 0));
       // per step U and R over the interior columns; the image once
-      timed_launch(is.s, "rtm_image", 2 * E * faqi_cells * (n0 + (pair ? n : 0)) + 2 * E * n0 * (double)g.nx * g.nz,
-                   0, [&] {
-                     hipLaunchKernelGGL((image_kernel<half>), dim3(g.nx, nshots), dim3(128), smem, is.s, k, br.dev,
-                                        first, taps, pair, 0);
-                   });
+      launch(quiet, is.s, "rtm_image", 2 * E * (fq0 + fq1) + 2 * E * mc0, 0, [&] {
+        hipLaunchKernelGGL((image_kernel<half>), dim3(nxmax, n), dim3(128), smem, is.s, k, dev, first, taps, pair, 0);
+      });
       HIPCHECK(hipEventRecord(is.done[q % ImageStream::NEV], is.s));
     }
     HIPCHECK(hipEventRecord(is.ready[0], is.s));
     HIPCHECK(hipStreamWaitEvent(stream, is.ready[0], 0));
-    hipLaunchKernelGGL((crop_kernel<T>), dim3(64, nshots), dim3(256), 0, stream, k, br.dev, img);
+    hipLaunchKernelGGL((crop_kernel<T>), dim3(64, n), dim3(256), 0, stream, k, dev);
     HIPCHECK(hipStreamSynchronize(stream));
   }
   HIPCHECK(hipGetLastError());
@@ -1039,51 +1154,83 @@ int run_batch(int mode, const GridSpec& g, int nshots, const ShotSpec* shots, co
   return SEIS_OK;
 }
 
-template int run_batch<float>(int, const GridSpec&, int, const ShotSpec*, const float*, const float*, const float*,
-                              const int*, float*, float*, hipStream_t, int, int);
+// Image destinations of the shots of a batch: consecutive [nx_s][nz] blocks.
+template <typename T>
+std::vector<T*> image_outs(const GridSpec& g, int nshots, const ShotSpec* shots, T* img) {
+  std::vector<T*> o(nshots, nullptr);
+  if (!img) return o;
+  size_t off = 0;
+  for (int s = 0; s < nshots; s++) {
+    o[s] = img + off;
+    off += (size_t)shot_width(g, shots[s]) * g.nz;
+  }
+  return o;
+}
 
 template <typename T>
-int run_rtm_plane(const GridSpec& g, int nshots, const ShotSpec* shots, const T* vel, const T* sou, const int* gz,
-                  const T* rec, T* img, hipStream_t stream) {
+int run_batch(int mode, const GridSpec& g, int nshots, const ShotSpec* shots, const T* vel, const T* dv,
+              const T* wav, const int* gz, T* rec, T* img, hipStream_t stream, int rtm_flags, int line_src,
+              const Workspace* ws) {
   int rc = check_grid(g);
   if (rc) return rc;
   if (nshots <= 0) return SEIS_OK;
-  if (g.nx < 4 || g.nz < 4)
-    return set_error("rtm_plane: the 4-layer boundary strips need nx >= 4 and nz >= 4"), SEIS_ERR_ARG;
+  if ((rc = check_shots(mode, g, nshots, shots, line_src))) return rc;
+  // the imaging pass holds U and R of a whole padded column in LDS
+  if (mode == RUN_RTM && sizeof(T) * 2 * (size_t)(g.nz + 2 * g.pml) > 64 * 1024)
+    return set_error("rtm: padded nz too large for the column Hilbert kernel"), SEIS_ERR_ARG;
+  std::vector<size_t> bytes(nshots);
+  for (int s = 0; s < nshots; s++) bytes[s] = shot_bytes(mode, g, shots[s], sizeof(T), sizeof(DevShot<T>));
+  size_t budget;
+  if (ws) budget = ws->bytes;
+  else if ((rc = batch_budget(&budget))) return rc;
+  std::vector<int> starts;
+  if ((rc = plan_chunks(bytes, budget, batch_max_shots(), starts))) return rc;
+  const std::vector<T*> outs = image_outs(g, nshots, shots, img);
+  for (size_t c = 0; c + 1 < starts.size(); c++) {
+    const int s0 = starts[c], n = starts[c + 1] - s0;
+    size_t need = 256;
+    for (int s = s0; s < s0 + n; s++) need += bytes[s];
+    ChunkMem m;
+    if (ws) {
+      m.p = ws->ptr;
+    } else {
+      HIPCHECK(hipMalloc(&m.p, need));
+      m.owned = true;
+    }
+    if ((rc = run_chunk<T>(mode, g, n, shots + s0, outs.data() + s0, vel, dv, wav, gz, rec, m.p, stream, rtm_flags,
+                           line_src, ws != nullptr)))
+      return rc;
+  }
+  return SEIS_OK;
+}
+
+template int run_batch<float>(int, const GridSpec&, int, const ShotSpec*, const float*, const float*, const float*,
+                              const int*, float*, float*, hipStream_t, int, int, const Workspace*);
+
+// adjointPlane for one chunk of shots (all of the grid's width).
+template <typename T>
+int run_rtm_plane_chunk(const GridSpec& g, int n, const ShotSpec* shots, T* const* outs, const T* vel, const T* sou,
+                        const int* gz, const T* rec, void* mem, hipStream_t stream) {
   const int NZ = g.nz + 2 * g.pml;
   const int NX = g.nx + 2 * g.pml;
-  if (sizeof(T) * 2 * (size_t)NZ > 64 * 1024)
-    return set_error("rtm_plane: padded nz too large for the column Hilbert kernel"), SEIS_ERR_ARG;
   const long long ld = field_ld(NZ);
   const long long fe = field_elems(NX, ld);
   int NTmax = 0;
-  size_t strip_elems = 0;
-  for (int s = 0; s < nshots; s++) {
-    const ShotSpec& q = shots[s];
-    if (q.nt <= 0 || q.ndel < 0) return set_error("shot: nt must be > 0 and ndel >= 0"), SEIS_ERR_ARG;
-    if (q.sz < 0 || q.sz >= g.nz) return set_error("shot: source row outside the model"), SEIS_ERR_ARG;
-    NTmax = std::max(NTmax, q.nt + q.ndel);
-    strip_elems += (size_t)(q.nt + q.ndel) * 8 * (g.nx + g.nz);
-  }
+  for (int s = 0; s < n; s++) NTmax = std::max(NTmax, shots[s].nt + shots[s].ndel);
   // per shot: source field and receiver field (3 levels each), their band
   // stencil values, the image and the U buffers; then strips of every step
   const int nslots = 6, nraw = 2;
-  const size_t per_shot = (size_t)fe * (nslots + nraw + 1 + NAUX);
-  const size_t total = per_shot * nshots + strip_elems;
-  BatchRun<T> br;
-  br.pool_bytes = total * sizeof(T);
-  HIPCHECK(hipMalloc(&br.pool, br.pool_bytes));
-  // all zero, strips included: the strips saved at the last step are those
-  // of the zero level the backward pass starts from
-  HIPCHECK(hipMemsetAsync(br.pool, 0, br.pool_bytes, stream));
-  T* p = reinterpret_cast<T*>(br.pool);
-  T* strip_p = p + per_shot * nshots;
+  const int nf = fields_per_shot(RUN_RTM_PLANE);
+  std::vector<DevShot<T>> host(n);
+  DevShot<T>* dev = reinterpret_cast<DevShot<T>*>(mem);
+  char* p = reinterpret_cast<char*>(mem) + (sizeof(DevShot<T>) * n + 255) / 256 * 256;
   const long long origin = (long long)MX * ld + MZ;
-  br.host.resize(nshots);
-  for (int s = 0; s < nshots; s++) {
+  std::vector<T*> bases(n);
+  for (int s = 0; s < n; s++) {
     const ShotSpec& q = shots[s];
-    DevShot<T>& d = br.host[s];
-    T* base = p + per_shot * s;
+    T* base = reinterpret_cast<T*>(p);
+    bases[s] = base;
+    DevShot<T>& d = host[s];
     d = DevShot<T>{};
     d.vel = vel + q.model;
     d.wav = sou + q.wav;
@@ -1094,8 +1241,10 @@ int run_rtm_plane(const GridSpec& g, int nshots, const ShotSpec* shots, const T*
     d.raw = base + (size_t)fe * nslots + origin;
     d.img = base + (size_t)fe * (nslots + nraw) + origin;
     d.aux = base + (size_t)fe * (nslots + nraw + 1) + origin;
-    d.strips = strip_p;
-    strip_p += (size_t)(q.nt + q.ndel) * 8 * (g.nx + g.nz);
+    d.out = outs[s];
+    const size_t st = (size_t)(q.nt + q.ndel) * 8 * (g.nx + g.nz) * sizeof(T);
+    d.strips = reinterpret_cast<T*>(p + (size_t)fe * nf * sizeof(T));
+    p += (size_t)fe * nf * sizeof(T) + (st + 255) / 256 * 256;
     d.nx = g.nx;
     d.NX = NX;
     d.nt = q.nt;
@@ -1104,56 +1253,61 @@ int run_rtm_plane(const GridSpec& g, int nshots, const ShotSpec* shots, const T*
     d.sz = q.sz;
     d.line_src = 1;
   }
-  HIPCHECK(hipMalloc(&br.dev, sizeof(DevShot<T>) * nshots));
-  HIPCHECK(hipMemcpyAsync(br.dev, br.host.data(), sizeof(DevShot<T>) * nshots, hipMemcpyHostToDevice, stream));
+  HIPCHECK(hipMemcpyAsync(dev, host.data(), sizeof(DevShot<T>) * n, hipMemcpyHostToDevice, stream));
+  hipLaunchKernelGGL((zero_fields_kernel<T>), dim3(512, n), dim3(256), 0, stream, dev, origin, nf);
+  // strips too: the strips saved at the last step are those of the zero
+  // level the backward pass starts from
+  for (int s = 0; s < n; s++)
+    HIPCHECK(hipMemsetAsync(host[s].strips, 0,
+                            sizeof(T) * (size_t)(shots[s].nt + shots[s].ndel) * 8 * (g.nx + g.nz), stream));
   const Consts<T> k = make_consts<T>(g, ld);
-  Layout L = make_layout(NX, NZ, g.pml);
-  const Ring B = band_ring(NX, NZ, g.pml);
+  std::vector<int> NXs(n, NX);
+  Layout L = make_layout(NXs.data(), n, NZ, std::max(g.pml, 4));
+  const Layout B = band_layout(NXs.data(), n, NZ, g.pml);
   {
     // strip length: a backward-pass step 0 is a trial that writes only what
     // the real step 0 writes again (its new level, band values, strips)
     struct Trial {
       const Consts<T>* k;
       const DevShot<T>* dev;
-      Layout L;
-      Ring B;
+      Layout L, B;
       int nshots;
       hipStream_t s;
     };
-    Trial tr{&k, br.dev, L, B, nshots, stream};
+    Trial tr{&k, dev, L, B, n, stream};
     auto fn = [](void* c, int bx) {
       const Trial* t = static_cast<const Trial*>(c);
       launch_step<T>(PW_BACK, *t->k, t->dev, with_strip(t->L, bx), t->B, t->nshots, 0, t->s);
     };
-    L = with_strip(L, tuned_strip("rtm_plane", NX, NZ, nshots, stream, fn, &tr));
+    L = with_strip(L, tuned_strip("rtm_plane", NX, NZ, n, stream, fn, &tr));
   }
   const double cells = (double)NX * NZ;
   const double E = sizeof(T);
   const double strip = 8.0 * (g.nx + g.nz) * E;
   auto active = [&](int mode, int st) {
-    int n = 0;
-    for (int s = 0; s < nshots; s++) n += step_it<T>(mode, br.host[s], st) >= 0;
-    return n;
+    int c = 0;
+    for (int s = 0; s < n; s++) c += step_it<T>(mode, host[s], st) >= 0;
+    return c;
   };
   // pass 1: receiver field backward in time
   for (int st = 0; st < NTmax; st++) {
-    const int n = active(PW_BACK, st);
-    timed_launch(stream, "step_plane_back", 4 * E * n * cells + strip * n, n * cells, [&] {
-      launch_step<T>(PW_BACK, k, br.dev, L, B, nshots, st, stream);
+    const int a = active(PW_BACK, st);
+    timed_launch(stream, "step_plane_back", 4 * E * a * cells + strip * a, a * cells, [&] {
+      launch_step<T>(PW_BACK, k, dev, L, B, n, st, stream);
     });
   }
   // its last two levels start the forward rebuild (fd.cpp:848-852): the
   // level of time 0 as previous, of time 1 as current; the source field
   // starts from zero
-  for (int s = 0; s < nshots; s++) {
+  for (int s = 0; s < n; s++) {
     const int NT = shots[s].nt + shots[s].ndel;
-    T* base = p + per_shot * s;
+    T* base = bases[s];
     HIPCHECK(hipMemcpyAsync(base + (size_t)fe * 3, base + (size_t)fe * ((NT + 1) % 3), sizeof(T) * fe,
                             hipMemcpyDeviceToDevice, stream));
     HIPCHECK(hipMemcpyAsync(base + (size_t)fe * 4, base + (size_t)fe * (NT % 3), sizeof(T) * fe,
                             hipMemcpyDeviceToDevice, stream));
-    HIPCHECK(hipMemsetAsync(base, 0, sizeof(T) * (size_t)fe * 3, stream));
   }
+  hipLaunchKernelGGL((zero_fields_kernel<T>), dim3(512, n), dim3(256), 0, stream, dev, origin, 3);
   // pass 2: line-source field forward, receiver field rebuilt forward,
   // FaQi imaging, strips restored into the receiver field's new level
   constexpr int half = HILBERT_HALF;
@@ -1162,19 +1316,46 @@ int run_rtm_plane(const GridSpec& g, int nshots, const ShotSpec* shots, const T*
   const size_t smem = sizeof(T) * 2 * NZ;  // at most 64 KiB (checked with the arguments)
   const double faqi_cells = (double)g.nx * NZ;
   for (int st = 0; st + 2 < NTmax; st++) {
-    const int n = active(PW_FWD, st);
-    timed_launch(stream, "step_plane_fwd", 8 * E * n * cells, 2 * n * cells, [&] {
-      launch_step<T>(PW_FWD, k, br.dev, L, B, nshots, st, stream);
+    const int a = active(PW_FWD, st);
+    timed_launch(stream, "step_plane_fwd", 8 * E * a * cells, 2 * a * cells, [&] {
+      launch_step<T>(PW_FWD, k, dev, L, B, n, st, stream);
     });
-    timed_launch(stream, "plane_image", 2 * E * n * faqi_cells + 2 * E * n * (double)g.nx * g.nz, 0, [&] {
-      hipLaunchKernelGGL((image_kernel<half>), dim3(g.nx, nshots), dim3(128), smem, stream, k, br.dev, st, taps, 0,
-                         1);
+    timed_launch(stream, "plane_image", 2 * E * a * faqi_cells + 2 * E * a * (double)g.nx * g.nz, 0, [&] {
+      hipLaunchKernelGGL((image_kernel<half>), dim3(g.nx, n), dim3(128), smem, stream, k, dev, st, taps, 0, 1);
     });
-    hipLaunchKernelGGL((restore_plane_strips_kernel<T>), dim3(16, nshots), dim3(256), 0, stream, k, br.dev, st);
+    hipLaunchKernelGGL((restore_plane_strips_kernel<T>), dim3(16, n), dim3(256), 0, stream, k, dev, st);
   }
-  hipLaunchKernelGGL((crop_kernel<T>), dim3(64, nshots), dim3(256), 0, stream, k, br.dev, img);
+  hipLaunchKernelGGL((crop_kernel<T>), dim3(64, n), dim3(256), 0, stream, k, dev);
   HIPCHECK(hipGetLastError());
   HIPCHECK(hipStreamSynchronize(stream));
+  return SEIS_OK;
+}
+
+template <typename T>
+int run_rtm_plane(const GridSpec& g, int nshots, const ShotSpec* shots, const T* vel, const T* sou, const int* gz,
+                  const T* rec, T* img, hipStream_t stream) {
+  int rc = check_grid(g);
+  if (rc) return rc;
+  if (nshots <= 0) return SEIS_OK;
+  if ((rc = check_shots(RUN_RTM_PLANE, g, nshots, shots, 1))) return rc;
+  if (sizeof(T) * 2 * (size_t)(g.nz + 2 * g.pml) > 64 * 1024)
+    return set_error("rtm_plane: padded nz too large for the column Hilbert kernel"), SEIS_ERR_ARG;
+  std::vector<size_t> bytes(nshots);
+  for (int s = 0; s < nshots; s++) bytes[s] = shot_bytes(RUN_RTM_PLANE, g, shots[s], sizeof(T), sizeof(DevShot<T>));
+  size_t budget;
+  if ((rc = batch_budget(&budget))) return rc;
+  std::vector<int> starts;
+  if ((rc = plan_chunks(bytes, budget, batch_max_shots(), starts))) return rc;
+  const std::vector<T*> outs = image_outs(g, nshots, shots, img);
+  for (size_t c = 0; c + 1 < starts.size(); c++) {
+    const int s0 = starts[c], n = starts[c + 1] - s0;
+    size_t need = 256;
+    for (int s = s0; s < s0 + n; s++) need += bytes[s];
+    ChunkMem m;
+    HIPCHECK(hipMalloc(&m.p, need));
+    m.owned = true;
+    if ((rc = run_rtm_plane_chunk<T>(g, n, shots + s0, outs.data() + s0, vel, sou, gz, rec, m.p, stream))) return rc;
+  }
   return SEIS_OK;
 }
 
@@ -1182,3 +1363,36 @@ template int run_rtm_plane<float>(const GridSpec&, int, const ShotSpec*, const f
                                   const float*, float*, hipStream_t);
 
 }  // namespace seis
+
+using namespace seis;
+
+extern "C" {
+
+int seis_set_batch_limits(size_t bytes, int shots) {
+  if (shots < 0) return set_error("seis_set_batch_limits: shots must be >= 0"), SEIS_ERR_ARG;
+  std::lock_guard<std::mutex> lk(g_limit_mu);
+  g_limit_bytes = bytes;
+  g_limit_shots = shots;
+  return SEIS_OK;
+}
+
+int seis_batch_plan(int mode, const seis_grid* grid, int nshots, const seis_shot* shots, size_t budget,
+                    int max_shots, int* starts, int* nchunks) {
+  if (!grid || (nshots > 0 && !shots) || !nchunks || nshots < 0 || max_shots < 0)
+    return set_error("seis_batch_plan: bad argument"), SEIS_ERR_ARG;
+  if (mode < SEIS_PLAN_MODEL || mode > SEIS_PLAN_RTM_PLANE) return set_error("seis_batch_plan: unknown mode"), SEIS_ERR_ARG;
+  const int rmode = mode == SEIS_PLAN_MODEL ? RUN_MODEL : mode == SEIS_PLAN_BORN ? RUN_BORN
+                    : mode == SEIS_PLAN_RTM ? RUN_RTM : mode == SEIS_PLAN_ILLUM ? RUN_ILLUM : RUN_RTM_PLANE;
+  int rc = check_grid(*grid);
+  if (rc) return rc;
+  if ((rc = check_shots(rmode, *grid, nshots, shots, rmode == RUN_RTM_PLANE))) return rc;
+  std::vector<size_t> bytes(nshots);
+  for (int s = 0; s < nshots; s++) bytes[s] = shot_bytes(rmode, *grid, shots[s], sizeof(float), sizeof(DevShot<float>));
+  std::vector<int> st;
+  if ((rc = plan_chunks(bytes, budget, max_shots, st))) return rc;
+  if (starts) std::copy(st.begin(), st.end(), starts);
+  *nchunks = (int)st.size() - 1;
+  return SEIS_OK;
+}
+
+}  // extern "C"

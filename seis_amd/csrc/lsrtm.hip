@@ -55,16 +55,16 @@ __global__ void band_terms_kernel(Ring R, int NX, int NZ, int pml, unsigned* __r
 // lam^{lev-1} at those cells (levels lev and lev+1 are complete now), both
 // stored for the step kernel.  `masks` holds band_terms of each ring cell.
 template <typename T>
-__global__ __launch_bounds__(256) void adj_prep_kernel(Consts<T> k, const DevShot<T>* __restrict__ shots, Ring R,
-                                                       const unsigned* __restrict__ masks, int step) {
+__global__ __launch_bounds__(256) void adj_prep_kernel(Consts<T> k, const DevShot<T>* __restrict__ shots, int step) {
   const DevShot<T>& s = shots[blockIdx.y];
   const int lev = s.nt + s.ndel + 1 - step;
   if (lev < 1) return;
+  const Ring R = shot_split(k.pml + 1, s.NX, k.NZ);  // ring P of this shot's width
   const long long j = (long long)blockIdx.x * 256 + threadIdx.x;
   if (j >= R.n) return;
   int x, z;
   ring_cell(R, s.NX, k.NZ, j, &x, &z);
-  const unsigned mk = SEIS_GL(masks)[j];
+  const unsigned mk = SEIS_GL(s.pmask)[j];
   const AdjLevel<T> L = level(s, lev);
   const long long i = fidx(k.ld, x, z);
   SEIS_GL(s.fld + (LS_AR + lev % 2) * s.fstride)[i] = adj_raw_m(k, s.nx, L, x, z, mk);
@@ -122,6 +122,7 @@ __global__ __launch_bounds__(AWG) void adj_step_kernel(Consts<T> k, const DevSho
   const Model<T> m{s.vel, s.nx, k.nz, pml};
   const RawAdj<T> A1{s.fld + (LS_AR + lev % 2) * s.fstride, L1.lam, ld, NX, NZ, pml};
   const RawAdj<T> A2{s.fld + (LS_AR + (lev + 1) % 2) * s.fstride, L2.lam, ld, NX, NZ, pml};
+  const Split sq = shot_split(F.m, NX, NZ);  // this shot's interior
   if ((int)blockIdx.x < F.nfwg) {
     // Cells at least 8 from the outer edge: every stencil reader within 4
     // cells uses the 9-point operator, so the stencil part is the 9-point
@@ -129,13 +130,13 @@ __global__ __launch_bounds__(AWG) void adj_step_kernel(Consts<T> k, const DevSho
     // pml from the edge add the one-way terms and store their band products.
     const int wave = threadIdx.x >> 6;
     const int item = xcd_tile(blockIdx.x, F.nfwg) * WAVES + wave;
-    if (item >= F.nzc * F.nxs) return;
+    if (item >= F.nzc * shot_strips(sq, F.bx)) return;
     const int zc = item % F.nzc, xs = item / F.nzc;
-    const int z0 = F.za + zc * 64;
-    const int x0 = F.xa + xs * F.bx, x1 = min(x0 + F.bx, F.xb);
+    const int z0 = sq.za + zc * 64;
+    const int x0 = sq.xa + xs * F.bx, x1 = min(x0 + F.bx, sq.xb);
     const int lane = threadIdx.x & 63;
     const int z = z0 + lane;
-    const bool act = z < F.zb;
+    const bool act = z < sq.zb;
     const int zl = min(z, NZ - 1);
     T* la = lds[wave];
     const bool halo = lane < 4 || lane >= 60;
@@ -245,10 +246,9 @@ __global__ __launch_bounds__(AWG) void adj_step_kernel(Consts<T> k, const DevSho
     return;
   }
   const long long j = (long long)(blockIdx.x - F.nfwg) * AWG + threadIdx.x;
-  if (j >= F.nband) return;
-  Ring R{F.xa, F.xb, F.za, F.zb, F.nband};
+  if (j >= sq.n) return;
   int x, z;
-  ring_cell(R, NX, NZ, j, &x, &z);
+  ring_cell(sq, NX, NZ, j, &x, &z);
   T ar;
   T val = adj_value(k, m, NX, L1, L2, A1, A2, x, z, &ar);
   val = adj_finish(k, s, lev, x, z, val, ar);
@@ -263,12 +263,18 @@ __global__ __launch_bounds__(AWG) void adj_step_kernel(Consts<T> k, const DevSho
   }
 }
 
-// Sum of the per-shot gradients in shot order: out[j] = sum_s grad_s[j].
+// Sum of the per-shot gradients in shot order into the image grid: cell j
+// (column j / nz) gets the gradient of every shot whose window holds that
+// column, out[j] = sum_s grad_s[j - x0_s nz].
 template <typename T>
-__global__ void sum_grads_kernel(const DevShot<T>* __restrict__ shots, int nshots, long long n, T* out) {
+__global__ void sum_grads_kernel(const DevShot<T>* __restrict__ shots, int nshots, int nz, long long n, T* out) {
   for (long long j = blockIdx.x * (long long)blockDim.x + threadIdx.x; j < n; j += (long long)gridDim.x * blockDim.x) {
+    const int col = (int)(j / nz);
     T a = T(0);
-    for (int s = 0; s < nshots; s++) a += shots[s].grad[j];
+    for (int s = 0; s < nshots; s++) {
+      const int c = col - shots[s].x0;
+      if (c >= 0 && c < shots[s].nx) a += shots[s].grad[j - (long long)shots[s].x0 * nz];
+    }
     out[j] = a;
   }
 }
@@ -349,9 +355,10 @@ __global__ void fill_kernel(T* p, T v, long long n) {
 template <typename T>
 struct LsrtmPlan {
   GridSpec g{};
-  int nshots = 0, snap_kind = 0, NX = 0, NZ = 0, NTmax = 0;
-  long long ld = 0, fe = 0, model = 0, nrec = 0;
+  int nshots = 0, snap_kind = 0, NXmax = 0, NZ = 0, NTmax = 0;
+  long long ld = 0, model = 0, nrec = 0;
   std::vector<ShotSpec> spec;
+  std::vector<int> NXs;    // padded width of each shot's window
   std::vector<DevShot<T>> host;
   DevShot<T>* dev = nullptr;
   T* pool = nullptr;       // LS_NSLOT fields per shot
@@ -363,12 +370,13 @@ struct LsrtmPlan {
   const T* mask = nullptr; // optional record mask (caller-owned, device)
   const T* weight = nullptr;  // optional image-space preconditioner W of CG (caller-owned, device)
   T* masked = nullptr;     // masked copy of an adjoint input
-  unsigned* pmask = nullptr;  // band_terms of the ring cells P
+  unsigned* pmask = nullptr;  // band_terms of the ring cells P, one table per distinct width
   Consts<T> k{};
   Layout F{};   // forward step split (interior sweep / band)
   Layout A{};   // adjoint step split
-  Ring P{};     // cells whose raw adjoint is precomputed
-  Ring B{};     // absorbing band of the forward steps
+  Layout P{};   // cells whose raw adjoint is precomputed (margin pml + 1)
+  Layout B{};   // absorbing band of the forward steps
+  long long model_cells = 0;  // sum over shots of the window's model cells
   bool adj_tuned = false;
   int adj_kind = SEIS_LSRTM_ADJ_EXACT;  // seis_lsrtm_set_adjoint
   hipStream_t stream = 0;
@@ -392,80 +400,98 @@ struct LsrtmPlan {
       return set_error("lsrtm: the padded grid must have fewer than 2^31 cells"), SEIS_ERR_ARG;
     if (kind < 0 || kind > 2) return set_error("lsrtm: snapshot kind must be 0, 1 or 2"), SEIS_ERR_ARG;
     if (ns <= 0) return set_error("lsrtm: no shots"), SEIS_ERR_ARG;
-    NX = g.nx + 2 * g.pml;
     NZ = g.nz + 2 * g.pml;
     ld = field_ld(NZ);
-    fe = field_elems(NX, ld);
     model = (long long)g.nx * g.nz;
     spec.assign(shots, shots + ns);
-    long long snap_total = 0, scale_total = 0;
+    NXs.resize(ns);
+    long long snap_total = 0, scale_total = 0, pool_total = 0;
     nrec = 0;
-    for (const ShotSpec& q : spec) {
-      if (q.nt <= 0 || q.ndel < 0 || q.sx < 0 || q.sx >= g.nx || q.sz < 0 || q.sz >= g.nz)
+    model_cells = 0;
+    for (int s = 0; s < ns; s++) {
+      const ShotSpec& q = spec[s];
+      const int nx = shot_width(g, q);
+      if (q.nt <= 0 || q.ndel < 0 || q.sx < 0 || q.sx >= nx || q.sz < 0 || q.sz >= g.nz)
         return set_error("lsrtm: bad shot"), SEIS_ERR_ARG;
+      if (q.nx < 0 || q.x0 < 0 || q.x0 + nx > g.nx)
+        return set_error("lsrtm: a shot's window [x0, x0 + nx) must lie in the grid's columns"), SEIS_ERR_ARG;
+      if (nx + 2 * g.pml < 8) return set_error("lsrtm: a shot's padded width must be at least 8"), SEIS_ERR_ARG;
+      NXs[s] = nx + 2 * g.pml;
+      NXmax = std::max(NXmax, NXs[s]);
       NTmax = std::max(NTmax, q.nt + q.ndel);
-      snap_total += (long long)(q.nt + q.ndel) * model;
+      snap_total += (long long)(q.nt + q.ndel) * nx * g.nz;
       scale_total += q.nt + q.ndel;
-      nrec = std::max(nrec, q.rec + (long long)g.nx * q.nt);
+      pool_total += field_elems(NXs[s], ld) * LS_NSLOT;
+      model_cells += (long long)nx * g.nz;
+      nrec = std::max(nrec, q.rec + (long long)nx * q.nt);
     }
-    LS_CHECK(hipMalloc(&pool, sizeof(T) * (size_t)fe * LS_NSLOT * ns));
+    LS_CHECK(hipMalloc(&pool, sizeof(T) * (size_t)pool_total));
     LS_CHECK(hipMalloc(&snaps, snap_elem() * (size_t)snap_total));
     LS_CHECK(hipMalloc(&scales, sizeof(float) * (size_t)scale_total));
-    LS_CHECK(hipMalloc(&grads, sizeof(T) * (size_t)model * ns));
+    LS_CHECK(hipMalloc(&grads, sizeof(T) * (size_t)model_cells));
     LS_CHECK(hipMalloc(&part, sizeof(double) * (RGRID + 8)));
     LS_CHECK(hipMemset(scales, 0, sizeof(float) * (size_t)scale_total));
     k = make_consts<T>(g, ld);
-    F = make_layout(NX, NZ, g.pml);
-    {
-      // adjoint sweep: every stencil reader within 4 cells is a 9-point cell
-      A = F;
-      A.xa = 8;
-      A.xb = NX - 8;
-      A.za = 8;
-      A.zb = NZ - 8;
-      if (A.xb <= A.xa || A.zb <= A.za) {
-        A.xa = A.xb = 0;
-        A.za = 0;
-        A.zb = NZ;
-      }
-      A.nband = (long long)NX * NZ - (long long)(A.xb - A.xa) * (A.zb - A.za);
-      A.nbwg = (int)((A.nband + AWG - 1) / AWG);
-      A = with_strip(A, A.bx);
+    F = make_layout(NXs.data(), ns, NZ, std::max(g.pml, 4));
+    // adjoint sweep: every stencil reader within 4 cells is a 9-point cell
+    A = make_layout(NXs.data(), ns, NZ, 8, AWG);
+    P = make_layout(NXs.data(), ns, NZ, g.pml + 1, 256);
+    B = band_layout(NXs.data(), ns, NZ, g.pml);
+    // band-term masks of ring P, one table per distinct window width
+    std::vector<int> widths(NXs);
+    std::sort(widths.begin(), widths.end());
+    widths.erase(std::unique(widths.begin(), widths.end()), widths.end());
+    std::vector<long long> moff(widths.size());
+    long long mtot = 0;
+    for (size_t w = 0; w < widths.size(); w++) {
+      moff[w] = mtot;
+      mtot += shot_split(g.pml + 1, widths[w], NZ).n;
     }
-    P = make_ring(NX, NZ, g.pml + 1, NX - g.pml - 1, g.pml + 1, NZ - g.pml - 1);
-    B = band_ring(NX, NZ, g.pml);
-    if (P.n > 0) {
-      LS_CHECK(hipMalloc(&pmask, sizeof(unsigned) * (size_t)P.n));
-      hipLaunchKernelGGL(band_terms_kernel, dim3((unsigned)((P.n + 255) / 256)), dim3(256), 0, stream, P, NX, NZ,
-                         g.pml, pmask);
+    if (mtot > 0) {
+      LS_CHECK(hipMalloc(&pmask, sizeof(unsigned) * (size_t)mtot));
+      for (size_t w = 0; w < widths.size(); w++) {
+        const Ring R = shot_split(g.pml + 1, widths[w], NZ);
+        if (R.n > 0)
+          hipLaunchKernelGGL(band_terms_kernel, dim3((unsigned)((R.n + 255) / 256)), dim3(256), 0, stream, R,
+                             widths[w], NZ, g.pml, pmask + moff[w]);
+      }
     }
     host.resize(ns);
     const long long origin = (long long)MX * ld + MZ;
     char* sp = reinterpret_cast<char*>(snaps);
     float* sc = scales;
+    T* fp = pool;
+    T* gp = grads;
     for (int s = 0; s < ns; s++) {
       const ShotSpec& q = spec[s];
+      const int nx = NXs[s] - 2 * g.pml;
+      const long long fe = field_elems(NXs[s], ld);
       DevShot<T>& d = host[s];
       std::memset(&d, 0, sizeof(d));
       d.vel = vel + q.model;
       d.wav = wav + q.wav;
       d.gz = gz + q.gz;
-      d.fld = pool + (size_t)fe * LS_NSLOT * s + origin;
+      d.fld = fp + origin;
       d.fstride = fe;
+      fp += fe * LS_NSLOT;
       // the forward passes propagate one field; the band-product slots are
       // free then (the adjoint clears every slot before use)
       d.raw = d.fld + (size_t)fe * LS_AIN;
       d.snap = sp;
       d.scale = sc;
-      d.grad = grads + (size_t)model * s;
+      d.grad = gp;
+      gp += (long long)nx * g.nz;
       d.snap_kind = kind;
-      d.nx = g.nx;
-      d.NX = NX;
+      d.nx = nx;
+      d.NX = NXs[s];
+      d.x0 = q.x0;
       d.nt = q.nt;
       d.ndel = q.ndel;
       d.sx = q.sx;
       d.sz = q.sz;
-      sp += snap_elem() * (size_t)(q.nt + q.ndel) * model;
+      d.pmask = pmask ? pmask + moff[std::lower_bound(widths.begin(), widths.end(), NXs[s]) - widths.begin()]
+                      : nullptr;
+      sp += snap_elem() * (size_t)(q.nt + q.ndel) * nx * g.nz;
       sc += q.nt + q.ndel;
     }
     LS_CHECK(hipMalloc(&dev, sizeof(DevShot<T>) * ns));
@@ -479,22 +505,29 @@ struct LsrtmPlan {
         LsrtmPlan* p = static_cast<LsrtmPlan*>(c);
         launch_step<T>(SNAP_MAX, p->k, p->dev, with_strip(p->F, bx), p->B, p->nshots, 0, p->stream);
       };
-      F = with_strip(F, tuned_strip("lsrtm_forward", NX, NZ, nshots, stream, fn, this));
+      F = with_strip(F, tuned_strip("lsrtm_forward", NXmax, NZ, nshots, stream, fn, this));
     }
     return make_snapshots();
   }
 
-  double cells() const { return (double)NX * NZ; }
-
-  int active_fwd(int st) const {
-    int n = 0;
-    for (const ShotSpec& q : spec) n += st < q.nt + q.ndel;
-    return n;
+  // padded cells and model cells of the shots still running at forward step st
+  double cells_fwd(int st, double* mcells = nullptr) const {
+    double c = 0, m = 0;
+    for (int s = 0; s < nshots; s++)
+      if (st < spec[s].nt + spec[s].ndel) {
+        c += (double)NXs[s] * NZ;
+        m += (double)(NXs[s] - 2 * g.pml) * g.nz;
+      }
+    if (mcells) *mcells = m;
+    return c;
   }
 
   int zero_fields(int first, int count) {
-    for (int s = 0; s < nshots; s++)
-      LS_CHECK(hipMemsetAsync(pool + (size_t)fe * (LS_NSLOT * s + first), 0, sizeof(T) * (size_t)fe * count, stream));
+    for (int s = 0; s < nshots; s++) {
+      const long long fe = host[s].fstride;
+      T* base = host[s].fld - ((long long)MX * ld + MZ);
+      LS_CHECK(hipMemsetAsync(base + (size_t)fe * first, 0, sizeof(T) * (size_t)fe * count, stream));
+    }
     return SEIS_OK;
   }
 
@@ -506,9 +539,10 @@ struct LsrtmPlan {
       int rc = zero_fields(LS_FIELD, 3);
       if (rc) return rc;
       for (int st = 0; st < NTmax; st++) {
-        const int n = active_fwd(st);
-        const double by = 4 * E * n * cells() + (mode == SNAP_STORE ? snap_elem() * n * (double)model : 0.0);
-        timed_launch(stream, mode == SNAP_MAX ? "lsrtm_snap_max" : "lsrtm_snap_store", by, n * cells(),
+        double mc;
+        const double c = cells_fwd(st, &mc);
+        const double by = 4 * E * c + (mode == SNAP_STORE ? snap_elem() * mc : 0.0);
+        timed_launch(stream, mode == SNAP_MAX ? "lsrtm_snap_max" : "lsrtm_snap_store", by, c,
                      [&] { launch_step<T>(mode, k, dev, F, B, nshots, st, stream); });
       }
     }
@@ -520,7 +554,7 @@ struct LsrtmPlan {
   // rec (device, packed like the shot specs) = L pert (device [nx][nz])
   int born(const T* pert, T* rec) {
     for (int s = 0; s < nshots; s++) {
-      host[s].dv = pert;
+      host[s].dv = pert + (long long)spec[s].x0 * g.nz;  // the shot's window of the perturbation
       host[s].rec = rec + spec[s].rec;
     }
     LS_CHECK(hipMemcpyAsync(dev, host.data(), sizeof(DevShot<T>) * nshots, hipMemcpyHostToDevice, stream));
@@ -528,9 +562,10 @@ struct LsrtmPlan {
     if (rc) return rc;
     const double E = sizeof(T);
     for (int st = 0; st < NTmax; st++) {
-      const int n = active_fwd(st);
-      const double by = 4 * E * n * cells() + (snap_elem() + E) * n * (double)model;
-      timed_launch(stream, "lsrtm_born", by, n * cells(), [&] { launch_step<T>(BORN_S, k, dev, F, B, nshots, st, stream); });
+      double mc;
+      const double c = cells_fwd(st, &mc);
+      const double by = 4 * E * c + (snap_elem() + E) * mc;
+      timed_launch(stream, "lsrtm_born", by, c, [&] { launch_step<T>(BORN_S, k, dev, F, B, nshots, st, stream); });
     }
     if (mask) hipLaunchKernelGGL((mul_kernel<T>), dim3(1024), dim3(256), 0, stream, rec, rec, mask, nrec);
     LS_CHECK(hipGetLastError());
@@ -560,15 +595,16 @@ struct LsrtmPlan {
     LS_CHECK(hipMemcpyAsync(dev, host.data(), sizeof(DevShot<T>) * nshots, hipMemcpyHostToDevice, stream));
     int rc = zero_fields(LS_FIELD, 3);
     if (rc) return rc;
-    LS_CHECK(hipMemsetAsync(grads, 0, sizeof(T) * (size_t)model * nshots, stream));
+    LS_CHECK(hipMemsetAsync(grads, 0, sizeof(T) * (size_t)model_cells, stream));
     const double E = sizeof(T);
     for (int st = 0; st < NTmax; st++) {
-      const int n = active_fwd(st);  // shot s runs backward steps st < nt + ndel
-      const double by = 4 * E * n * cells() + (snap_elem() + 2 * E) * n * (double)model;
-      timed_launch(stream, "lsrtm_adjoint_xcorr", by, n * cells(),
+      double mc;
+      const double c = cells_fwd(st, &mc);  // shot s runs backward steps st < nt + ndel
+      const double by = 4 * E * c + (snap_elem() + 2 * E) * mc;
+      timed_launch(stream, "lsrtm_adjoint_xcorr", by, c,
                    [&] { launch_step<T>(ADJ_X, k, dev, F, B, nshots, st, stream); });
     }
-    hipLaunchKernelGGL((sum_grads_kernel<T>), dim3(256), dim3(256), 0, stream, dev, nshots, model, grad);
+    hipLaunchKernelGGL((sum_grads_kernel<T>), dim3(256), dim3(256), 0, stream, dev, nshots, g.nz, model, grad);
     LS_CHECK(hipGetLastError());
     LS_CHECK(hipStreamSynchronize(stream));
     return SEIS_OK;
@@ -592,26 +628,30 @@ struct LsrtmPlan {
         hipLaunchKernelGGL((adj_step_kernel<T>), dim3(a.nfwg + a.nbwg, p->nshots), dim3(AWG), 0, p->stream, p->k,
                            p->dev, a, 0);
       };
-      A = with_strip(A, tuned_strip("lsrtm_adjoint", NX, NZ, nshots, stream, fn, this));
+      A = with_strip(A, tuned_strip("lsrtm_adjoint", NXmax, NZ, nshots, stream, fn, this));
       adj_tuned = true;
     }
     int rc = zero_fields(0, LS_NSLOT);
     if (rc) return rc;
-    LS_CHECK(hipMemsetAsync(grads, 0, sizeof(T) * (size_t)model * nshots, stream));
+    LS_CHECK(hipMemsetAsync(grads, 0, sizeof(T) * (size_t)model_cells, stream));
     const double E = sizeof(T);
-    const int pwg = (int)((P.n + 255) / 256);
     for (int st = 0; st <= NTmax; st++) {
-      int n = 0;
-      for (const ShotSpec& q : spec) n += (q.nt + q.ndel + 1 - st) >= 1;
-      timed_launch(stream, "lsrtm_adjoint_prep", 8 * E * n * (double)P.n, 0, [&] {
-        hipLaunchKernelGGL((adj_prep_kernel<T>), dim3(pwg, nshots), dim3(256), 0, stream, k, dev, P, pmask, st);
+      double pc = 0, c = 0, mc = 0;
+      for (int s = 0; s < nshots; s++)
+        if ((spec[s].nt + spec[s].ndel + 1 - st) >= 1) {
+          pc += (double)shot_split(g.pml + 1, NXs[s], NZ).n;
+          c += (double)NXs[s] * NZ;
+          mc += (double)(NXs[s] - 2 * g.pml) * g.nz;
+        }
+      timed_launch(stream, "lsrtm_adjoint_prep", 8 * E * pc, 0, [&] {
+        hipLaunchKernelGGL((adj_prep_kernel<T>), dim3(P.nbwg, nshots), dim3(256), 0, stream, k, dev, st);
       });
-      const double by = 4 * E * n * cells() + (snap_elem() + 2 * E) * n * (double)model;
-      timed_launch(stream, "lsrtm_adjoint", by, n * cells(), [&] {
+      const double by = 4 * E * c + (snap_elem() + 2 * E) * mc;
+      timed_launch(stream, "lsrtm_adjoint", by, c, [&] {
         hipLaunchKernelGGL((adj_step_kernel<T>), dim3(A.nfwg + A.nbwg, nshots), dim3(AWG), 0, stream, k, dev, A, st);
       });
     }
-    hipLaunchKernelGGL((sum_grads_kernel<T>), dim3(256), dim3(256), 0, stream, dev, nshots, model, grad);
+    hipLaunchKernelGGL((sum_grads_kernel<T>), dim3(256), dim3(256), 0, stream, dev, nshots, g.nz, model, grad);
     LS_CHECK(hipGetLastError());
     LS_CHECK(hipStreamSynchronize(stream));
     return SEIS_OK;

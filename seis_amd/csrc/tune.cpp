@@ -76,6 +76,17 @@ int tuned_strip(const char* kind, int NX, int NZ, int nshots, hipStream_t stream
   return best;
 }
 
+int strip_lookup(const char* kind, int NX, int NZ, int nshots) {
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (g_on) {
+    auto it = g_table.find(std::make_tuple(std::string(kind), dev, NX, NZ, nshots));
+    if (it != g_table.end()) return it->second;
+  }
+  return kDefaultStrip;
+}
+
 }  // namespace seis
 
 using namespace seis;

@@ -19,10 +19,17 @@ ADJ_EXACT, ADJ_XCORR = 0, 1  # exact transpose / the reference's cross-correlati
 
 
 class LsrtmPlan:
+    """A plan over shots of one model vel [nx][nz]; a shot with Shot.nx > 0
+    propagates on its window, columns [Shot.x0, Shot.x0 + Shot.nx) of the
+    model, and its records are [Shot.nx][nt]."""
+
     def __init__(self, grid, shots, vel, wavs, gzs, snapshot=SNAP_BF16, fp64=False):
         self.grid, self.shots, self.fp64 = grid, shots, fp64
         self.dtype = np.float64 if fp64 else np.float32
         arr, nw, ng, nr = _layout(grid, shots)
+        # every shot's velocity is its window [x0, x0 + nx) of the one model
+        for j, s in enumerate(shots):
+            arr[j].model = s.x0 * grid.nz
         self._arr = arr
         self.nrec = nr
         self.model = (grid.nx, grid.nz)

@@ -88,3 +88,13 @@ def salt_body(v, dx, dz, speed=4500.0, seed=0):
     out = np.array(v, np.float32, copy=True)
     out[np.hypot(x, z) < r] = speed
     return out
+
+
+def spread_window(nx, sx, first, last, pad):
+    """Model window of one shot: columns [x0, x0 + n) covering the source at
+    column sx and receivers at columns first..last, widened by `pad` columns
+    on each side and clipped to the model's nx columns (the aperture of
+    getVel, lsrtm/inversion.cpp:552-598, without its edge replication)."""
+    lo = max(0, min(sx, first) - pad)
+    hi = min(nx - 1, max(sx, last) + pad)
+    return lo, hi - lo + 1
