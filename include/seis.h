@@ -177,6 +177,21 @@ int seis_modeling_batch(const seis_grid* grid, int nshots, const seis_shot* shot
                         const float* wav, const int* gz, float* rec);
 int seis_born_batch(const seis_grid* grid, int nshots, const seis_shot* shots, const float* vel,
                     const float* dv, const float* wav, const int* gz, float* rec);
+/* seis_modeling_batch without device allocation, for callers that cannot
+ * call hipMalloc (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid allocation).  No counterpart in the reference.
+ * Same arguments and results as seis_modeling_batch; the wavefields and shot
+ * descriptors live in a static device scratch of 64 MiB per GPU (held until
+ * the process ends), which holds 4 padded fields per shot, e.g. about 48
+ * shots of 300 x 200 cells with pml 14 at once; larger batches run in
+ * consecutive chunks of the scratch.  A shot whose fields alone exceed the
+ * scratch is SEIS_ERR_ARG.  Every call on a GPU shares that GPU's scratch:
+ * calls that overlap on one GPU (other host threads) race.  The strip length
+ * is the one the tuner chose earlier for the configuration, else 16; no
+ * profiler events are recorded. */
+int seis_modeling_batch_static(const seis_grid* grid, int nshots, const seis_shot* shots, const float* vel,
+                               const float* wav, const int* gz, float* rec);
+
 /* img: nshots consecutive [nx][nz] images, one per shot in shot order, each
  * of its shot's width (the per-shot CSG_IMAGE<is> of inversion.cpp:434-439). */
 int seis_rtm_batch(const seis_grid* grid, int nshots, const seis_shot* shots, const float* vel,

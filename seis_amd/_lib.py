@@ -55,6 +55,7 @@ _SIGS = {
     "seis_rtm_plane_batch": (_I, [ctypes.POINTER(seis_grid), _I, ctypes.POINTER(seis_shot), _P, _P, _P, _P, _P]),
     "seis_illum_batch": (_I, [ctypes.POINTER(seis_grid), _I, ctypes.POINTER(seis_shot), _P, _P, _P]),
     "seis_modeling_batch": (_I, [ctypes.POINTER(seis_grid), _I, ctypes.POINTER(seis_shot), _P, _P, _P, _P]),
+    "seis_modeling_batch_static": (_I, [ctypes.POINTER(seis_grid), _I, ctypes.POINTER(seis_shot), _P, _P, _P, _P]),
     "seis_born_batch": (_I, [ctypes.POINTER(seis_grid), _I, ctypes.POINTER(seis_shot), _P, _P, _P, _P, _P]),
     "seis_rtm_batch": (_I, [ctypes.POINTER(seis_grid), _I, ctypes.POINTER(seis_shot), _P, _P, _P, _P, _P]),
     "seis_rtm_batch_ex": (_I, [ctypes.POINTER(seis_grid), _I, ctypes.POINTER(seis_shot), _P, _P, _P, _P, _P, _I]),

@@ -1362,11 +1362,60 @@ int run_rtm_plane(const GridSpec& g, int nshots, const ShotSpec* shots, const T*
 template int run_rtm_plane<float>(const GridSpec&, int, const ShotSpec*, const float*, const float*, const int*,
                                   const float*, float*, hipStream_t);
 
+// Device scratch of seis_modeling_batch_static: fields and shot descriptors
+// of a modeling batch, for callers that cannot allocate device memory.  One
+// copy per GPU, held until the process ends.  In host code `scratch` names a
+// host-side stand-in; its device address comes from hipGetSymbolAddress.
+constexpr size_t kStaticScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};  // device address of `scratch`, per device
+}  // namespace
+
+// Device address of the current device's scratch, looked up once per device
+// (with that device current) on its first use.
+int static_scratch(void** out) {
+  int dev = 0;
+  HIPCHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return set_error("static scratch: device index out of range"), SEIS_ERR_ARG;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    HIPCHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return SEIS_OK;
+}
+
 }  // namespace seis
 
 using namespace seis;
 
 extern "C" {
+
+int seis_modeling_batch_static(const seis_grid* grid, int nshots, const seis_shot* shots, const float* vel,
+                               const float* wav, const int* gz, float* rec) {
+  if (!grid || nshots < 0 || (nshots > 0 && !shots)) return set_error("bad grid or shot list"), SEIS_ERR_ARG;
+  int rc = check_grid(*grid);
+  if (rc) return rc;
+  if ((rc = check_shots(RUN_MODEL, *grid, nshots, shots, 0))) return rc;
+  for (int s = 0; s < nshots; s++) {
+    const size_t b = shot_bytes(RUN_MODEL, *grid, shots[s], sizeof(float), sizeof(DevShot<float>)) + 256;
+    if (b > kStaticScratchBytes)
+      return set_error("seis_modeling_batch_static: shot " + std::to_string(s) + " needs " + std::to_string(b) +
+                       " bytes of scratch, more than the " + std::to_string(kStaticScratchBytes) +
+                       " bytes of the static scratch"),
+             SEIS_ERR_ARG;
+  }
+  void* p = nullptr;
+  if ((rc = static_scratch(&p))) return rc;
+  const Workspace ws{p, kStaticScratchBytes};
+  return run_batch<float>(RUN_MODEL, *grid, nshots, shots, vel, nullptr, wav, gz, rec, nullptr, 0, 0, 0, &ws);
+}
 
 int seis_set_batch_limits(size_t bytes, int shots) {
   if (shots < 0) return set_error("seis_set_batch_limits: shots must be >= 0"), SEIS_ERR_ARG;
