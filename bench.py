@@ -1,25 +1,38 @@
 """Benchmark of the MI355X acoustic finite-difference engine.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--shots S]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--shots-per-step S]
 
-Default workload: reference-semantics RTM of a batch of 96 shots on a seeded
-layered-and-faulted model of 751 x 2301 cells at 4 m (pml 30 -> 811 x 2361
-padded cells), dt = 0.4 ms, nt = 5000, Ricker 12 Hz, migration velocity 10x
-smoother than the true model.  The input records are forward-modeled from the
-true model on the GPU before timing (inputs resident in HBM when the timed
-region starts).  A step is one RTM of the whole batch: forward propagation
-with boundary-strip saving, then source reconstruction + receiver
-back-propagation + imaging, i.e. 3 wavefield propagations per shot.
+Headline workload: the large batched RTM survey.  2000 x 8000 cells @ 5 m,
+pml 30, a seeded layered-and-faulted model (1500-5500 m/s, 40 dipping layers,
+3 normal faults) with a 4500 m/s salt body inside a random smooth closed
+contour; migration velocity 20 cells smoother.  512 shots between columns 500
+and 7500, each recorded by 1000 receivers (every column from sx - 500 to
+sx + 499, depth 10 m), dt = 0.5 ms, nt = 6000, Ricker 12 Hz.  Each shot runs
+on its own window of the model, its spread widened by 50 columns on each side
+(1100 columns, 1160 x 2060 padded cells), as the reference's getVel windows
+every shot (lsrtm/inversion.cpp:552-598).  The records are forward-modeled
+from the true model on the GPU before timing, so every input is resident in
+HBM when the timed region starts.
+
+A step is the reference-semantics RTM of `--shots-per-step` consecutive shots
+of the survey at the full record length: forward propagation saving the
+boundary strips, then source reconstruction, receiver back-propagation and
+the imaging condition with FaQi, i.e. 3 wavefield propagations per shot, all
+shots of the step in the same launches.  Step k takes shots S k .. S k + S - 1
+(mod 512): with the default 26 shots per step the 20 timed steps image the
+whole survey once.
 
 Prints one JSON line: Gcell-updates/s over the timed steps, the dominant
 kernel's HBM roofline fraction (HIP-event durations on the launch stream and
-algorithmic bytes), and the reference's CPU path timed on this host on a
-bounded sample of the same workload.
+algorithmic bytes), the reference's CPU path timed on this host on a bounded
+sample of the same workload, and the other workloads as extra objects.
 """
 import argparse
 import contextlib
+import ctypes
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -30,55 +43,11 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Gcell-updates/s (batched shots) + LSRTM s/CG-iter vs plain FP32 at stated error"
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of each kernel for the 96-shot workload, from
-# rocprofv3 FETCH_SIZE / WRITE_SIZE (calibrated against a 1 GiB copy; the
-# commands and counter values are in profiles/rtm_batch_hbm_traffic.md).
-# Scaled by the number of shots in a launch.
-MEASURED_TRAFFIC_96 = {"step_rtm_forward": 4.474e+09, "step_rtm_reverse": 8.846e+09, "rtm_image": 4.291e+09}
-
-
-def rtm_workload(nshots, nt=5000):
-    from seis_amd import models
-    nx, nz, pml = 2301, 751, 30
-    dx = dz = 4.0
-    dt, fr = 0.0004, 12.0
-    vtrue = models.layered_faulted(nx, nz, dx, dz, seed=1001)
-    vmig = models.smooth_model(vtrue, 20.0)
-    delay = int(round(1.0 / (fr * dt)))
-    sxs = [12 + 24 * i for i in range(96)][:nshots]
-    return dict(nx=nx, nz=nz, pml=pml, dx=dx, dz=dz, dt=dt, nt=nt, fr=fr, delay=delay, vtrue=vtrue, vmig=vmig,
-                sxs=sxs, sz=2, gz=2, spread=240, rstep=3)
-
-
-def spread_mask(w, sx):
-    cols = np.arange(w["nx"])
-    off = cols - sx
-    return (np.abs(off) <= w["spread"] * w["rstep"]) & (off % w["rstep"] == 0)
-
-
-def cpu_baseline(w, nt_sample, threads):
-    """Reference RTM (compiled reference engine if present, else the oracle's
-    restatement) of one shot with a shortened record, on `threads` host cores."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_lib
-    kind = "reference" if oracle_lib.have_reference() else "port"
-    lib = oracle_lib.Reference() if kind == "reference" else oracle_lib.Oracle()
-    lib.set_threads(threads)
-    nx, nz, pml = w["nx"], w["nz"], w["pml"]
-    wav = lib.ricker(w["dt"], w["fr"], w["delay"], nt_sample)
-    gz = np.full(nx, w["gz"], np.int32)
-    rec = np.zeros((nx, nt_sample), np.float32)
-    sx = w["sxs"][len(w["sxs"]) // 2]
-    rec[spread_mask(w, sx)] = np.random.default_rng(5).standard_normal((int(spread_mask(w, sx).sum()), nt_sample))
-    t0 = time.perf_counter()
-    with stdout_to_stderr():
-        lib.rtm(w["dt"], w["dx"], w["dz"], nt_sample, 0, nx, nz, pml, sx, w["sz"], gz, wav, w["vmig"], rec)
-    el = time.perf_counter() - t0
-    cells = (nx + 2 * pml) * (nz + 2 * pml)
-    updates = 3.0 * nt_sample * cells
-    return dict(value=updates / el / 1e9, unit="Gcell-updates/s", cores=threads, kind=kind,
-                sample="one shot of the same RTM workload with nt=%d (%.1f s of CPU work)" % (nt_sample, el),
-                cpu_model=host_cpu_model())
+# HBM bytes per launch of the headline's kernels (26 shots per launch), from
+# rocprofv3 FETCH_SIZE / WRITE_SIZE calibrated against a 1 GiB copy; the
+# commands and counter values are in profiles/c5_rtm_hbm_traffic.md.  Scaled
+# by the shots of a launch.
+MEASURED_TRAFFIC = {"shots": 26, "bytes": {}}
 
 
 def host_cpu_model():
@@ -112,6 +81,252 @@ def stdout_to_stderr():
         os.close(saved)
 
 
+def c5_survey(nt=6000):
+    """The headline survey: model, migration model and per-shot windows."""
+    from seis_amd import models
+    nx, nz, pml, dx, dt, fr = 8000, 2000, 30, 5.0, 0.0005, 12.0
+    t0 = time.perf_counter()
+    v = models.salt_body(models.layered_faulted(nx, nz, dx, dx, seed=1005), dx, dx, seed=1005)
+    vm = models.smooth_model(v, 20.0)
+    log("C5 model built in %.1f s" % (time.perf_counter() - t0))
+    sxs = [500 + int(round(i * 7000 / 511)) for i in range(512)]
+    wins = [models.spread_window(nx, sx, sx - 500, sx + 499, 50) for sx in sxs]
+    return dict(nx=nx, nz=nz, pml=pml, dx=dx, dt=dt, fr=fr, nt=nt, delay=int(round(1.0 / (fr * dt))), v=v, vm=vm,
+                sxs=sxs, wins=wins, sz=2, gz=2)
+
+
+def window_shots(grid, sxs, wins, nts, sz=2):
+    """Shots on their windows [x0, x0 + nxw) of one model (views: model offset
+    x0 nz), the packed arrays' offsets in shot order."""
+    from seis_amd.batch import Shot, _layout
+    shots = [Shot(sx=sx - x0, sz=sz, nt=int(nt), nx=n, x0=x0) for sx, (x0, n), nt in zip(sxs, wins, nts)]
+    arr, nw, ng, nr = _layout(grid, shots, views=True)
+    return shots, arr, nw, ng, nr
+
+
+def sub_array(arr, idx):
+    """seis_shot array of the shots idx of arr (offsets kept)."""
+    from seis_amd._lib import seis_shot
+    out = (seis_shot * len(idx))()
+    for j, i in enumerate(idx):
+        out[j] = arr[i]
+    return out
+
+
+def cpu_info():
+    """lscpu facts of this host and the threads the CPU baseline may use."""
+    info = {"model": host_cpu_model()}
+    try:
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in txt.splitlines():
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "Model name":
+                info["model"] = v
+            elif k == "Socket(s)":
+                info["sockets"] = int(v)
+            elif k == "Core(s) per socket":
+                info["cores_per_socket"] = int(v)
+            elif k == "Thread(s) per core":
+                info["threads_per_core"] = int(v)
+            elif k == "CPU(s)":
+                info["cpus"] = int(v)
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    aff = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS")
+    info["affinity_cpus"] = aff
+    info["threads_used"] = min(aff, int(omp)) if omp and omp.isdigit() else aff
+    return info
+
+
+def cpu_baseline(w, nt_sample, threads):
+    """Reference RTM (the compiled reference engine if present, else the
+    oracle's restatement) of one windowed shot of the headline survey with a
+    shortened record, on `threads` host cores."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+    kind = "reference" if oracle_lib.have_reference() else "port"
+    lib = oracle_lib.Reference() if kind == "reference" else oracle_lib.Oracle()
+    lib.set_threads(threads)
+    j = len(w["sxs"]) // 2
+    x0, n = w["wins"][j]
+    sx = w["sxs"][j] - x0
+    wav = lib.ricker(w["dt"], w["fr"], w["delay"], nt_sample)
+    gz = np.full(n, w["gz"], np.int32)
+    rec = np.random.default_rng(5).standard_normal((n, nt_sample)).astype(np.float32)
+    vel = np.ascontiguousarray(w["vm"][x0:x0 + n])
+    t0 = time.perf_counter()
+    with stdout_to_stderr():
+        lib.rtm(w["dt"], w["dx"], w["dx"], nt_sample, 0, n, w["nz"], w["pml"], sx, w["sz"], gz, wav, vel, rec)
+    el = time.perf_counter() - t0
+    cells = (n + 2 * w["pml"]) * (w["nz"] + 2 * w["pml"])
+    updates = 3.0 * nt_sample * cells
+    return dict(value=updates / el / 1e9, unit="Gcell-updates/s", cores=threads, kind=kind,
+                sample="RTM of one shot of the same survey on its %d-column window with nt=%d (%.1f s of CPU work)"
+                       % (n, nt_sample, el))
+
+
+def c2_rtm_batch(nt=5000):
+    """The 751 x 2301 @ 4 m RTM workload: 96 shots every 96 m, 480 receivers
+    at 12 m (every third column, split spread), each shot on its window
+    (spread + 50 columns), dt = 0.4 ms, nt = 5000, Ricker 12 Hz, one batch."""
+    import seis_amd
+    from seis_amd import DeviceArray, models
+    from seis_amd.batch import Grid, RTM_ISOLATE_BAND
+    from seis_amd._lib import check
+    L = seis_amd.lib()
+    nx, nz, pml, dx, dt, fr = 2301, 751, 30, 4.0, 0.0004, 12.0
+    vtrue = models.layered_faulted(nx, nz, dx, dx, seed=1001)
+    vmig = models.smooth_model(vtrue, 20.0)
+    delay = int(round(1.0 / (fr * dt)))
+    sxs = [12 + 24 * i for i in range(96)]
+    wins = []
+    for sx in sxs:
+        lo, hi = max(0, sx - 720), min(nx - 1, sx + 720)
+        wins.append(models.spread_window(nx, sx, lo, hi, 50))
+    grid = Grid(nx, nz, pml, dt, dx, dx)
+    g = grid.c()
+    shots, arr, nw, ng, nr = window_shots(grid, sxs, wins, [nt] * 96)
+    d_vt, d_vm = DeviceArray.from_host(vtrue), DeviceArray.from_host(vmig)
+    d_wav = DeviceArray.from_host(np.tile(seis_amd.ricker(dt, fr, delay, nt), 96))
+    d_gz = DeviceArray.from_host(np.full(ng, 2, np.int32))
+    d_rec = DeviceArray((nr,), np.float32, zero=True)
+    nimg = sum(n for _, n in wins) * nz
+    d_img = DeviceArray((nimg,), np.float32, zero=True)
+    check(L.seis_modeling_batch(ctypes.byref(g), 96, arr, d_vt.ptr, d_wav.ptr, d_gz.ptr, d_rec.ptr))
+    # receivers: every third column of the spread; the pad columns record nothing
+    rec = d_rec.download()
+    off = 0
+    for sx, (x0, n) in zip(sxs, wins):
+        r = rec[off:off + n * nt].reshape(n, nt)
+        cols = np.arange(x0, x0 + n)
+        r[~((np.abs(cols - sx) <= 720) & ((cols - sx) % 3 == 0))] = 0.0
+        off += n * nt
+    d_rec.upload(rec)
+    del rec
+    seis_amd.synchronize()
+    seis_amd.profile_start()
+    t0 = time.perf_counter()
+    check(L.seis_rtm_batch_ex(ctypes.byref(g), 96, arr, d_vm.ptr, d_wav.ptr, d_gz.ptr, d_rec.ptr, d_img.ptr,
+                              RTM_ISOLATE_BAND))
+    seis_amd.synchronize()
+    el = time.perf_counter() - t0
+    st = seis_amd.profile_stop()
+    cells = sum((n + 2 * pml) * (nz + 2 * pml) for _, n in wins)
+    img = d_img.download()
+    log("C2: RTM of 96 windowed shots in %.1f s" % el)
+    return {"workload": "rtm_batch: 96 shots every 96 m on 751x2301 cells @ 4 m, 480 receivers @ 12 m per shot, "
+                        "each shot on its window (spread + 50 columns, %d..%d columns), pml 30, nt=%d, dt=0.4 ms"
+                        % (min(n for _, n in wins), max(n for _, n in wins), nt),
+            "s": round(el, 3), "s_per_shot": round(el / 96, 4),
+            "gcell_updates_per_s": round(3.0 * nt * cells / el / 1e9, 1),
+            "kernels": {k: {"launches": v["launches"], "ms": round(v["total_ms"], 1),
+                            "GB/s": round(v["alg_bytes"] / max(v["total_ms"], 1e-9) / 1e6, 1)} for k, v in st.items()},
+            "image_finite": bool(np.isfinite(img).all())}
+
+
+def c3_born_adjoint(nshots, pair_shots, seed=1003):
+    """Batched Born modeling with uneven shots: 500 x 1600 cells @ 5 m,
+    pml 30, dt = 0.5 ms, record lengths drawn in 2000..4000 and spreads of
+    200..1200 receivers (every column, centred on the source), each shot on
+    its window (spread + 50 columns), a sparse spike perturbation (2 %
+    density, N(0,1)), every shot in one batch.  Then the BF16-snapshot Born /
+    exact-adjoint pair on `pair_shots` of them with the dot-product test, and
+    the same test in FP64 on 4."""
+    import seis_amd
+    from seis_amd import DeviceArray, models
+    from seis_amd.batch import Grid
+    from seis_amd._lib import check
+    L = seis_amd.lib()
+    nx, nz, pml, dx, dt, fr = 1600, 500, 30, 5.0, 0.0005, 15.0
+    rng = np.random.default_rng(seed)
+    # 1500-4500 m/s keeps v dt / dx at 0.45, inside the 8th-order scheme's
+    # stability limit (0.55) with a margin
+    v = models.layered_faulted(nx, nz, dx, dx, vmax=4500.0, seed=seed)
+    dv = models.sparse_spikes(nx, nz, 0.02, seed)
+    delay = int(round(1.0 / (fr * dt)))
+    nts = rng.integers(2000, 4001, nshots)
+    ngs = rng.integers(200, 1201, nshots)
+    sxs = np.linspace(5, nx - 6, nshots).astype(int)
+    wins = [models.spread_window(nx, int(sx), int(sx) - int(ng) // 2, int(sx) + (int(ng) - 1) // 2, 50)
+            for sx, ng in zip(sxs, ngs)]
+    grid = Grid(nx, nz, pml, dt, dx, dx)
+    g = grid.c()
+    shots, arr, nw, ng_off, nr = window_shots(grid, [int(s) for s in sxs], wins, nts)
+    wav = np.concatenate([seis_amd.ricker(dt, fr, delay, int(n)) for n in nts])
+    d_v, d_dv = DeviceArray.from_host(v), DeviceArray.from_host(dv)
+    d_wav = DeviceArray.from_host(wav)
+    d_gz = DeviceArray.from_host(np.full(ng_off, 2, np.int32))
+    d_rec = DeviceArray((nr,), np.float32, zero=True)
+    cells = [(n + 2 * pml) * (nz + 2 * pml) for _, n in wins]
+    out = {"workload": "born_batch: %d shots in one batch, 500x1600 cells @ 5 m, pml 30, dt=0.5 ms, nt in "
+                       "2000..4000, 200..1200 receivers, each shot on its window (%d..%d columns), 2%% sparse spike "
+                       "perturbation" % (nshots, min(n for _, n in wins), max(n for _, n in wins)),
+           "nt_range": [int(nts.min()), int(nts.max())]}
+    # (the strip-length trial of this configuration runs inside: 12 launches)
+    t0 = time.perf_counter()
+    check(L.seis_born_batch(ctypes.byref(g), nshots, arr, d_v.ptr, d_dv.ptr, d_wav.ptr, d_gz.ptr, d_rec.ptr))
+    seis_amd.synchronize()
+    el = time.perf_counter() - t0
+    out["born_s"] = round(el, 3)
+    out["s_per_shot"] = round(el / nshots, 4)
+    out["born_gcell_updates_per_s"] = round(2.0 * sum(c * float(n) for c, n in zip(cells, nts)) / el / 1e9, 1)
+    log("C3: Born batch of %d windowed shots in %.1f s" % (nshots, el))
+    del d_rec
+    # the Born / adjoint pair with BF16 snapshots on a subset (receivers: the
+    # spread columns of each window)
+    from seis_amd.lsrtm import LsrtmPlan, SNAP_BF16
+    sub = list(range(pair_shots))
+    sshots = [shots[i] for i in sub]
+    mask = []
+    for i in sub:
+        x0, n = wins[i]
+        cols = np.arange(x0, x0 + n)
+        m = np.zeros((n, int(nts[i])), np.float32)
+        m[np.abs(cols - sxs[i]) <= int(ngs[i]) // 2] = 1.0
+        mask.append(m.ravel())
+    mask = np.concatenate(mask)
+    t0 = time.perf_counter()
+    plan = LsrtmPlan(grid, sshots, v, [seis_amd.ricker(dt, fr, delay, s.nt) for s in sshots],
+                     [np.full(s.nx, 2, np.int32) for s in sshots], snapshot=SNAP_BF16)
+    plan.set_mask(mask)
+    seis_amd.synchronize()
+    out["pair_snapshot_s"] = round(time.perf_counter() - t0, 3)
+    r = DeviceArray.from_host((rng.standard_normal(plan.nrec) * mask).astype(np.float32))
+    t0 = time.perf_counter()
+    lm = plan.born(d_dv)
+    t1 = time.perf_counter()
+    ltr = plan.adjoint(r)
+    t2 = time.perf_counter()
+    rr = r.download()
+    a, b = float(np.dot(lm.astype(np.float64), rr)), float(np.sum(dv.astype(np.float64) * ltr))
+    rel = abs(a - b) / max(abs(a), abs(b), 1e-300)
+    reln = abs(a - b) / max(float(np.linalg.norm(lm) * np.linalg.norm(rr)), 1e-300)
+    plan.close()
+    pair_cells = sum(cells[i] * float(nts[i]) for i in sub)
+    out["pair_shots"] = pair_shots
+    out["pair_born_gcell_updates_per_s"] = round(pair_cells / (t1 - t0) / 1e9, 1)
+    out["pair_adjoint_gcell_updates_per_s"] = round(pair_cells / (t2 - t1) / 1e9, 1)
+    # |<L m, r> - <m, L^T r>| relative to the value and to ||L m|| ||r||: a
+    # random r makes the two products cancel to ~1/sqrt(n) of the norms, so
+    # FP32 rounding shows up far larger against the value
+    out["dot_test_fp32_rel"] = float("%.3e" % rel)
+    out["dot_test_fp32_rel_to_norms"] = float("%.3e" % reln)
+    sub64 = sshots[:4]
+    p64 = LsrtmPlan(grid, sub64, v.astype(np.float64), [seis_amd.ricker(dt, fr, delay, s.nt) for s in sub64],
+                    [np.full(s.nx, 2, np.int32) for s in sub64], fp64=True)
+    m64 = dv.astype(np.float64)
+    r64 = np.random.default_rng(seed + 1).standard_normal(p64.nrec) * mask[:p64.nrec]
+    a, b = float(np.dot(p64.born(m64), r64)), float(np.sum(m64 * p64.adjoint(r64)))
+    p64.close()
+    out["dot_test_fp64_rel"] = float("%.3e" % (abs(a - b) / max(abs(a), abs(b), 1e-300)))
+    out["dot_test_fp64_shots"] = 4
+    out["dot_test_note"] = ("FP32: FP32 propagation, BF16 snapshots, FP64 inner products; FP64: every step in FP64 "
+                            "(the stated bar, <= 1e-6)")
+    log("C3: pair on %d shots; dot test FP32 %.2e (%.2e of the norms), FP64 %s" % (pair_shots, rel, reln,
+                                                                                   out["dot_test_fp64_rel"]))
+    return out
 def c1_single_shot(calls, with_cpu, threads):
     """The reference's CPU-runnable case: one-shot forward modeling of the
     3-layer model, 201 x 301 cells @ 10 m, pml 30, nt = 1500, through the
@@ -148,164 +363,6 @@ def c1_single_shot(calls, with_cpu, threads):
         out["cpu"] = {"kind": kind, "cores": threads}
         out["equals_cpu"] = bool(np.array_equal(ref, rec))
     return out
-
-
-def c3_born_adjoint(nshots, pair_shots, seed=1003):
-    """Batched Born modeling with uneven shots: 500 x 1600 cells @ 5 m,
-    pml 30, dt = 0.5 ms, record lengths drawn in 2000..4000 and spreads of
-    200..1200 receivers, a sparse spike perturbation (2 % density, N(0,1)),
-    every shot in one batch.  Then the BF16-snapshot Born / exact-adjoint pair
-    on the first `pair_shots` shots with its FP64 dot-product test."""
-    import ctypes
-    import seis_amd
-    from seis_amd import DeviceArray, models
-    from seis_amd.batch import _layout, Grid, Shot
-    from seis_amd._lib import check
-    L = seis_amd.lib()
-    nx, nz, pml, dx, dt, fr = 1600, 500, 30, 5.0, 0.0005, 15.0
-    rng = np.random.default_rng(seed)
-    # 1500-4500 m/s keeps v dt / dx at 0.45, inside the 8th-order scheme's
-    # stability limit (0.55) with a margin
-    v = models.layered_faulted(nx, nz, dx, dx, vmax=4500.0, seed=seed)
-    dv = models.sparse_spikes(nx, nz, 0.02, seed)
-    delay = int(round(1.0 / (fr * dt)))
-    nts = rng.integers(2000, 4001, nshots)
-    ngs = rng.integers(200, 1201, nshots)
-    sxs = np.linspace(5, nx - 6, nshots).astype(int)
-    shots = [Shot(sx=int(sx), sz=2, nt=int(n)) for sx, n in zip(sxs, nts)]
-    grid = Grid(nx, nz, pml, dt, dx, dx)
-    g = grid.c()
-    arr, nw, ngz, nr = _layout(grid, shots)
-    wav = np.concatenate([seis_amd.ricker(dt, fr, delay, int(n)) for n in nts])
-    d_v, d_dv = DeviceArray.from_host(v), DeviceArray.from_host(dv)
-    d_wav = DeviceArray.from_host(wav)
-    d_gz = DeviceArray.from_host(np.full(nx * nshots, 2, np.int32))
-    d_rec = DeviceArray((nr,), np.float32, zero=True)
-    cells = (nx + 2 * pml) * (nz + 2 * pml)
-    out = {"workload": "born_batch: %d shots in one batch, 500x1600 cells @ 5 m, pml 30, dt=0.5 ms, nt in "
-                       "2000..4000, 200..1200 receivers, 2%% sparse spike perturbation" % nshots,
-           "nt_range": [int(nts.min()), int(nts.max())]}
-    # (the strip-length trial of this configuration runs inside: 12 launches)
-    t0 = time.perf_counter()
-    check(L.seis_born_batch(ctypes.byref(g), nshots, arr, d_v.ptr, d_dv.ptr, d_wav.ptr, d_gz.ptr, d_rec.ptr))
-    seis_amd.synchronize()
-    el = time.perf_counter() - t0
-    out["born_s"] = round(el, 3)
-    out["born_gcell_updates_per_s"] = round(2.0 * cells * float(nts.sum()) / el / 1e9, 1)
-    log("C3: Born batch of %d shots in %.1f s" % (nshots, el))
-    del d_rec
-    # the Born / adjoint pair with BF16 snapshots on a subset
-    sub = shots[:pair_shots]
-    sarr, snw, sng, snr = _layout(grid, sub)
-    mask = np.zeros(snr, np.float32)
-    off = 0
-    for s, ng in zip(sub, ngs[:pair_shots]):
-        lo, hi = max(0, s.sx - int(ng) // 2), min(nx, s.sx + int(ng) // 2)
-        mask[off:off + nx * s.nt].reshape(nx, s.nt)[lo:hi] = 1.0
-        off += nx * s.nt
-    sw = DeviceArray.from_host(np.concatenate([seis_amd.ricker(dt, fr, delay, s.nt) for s in sub]))
-    sgz = DeviceArray.from_host(np.full(nx * pair_shots, 2, np.int32))
-    plan = ctypes.c_void_p()
-    t0 = time.perf_counter()
-    check(L.seis_lsrtm_create(ctypes.byref(plan), ctypes.byref(g), pair_shots, sarr, d_v.ptr, sw.ptr, sgz.ptr, 1))
-    dmask = DeviceArray.from_host(mask)
-    check(L.seis_lsrtm_set_mask(plan, dmask.ptr))
-    seis_amd.synchronize()
-    out["pair_snapshot_s"] = round(time.perf_counter() - t0, 3)
-    r = DeviceArray.from_host((rng.standard_normal(snr) * mask).astype(np.float32))
-    lm = DeviceArray((snr,), np.float32, zero=True)
-    ltr = DeviceArray((nx, nz), np.float32, zero=True)
-    t0 = time.perf_counter()
-    check(L.seis_lsrtm_born(plan, d_dv.ptr, lm.ptr))
-    seis_amd.synchronize()
-    t1 = time.perf_counter()
-    check(L.seis_lsrtm_adjoint(plan, r.ptr, ltr.ptr))
-    seis_amd.synchronize()
-    t2 = time.perf_counter()
-    def dot(pl, x, y, n):
-        o = ctypes.c_double()
-        check(L.seis_lsrtm_dot(pl, x.ptr, y.ptr, n, ctypes.byref(o)))
-        return o.value
-
-    def dot_test(pl, m, lm, r, ltr, nrec):
-        a, b = dot(pl, lm, r, nrec), dot(pl, m, ltr, nx * nz)
-        norms = np.sqrt(dot(pl, lm, lm, nrec) * dot(pl, r, r, nrec))
-        return abs(a - b) / max(abs(a), abs(b), 1e-300), abs(a - b) / max(norms, 1e-300)
-
-    rel, reln = dot_test(plan, d_dv, lm, r, ltr, snr)
-    check(L.seis_lsrtm_destroy(plan))
-    pair_cells = cells * float(sum(s.nt for s in sub))
-    out["pair_shots"] = pair_shots
-    out["pair_born_gcell_updates_per_s"] = round(pair_cells / (t1 - t0) / 1e9, 1)
-    out["pair_adjoint_gcell_updates_per_s"] = round(pair_cells / (t2 - t1) / 1e9, 1)
-    # |<L m, r> - <m, L^T r>| relative to the value and to ||L m|| ||r||: a
-    # random r makes the two products cancel to ~1/sqrt(n) of the norms, so
-    # FP32 rounding shows up far larger against the value
-    out["dot_test_fp32_rel"] = float("%.3e" % rel)
-    out["dot_test_fp32_rel_to_norms"] = float("%.3e" % reln)
-    # the same test in FP64 (propagation, snapshots and products) on 4 shots
-    n64 = min(4, pair_shots)
-    sub64 = sub[:n64]
-    a64, nw64, ng64, nr64 = _layout(grid, sub64)
-    v64 = DeviceArray.from_host(v.astype(np.float64))
-    w64 = DeviceArray.from_host(np.concatenate([seis_amd.ricker(dt, fr, delay, s.nt) for s in sub64]).astype(np.float64))
-    p64 = ctypes.c_void_p()
-    check(L.seis_lsrtm_create_f64(ctypes.byref(p64), ctypes.byref(g), n64, a64, v64.ptr, w64.ptr, sgz.ptr))
-    m64 = DeviceArray.from_host(dv.astype(np.float64))
-    r64 = DeviceArray.from_host((rng.standard_normal(nr64) * mask[:nr64]).astype(np.float64))
-    lm64 = DeviceArray((nr64,), np.float64, zero=True)
-    ltr64 = DeviceArray((nx, nz), np.float64, zero=True)
-    check(L.seis_lsrtm_born(p64, m64.ptr, lm64.ptr))
-    check(L.seis_lsrtm_adjoint(p64, r64.ptr, ltr64.ptr))
-    rel64, reln64 = dot_test(p64, m64, lm64, r64, ltr64, nr64)
-    check(L.seis_lsrtm_destroy(p64))
-    out["dot_test_fp64_rel"] = float("%.3e" % rel64)
-    out["dot_test_fp64_shots"] = n64
-    out["dot_test_note"] = ("FP32: FP32 propagation, BF16 snapshots, FP64 inner products; FP64: every step in FP64 "
-                            "(the stated bar, <= 1e-6)")
-    log("C3: pair on %d shots; dot test FP32 %.2e (%.2e of the norms), FP64 %.2e" % (pair_shots, rel, reln, rel64))
-    return out
-
-
-def c5_sample(nshots, nt, seed=1005):
-    """RTM throughput on the large grid: 2000 x 8000 cells @ 5 m, pml 30,
-    layered-faulted model with a 4500 m/s salt body, dt = 0.5 ms, a short
-    record of the workload's shots."""
-    import ctypes
-    import seis_amd
-    from seis_amd import DeviceArray, models
-    from seis_amd.batch import _layout, Grid, Shot, RTM_ISOLATE_BAND
-    from seis_amd._lib import check
-    L = seis_amd.lib()
-    nx, nz, pml, dx, dt, fr = 8000, 2000, 30, 5.0, 0.0005, 12.0
-    v = models.salt_body(models.layered_faulted(nx, nz, dx, dx, seed=seed), dx, dx, seed=seed)
-    vm = models.smooth_model(v, 20.0)
-    delay = int(round(1.0 / (fr * dt)))
-    sxs = np.linspace(500, nx - 500, nshots).astype(int)
-    shots = [Shot(sx=int(sx), sz=2, nt=nt) for sx in sxs]
-    grid = Grid(nx, nz, pml, dt, dx, dx)
-    g = grid.c()
-    arr, nw, ngz, nr = _layout(grid, shots)
-    d_v, d_vm = DeviceArray.from_host(v), DeviceArray.from_host(vm)
-    d_wav = DeviceArray.from_host(np.tile(seis_amd.ricker(dt, fr, delay, nt), nshots))
-    d_gz = DeviceArray.from_host(np.full(nx * nshots, 2, np.int32))
-    d_rec = DeviceArray((nr,), np.float32, zero=True)
-    d_img = DeviceArray((nshots * nx * nz,), np.float32, zero=True)
-    check(L.seis_modeling_batch(ctypes.byref(g), nshots, arr, d_v.ptr, d_wav.ptr, d_gz.ptr, d_rec.ptr))
-    seis_amd.synchronize()
-    t0 = time.perf_counter()
-    check(L.seis_rtm_batch_ex(ctypes.byref(g), nshots, arr, d_vm.ptr, d_wav.ptr, d_gz.ptr, d_rec.ptr, d_img.ptr,
-                              RTM_ISOLATE_BAND))
-    seis_amd.synchronize()
-    el = time.perf_counter() - t0
-    cells = (nx + 2 * pml) * (nz + 2 * pml)
-    img = d_img.download()
-    log("C5 sample: RTM of %d shots x %d steps in %.1f s" % (nshots, nt, el))
-    return {"workload": "rtm_batch sample: %d shots, 2000x8000 cells @ 5 m, pml 30, nt=%d of the workload's 6000, "
-                        "dt=0.5 ms, layered-faulted model with a salt body, isolated source band" % (nshots, nt),
-            "s": round(el, 3), "s_per_shot_per_1000_steps": round(el / nshots * 1000.0 / nt, 3),
-            "gcell_updates_per_s": round(3.0 * nt * cells * nshots / el / 1e9, 1),
-            "image_finite": bool(np.isfinite(img).all())}
 
 
 def plane_prepare_leg(nshots, ng, nt, n_p, with_cpu, seed=1006):
@@ -451,20 +508,24 @@ def snapshot_vs_recompute(nshots, nt, seed=1007):
             "image_finite": bool(np.isfinite(img).all())}
 
 
-def lsrtm_bench(nshots, nt, iters, err_shots, err_iters):
-    """Seconds per CG iteration of least-squares RTM on the 376 x 1151 @ 8 m
-    workload (the RTM model at half resolution, nt = 3000, dt = 0.8 ms, 400
-    receivers per shot), with BF16 snapshots, beside the plain FP32 GPU path
-    (per iteration one Born batch and one RTM batch, as the reference's
-    forward_MPI + adjoint_MPI would), and the image error of BF16 and FP32
-    snapshots against the FP64 GPU plan on a small batch."""
-    import ctypes
+def lsrtm_bench(iters, err_shots, plain_iters, xcorr_iters):
+    """Least-squares RTM on the 376 x 1151 @ 8 m workload (the RTM model at
+    half resolution, nt = 3000, dt = 0.8 ms): all 128 shots, 400 receivers
+    each (every column from sx - 200 to sx + 199), each shot on its window
+    (spread + 40 columns); observed data = Born records of the true
+    perturbation + 5 % Gaussian noise.  CGLS with BF16 snapshots and the
+    exact adjoint for `iters` iterations, the same with the cross-correlation
+    adjoint, the plain FP32 GPU path (per iteration one Born batch and one RTM
+    batch of the same shots, as the reference's forward_MPI + adjoint_MPI),
+    and, on `err_shots` of the shots, the image of every iteration with
+    FP32 / BF16 / FP16 snapshots against the FP64 plan."""
     import seis_amd
     from seis_amd import DeviceArray, models
-    from seis_amd.batch import _layout, Grid, Shot
+    from seis_amd.batch import Grid, RTM_ISOLATE_BAND
+    from seis_amd.lsrtm import LsrtmPlan, SNAP_BF16, SNAP_FP16, SNAP_FP32
     from seis_amd._lib import check
     L = seis_amd.lib()
-    nx, nz, pml, dx, dt, fr = 1151, 376, 30, 8.0, 0.0008, 12.0
+    nx, nz, pml, dx, dt, fr, nt = 1151, 376, 30, 8.0, 0.0008, 12.0, 3000
     v = np.ascontiguousarray(models.layered_faulted(2301, 751, 4.0, 4.0, seed=1001)[::2, ::2])
     v0 = models.smooth_model(v, 10.0)
     dv_true = (v - v0).astype(np.float32)
@@ -472,171 +533,143 @@ def lsrtm_bench(nshots, nt, iters, err_shots, err_iters):
     grid = Grid(nx, nz, pml, dt, dx, dx)
     g = grid.c()
     all_sx = [4 + 9 * i for i in range(128)]
+    wav1 = seis_amd.ricker(dt, fr, delay, nt)
 
-    def survey(sxs, nt_):
-        shots = [Shot(sx=sx, sz=2, nt=nt_) for sx in sxs]
-        arr, nw, ng, nr = _layout(grid, shots)
-        mask = np.zeros((len(sxs), nx, nt_), np.float32)
-        for j, sx in enumerate(sxs):
-            mask[j, max(0, sx - 200):min(nx, sx + 200)] = 1.0
-        return shots, arr, nr, mask.ravel()
+    def survey(sxs):
+        wins = [models.spread_window(nx, sx, max(0, sx - 200), min(nx - 1, sx + 199), 40) for sx in sxs]
+        shots, arr, nw, ng, nr = window_shots(grid, sxs, wins, [nt] * len(sxs))
+        mask = []
+        for sx, (x0, n) in zip(sxs, wins):
+            cols = np.arange(x0, x0 + n)
+            m = np.zeros((n, nt), np.float32)
+            m[(cols >= sx - 200) & (cols <= sx + 199)] = 1.0
+            mask.append(m.ravel())
+        return shots, arr, ng, nr, np.concatenate(mask), wins
 
-    def born_data(shots, arr, nr, mask, vel, wav, gz, noise_seed):
-        d_dv = DeviceArray.from_host(dv_true)
+    def born_data(shots, arr, ng, nr, mask, noise_seed):
+        d_v0, d_dv = DeviceArray.from_host(v0), DeviceArray.from_host(dv_true)
+        d_w = DeviceArray.from_host(np.tile(wav1, len(shots)))
+        d_g = DeviceArray.from_host(np.full(ng, 2, np.int32))
         d_rec = DeviceArray((nr,), np.float32, zero=True)
-        check(L.seis_born_batch(ctypes.byref(g), len(shots), arr, vel.ptr, d_dv.ptr, wav.ptr, gz.ptr, d_rec.ptr))
+        check(L.seis_born_batch(ctypes.byref(g), len(shots), arr, d_v0.ptr, d_dv.ptr, d_w.ptr, d_g.ptr, d_rec.ptr))
         d = d_rec.download() * mask
-        rms = float(np.sqrt(np.mean(d[mask > 0] ** 2)))
+        rms = float(np.sqrt(np.mean(d[mask > 0].astype(np.float64) ** 2)))
         d += (0.05 * rms * np.random.default_rng(noise_seed).standard_normal(d.shape)).astype(np.float32) * mask
         return d
 
-    out = {"workload": "lsrtm_cg: %d of the 128 shots, 376x1151 cells @ 8 m, pml 30, nt=%d, dt=0.8 ms, "
-                       "Ricker 12 Hz, 400 receivers per shot, Born data of the true perturbation + 5%% noise"
-                       % (nshots, nt)}
-    # --- timed: BF16-snapshot CGLS against the plain FP32 path
-    sxs = all_sx[::2][:nshots]
-    shots, arr, nr, mask = survey(sxs, nt)
+    shots, arr, ng, nr, mask, wins = survey(all_sx)
     ns = len(shots)
-    wav1 = seis_amd.ricker(dt, fr, delay, nt)
-    d_v0 = DeviceArray.from_host(v0)
-    d_v = DeviceArray.from_host(v)
-    d_wav = DeviceArray.from_host(np.tile(wav1, ns))
-    d_gz = DeviceArray.from_host(np.full(nx * ns, 2, np.int32))
-    d_mask = DeviceArray.from_host(mask)
-    dobs = DeviceArray.from_host(born_data(shots, arr, nr, mask, d_v0, d_wav, d_gz, 11))
+    cells = float(sum((n + 2 * pml) * (nz + 2 * pml) for _, n in wins)) * nt  # cell-updates per propagation
+    out = {"workload": "lsrtm_cg: 128 shots, 376x1151 cells @ 8 m, pml 30, nt=%d, dt=0.8 ms, Ricker 12 Hz, 400 "
+                       "receivers per shot, each shot on its window (%d..%d columns), Born data of the true "
+                       "perturbation + 5%% noise" % (nt, min(n for _, n in wins), max(n for _, n in wins)),
+           "cells_per_propagation": cells}
+    dobs = born_data(shots, arr, ng, nr, mask, 11)
     log("lsrtm: observed data for %d shots ready" % ns)
     t0 = time.perf_counter()
-    plan = ctypes.c_void_p()
-    check(L.seis_lsrtm_create(ctypes.byref(plan), ctypes.byref(g), ns, arr, d_v0.ptr, d_wav.ptr, d_gz.ptr, 1))
+    plan = LsrtmPlan(grid, shots, v0, [wav1] * ns, [np.full(s.nx, 2, np.int32) for s in shots], snapshot=SNAP_BF16)
+    plan.set_mask(mask)
     seis_amd.synchronize()
     out["snapshot_s"] = round(time.perf_counter() - t0, 3)
-    check(L.seis_lsrtm_set_mask(plan, d_mask.ptr))
-    img = DeviceArray((nx, nz), np.float32, zero=True)
-    hist = (ctypes.c_double * (2 * (iters + 1)))()
-    done = ctypes.c_int(0)
+    out["snapshot_gb"] = round(2.0 * sum(n for _, n in wins) * nz * nt / 1e9, 1)
+    d_obs = DeviceArray.from_host(dobs)
     seis_amd.profile_start()
     t0 = time.perf_counter()
-    check(L.seis_lsrtm_cg(plan, dobs.ptr, iters, 0.0, img.ptr, hist, ctypes.byref(done)))
-    seis_amd.synchronize()
+    img, hist = plan.cg(d_obs, iters)
     el = time.perf_counter() - t0
     st = seis_amd.profile_stop()
-    out["kernels"] = {k: {"launches": v["launches"], "ms": round(v["total_ms"], 2),
-                          "GB/s": round(v["alg_bytes"] / max(v["total_ms"], 1e-9) / 1e6, 1)} for k, v in st.items()}
-    # the same CG with the cross-correlation adjoint (the reference's imaging
-    # condition on the snapshots, SEIS_LSRTM_ADJ_XCORR)
-    check(L.seis_lsrtm_set_adjoint(plan, 1))
-    img_x = DeviceArray((nx, nz), np.float32, zero=True)
-    done_x = ctypes.c_int(0)
-    seis_amd.profile_start()
-    t0 = time.perf_counter()
-    check(L.seis_lsrtm_cg(plan, dobs.ptr, iters, 0.0, img_x.ptr, None, ctypes.byref(done_x)))
-    seis_amd.synchronize()
-    el_x = time.perf_counter() - t0
-    st_x = seis_amd.profile_stop()
-    check(L.seis_lsrtm_destroy(plan))
-    out["kernels_xcorr"] = {k: {"launches": v["launches"], "ms": round(v["total_ms"], 2)} for k, v in st_x.items()}
-    out["s_per_cg_iter_bf16_xcorr"] = round(el_x / max(done_x.value, 1), 3)
-    if done_x.value > 0 and "lsrtm_born" in st_x and "lsrtm_adjoint_xcorr" in st_x:
-        l_s = st_x["lsrtm_born"]["total_ms"] / done_x.value / 1e3
-        lt_s = st_x["lsrtm_adjoint_xcorr"]["total_ms"] / (done_x.value + 1) / 1e3
-        out["s_per_cg_iter_bf16_xcorr_marginal"] = round(l_s + lt_s, 3)
-        # the workload's 20 iterations: the initial L^T d spread over them
-        out["s_per_cg_iter_bf16_xcorr_20"] = round((lt_s + 20 * (l_s + lt_s)) / 20, 3)
-    log("lsrtm: %d CG iterations with the cross-correlation adjoint in %.1f s" % (done_x.value, el_x))
-    h = np.array(hist[:]).reshape(-1, 2)[: done.value + 1]
-    out["cg_iterations"] = done.value
-    out["s_per_cg_iter_bf16"] = round(el / max(done.value, 1), 3)
-    # one CGLS iteration is one L and one L^T; the run also holds the initial
-    # L^T dobs, so the per-iteration cost without it:
-    if done.value > 0 and "lsrtm_born" in st:
-        adj = sum(st[k]["total_ms"] for k in ("lsrtm_adjoint", "lsrtm_adjoint_prep") if k in st)
-        l_s, lt_s = st["lsrtm_born"]["total_ms"] / done.value / 1e3, adj / (done.value + 1) / 1e3
-        out["s_per_cg_iter_bf16_marginal"] = round(l_s + lt_s, 3)
-        out["s_per_cg_iter_bf16_20"] = round((lt_s + 20 * (l_s + lt_s)) / 20, 3)
-    out["residual_rel"] = [round(float(np.sqrt(x / h[0, 0])), 5) for x in h[:, 0]]
-    log("lsrtm: %d CG iterations (BF16 snapshots) in %.1f s" % (done.value, el))
-    # plain FP32 GPU path: Born batch + RTM batch per iteration
-    d_pert = DeviceArray.from_host(img.download())
+    done = len(hist) - 1
+    out["cg_iterations"] = done
+    out["s_per_cg_iter_bf16"] = round(el / max(done, 1), 3)
+    out["kernels"] = {k: {"launches": v_["launches"], "ms": round(v_["total_ms"], 1),
+                          "GB/s": round(v_["alg_bytes"] / max(v_["total_ms"], 1e-9) / 1e6, 1)} for k, v_ in st.items()}
+    out["residual_rel"] = [round(float(np.sqrt(x / hist[0, 0])), 5) for x in hist[:, 0]]
+    log("lsrtm: %d CG iterations (BF16 snapshots, exact adjoint) in %.1f s" % (done, el))
+    if xcorr_iters > 0:
+        plan.set_adjoint(1)
+        t0 = time.perf_counter()
+        img_x, hist_x = plan.cg(d_obs, xcorr_iters)
+        el_x = time.perf_counter() - t0
+        out["s_per_cg_iter_bf16_xcorr"] = round(el_x / max(len(hist_x) - 1, 1), 3)
+        out["xcorr_iterations"] = len(hist_x) - 1
+        log("lsrtm: %d CG iterations with the cross-correlation adjoint in %.1f s" % (len(hist_x) - 1, el_x))
+    plan.close()
+    # plain FP32 GPU path, per iteration: Born batch of the current image and
+    # RTM batch of the residual, over the same windows
+    d_v0 = DeviceArray.from_host(v0)
+    d_w = DeviceArray.from_host(np.tile(wav1, ns))
+    d_g = DeviceArray.from_host(np.full(ng, 2, np.int32))
+    d_pert = DeviceArray.from_host(img)
     d_rec = DeviceArray((nr,), np.float32, zero=True)
-    d_img = DeviceArray((ns * nx * nz,), np.float32, zero=True)
+    d_img = DeviceArray((sum(n for _, n in wins) * nz,), np.float32, zero=True)
     t0 = time.perf_counter()
-    check(L.seis_born_batch(ctypes.byref(g), ns, arr, d_v0.ptr, d_pert.ptr, d_wav.ptr, d_gz.ptr, d_rec.ptr))
-    check(L.seis_rtm_batch_ex(ctypes.byref(g), ns, arr, d_v0.ptr, d_wav.ptr, d_gz.ptr, dobs.ptr, d_img.ptr, 1))
+    for _ in range(plain_iters):
+        check(L.seis_born_batch(ctypes.byref(g), ns, arr, d_v0.ptr, d_pert.ptr, d_w.ptr, d_g.ptr, d_rec.ptr))
+        check(L.seis_rtm_batch_ex(ctypes.byref(g), ns, arr, d_v0.ptr, d_w.ptr, d_g.ptr, d_obs.ptr, d_img.ptr,
+                                  RTM_ISOLATE_BAND))
     seis_amd.synchronize()
-    plain = time.perf_counter() - t0
+    plain = (time.perf_counter() - t0) / plain_iters
+    del d_img, d_rec, d_pert, d_obs
     out["s_per_iter_plain_fp32"] = round(plain, 3)
-    out["cells_per_propagation"] = float((nx + 2 * pml) * (nz + 2 * pml)) * nt * ns
+    out["plain_iterations_timed"] = plain_iters
     out["speedup_vs_plain_fp32"] = round(plain / out["s_per_cg_iter_bf16"], 2)
-    if "s_per_cg_iter_bf16_marginal" in out:
-        out["speedup_vs_plain_fp32_marginal"] = round(plain / out["s_per_cg_iter_bf16_marginal"], 2)
-    out["speedup_vs_plain_fp32_xcorr"] = round(plain / out["s_per_cg_iter_bf16_xcorr"], 2)
-    for key in ("", "_xcorr"):
-        if "s_per_cg_iter_bf16%s_20" % key in out:
-            out["speedup_vs_plain_fp32%s_20_iters" % key] = round(plain / out["s_per_cg_iter_bf16%s_20" % key], 2)
-    if "s_per_cg_iter_bf16_xcorr_marginal" in out:
-        out["speedup_vs_plain_fp32_xcorr_marginal"] = round(plain / out["s_per_cg_iter_bf16_xcorr_marginal"], 2)
-    log("lsrtm: plain FP32 path, one iteration's Born + RTM in %.1f s" % plain)
-    del d_img, d_rec, d_pert, dobs
-    # --- image error of 16/32-bit snapshots against the FP64 plan (small batch)
+    if "s_per_cg_iter_bf16_xcorr" in out:
+        out["speedup_vs_plain_fp32_xcorr"] = round(plain / out["s_per_cg_iter_bf16_xcorr"], 2)
+    log("lsrtm: plain FP32 path, %.2f s per iteration (Born + RTM batches)" % plain)
+    # --- image error per iteration against the FP64 plan, on a subset
     if err_shots > 0:
-        esx = all_sx[32:32 + 8 * err_shots:8][:err_shots]
-        eshots, earr, enr, emask = survey(esx, nt)
-        ew = DeviceArray.from_host(np.tile(wav1, len(eshots)))
-        egz = DeviceArray.from_host(np.full(nx * len(eshots), 2, np.int32))
-        ed = born_data(eshots, earr, enr, emask, d_v0, ew, egz, 12)
-        imgs = {}
-        for kind, name, adj in ((0, "fp32", 0), (1, "bf16", 0), (2, "fp16", 0), (-1, "fp64", 0),
-                                (1, "bf16_xcorr", 1), (-1, "fp64_xcorr", 1)):
-            pl = ctypes.c_void_p()
-            if kind < 0:
-                dt64 = np.float64
-                # the plan reads these during its lifetime: keep them referenced
-                v64 = DeviceArray.from_host(v0.astype(dt64))
-                w64 = DeviceArray.from_host(np.tile(wav1, len(eshots)).astype(dt64))
-                check(L.seis_lsrtm_create_f64(ctypes.byref(pl), ctypes.byref(g), len(eshots), earr, v64.ptr, w64.ptr,
-                                              egz.ptr))
+        esx = all_sx[4::128 // err_shots][:err_shots]
+        eshots, earr, eng, enr, emask, ewins = survey(esx)
+        ed = born_data(eshots, earr, eng, enr, emask, 12)
+        egz = [np.full(s.nx, 2, np.int32) for s in eshots]
+        its = {}
+        hists = {}
+        for name, kind in (("fp64", None), ("fp32", SNAP_FP32), ("bf16", SNAP_BF16), ("fp16", SNAP_FP16)):
+            if kind is None:
+                p = LsrtmPlan(grid, eshots, v0.astype(np.float64), [wav1.astype(np.float64)] * len(eshots), egz,
+                              fp64=True)
+                dd = ed.astype(np.float64)
+                p.set_mask(emask.astype(np.float64))
             else:
-                dt64 = np.float32
-                check(L.seis_lsrtm_create(ctypes.byref(pl), ctypes.byref(g), len(eshots), earr, d_v0.ptr, ew.ptr,
-                                          egz.ptr, kind))
-            m = DeviceArray.from_host(emask.astype(dt64))
-            check(L.seis_lsrtm_set_mask(pl, m.ptr))
-            check(L.seis_lsrtm_set_adjoint(pl, adj))
-            dd = DeviceArray.from_host(ed.astype(dt64))
-            im = DeviceArray((nx, nz), dt64, zero=True)
-            check(L.seis_lsrtm_cg(pl, dd.ptr, err_iters, 0.0, im.ptr, None, None))
-            imgs[name] = im.download().astype(np.float64)
-            check(L.seis_lsrtm_destroy(pl))
-        ref = imgs["fp64"]
-        out["image_rel_l2_vs_fp64"] = {k: float("%.3e" % (np.linalg.norm(imgs[k] - ref) / np.linalg.norm(ref)))
-                                       for k in ("fp32", "bf16", "fp16")}
-        refx = imgs["fp64_xcorr"]
-        out["image_rel_l2_vs_fp64"]["bf16_xcorr"] = float("%.3e" % (np.linalg.norm(imgs["bf16_xcorr"] - refx)
-                                                                    / np.linalg.norm(refx)))
-        out["xcorr_vs_exact_fp64_image_rel"] = float("%.3e" % (np.linalg.norm(refx - ref) / np.linalg.norm(ref)))
-        out["error_case"] = "%d shots, %d CG iterations, same grid and nt" % (len(eshots), err_iters)
-        log("lsrtm: image errors vs FP64 %s" % out["image_rel_l2_vs_fp64"])
+                p = LsrtmPlan(grid, eshots, v0, [wav1] * len(eshots), egz, snapshot=kind)
+                dd = ed
+                p.set_mask(emask)
+            _, h, it = p.cg(dd, iters, iterates=True)
+            p.close()
+            its[name] = it.astype(np.float64)
+            hists[name] = h
+        ref = its["fp64"]
+        n = min(len(ref), *(len(its[k]) for k in ("fp32", "bf16", "fp16")))
+        err = {k: [float("%.3e" % (np.linalg.norm(its[k][i] - ref[i]) / max(np.linalg.norm(ref[i]), 1e-300)))
+                   for i in range(n)] for k in ("fp32", "bf16", "fp16")}
+        out["image_rel_l2_vs_fp64_per_iter"] = err
+        out["image_rel_l2_vs_fp64_final"] = {k: e[-1] for k, e in err.items()}
+        out["residual_rel_fp64_per_iter"] = [round(float(np.sqrt(x / hists["fp64"][0, 0])), 5)
+                                             for x in hists["fp64"][:, 0]]
+        out["error_case"] = "%d of the 128 shots, %d CG iterations, same grid, windows and nt" % (len(eshots), n)
+        log("lsrtm: final image errors vs FP64 %s" % out["image_rel_l2_vs_fp64_final"])
     return out
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=0)
-    ap.add_argument("--shots", type=int, default=96)
-    ap.add_argument("--cpu-nt", type=int, default=900)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--shots-per-step", type=int, default=26)
+    ap.add_argument("--nt", type=int, default=6000,
+                    help="record length of the headline survey; shorter runs keep every launch's size "
+                         "(for counter collection)")
+    ap.add_argument("--cpu-nt", type=int, default=600)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--nt", type=int, default=5000,
-                    help="record length; shorter runs keep every launch's size (for counter collection)")
-    ap.add_argument("--lsrtm-shots", type=int, default=64, help="shots of the LSRTM timing (0 skips it)")
-    ap.add_argument("--lsrtm-nt", type=int, default=3000)
-    ap.add_argument("--lsrtm-iters", type=int, default=3)
+    ap.add_argument("--lsrtm-iters", type=int, default=20, help="CG iterations of the LSRTM leg (0 skips it)")
+    ap.add_argument("--lsrtm-xcorr-iters", type=int, default=20)
+    ap.add_argument("--lsrtm-plain-iters", type=int, default=1)
+    ap.add_argument("--lsrtm-err-shots", type=int, default=8)
     ap.add_argument("--c1-calls", type=int, default=10, help="timed calls of the C1 single-shot case (0 skips it)")
+    ap.add_argument("--c2", type=int, default=1, help="run the 96-shot RTM batch on the 751 x 2301 grid (0 skips)")
     ap.add_argument("--c3-shots", type=int, default=256, help="shots of the C3 Born batch (0 skips it)")
     ap.add_argument("--c3-pair-shots", type=int, default=16)
-    ap.add_argument("--c5-shots", type=int, default=2, help="shots of the C5 RTM sample (0 skips it)")
-    ap.add_argument("--c5-nt", type=int, default=500)
     ap.add_argument("--snap-shots", type=int, default=4,
                     help="shots of the snapshot-storage vs recomputation comparison (0 skips it)")
     ap.add_argument("--plane-shots", type=int, default=96, help="shots of the plane-wave synthesis leg (0 skips it)")
@@ -660,69 +693,68 @@ def main():
         seis_amd.synchronize()
         log("copied 8 x %d bytes device to device" % (4 * n))
         return
-    from seis_amd.batch import _layout, Grid, Shot, RTM_ISOLATE_BAND
-    import ctypes
+    from seis_amd.batch import Grid, RTM_ISOLATE_BAND
 
-    w = rtm_workload(args.shots, args.nt)
+    w = c5_survey(args.nt)
     if not args.no_cpu and args.cpu_nt <= w["delay"]:
         # the reference's engine exits the process when the wavelet delay
         # does not fit the record
         print("--cpu-nt must exceed the wavelet delay (%d samples)" % w["delay"], file=sys.stderr)
         sys.exit(2)
     nx, nz, pml, nt = w["nx"], w["nz"], w["pml"], w["nt"]
-    grid = Grid(nx, nz, pml, w["dt"], w["dx"], w["dz"])
-    shots = [Shot(sx=sx, sz=w["sz"], nt=nt, ndel=0, model=0) for sx in w["sxs"]]
-    ns = len(shots)
-    wav1 = seis_amd.ricker(w["dt"], w["fr"], w["delay"], nt)
-    arr, nw, ng, nr = _layout(grid, shots)
+    grid = Grid(nx, nz, pml, w["dt"], w["dx"], w["dx"])
     g = grid.c()
-    d_vtrue = DeviceArray.from_host(w["vtrue"])
-    d_vmig = DeviceArray.from_host(w["vmig"])
-    d_wav = DeviceArray.from_host(np.tile(wav1, ns))
-    d_gz = DeviceArray.from_host(np.full(nx * ns, w["gz"], np.int32))
-    d_rec = DeviceArray((nr,), np.float32, zero=True)
-    d_img = DeviceArray((ns * nx * nz,), np.float32, zero=True)
+    ns = len(w["sxs"])
+    shots, arr, nw, ng, nr = window_shots(grid, w["sxs"], w["wins"], [nt] * ns, w["sz"])
+    wav1 = seis_amd.ricker(w["dt"], w["fr"], w["delay"], nt)
     L = seis_amd.lib()
-    from seis_amd._lib import check
-
-    # observed data: forward modeling through the true model, receivers limited
-    # to each shot's 480-channel split spread
+    d_vtrue = DeviceArray.from_host(w["v"])
+    d_vmig = DeviceArray.from_host(w["vm"])
+    d_wav = DeviceArray.from_host(np.tile(wav1, ns))
+    d_gz = DeviceArray.from_host(np.full(ng, w["gz"], np.int32))
+    d_rec = DeviceArray((nr,), np.float32, zero=True)
+    # observed data: forward modeling of every shot through the true model
     t0 = time.perf_counter()
     check(L.seis_modeling_batch(ctypes.byref(g), ns, arr, d_vtrue.ptr, d_wav.ptr, d_gz.ptr, d_rec.ptr))
-    rec = d_rec.download().reshape(ns, nx, nt)
-    for j, s in enumerate(shots):
-        rec[j][~spread_mask(w, s.sx)] = 0.0
-    d_rec.upload(rec.ravel())
+    seis_amd.synchronize()
     prep_s = time.perf_counter() - t0
     log("observed records modeled for %d shots in %.1f s" % (ns, prep_s))
+    del d_vtrue
+    S = args.shots_per_step
+    cells_of = [(n + 2 * pml) * (nz + 2 * pml) for _, n in w["wins"]]
+    img_elems = max(n for _, n in w["wins"]) * nz * S
+    d_img = DeviceArray((img_elems,), np.float32, zero=True)
 
     # The reference's backward source reconstruction diverges in the absorbing
-    # band on this model (inf/NaN images by nt = 5000, the reference included);
-    # SEIS_RTM_ISOLATE_BAND keeps the band of that field at zero.  Same kernels
-    # and work; the image equals the reference's beyond 34 cells of the edges.
-    def step():
-        check(L.seis_rtm_batch_ex(ctypes.byref(g), ns, arr, d_vmig.ptr, d_wav.ptr, d_gz.ptr, d_rec.ptr, d_img.ptr,
-                                  RTM_ISOLATE_BAND))
+    # band on long records in fast media (inf/NaN images, the reference
+    # included); SEIS_RTM_ISOLATE_BAND keeps the band of that field at zero.
+    # Same kernels and work; the image equals the reference's beyond 34 cells
+    # of the edges.
+    def step(k):
+        idx = [(k * S + j) % ns for j in range(S)]
+        check(L.seis_rtm_batch_ex(ctypes.byref(g), S, sub_array(arr, idx), d_vmig.ptr, d_wav.ptr, d_gz.ptr,
+                                  d_rec.ptr, d_img.ptr, RTM_ISOLATE_BAND))
+        return 3.0 * nt * sum(cells_of[i] for i in idx)
 
     for i in range(args.warmup):
-        step()
+        step(i)
         log("warmup step %d/%d done" % (i + 1, args.warmup))
     seis_amd.synchronize()
     seis_amd.profile_start()
     t0 = time.perf_counter()
+    updates = 0.0
     for i in range(args.steps):
-        step()  # returns when the batch has finished on the device
+        updates += step(i)  # returns when the step's shots have finished on the device
         log("timed step %d/%d done at %.1f s" % (i + 1, args.steps, time.perf_counter() - t0))
     seis_amd.synchronize()
     el = time.perf_counter() - t0
     stats = seis_amd.profile_stop()
-
-    cells = (nx + 2 * pml) * (nz + 2 * pml)
-    updates = 3.0 * nt * cells * ns * args.steps
     value = updates / el / 1e9
     dom = max(stats.items(), key=lambda kv: kv[1]["total_ms"])
     achieved = dom[1]["alg_bytes"] / (dom[1]["total_ms"] * 1e-3) / 1e9
     img = d_img.download()
+    s_per_shot = el / (args.steps * S)
+    traffic = MEASURED_TRAFFIC["bytes"].get(dom[0])
     out = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -735,50 +767,59 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded layered-faulted model; records forward-modeled on the GPU before timing)",
-        "config": {"workload": "rtm_batch: %d shots, 751x2301 cells @ 4 m, pml 30, nt=%d, dt=0.4 ms, "
-                               "Ricker 12 Hz, 480-channel split spread @ 12 m, reference imaging condition" % (ns, nt),
-                   "shots": ns, "grid": [nz, nx], "padded_cells": cells, "nt": nt,
-                   "propagations_per_shot": 3, "source_band": "isolated (SEIS_RTM_ISOLATE_BAND)",
-                   "strip_length": seis_amd.tuned_strip("rtm", nx + 2 * w["pml"], nz + 2 * w["pml"], ns)},
+        "data": "synthetic (seeded layered-faulted model with a salt body; records forward-modeled on the GPU "
+                "before timing)",
+        "config": {"workload": "rtm_survey (the large batched RTM/LSRTM stress case): 512 shots, 2000x8000 cells "
+                               "@ 5 m with a salt body, pml 30, nt=%d, dt=0.5 ms, Ricker 12 Hz, 1000 receivers "
+                               "per shot, each shot on its 1100-column window, %d shots per step (step k images "
+                               "shots %dk..%dk+%d mod 512), reference imaging condition" % (nt, S, S, S, S - 1),
+                   "shots": ns, "shots_per_step": S, "grid": [nz, nx], "window_padded_cells": cells_of[0],
+                   "nt": nt, "propagations_per_shot": 3, "source_band": "isolated (SEIS_RTM_ISOLATE_BAND)",
+                   "strip_length": seis_amd.tuned_strip("rtm", max(n for _, n in w["wins"]) + 2 * pml,
+                                                        nz + 2 * pml, S)},
         "roofline": {"bound": "hbm", "kernel": dom[0], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": (MEASURED_TRAFFIC_96[dom[0]] * ns / 96 if dom[0] in MEASURED_TRAFFIC_96 else None),
+                     "traffic": traffic * S / MEASURED_TRAFFIC["shots"] if traffic else None,
                      "avg_launch_ms": round(dom[1]["total_ms"] / dom[1]["launches"], 4),
                      "alg_bytes_per_launch": dom[1]["alg_bytes"] / dom[1]["launches"]},
         "kernels": {k: {"launches": v["launches"], "ms": round(v["total_ms"], 2),
                         "GB/s": round(v["alg_bytes"] / max(v["total_ms"], 1e-9) / 1e6, 1)}
                     for k, v in stats.items()},
+        "survey": {"s_per_shot": round(s_per_shot, 4), "s_for_512_shots": round(s_per_shot * ns, 1),
+                   "shot_rtms_timed": args.steps * S},
         "prep_s": round(prep_s, 2),
         "image_finite": bool(np.isfinite(img).all()),
     }
-    threads = min(16, len(os.sched_getaffinity(0)))
+    del d_rec, d_img, d_wav, d_gz, d_vmig
+    ci = cpu_info()
+    threads = ci["threads_used"]
+    if args.c2:
+        out["c2_rtm_batch"] = c2_rtm_batch()
     if args.c3_shots > 0:
-        del d_rec, d_img
         out["c3_born_adjoint"] = c3_born_adjoint(args.c3_shots, args.c3_pair_shots)
-    if args.c5_shots > 0:
-        out["c5_sample"] = c5_sample(args.c5_shots, args.c5_nt)
+    if args.lsrtm_iters > 0:
+        out["lsrtm"] = lsrtm_bench(args.lsrtm_iters, args.lsrtm_err_shots, args.lsrtm_plain_iters,
+                                   args.lsrtm_xcorr_iters)
     if args.c1_calls > 0:
         log("C1 single-shot modeling, %d calls per mode" % args.c1_calls)
         out["c1_single_shot"] = c1_single_shot(args.c1_calls, not args.no_cpu, threads)
     if args.snap_shots > 0:
-        out["snapshot_vs_recompute"] = snapshot_vs_recompute(args.snap_shots, args.nt)
+        out["snapshot_vs_recompute"] = snapshot_vs_recompute(args.snap_shots, 5000)
     if args.plane_shots > 0:
         out["plane_prepare"] = plane_prepare_leg(args.plane_shots, 480, 5000, 8, not args.no_cpu)
-    if args.lsrtm_shots > 0:
-        out["lsrtm"] = lsrtm_bench(args.lsrtm_shots, args.lsrtm_nt, args.lsrtm_iters, 2, 3)
     if not args.no_cpu:
         log("timing the CPU baseline on %d threads" % threads)
         cb = cpu_baseline(w, args.cpu_nt, threads)
         cb["gpu_over_cpu"] = round(value / cb["value"], 1)
+        cb["cpu"] = ci
         out["cpu_baseline"] = cb
         ls = out.get("lsrtm")
-        if ls and "s_per_cg_iter_bf16_marginal" in ls:
+        if ls:
             # the reference's iteration is one Born (2 propagations) and one
             # RTM (3) per shot; priced at the measured reference rate
             cpu_iter = 5.0 * ls["cells_per_propagation"] / (cb["value"] * 1e9)
             ls["s_per_iter_cpu_reference_est"] = round(cpu_iter, 1)
-            ls["speedup_vs_cpu_reference"] = round(cpu_iter / ls["s_per_cg_iter_bf16_marginal"], 1)
+            ls["speedup_vs_cpu_reference"] = round(cpu_iter / ls["s_per_cg_iter_bf16"], 1)
     print(json.dumps(out))
 
 

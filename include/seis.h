@@ -333,6 +333,12 @@ int seis_lsrtm_set_weight(seis_lsrtm* plan, const void* weight);
  * *iters (may be NULL) = iterations done. */
 int seis_lsrtm_cg(seis_lsrtm* plan, const void* dobs, int niter, double rtol, void* image, double* hist,
                   int* iters);
+/* seis_lsrtm_cg that also writes the image after every iteration i (1..
+ * iterations done) into iterates + (i-1) nx nz (device, niter images in the
+ * plan's precision; NULL writes none), e.g. to follow the image error per
+ * iteration.  No counterpart in the reference. */
+int seis_lsrtm_cg_history(seis_lsrtm* plan, const void* dobs, int niter, double rtol, void* image, double* hist,
+                          int* iters, void* iterates);
 
 /* Strip-length autotuning of the interior sweeps (no counterpart in the
  * reference).  On by default: the first run of a configuration (kind of

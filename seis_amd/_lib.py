@@ -71,6 +71,8 @@ _SIGS = {
     "seis_lsrtm_set_adjoint": (_I, [_P, _I]),
     "seis_lsrtm_set_weight": (_I, [_P, _P]),
     "seis_lsrtm_cg": (_I, [_P, _P, _I, ctypes.c_double, _P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I)]),
+    "seis_lsrtm_cg_history": (_I, [_P, _P, _I, ctypes.c_double, _P, ctypes.POINTER(ctypes.c_double),
+                                   ctypes.POINTER(_I), _P]),
     "seis_ricker": (_I, [_F, _F, _I, _I, _P]),
     "seis_harmonic": (_I, [_P, _P, _I, _F, _F]),
     "seis_plane_prepare": (_I, [_I, _I, _I, _I, _I, _F, _F, _P, _P, _P, _P, _P, _P, _P, _P]),

@@ -44,17 +44,21 @@ def width(grid, s):
     return s.nx if s.nx > 0 else grid.nx
 
 
-def _layout(grid, shots):
+def _layout(grid, shots, views=False):
     """Offsets of each shot's model / wavelet / gz row / record in packed
-    batch arrays.  Models are stacked by index for full-width shots; a shot
-    with its own width nx takes its model at its own slot of the packed
-    window models (see pack_models)."""
+    batch arrays.  Models are stacked by index for full-width shots; with
+    windowed shots (Shot.nx > 0) each shot takes its window at its own slot
+    of the packed window models, or, with views=True, its window is the
+    column range [x0, x0 + nx) of one [nx][nz] model (offset x0 nz)."""
     arr = (seis_shot * len(shots))()
     wav_off = gz_off = rec_off = mod_off = 0
     windowed = any(s.nx > 0 for s in shots)
     for j, s in enumerate(shots):
         nx = width(grid, s)
-        model = mod_off if windowed else s.model * grid.nx * grid.nz
+        if views:
+            model = s.x0 * grid.nz
+        else:
+            model = mod_off if windowed else s.model * grid.nx * grid.nz
         arr[j] = seis_shot(s.sx, s.sz, s.nt, s.ndel, model, wav_off, gz_off, rec_off, s.nx, s.x0)
         wav_off += s.nt + s.ndel
         gz_off += nx

@@ -624,80 +624,109 @@ struct HilbertTaps {
   float h[2 * HILBERT_HALF + 1];
 };
 
-// One step's term at row z of a column held in LDS (sU, sR over NZ rows),
-// added to img in the reference's order: U R first unless `faqi_only`
-// (adjointPlane images with FaQi alone, fd.cpp:868-870).
-template <int HALF>
-__device__ __forceinline__ float image_term(const HilbertTaps& taps, const float* sU, const float* sR, int z, int NZ,
-                                            float img, bool faqi_only = false) {
-  float uh = 0.f, rh = 0.f;
-  if (z >= HALF && z + HALF <= NZ - 1) {
-    // full window: inputs z-HALF+1 .. z+HALF-1 at odd offsets, tap index
-    // z - i + HALF, summed in increasing i
-#pragma unroll
-    for (int j = 0; j < HALF; j++) {
-      const int i = z - HALF + 1 + 2 * j;
-      const float h = taps.h[2 * HALF - 1 - 2 * j];
-      uh += sU[i] * h;
-      rh += sR[i] * h;
-    }
-  } else {
-    const int i0 = max(0, z - HALF), i1 = min(NZ - 1, z + HALF);
-    for (int i = i0 + (((z - i0) & 1) ? 0 : 1); i <= i1; i += 2) {
-      const float h = taps.h[z - i + HALF];
-      uh += sU[i] * h;
-      rh += sR[i] * h;
-    }
-  }
-  const float scal = 2.0f / 4.0f;
-  const float ur = sU[z] * sR[z];
-  if (!faqi_only) img += ur;
-  img += scal * (ur - uh * rh);
-  return img;
-}
+// The imaging pass works on tiles of FQ_TILE interior rows of one column.
+// Output row z uses the inputs z-29, z-27, ..., z+29, all of the other
+// parity, so each tile's U and R (rows z0-29 .. z0+FQ_TILE+28) are split by
+// parity into LDS arrays: output n of parity p (z = z0 + p + 2n) reads
+// A_p[n .. n+29] with A_p[k] = in[z0 - 29 + p + 2k].  A thread forms 4
+// outputs of one parity from A_p[4t .. 4t+32], read as 9 aligned float4; each
+// input is added into every accumulator whose window holds it, in increasing
+// order of the input row, so every sum is the reference's sequence of
+// roundings (hilbert.h:42-54; the terms with a zero tap or an input outside
+// the column are left out: each would add a zero product to a partial sum
+// that is never -0, which leaves it unchanged).
+constexpr int FQ_WG = 256;
+constexpr int FQ_TILE = 1024;             // output rows per work-group (512 of each parity)
+constexpr int FQ_PAR = FQ_TILE / 2 + 32;  // inputs per parity array (multiple of 4)
 
-// Reverse steps `step` and, when `pair`, step + 1 applied to one interior
-// column (the image is read and written once for both).  U of step st is in
-// aux field st % NAUX; R of step st is in field slot 3 + (st + 2) % 3.
+// Reverse steps `step` and, when `pair`, step + 1 applied to one tile of one
+// interior column (the image is read and written once for both).  U of step
+// st is in aux field st % NAUX; R of step st is in field slot 3 + (st + 2) % 3.
 // `plane`: step `step` of adjointPlane's forward pass instead (it = step+2,
 // imaged when it > ndel, FaQi term only; never paired).
+// Grid: (row tiles, interior columns of the widest shot, shots).
 template <int HALF>
-__global__ __launch_bounds__(128) void image_kernel(Consts<float> k, const DevShot<float>* __restrict__ shots,
-                                                    int step, HilbertTaps taps, int pair, int plane) {
-  extern __shared__ unsigned char smem_raw[];
-  float* sU0 = reinterpret_cast<float*>(smem_raw);
-  const DevShot<float>& s = shots[blockIdx.y];
-  const int x = k.pml + blockIdx.x;
+__global__ __launch_bounds__(FQ_WG) void image_kernel(Consts<float> k, const DevShot<float>* __restrict__ shots,
+                                                      int step, HilbertTaps taps, int pair, int plane) {
+  static_assert(HALF == 30, "the tile arithmetic assumes FaQi's 61-tap filter");
+  __shared__ __align__(16) float A[2][2][2][FQ_PAR];  // [step][U, R][parity][k]
+  __shared__ float HP[2][FQ_TILE];                    // H[U] H[R] per step and output
+  const DevShot<float>& s = shots[blockIdx.z];
+  const int x = k.pml + blockIdx.y;
   if (x >= s.NX - k.pml) return;
   const int it = plane ? step + 2 : s.nt + s.ndel - 1 - step;
   if (plane ? (it >= s.nt + s.ndel || it - s.ndel <= 0) : it < s.ndel) return;
   const bool two = !plane && pair && it - 1 >= s.ndel;
   const int NZ = k.NZ;
-  float* sR0 = sU0 + NZ;
-  float* sU1 = sR0 + NZ;
-  float* sR1 = sU1 + NZ;
+  const int zhi = NZ - k.pml;
+  const int z0 = k.pml + blockIdx.x * FQ_TILE;
+  if (z0 >= zhi) return;
   const long long base = fidx(k.ld, x, 0);
-  const auto U0 = SEIS_GL(s.aux + (step % NAUX) * s.fstride + base);
-  const auto R0 = SEIS_GL(s.fld + (3 + (step + 2) % 3) * s.fstride + base);
-  const auto U1 = SEIS_GL(s.aux + ((step + 1) % NAUX) * s.fstride + base);
-  const auto R1 = SEIS_GL(s.fld + (3 + (step + 3) % 3) * s.fstride + base);
-  const auto IMG = SEIS_GL(s.img + base);
-  for (int z = threadIdx.x; z < NZ; z += blockDim.x) {
-    sU0[z] = U0[z];
-    sR0[z] = R0[z];
-    if (two) {
-      sU1[z] = U1[z];
-      sR1[z] = R1[z];
-    }
+  const float* U0 = s.aux + (step % NAUX) * s.fstride + base;
+  const float* R0 = s.fld + (3 + (step + 2) % 3) * s.fstride + base;
+  const float* U1 = s.aux + ((step + 1) % NAUX) * s.fstride + base;
+  const float* R1 = s.fld + (3 + (step + 3) % 3) * s.fstride + base;
+  const int nst = two ? 2 : 1;
+  // rows z0-29+q, q < 2 FQ_PAR, into A[st][f][q & 1][q >> 1] (coalesced reads)
+  for (int w = threadIdx.x; w < nst * 2 * 2 * FQ_PAR; w += FQ_WG) {
+    const int q = w % (2 * FQ_PAR), sf = w / (2 * FQ_PAR);
+    const int st = sf >> 1, f = sf & 1;
+    const float* src = st == 0 ? (f == 0 ? U0 : R0) : (f == 0 ? U1 : R1);
+    const int r = z0 - 29 + q;
+    A[st][f][q & 1][q >> 1] = (r >= 0 && r < NZ) ? SEIS_GL(src)[r] : 0.f;
   }
   __syncthreads();
-  for (int z = k.pml + threadIdx.x; z < NZ - k.pml; z += blockDim.x) {
+  const int p = threadIdx.x >> 7;   // parity of this thread's outputs
+  const int t = threadIdx.x & 127;  // outputs n = 4t .. 4t + 3
+  for (int st = 0; st < nst; st++) {
+    const float4* u4 = reinterpret_cast<const float4*>(&A[st][0][p][4 * t]);
+    const float4* r4 = reinterpret_cast<const float4*>(&A[st][1][p][4 * t]);
+    float uh[4] = {0.f, 0.f, 0.f, 0.f}, rh[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int v = 0; v < 9; v++) {
+      const float4 qu = u4[v], qr = r4[v];
+      const float xu[4] = {qu.x, qu.y, qu.z, qu.w};
+      const float xr[4] = {qr.x, qr.y, qr.z, qr.w};
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        const int w = 4 * v + e;  // input A_p[4t + w]
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+          const int u = w - m;  // its index j in output (4t + m)'s window: tap h[59 - 2j]
+          if (u >= 0 && u < HALF) {
+            const float h = taps.h[2 * HALF - 1 - 2 * u];
+            uh[m] = uh[m] + xu[e] * h;
+            rh[m] = rh[m] + xr[e] * h;
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < 4; m++) HP[st][p + 2 * (4 * t + m)] = uh[m] * rh[m];
+  }
+  __syncthreads();
+  // img += U R (unless `plane`), then img += 1/2 (U R - H[U] H[R]), step by step
+  const float scal = 2.0f / 4.0f;
+  const auto IMG = SEIS_GL(s.img + base);
+  for (int q = threadIdx.x; q < FQ_TILE; q += FQ_WG) {
+    const int z = z0 + q;
+    if (z >= zhi) break;
+    const int pp = (q + 29) & 1, kk = (q + 29) >> 1;  // row z in the parity arrays
     float img = IMG[z];
-    img = image_term<HALF>(taps, sU0, sR0, z, NZ, img, plane != 0);
-    if (two) img = image_term<HALF>(taps, sU1, sR1, z, NZ, img);
+    const float ur0 = A[0][0][pp][kk] * A[0][1][pp][kk];
+    if (!plane) img += ur0;
+    img += scal * (ur0 - HP[0][q]);
+    if (two) {
+      const float ur1 = A[1][0][pp][kk] * A[1][1][pp][kk];
+      img += ur1;
+      img += scal * (ur1 - HP[1][q]);
+    }
     IMG[z] = img;
   }
 }
+
+// Row tiles of the imaging pass for padded depth NZ.
+inline int image_tiles(int NZ, int pml) { return (NZ - 2 * pml + FQ_TILE - 1) / FQ_TILE; }
 
 // Writes the strips of time sample `itp` into a field (fd.cpp:1204-1223).
 template <typename T>
@@ -1113,10 +1142,9 @@ int run_chunk(int mode, const GridSpec& g, int n, const ShotSpec* shots, T* cons
     constexpr int half = HILBERT_HALF;
     HilbertTaps taps{};
     hilbert_taps_host<float>(half, taps.h);
-    const size_t smem1 = sizeof(T) * 2 * NZ;  // at most 64 KiB (checked with the arguments)
-    // two reverse steps per imaging pass where four columns fit in LDS
-    const int per = 2 * smem1 <= 64 * 1024 ? 2 : 1;
-    const size_t smem = per * smem1;
+    // two reverse steps per imaging pass
+    const int per = 2;
+    const dim3 igrid((unsigned)image_tiles(NZ, g.pml), (unsigned)nxmax, (unsigned)n);
     // Imaging pass q covers reverse steps [per q, per q + per) and runs on a
     // second stream after the last of them, beside the next reverse step.
     // Step st rewrites the receiver slot of step st-3 (and the U buffer of
@@ -1140,7 +1168,7 @@ int run_chunk(int mode, const GridSpec& g, int n, const ShotSpec* shots, T* cons
       HIPCHECK(hipStreamWaitEvent(is.s, is.ready[q % ImageStream::NEV], 0));
       // per step U and R over the interior columns; the image once
       launch(quiet, is.s, "rtm_image", 2 * E * (fq0 + fq1) + 2 * E * mc0, 0, [&] {
-        hipLaunchKernelGGL((image_kernel<half>), dim3(nxmax, n), dim3(128), smem, is.s, k, dev, first, taps, pair, 0);
+        hipLaunchKernelGGL((image_kernel<half>), igrid, dim3(FQ_WG), 0, is.s, k, dev, first, taps, pair, 0);
       });
       HIPCHECK(hipEventRecord(is.done[q % ImageStream::NEV], is.s));
     }
@@ -1175,9 +1203,6 @@ int run_batch(int mode, const GridSpec& g, int nshots, const ShotSpec* shots, co
   if (rc) return rc;
   if (nshots <= 0) return SEIS_OK;
   if ((rc = check_shots(mode, g, nshots, shots, line_src))) return rc;
-  // the imaging pass holds U and R of a whole padded column in LDS
-  if (mode == RUN_RTM && sizeof(T) * 2 * (size_t)(g.nz + 2 * g.pml) > 64 * 1024)
-    return set_error("rtm: padded nz too large for the column Hilbert kernel"), SEIS_ERR_ARG;
   std::vector<size_t> bytes(nshots);
   for (int s = 0; s < nshots; s++) bytes[s] = shot_bytes(mode, g, shots[s], sizeof(T), sizeof(DevShot<T>));
   size_t budget;
@@ -1313,7 +1338,7 @@ int run_rtm_plane_chunk(const GridSpec& g, int n, const ShotSpec* shots, T* cons
   constexpr int half = HILBERT_HALF;
   HilbertTaps taps{};
   hilbert_taps_host<float>(half, taps.h);
-  const size_t smem = sizeof(T) * 2 * NZ;  // at most 64 KiB (checked with the arguments)
+  const dim3 igrid((unsigned)image_tiles(NZ, g.pml), (unsigned)g.nx, (unsigned)n);
   const double faqi_cells = (double)g.nx * NZ;
   for (int st = 0; st + 2 < NTmax; st++) {
     const int a = active(PW_FWD, st);
@@ -1321,7 +1346,7 @@ int run_rtm_plane_chunk(const GridSpec& g, int n, const ShotSpec* shots, T* cons
       launch_step<T>(PW_FWD, k, dev, L, B, n, st, stream);
     });
     timed_launch(stream, "plane_image", 2 * E * a * faqi_cells + 2 * E * a * (double)g.nx * g.nz, 0, [&] {
-      hipLaunchKernelGGL((image_kernel<half>), dim3(g.nx, n), dim3(128), smem, stream, k, dev, st, taps, 0, 1);
+      hipLaunchKernelGGL((image_kernel<half>), igrid, dim3(FQ_WG), 0, stream, k, dev, st, taps, 0, 1);
     });
     hipLaunchKernelGGL((restore_plane_strips_kernel<T>), dim3(16, n), dim3(256), 0, stream, k, dev, st);
   }
@@ -1338,8 +1363,6 @@ int run_rtm_plane(const GridSpec& g, int nshots, const ShotSpec* shots, const T*
   if (rc) return rc;
   if (nshots <= 0) return SEIS_OK;
   if ((rc = check_shots(RUN_RTM_PLANE, g, nshots, shots, 1))) return rc;
-  if (sizeof(T) * 2 * (size_t)(g.nz + 2 * g.pml) > 64 * 1024)
-    return set_error("rtm_plane: padded nz too large for the column Hilbert kernel"), SEIS_ERR_ARG;
   std::vector<size_t> bytes(nshots);
   for (int s = 0; s < nshots; s++) bytes[s] = shot_bytes(RUN_RTM_PLANE, g, shots[s], sizeof(T), sizeof(DevShot<T>));
   size_t budget;

@@ -334,6 +334,27 @@ __global__ void mul_kernel(T* y, const T* a, const T* b, long long n) {
     y[j] = a[j] * b[j];
 }
 
+// y = y + alpha x with x in FP64 (the CG iterate update from the FP64 direction)
+template <typename T>
+__global__ void axpy_from64_kernel(T* y, const double* x, double alpha, long long n) {
+  for (long long j = blockIdx.x * (long long)blockDim.x + threadIdx.x; j < n; j += (long long)gridDim.x * blockDim.x)
+    y[j] = (T)((double)y[j] + alpha * x[j]);
+}
+
+// dst = a * (double)src (widening copy with a scale)
+template <typename S>
+__global__ void scale_to64_kernel(double* dst, const S* src, double a, long long n) {
+  for (long long j = blockIdx.x * (long long)blockDim.x + threadIdx.x; j < n; j += (long long)gridDim.x * blockDim.x)
+    dst[j] = a * (double)src[j];
+}
+
+// dst = (T)src
+template <typename T>
+__global__ void from64_kernel(T* dst, const double* src, long long n) {
+  for (long long j = blockIdx.x * (long long)blockDim.x + threadIdx.x; j < n; j += (long long)gridDim.x * blockDim.x)
+    dst[j] = (T)src[j];
+}
+
 template <typename T>
 __global__ void fill_kernel(T* p, T v, long long n) {
   for (long long j = blockIdx.x * (long long)blockDim.x + threadIdx.x; j < n; j += (long long)gridDim.x * blockDim.x)
@@ -366,7 +387,9 @@ struct LsrtmPlan {
   float* scales = nullptr;
   T* grads = nullptr;      // per-shot gradients
   double* part = nullptr;  // reduction scratch (RGRID partials + results)
-  T* cg = nullptr;         // CG vectors: x, s, p (model) and r, q (records)
+  T* cg = nullptr;         // CG vectors: s, p, W p (model) and r, q (records)
+  double* cg64 = nullptr;  // FP64 CG vectors: s, p and the orthonormal gradients
+  size_t cg64_elems = 0;
   const T* mask = nullptr; // optional record mask (caller-owned, device)
   const T* weight = nullptr;  // optional image-space preconditioner W of CG (caller-owned, device)
   T* masked = nullptr;     // masked copy of an adjoint input
@@ -382,7 +405,8 @@ struct LsrtmPlan {
   hipStream_t stream = 0;
 
   ~LsrtmPlan() {
-    for (void* p : {(void*)dev, (void*)pool, snaps, (void*)scales, (void*)grads, (void*)part, (void*)cg, (void*)masked, (void*)pmask})
+    for (void* p : {(void*)dev, (void*)pool, snaps, (void*)scales, (void*)grads, (void*)part, (void*)cg, (void*)cg64,
+                    (void*)masked, (void*)pmask})
       if (p) (void)hipFree(p);
   }
 
@@ -666,14 +690,32 @@ struct LsrtmPlan {
   }
 
   // CGLS on min ||L x - d||^2 from x = 0 (with the record mask, if set, in
-  // L).  hist[2*i] = ||r_i||^2, hist[2*i+1] = ||L^T r_i||^2 for i = 0..done;
-  // stops after niter iterations or when ||L^T r|| <= rtol ||L^T d||.
+  // L).  hist[2*i] = ||r_i||^2, hist[2*i+1] = ||s_i||^2 for i = 0..done;
+  // stops after niter iterations or when ||s|| <= rtol ||s_0||.
   // With a weight W set, CGLS runs on the operator L W (right
   // preconditioning: x is the preconditioned image) and returns m = W x.
-  int cgls(const T* d, int niter, double rtol, T* x_out, double* hist, int* done) {
-    if (!cg) LS_CHECK(hipMalloc(&cg, sizeof(T) * (size_t)(4 * model + 2 * nrec)));
-    T *x = x_out, *s = cg, *p = cg + model, *r = cg + 2 * model, *q = cg + 2 * model + nrec;
+  //
+  // The gradients s_i = W L^T r_i, orthogonal in exact arithmetic, are kept
+  // orthogonal: each new s is Gram-Schmidt-orthogonalised in FP64 against the
+  // earlier ones (stored normalised, FP64).  Without it CG loses
+  // orthogonality within ~10 iterations on these ill-conditioned operators
+  // and its iterates drift from the exact-arithmetic ones by far more than
+  // the rounding of the operator (FP64 included); with it the FP32 plan's
+  // iterates stay within the operator's rounding of the FP64 plan's.  The
+  // directions p, the gradients and the inner products are FP64; the
+  // operator applications run in the plan's precision.
+  int cgls(const T* d, int niter, double rtol, T* x_out, double* hist, int* done, T* iterates = nullptr) {
+    const size_t need64 = (size_t)model * (2 + (niter + 1));
+    if (!cg) LS_CHECK(hipMalloc(&cg, sizeof(T) * (size_t)(3 * model + 2 * nrec)));
+    if (cg64_elems < need64) {
+      if (cg64) (void)hipFree(cg64);
+      cg64 = nullptr;
+      LS_CHECK(hipMalloc(&cg64, sizeof(double) * need64));
+      cg64_elems = need64;
+    }
+    T *x = x_out, *s = cg, *pt = cg + model, *r = cg + 2 * model, *q = cg + 2 * model + nrec;
     T* wp = cg + 2 * model + 2 * nrec;  // W p
+    double *s64 = cg64, *p64 = cg64 + model, *U = cg64 + 2 * model;  // U: orthonormal gradients
     const int G = 1024, B = 256;
     // the operator's adjoint side: s = W (L^T r)
     auto adjoint_w = [&](const T* rr, T* ss) {
@@ -681,13 +723,31 @@ struct LsrtmPlan {
       if (!e && weight) hipLaunchKernelGGL((mul_kernel<T>), dim3(G), dim3(B), 0, stream, ss, ss, weight, model);
       return e;
     };
+    // s64 = s orthogonalised against the k stored gradients, stored as U[k];
+    // returns ||s64||^2
+    auto orthogonalise = [&](int k, double* norm2) {
+      hipLaunchKernelGGL((scale_to64_kernel<T>), dim3(G), dim3(B), 0, stream, s64, s, 1.0, model);
+      for (int j = 0; j < k; j++) {
+        double c;
+        int e = dot64(s64, U + (size_t)j * model, model, &c);
+        if (e) return e;
+        hipLaunchKernelGGL((axpy_kernel<double>), dim3(G), dim3(B), 0, stream, s64, U + (size_t)j * model, -c,
+                           model);
+      }
+      int e = dot64(s64, s64, model, norm2);
+      if (e) return e;
+      if (*norm2 > 0)
+        hipLaunchKernelGGL((scale_to64_kernel<double>), dim3(G), dim3(B), 0, stream, U + (size_t)k * model, s64,
+                           1.0 / std::sqrt(*norm2), model);
+      return SEIS_OK;
+    };
     hipLaunchKernelGGL((fill_kernel<T>), dim3(G), dim3(B), 0, stream, x, T(0), model);
     LS_CHECK(hipMemcpyAsync(r, d, sizeof(T) * (size_t)nrec, hipMemcpyDeviceToDevice, stream));
     int rc = adjoint_w(r, s);
     if (rc) return rc;
-    LS_CHECK(hipMemcpyAsync(p, s, sizeof(T) * (size_t)model, hipMemcpyDeviceToDevice, stream));
     double gamma, rr;
-    if ((rc = dot(s, s, model, &gamma)) || (rc = dot(r, r, nrec, &rr))) return rc;
+    if ((rc = orthogonalise(0, &gamma)) || (rc = dot(r, r, nrec, &rr))) return rc;
+    LS_CHECK(hipMemcpyAsync(p64, s64, sizeof(double) * (size_t)model, hipMemcpyDeviceToDevice, stream));
     if (hist) {
       hist[0] = rr;
       hist[1] = gamma;
@@ -696,24 +756,32 @@ struct LsrtmPlan {
     const double gamma0 = gamma;
     for (int it = 0; it < niter; it++) {
       if (!(gamma > 0) || (rtol > 0 && gamma <= rtol * rtol * gamma0)) break;
+      hipLaunchKernelGGL((from64_kernel<T>), dim3(G), dim3(B), 0, stream, pt, p64, model);
       if (weight) {
-        hipLaunchKernelGGL((mul_kernel<T>), dim3(G), dim3(B), 0, stream, wp, p, weight, model);
+        hipLaunchKernelGGL((mul_kernel<T>), dim3(G), dim3(B), 0, stream, wp, pt, weight, model);
         if ((rc = born(wp, q))) return rc;
-      } else if ((rc = born(p, q))) {
+      } else if ((rc = born(pt, q))) {
         return rc;
       }
       double delta;
       if ((rc = dot(q, q, nrec, &delta))) return rc;
       if (!(delta > 0)) break;
       const double alpha = gamma / delta;
-      hipLaunchKernelGGL((axpy_kernel<T>), dim3(G), dim3(B), 0, stream, x, p, alpha, model);
+      hipLaunchKernelGGL((axpy_from64_kernel<T>), dim3(G), dim3(B), 0, stream, x, p64, alpha, model);
       hipLaunchKernelGGL((axpy_kernel<T>), dim3(G), dim3(B), 0, stream, r, q, -alpha, nrec);
+      if (iterates) {  // the image after this iteration (W x with a weight)
+        T* xi = iterates + (size_t)it * model;
+        if (weight)
+          hipLaunchKernelGGL((mul_kernel<T>), dim3(G), dim3(B), 0, stream, xi, x, weight, model);
+        else
+          LS_CHECK(hipMemcpyAsync(xi, x, sizeof(T) * (size_t)model, hipMemcpyDeviceToDevice, stream));
+      }
       if ((rc = adjoint_w(r, s))) return rc;
       double gnew;
-      if ((rc = dot(s, s, model, &gnew)) || (rc = dot(r, r, nrec, &rr))) return rc;
+      if ((rc = orthogonalise(it + 1, &gnew)) || (rc = dot(r, r, nrec, &rr))) return rc;
       const double beta = gnew / gamma;
       gamma = gnew;
-      hipLaunchKernelGGL((xpby_kernel<T>), dim3(G), dim3(B), 0, stream, p, s, beta, model);
+      hipLaunchKernelGGL((xpby_kernel<double>), dim3(G), dim3(B), 0, stream, p64, s64, beta, model);
       if (hist) {
         hist[2 * (it + 1)] = rr;
         hist[2 * (it + 1) + 1] = gamma;
@@ -722,6 +790,14 @@ struct LsrtmPlan {
     }
     if (weight) hipLaunchKernelGGL((mul_kernel<T>), dim3(G), dim3(B), 0, stream, x, x, weight, model);
     LS_CHECK(hipGetLastError());
+    LS_CHECK(hipStreamSynchronize(stream));
+    return SEIS_OK;
+  }
+
+  int dot64(const double* a, const double* b, long long n, double* out) {
+    hipLaunchKernelGGL((dot_partial_kernel<double>), dim3(RGRID), dim3(RB), 0, stream, a, b, n, part);
+    hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(RB), 0, stream, part, RGRID, part + RGRID);
+    LS_CHECK(hipMemcpyAsync(out, part + RGRID, sizeof(double), hipMemcpyDeviceToHost, stream));
     LS_CHECK(hipStreamSynchronize(stream));
     return SEIS_OK;
   }
@@ -825,9 +901,14 @@ int seis_lsrtm_set_mask(seis_lsrtm* plan, const void* mask) {
 
 int seis_lsrtm_cg(seis_lsrtm* plan, const void* dobs, int niter, double rtol, void* image, double* hist,
                   int* iters) {
+  return seis_lsrtm_cg_history(plan, dobs, niter, rtol, image, hist, iters, nullptr);
+}
+
+int seis_lsrtm_cg_history(seis_lsrtm* plan, const void* dobs, int niter, double rtol, void* image, double* hist,
+                          int* iters, void* iterates) {
   if (!plan || !dobs || !image || niter < 0) return set_error("lsrtm: bad argument"), SEIS_ERR_ARG;
-  return plan->fp64 ? plan->d->cgls((const double*)dobs, niter, rtol, (double*)image, hist, iters)
-                    : plan->f->cgls((const float*)dobs, niter, rtol, (float*)image, hist, iters);
+  return plan->fp64 ? plan->d->cgls((const double*)dobs, niter, rtol, (double*)image, hist, iters, (double*)iterates)
+                    : plan->f->cgls((const float*)dobs, niter, rtol, (float*)image, hist, iters, (float*)iterates);
 }
 
 }  // extern "C"

@@ -26,10 +26,8 @@ class LsrtmPlan:
     def __init__(self, grid, shots, vel, wavs, gzs, snapshot=SNAP_BF16, fp64=False):
         self.grid, self.shots, self.fp64 = grid, shots, fp64
         self.dtype = np.float64 if fp64 else np.float32
-        arr, nw, ng, nr = _layout(grid, shots)
         # every shot's velocity is its window [x0, x0 + nx) of the one model
-        for j, s in enumerate(shots):
-            arr[j].model = s.x0 * grid.nz
+        arr, nw, ng, nr = _layout(grid, shots, views=True)
         self._arr = arr
         self.nrec = nr
         self.model = (grid.nx, grid.nz)
@@ -108,11 +106,18 @@ class LsrtmPlan:
         self._mask = self._dev(mask, (self.nrec,))
         check(lib().seis_lsrtm_set_mask(self.handle, self._mask.ptr))
 
-    def cg(self, dobs, niter, rtol=0.0):
-        """CGLS image and history [(||r||^2, ||L^T r||^2)] per completed iteration."""
+    def cg(self, dobs, niter, rtol=0.0, iterates=False):
+        """CGLS image and history [(||r||^2, ||L^T r||^2)] per completed
+        iteration; with iterates=True also the image after every iteration
+        ([iterations][nx][nz])."""
         dd = self._dev(dobs, (self.nrec,))
         img = DeviceArray(self.model, self.dtype, zero=True)
         hist = (ctypes.c_double * (2 * (niter + 1)))()
         done = ctypes.c_int(0)
-        check(lib().seis_lsrtm_cg(self.handle, dd.ptr, int(niter), float(rtol), img.ptr, hist, ctypes.byref(done)))
-        return img.download(), np.array(hist[:]).reshape(-1, 2)[: done.value + 1]
+        its = DeviceArray((max(niter, 1),) + self.model, self.dtype, zero=True) if iterates else None
+        check(lib().seis_lsrtm_cg_history(self.handle, dd.ptr, int(niter), float(rtol), img.ptr, hist,
+                                          ctypes.byref(done), its.ptr if its is not None else None))
+        h = np.array(hist[:]).reshape(-1, 2)[: done.value + 1]
+        if iterates:
+            return img.download(), h, its.download()[: done.value]
+        return img.download(), h
