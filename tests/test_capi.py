@@ -79,18 +79,26 @@ def test_plane_prepare_argument_errors_without_gpu():
     assert L.seis_harmonic(None, None, 4, 1e-3, 10.0) == 1
 
 
-def test_rtm_column_limit_without_gpu():
-    """The imaging pass holds a whole padded column (U and R) in LDS: a padded
-    depth beyond 8192 cells is rejected with the arguments, before any device
-    work (so also here, without a GPU)."""
+def test_rtm_shot_checks_without_gpu():
+    """Shot arguments are checked before any device work (so also here,
+    without a GPU): an RTM window narrower than the 4-layer strips, a
+    negative width and a source outside its window are SEIS_ERR_ARG; a tall
+    column is fine (the imaging pass tiles it)."""
     import ctypes
     import seis_amd
     from seis_amd._lib import seis_grid, seis_shot
     L = seis_amd.lib()
-    g = seis_grid(8, 8200, 0, 0.001, 10.0, 10.0)
-    shots = (seis_shot * 1)(seis_shot(2, 2, 10, 0, 0, 0, 0, 0))
-    assert L.seis_rtm_batch(ctypes.byref(g), 1, shots, None, None, None, None, None) == 1
-    assert "too large" in L.seis_last_error().decode()
+    g = seis_grid(40, 8200, 4, 0.001, 10.0, 10.0)
+    for shot, msg in ((seis_shot(1, 2, 10, 0, 0, 0, 0, 0, 3, 0), "4-layer"),
+                      (seis_shot(1, 2, 10, 0, 0, 0, 0, 0, -1, 0), "nx must be"),
+                      (seis_shot(12, 2, 10, 0, 0, 0, 0, 0, 10, 0), "source cell")):
+        shots = (seis_shot * 1)(shot)
+        assert L.seis_rtm_batch(ctypes.byref(g), 1, shots, None, None, None, None, None) == 1
+        assert msg in L.seis_last_error().decode()
+    starts = (ctypes.c_int * 2)()
+    n = ctypes.c_int(0)
+    ok = (seis_shot * 1)(seis_shot(2, 2, 10, 0, 0, 0, 0, 0, 0, 0))
+    assert L.seis_batch_plan(2, ctypes.byref(g), 1, ok, 1 << 40, 0, starts, ctypes.byref(n)) == 0 and n.value == 1
 
 
 def test_tuning_switch_without_gpu():
