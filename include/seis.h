@@ -165,8 +165,18 @@ int seis_su_split(const char* su_file, const char* info_file, const char* coord_
 
 /* Parses a parameter file like Inversion::readParamFile (inversion.cpp:38-79,
  * grammar of parser.cpp:16-168) and writes "key = value" lines into out
- * (out_len bytes, NUL-terminated); errors name the missing key. */
+ * (out_len bytes, NUL-terminated; floats with 9 significant digits, np / dp /
+ * pmin only read with planeTag != 0, maskfile only required with mask != 0);
+ * errors name the missing key. */
 int seis_read_params(const char* param_file, char* out, int out_len);
+
+/* Reads a coordinates file as Coords does (coord.cpp:22-176): one trace per
+ * line "sx sz gx gz", a shot is a run of lines with the same sx.  *ns and
+ * *ntraces get the shot and trace counts; when they fit max_shots /
+ * max_traces, ng[ns] the traces per shot, sxz[ns][2] each shot's (sx, sz)
+ * and gxz[ntraces][2] each trace's (gx, gz), shot after shot. */
+int seis_read_coords(const char* coord_file, int max_shots, int max_traces, int* ns, int* ntraces, int* ng,
+                     float* sxz, float* gxz);
 
 /* Batched variants: all array arguments are DEVICE pointers (seis_malloc);
  * shots[] is a host array.  Shots run concurrently in the same launches;
@@ -182,8 +192,8 @@ int seis_born_batch(const seis_grid* grid, int nshots, const seis_shot* shots, c
  * harnesses that forbid allocation).  No counterpart in the reference.
  * Same arguments and results as seis_modeling_batch; the wavefields and shot
  * descriptors live in a static device scratch of 64 MiB per GPU (held until
- * the process ends), which holds 4 padded fields per shot, e.g. about 48
- * shots of 300 x 200 cells with pml 14 at once; larger batches run in
+ * the process ends), which holds 4 padded fields per shot, e.g. 39 shots
+ * of 300 x 200 cells with pml 14 at once; larger batches run in
  * consecutive chunks of the scratch.  A shot whose fields alone exceed the
  * scratch is SEIS_ERR_ARG.  Every call on a GPU shares that GPU's scratch:
  * calls that overlap on one GPU (other host threads) race.  The strip length

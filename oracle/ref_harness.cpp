@@ -20,6 +20,11 @@
 #include "InitInversion.h"
 #include "../filter/hilbert.h"
 #include "io.h"
+#include "parser.h"
+#include "coord.h"
+#include "../lsrtm/datastr.h"
+#include <cstdio>
+#include <string>
 
 namespace {
 
@@ -135,6 +140,97 @@ int ref_rtm(float dt, float dx, float dz, int nt, int ndel, int nx, int nz, int 
                r.p),
        img, (size_t)nz * nx);
   return 0;
+}
+
+// Inversion::readParamFile's sequence of parser calls (inversion.cpp:38-79)
+// on a paramLSRTM (datastr.h:51-75), printed as "key = value" lines in the
+// order of the driver's seis_read_params (floats with 9 significant digits).
+// Only files whose required keys are present: the parser exits otherwise.
+void ref_read_params(const char* file, char* out, int len) {
+  ParserFromFile pff{std::string(file)};
+  paramLSRTM param;
+  pff.getInt(param.nthread);
+  pff.getInt(param.ns);
+  pff.getInt(param.ngmax);
+  pff.getInt(param.nx);
+  pff.getInt(param.nz);
+  pff.getInt(param.nt);
+  pff.getInt(param.npml);
+  pff.getInt(param.delay);
+  pff.getInt(param.delaycal);
+  pff.getInt(param.mask);
+  pff.getInt(param.maxiter);
+  pff.getInt(param.lpad);
+  pff.getInt(param.rpad);
+  pff.getInt(param.planeTag);
+  pff.getInt(param.sbegin);
+  pff.getInt(param.send);
+  pff.getFloat(param.dx);
+  pff.getFloat(param.dz);
+  pff.getFloat(param.dt);
+  pff.getFloat(param.fr);
+  pff.getFloat(param.velfx);
+  pff.getString(param.precsg);
+  pff.getString(param.wdir);
+  pff.getString(param.v0file);
+  pff.getString(param.vfile);
+  pff.getString(param.coordfile);
+  if (param.mask.val) {
+    param.maskfile.opt = 0;
+    pff.getString(param.maskfile);
+  }
+  if (param.planeTag.val) {
+    pff.getInt(param.np);
+    pff.getFloat(param.dp);
+    pff.getFloat(param.pmin);
+  }
+  std::string o;
+  char b[512];
+  for (paramInt* q : {&param.nthread, &param.ns, &param.ngmax, &param.nx, &param.nz, &param.nt, &param.npml,
+                      &param.delay, &param.delaycal, &param.mask, &param.maxiter, &param.lpad, &param.rpad,
+                      &param.planeTag, &param.sbegin, &param.send, &param.np}) {
+    std::snprintf(b, sizeof(b), "%s = %d\n", q->key.c_str(), q->val);
+    o += b;
+  }
+  for (paramFloat* q : {&param.dx, &param.dz, &param.dt, &param.fr, &param.velfx, &param.dp, &param.pmin}) {
+    std::snprintf(b, sizeof(b), "%s = %.9g\n", q->key.c_str(), (double)q->val);
+    o += b;
+  }
+  for (paramString* q : {&param.precsg, &param.wdir, &param.v0file, &param.vfile, &param.coordfile, &param.maskfile})
+    o += q->key + " = " + q->val + "\n";
+  std::snprintf(out, len, "%s", o.c_str());
+}
+
+// Coords (coord.cpp:22-176): shots, receivers per shot, source (x, z) per
+// shot, receiver (x, z) per trace packed shot after shot.  Returns the number
+// of shots; fills the arrays when they hold cap_shots / cap_traces entries.
+int ref_coords(const char* file, int cap_shots, int cap_traces, int* ng, float* sxz, float* gxz) {
+  const std::string f(file);
+  const int ns = Coords::countShots(f);
+  int* n = nullptr;
+  Coords::countReceivers(f, ns, n);
+  float** sc = nullptr;
+  Coords::getSources(f, ns, sc);
+  float*** gc = nullptr;
+  Coords::getReceivers(f, ns, n, gc);
+  int total = 0;
+  for (int s = 0; s < ns; s++) total += n[s];
+  if (ns <= cap_shots && total <= cap_traces) {
+    int t = 0;
+    for (int s = 0; s < ns; s++) {
+      ng[s] = n[s];
+      sxz[2 * s] = sc[s][0];
+      sxz[2 * s + 1] = sc[s][1];
+      for (int g = 0; g < n[s]; g++, t++) {
+        gxz[2 * t] = gc[s][0][g];
+        gxz[2 * t + 1] = gc[s][1][g];
+      }
+    }
+  }
+  MyAlloc<int>::free(n);
+  MyAlloc<float>::free(sc);
+  MyAlloc<float>::free(gc);
+  return ns;
 }
 
 }  // extern "C"

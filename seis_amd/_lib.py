@@ -81,6 +81,7 @@ _SIGS = {
     "seis_fd_free": (None, [_P]),
     "seis_run": (_I, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p]),
     "seis_read_params": (_I, [ctypes.c_char_p, ctypes.c_char_p, _I]),
+    "seis_read_coords": (_I, [ctypes.c_char_p, _I, _I, ctypes.POINTER(_I), ctypes.POINTER(_I), _P, _P, _P]),
     "seis_set_batch_limits": (_I, [_SZ, _I]),
     "seis_batch_plan": (_I, [_I, ctypes.POINTER(seis_grid), _I, ctypes.POINTER(seis_shot), _SZ, _I,
                              ctypes.POINTER(_I), ctypes.POINTER(_I)]),

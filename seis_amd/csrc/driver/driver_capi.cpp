@@ -34,6 +34,30 @@ int seis_read_params(const char* param_file, char* out, int out_len) {
   return SEIS_OK;
 }
 
+int seis_read_coords(const char* coord_file, int max_shots, int max_traces, int* ns, int* ntraces, int* ng,
+                     float* sxz, float* gxz) {
+  if (!coord_file || !ns || !ntraces) return set_error("seis_read_coords: null argument"), SEIS_ERR_ARG;
+  Survey s;
+  std::string err;
+  if (!read_coords(coord_file, s, err)) return set_error(err), SEIS_ERR_ARG;
+  int total = 0;
+  for (int is = 0; is < s.ns(); is++) total += s.ng(is);
+  *ns = s.ns();
+  *ntraces = total;
+  if (s.ns() > max_shots || total > max_traces || !ng || !sxz || !gxz) return SEIS_OK;  // sizes only
+  int t = 0;
+  for (int is = 0; is < s.ns(); is++) {
+    ng[is] = s.ng(is);
+    sxz[2 * is] = s.sx[is];
+    sxz[2 * is + 1] = s.sz[is];
+    for (int g = 0; g < s.ng(is); g++, t++) {
+      gxz[2 * t] = s.gx[is][g];
+      gxz[2 * t + 1] = s.gz[is][g];
+    }
+  }
+  return SEIS_OK;
+}
+
 int seis_run(const char* param_file, const char* mode, const char* model_file, const char* out_file) {
   if (!param_file || !mode) return set_error("seis_run: null argument"), SEIS_ERR_ARG;
   Inversion inv;

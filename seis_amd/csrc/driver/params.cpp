@@ -1,6 +1,7 @@
 #include "params.h"
 
 #include <cctype>
+#include <cstdio>
 #include <fstream>
 #include <sstream>
 
@@ -114,9 +115,13 @@ bool read_params(const std::string& path, Params& p, std::string& err) {
 std::string format_params(const Params& p) {
   std::ostringstream o;
   for (const IntParam* q : {&p.nthread, &p.ns, &p.ngmax, &p.nx, &p.nz, &p.nt, &p.npml, &p.delay, &p.delaycal,
-                            &p.mask, &p.maxiter, &p.lpad, &p.rpad, &p.planeTag, &p.sbegin, &p.send})
+                            &p.mask, &p.maxiter, &p.lpad, &p.rpad, &p.planeTag, &p.sbegin, &p.send, &p.np})
     o << q->key << " = " << q->val << "\n";
-  for (const FloatParam* q : {&p.dx, &p.dz, &p.dt, &p.fr, &p.velfx}) o << q->key << " = " << q->val << "\n";
+  char b[64];
+  for (const FloatParam* q : {&p.dx, &p.dz, &p.dt, &p.fr, &p.velfx, &p.dp, &p.pmin}) {
+    std::snprintf(b, sizeof(b), "%.9g", (double)q->val);  // exact for float
+    o << q->key << " = " << b << "\n";
+  }
   for (const StringParam* q : {&p.precsg, &p.wdir, &p.v0file, &p.vfile, &p.coordfile, &p.maskfile})
     o << q->key << " = " << q->val << "\n";
   return o.str();

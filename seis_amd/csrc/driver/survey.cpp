@@ -5,6 +5,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <iterator>
 #include <sstream>
 
 namespace seis {
@@ -16,28 +17,88 @@ int Survey::ngmax() const {
 }
 
 bool read_coords(const std::string& path, Survey& s, std::string& err) {
-  std::ifstream in(path);
+  // The reference reads the file line by line with getline until eof and
+  // scans each of the four fields on its own (coord.cpp:22-176); a field that
+  // does not scan keeps the previous line's value.  A line whose source x
+  // differs from the previous line's starts a shot, every other line adds a
+  // receiver to the current one.  Its counts have two quirks kept here for
+  // identical geometry: the last shot's count is one less than its lines
+  // (a file that ends with a newline reads one empty line more, which repeats
+  // the last receiver, so the count comes out right; without that newline the
+  // last receiver is dropped), and a first shot of a single line keeps the
+  // count 0 when another shot follows (countReceivers sets a shot's count
+  // only from its second line on).
+  std::ifstream in(path, std::ios::binary);
   if (!in) {
     err = "Unable to open " + path;
     return false;
   }
-  s = Survey();
-  std::string line;
-  bool first = true;
-  float last_sx = 0.f;
-  while (std::getline(in, line)) {
-    float a, b, c, d;
-    if (std::sscanf(line.c_str(), "%f %f %f %f", &a, &b, &c, &d) != 4) continue;
-    if (first || a != last_sx) {
-      s.sx.push_back(a);
-      s.sz.push_back(b);
-      s.gx.emplace_back();
-      s.gz.emplace_back();
-      first = false;
+  std::string text((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+  if (text.find_first_not_of(" \t\r\n") == std::string::npos) {
+    err = "no traces in " + path;
+    return false;
+  }
+  std::vector<std::string> lines;
+  size_t at = 0;
+  while (at <= text.size()) {  // getline until eof: a final newline yields one empty line more
+    const size_t e = text.find('\n', at);
+    if (e == std::string::npos) {
+      if (at < text.size() || lines.empty()) lines.push_back(text.substr(at));
+      else lines.push_back("");
+      break;
     }
-    s.gx.back().push_back(c);
-    s.gz.back().push_back(d);
-    last_sx = a;
+    lines.push_back(text.substr(at, e - at));
+    at = e + 1;
+    if (at == text.size()) {
+      lines.push_back("");
+      break;
+    }
+  }
+  s = Survey();
+  float f[4] = {0.f, 0.f, 0.f, 0.f};
+  auto scan = [&](const std::string& l) {
+    // each field on its own, as sscanf(buffer, "%*s%*s%f", ...) does
+    const char* fmt[4] = {"%f", "%*s%f", "%*s%*s%f", "%*s%*s%*s%f"};
+    for (int k = 0; k < 4; k++) std::sscanf(l.c_str(), fmt[k], &f[k]);
+  };
+  std::vector<int> ng;
+  std::vector<std::vector<float>> gx, gz;
+  scan(lines[0]);
+  float hs0 = f[0];
+  s.sx.push_back(f[0]);
+  s.sz.push_back(f[1]);
+  gx.push_back({f[2]});
+  gz.push_back({f[3]});
+  ng.push_back(0);
+  int cs = 0, ig = 1;
+  float hg0 = f[2], vg0 = f[3];
+  for (size_t i = 1; i < lines.size(); i++) {
+    scan(lines[i]);
+    if (f[0] == hs0) {
+      ig++;
+    } else {
+      cs++;
+      ig = 1;
+      s.sx.push_back(f[0]);
+      s.sz.push_back(f[1]);
+      gx.emplace_back();
+      gz.emplace_back();
+      ng.push_back(0);
+    }
+    ng[cs] = ig;
+    gx[cs].push_back(f[2]);
+    gz[cs].push_back(f[3]);
+    hs0 = f[0];
+    hg0 = f[2];
+    vg0 = f[3];
+  }
+  if (hg0 == f[2] && vg0 == f[3]) ng[cs] = ig - 1;
+  s.gx.resize(gx.size());
+  s.gz.resize(gz.size());
+  for (size_t is = 0; is < gx.size(); is++) {
+    const size_t n = (size_t)std::max(ng[is], 0);
+    s.gx[is].assign(gx[is].begin(), gx[is].begin() + std::min(n, gx[is].size()));
+    s.gz[is].assign(gz[is].begin(), gz[is].begin() + std::min(n, gz[is].size()));
   }
   if (s.sx.empty()) {
     err = "no traces in " + path;

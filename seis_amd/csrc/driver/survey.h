@@ -10,8 +10,8 @@
 namespace seis {
 
 // Coordinates file: one trace per line "sx sz gx gz" (metres); a shot is a
-// run of consecutive lines with the same sx (coord.cpp:22-176).  Lines that do
-// not hold four numbers are skipped.
+// run of consecutive lines with the same sx, read with the reference's
+// counting rules (coord.cpp:22-176; see read_coords).
 struct Survey {
   std::vector<float> sx, sz;               // per shot
   std::vector<std::vector<float>> gx, gz;  // per shot, per receiver

@@ -898,7 +898,7 @@ int fields_per_shot(int mode) {
 // its descriptor), each part a multiple of 256 bytes.
 size_t shot_bytes(int mode, const GridSpec& g, const ShotSpec& q, size_t elem, size_t desc) {
   const int NX = shot_width(g, q) + 2 * g.pml, NZ = g.nz + 2 * g.pml;
-  const long long ld = field_ld(NZ);
+  const long long ld = field_ld(NZ, g.pml);
   size_t b = (size_t)field_elems(NX, ld) * fields_per_shot(mode) * elem;
   long long strip_t = mode == RUN_RTM ? q.nt : mode == RUN_RTM_PLANE ? q.nt + q.ndel : 0;
   size_t st = (size_t)strip_t * 8 * (shot_width(g, q) + g.nz) * elem;
@@ -993,7 +993,7 @@ int run_chunk(int mode, const GridSpec& g, int n, const ShotSpec* shots, T* cons
               const T* wav, const int* gz, T* rec, void* mem, hipStream_t stream, int rtm_flags, int line_src,
               bool quiet) {
   const int NZ = g.nz + 2 * g.pml;
-  const long long ld = field_ld(NZ);
+  const long long ld = field_ld(NZ, g.pml);
   const bool ilm = (mode == RUN_ILLUM);
   const int nslots = (mode == RUN_MODEL || ilm) ? 3 : 6;
   const bool rtm = (mode == RUN_RTM);
@@ -1013,7 +1013,7 @@ int run_chunk(int mode, const GridSpec& g, int n, const ShotSpec* shots, T* cons
   std::vector<DevShot<T>> host(n);
   DevShot<T>* dev = reinterpret_cast<DevShot<T>*>(mem);
   char* p = reinterpret_cast<char*>(mem) + (sizeof(DevShot<T>) * n + 255) / 256 * 256;
-  const long long origin = (long long)MX * ld + MZ;
+  const long long origin = field_origin(ld, g.pml);
   for (int s = 0; s < n; s++) {
     const ShotSpec& q = shots[s];
     const int nx = shot_width(g, q);
@@ -1238,7 +1238,7 @@ int run_rtm_plane_chunk(const GridSpec& g, int n, const ShotSpec* shots, T* cons
                         const int* gz, const T* rec, void* mem, hipStream_t stream) {
   const int NZ = g.nz + 2 * g.pml;
   const int NX = g.nx + 2 * g.pml;
-  const long long ld = field_ld(NZ);
+  const long long ld = field_ld(NZ, g.pml);
   const long long fe = field_elems(NX, ld);
   int NTmax = 0;
   for (int s = 0; s < n; s++) NTmax = std::max(NTmax, shots[s].nt + shots[s].ndel);
@@ -1249,7 +1249,7 @@ int run_rtm_plane_chunk(const GridSpec& g, int n, const ShotSpec* shots, T* cons
   std::vector<DevShot<T>> host(n);
   DevShot<T>* dev = reinterpret_cast<DevShot<T>*>(mem);
   char* p = reinterpret_cast<char*>(mem) + (sizeof(DevShot<T>) * n + 255) / 256 * 256;
-  const long long origin = (long long)MX * ld + MZ;
+  const long long origin = field_origin(ld, g.pml);
   std::vector<T*> bases(n);
   for (int s = 0; s < n; s++) {
     const ShotSpec& q = shots[s];

@@ -425,7 +425,7 @@ struct LsrtmPlan {
     if (kind < 0 || kind > 2) return set_error("lsrtm: snapshot kind must be 0, 1 or 2"), SEIS_ERR_ARG;
     if (ns <= 0) return set_error("lsrtm: no shots"), SEIS_ERR_ARG;
     NZ = g.nz + 2 * g.pml;
-    ld = field_ld(NZ);
+    ld = field_ld(NZ, g.pml);
     model = (long long)g.nx * g.nz;
     spec.assign(shots, shots + ns);
     NXs.resize(ns);
@@ -481,7 +481,7 @@ struct LsrtmPlan {
       }
     }
     host.resize(ns);
-    const long long origin = (long long)MX * ld + MZ;
+    const long long origin = field_origin(ld, g.pml);
     char* sp = reinterpret_cast<char*>(snaps);
     float* sc = scales;
     T* fp = pool;
@@ -549,7 +549,7 @@ struct LsrtmPlan {
   int zero_fields(int first, int count) {
     for (int s = 0; s < nshots; s++) {
       const long long fe = host[s].fstride;
-      T* base = host[s].fld - ((long long)MX * ld + MZ);
+      T* base = host[s].fld - (field_origin(ld, g.pml));
       LS_CHECK(hipMemsetAsync(base + (size_t)fe * first, 0, sizeof(T) * (size_t)fe * count, stream));
     }
     return SEIS_OK;

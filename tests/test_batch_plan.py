@@ -1,8 +1,8 @@
 """Cutting a batch into chunks that fit a device-memory budget (host-side
 arithmetic of the batch calls; no GPU needed).
 
-A shot holds nf padded fields of (NX + 8) x ld floats (ld = NZ + 8 rounded
-up to 64) for the run, plus, for RTM, its boundary strips of nt * 8 * (nx +
+A shot holds nf padded fields of (NX + 8) x ld floats (ld = NZ + 4 + a top
+margin that aligns row max(pml, 4), rounded up to 64) for the run, plus, for RTM, its boundary strips of nt * 8 * (nx +
 nz) floats, plus its descriptor.  The cut must be consecutive, cover every
 shot, keep each chunk within the budget, and be greedy: the next shot would
 not have fitted.  Also the host-side checks of the batch wrappers."""
@@ -21,7 +21,10 @@ def _fixed_bytes(mode, grid, s):
     """Fields and strips of one shot (its descriptor excluded)."""
     nx = s.nx if s.nx > 0 else grid.nx
     NX, NZ = nx + 2 * grid.pml, grid.nz + 2 * grid.pml
-    ld = (NZ + 8 + 63) // 64 * 64
+    m = max(grid.pml, 4) % 64          # top margin: row max(pml, 4) on a 64-float boundary
+    mz = (64 - m) % 64
+    mz = mz + 64 if mz < 4 else mz
+    ld = (mz + NZ + 4 + 63) // 64 * 64
     b = (NX + 8) * ld * FIELDS[mode] * 4
     nst = s.nt if mode == PLAN_RTM else s.nt + s.ndel if mode == PLAN_RTM_PLANE else 0
     st = nst * 8 * (nx + grid.nz) * 4

@@ -48,8 +48,8 @@ def test_static_modeling_equals_allocating_entry_point(engine):
 
 
 def test_static_modeling_chunks_within_scratch(engine):
-    """More shots than the 64 MiB scratch holds at once (48 of these):
-    the call runs them in consecutive chunks in the scratch (48 per chunk)."""
+    """More shots than the 64 MiB scratch holds at once (39 of these):
+    the call runs them in consecutive chunks in the scratch (39 per chunk)."""
     print("case no_alloc chunked: 70 shots 300x200", flush=True)
     a, b = _both(_case(70, 300, 200, 5))
     assert np.array_equal(a, b)
