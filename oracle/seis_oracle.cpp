@@ -30,9 +30,10 @@
 //   habc_update, modeling, born, rtm  : golden vectors of the compiled reference
 //   born_adjoint (exact transpose)    : dot-product test (an invariant), and
 //                                       linearization of modeling
-//   cg_solve                          : dense FP64 solve on small SPD systems
 //   born_adjoint_xcorr                : equals born_adjoint where no wave reaches
 //                                       the band (FP64), and the dot test with born there
+// (The CG solver has no CPU restatement: the GPU CGLS is checked against CGLS
+// written out in numpy on the explicit matrix of the operator, in tests/.)
 
 #include <cmath>
 #include <cstddef>

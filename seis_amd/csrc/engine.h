@@ -43,8 +43,8 @@ struct DevShot {
 };
 
 // U buffers of the RTM reverse pass: the imaging pass applies two
-// consecutive steps at once while the next two steps run.
-constexpr int NAUX = 4;
+// consecutive steps at once, after the second of them.
+constexpr int NAUX = 2;
 
 // Step kernel modes.  MODEL / BORN / RTM_* follow the reference's modeling,
 // forward and adjoint (fd.cpp:15-177, 592-719); SNAP_* run the background of

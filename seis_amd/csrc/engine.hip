@@ -816,31 +816,6 @@ template void launch_step<float>(int, const Consts<float>&, const DevShot<float>
 template void launch_step<double>(int, const Consts<double>&, const DevShot<double>*, const Layout&, const Layout&,
                                   int, int, hipStream_t);
 
-// Second stream of the RTM reverse pass and its ordering events (default
-// priority; non-blocking so that it also overlaps work on the null stream).
-struct ImageStream {
-  static constexpr int NEV = 4;
-  hipStream_t s = nullptr;
-  hipEvent_t ready[NEV] = {}, done[NEV] = {};
-  int init() {
-    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess)
-      return set_error("rtm: hipStreamCreateWithFlags failed"), SEIS_ERR_HIP;
-    for (int i = 0; i < NEV; i++)
-      if (hipEventCreateWithFlags(&ready[i], hipEventDisableTiming) != hipSuccess ||
-          hipEventCreateWithFlags(&done[i], hipEventDisableTiming) != hipSuccess)
-        return set_error("rtm: hipEventCreateWithFlags failed"), SEIS_ERR_HIP;
-    return SEIS_OK;
-  }
-  ~ImageStream() {
-    if (s) (void)hipStreamSynchronize(s);
-    for (int i = 0; i < NEV; i++) {
-      if (ready[i]) (void)hipEventDestroy(ready[i]);
-      if (done[i]) (void)hipEventDestroy(done[i]);
-    }
-    if (s) (void)hipStreamDestroy(s);
-  }
-};
-
 // Zeroes the first `nfields` field slots of every shot (a fill kernel, so
 // that a caller may capture the time loop in a graph).
 template <typename T>
@@ -1145,17 +1120,14 @@ int run_chunk(int mode, const GridSpec& g, int n, const ShotSpec* shots, T* cons
     // two reverse steps per imaging pass
     const int per = 2;
     const dim3 igrid((unsigned)image_tiles(NZ, g.pml), (unsigned)nxmax, (unsigned)n);
-    // Imaging pass q covers reverse steps [per q, per q + per) and runs on a
-    // second stream after the last of them, beside the next reverse step.
-    // Step st rewrites the receiver slot of step st-3 (and the U buffer of
-    // step st-4), so it first waits for the pass that covers step st-3.
-    ImageStream is;
-    int rc;
-    if ((rc = is.init())) return rc;
+    // Imaging pass q covers reverse steps [per q, per q + per) and runs on the
+    // same stream right after the last of them, while their U buffers and
+    // receiver levels are intact.  (On a second stream beside the next reverse
+    // step it measured no faster: both are bound by HBM bandwidth, and the
+    // reverse step then loses the occupancy it needs to hide its latency.)
     for (int st = 0; st < maxbwd; st++) {
       double sb;
       const double a = cells_bwd(st, &sb, nullptr, nullptr);
-      if (st >= 3) HIPCHECK(hipStreamWaitEvent(stream, is.done[((st - 3) / per) % ImageStream::NEV], 0));
       launch(quiet, stream, "step_rtm_reverse", 8 * E * a + sb, 2 * a,
              [&] { launch_step<T>(RTM_BWD, k, dev, L, B, n, st, stream); });
       if ((st + 1) % per != 0 && st != maxbwd - 1) continue;
@@ -1164,16 +1136,11 @@ int run_chunk(int mode, const GridSpec& g, int n, const ShotSpec* shots, T* cons
       double fq0, mc0, fq1 = 0;
       cells_bwd(first, nullptr, &fq0, &mc0);
       if (pair) cells_bwd(st, nullptr, &fq1, nullptr);
-      HIPCHECK(hipEventRecord(is.ready[q % ImageStream::NEV], stream));
-      HIPCHECK(hipStreamWaitEvent(is.s, is.ready[q % ImageStream::NEV], 0));
       // per step U and R over the interior columns; the image once
-      launch(quiet, is.s, "rtm_image", 2 * E * (fq0 + fq1) + 2 * E * mc0, 0, [&] {
-        hipLaunchKernelGGL((image_kernel<half>), igrid, dim3(FQ_WG), 0, is.s, k, dev, first, taps, pair, 0);
+      launch(quiet, stream, "rtm_image", 2 * E * (fq0 + fq1) + 2 * E * mc0, 0, [&] {
+        hipLaunchKernelGGL((image_kernel<half>), igrid, dim3(FQ_WG), 0, stream, k, dev, first, taps, pair, 0);
       });
-      HIPCHECK(hipEventRecord(is.done[q % ImageStream::NEV], is.s));
     }
-    HIPCHECK(hipEventRecord(is.ready[0], is.s));
-    HIPCHECK(hipStreamWaitEvent(stream, is.ready[0], 0));
     hipLaunchKernelGGL((crop_kernel<T>), dim3(64, n), dim3(256), 0, stream, k, dev);
     HIPCHECK(hipStreamSynchronize(stream));
   }
