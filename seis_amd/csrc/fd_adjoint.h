@@ -11,8 +11,8 @@
 //             + the same for prev(.) with level m+1 + record injection.
 //
 // Every term is a gather, so each cell of lam^{m-1} is a pure function of
-// lam^m, lam^{m+1} and per-band-cell products stored when those levels were
-// written (BandAdj below).  The scatter-form statement-by-statement transpose
+// lam^m, lam^{m+1} and per-band-cell products of those levels, each a fixed
+// coefficient times lam(b) (BandAdj below).  The scatter-form statement-by-statement transpose
 // in oracle/seis_oracle.cpp (born_adjoint) is the checker.
 //
 // The reference has no transpose of its Born operator (its `adjoint`,
@@ -51,7 +51,7 @@ SEIS_HD inline BandCell classify(int x, int z, int NX, int NZ, int pml) {
   return b;
 }
 
-// Products stored for every band cell b of an adjoint level lam, with
+// Products of every band cell b of an adjoint level lam, with
 // at = -(1-w) lam(b) / den the adjoint of the one-way quantity t of b:
 //   sides  : in = (-a1+a2+2a3) at   (raw(b_in) and prev(b) coefficient)
 //            t  = -a3 at            (raw and prev tangent neighbours)
@@ -88,20 +88,11 @@ SEIS_HD inline BandAdj<T> band_adj(const Consts<T>& k, const Model<T>& m, int x,
   return r;
 }
 
-// One adjoint level with its band products (all indexed like fields).
-template <typename T>
-struct AdjLevel {
-  const T* lam;
-  const T* ain;
-  const T* at;
-  const T* ac;
-};
-
 // Which band-adjoint terms reach cell (x, z): one bit per (product, neighbour)
-// that adj_raw, adj_oneway_cur and adj_oneway_prev add.  The set depends only
+// that the raw adjoint and the one-way cur / prev terms add.  The set depends only
 // on the cell's position, so a plan can compute it once per cell.
 enum BandTerm : unsigned {
-  // adj_raw: band cells that read raw(x, z)
+  // raw adjoint: band cells that read raw(x, z)
   BT_AIN_ZM = 1u << 0,     // ain(x, z-1): top side or top corner above
   BT_AT_ZM_XM = 1u << 1,   // at(x-1, z-1): top side (tangent reader)
   BT_AT_ZM_XP = 1u << 2,   // at(x+1, z-1)
@@ -116,13 +107,13 @@ enum BandTerm : unsigned {
   BT_ATC_XP = 1u << 11,    // at(x+1, z): right corner
   BT_AT_XP_ZM = 1u << 12,  // at(x+1, z-1)
   BT_AT_XP_ZP = 1u << 13,  // at(x+1, z+1)
-  // adj_oneway_cur: cur(b) by band cell b, cur(b_in) by the side cell b
+  // one-way cur: cur(b) by band cell b, cur(b_in) by the side cell b
   BT_AC_SELF = 1u << 14,
   BT_AC_ZM = 1u << 15,
   BT_AC_ZP = 1u << 16,
   BT_AC_XM = 1u << 17,
   BT_AC_XP = 1u << 18,
-  // adj_oneway_prev: prev(b) and prev(b_in) of sides, tangent prev reads
+  // one-way prev: prev(b) and prev(b_in) of sides, tangent prev reads
   BT_AIN_SELF = 1u << 19,
   BT_PIN_ZM = 1u << 20,
   BT_PIN_ZP = 1u << 21,
@@ -175,38 +166,6 @@ SEIS_HD inline unsigned band_terms(int x, int z, int NX, int NZ, int pml) {
   return mk;
 }
 
-// Adjoint of the raw (pre-blend) value of cell (x, z) at a level:
-// w lam(c) + sum over the band cells that read raw(c) of their stored
-// product; `mk` = band_terms(x, z, ...).
-template <typename T>
-SEIS_HD inline T adj_raw_m(const Consts<T>& k, int nx, const AdjLevel<T>& L, int x, int z, unsigned mk) {
-  const long long ld = k.ld;
-  const T w = sponge_weight<T>(x, z, nx, k.nz, k.pml);
-  auto ain = [&](int xx, int zz) { return SEIS_GL(L.ain)[fidx(ld, xx, zz)]; };
-  auto at = [&](int xx, int zz) { return SEIS_GL(L.at)[fidx(ld, xx, zz)]; };
-  T r = w * SEIS_GL(L.lam)[fidx(ld, x, z)];
-  if (mk & BT_AIN_ZM) r += ain(x, z - 1);
-  if (mk & BT_AT_ZM_XM) r += at(x - 1, z - 1);
-  if (mk & BT_AT_ZM_XP) r += at(x + 1, z - 1);
-  if (mk & BT_AIN_ZP) r += ain(x, z + 1);
-  if (mk & BT_AT_ZP_XM) r += at(x - 1, z + 1);
-  if (mk & BT_AT_ZP_XP) r += at(x + 1, z + 1);
-  if (mk & BT_AIN_XM) r += ain(x - 1, z);
-  if (mk & BT_ATC_XM) r += at(x - 1, z);
-  if (mk & BT_AT_XM_ZM) r += at(x - 1, z - 1);
-  if (mk & BT_AT_XM_ZP) r += at(x - 1, z + 1);
-  if (mk & BT_AIN_XP) r += ain(x + 1, z);
-  if (mk & BT_ATC_XP) r += at(x + 1, z);
-  if (mk & BT_AT_XP_ZM) r += at(x + 1, z - 1);
-  if (mk & BT_AT_XP_ZP) r += at(x + 1, z + 1);
-  return r;
-}
-
-template <typename T>
-SEIS_HD inline T adj_raw(const Consts<T>& k, int NX, int nx, const AdjLevel<T>& L, int x, int z) {
-  return adj_raw_m(k, nx, L, x, z, band_terms(x, z, NX, k.NZ, k.pml));
-}
-
 // Derivative of the ring's extrapolated term 2 d1 - d2 (fd.cpp:1072-1091),
 // d1 = c1 (u0 + u2 - 2 u1), d2 = c2 (u1 + u3 - 2 u2), with respect to u_j
 // (j = 0..3): 2 c1 (e0 + e2 - 2 e1) - c2 (e1 + e3 - 2 e2), e_i = [j == i].
@@ -224,65 +183,16 @@ SEIS_HD inline bool is_ring(int x, int z, int NX, int NZ) { return x == 0 || z =
 
 SEIS_HD inline bool high_order(int x, int z, int NX, int NZ) { return x >= 4 && x <= NX - 5 && z >= 4 && z <= NZ - 5; }
 
-// Adjoint of the one-way band update's reads of the current level at cell
-// (x, z): cur(b) by band cell b itself and cur(b_in) by the side cell b whose
-// inner neighbour is (x, z).  Zero unless edge_dist(x, z) <= pml.
-template <typename T>
-SEIS_HD inline T adj_oneway_cur_m(const Consts<T>& k, const AdjLevel<T>& L1, int x, int z, unsigned mk) {
-  const long long ld = k.ld;
-  auto ac = [&](int xx, int zz) { return SEIS_GL(L1.ac)[fidx(ld, xx, zz)]; };
-  T acc = T(0);
-  if (mk & BT_AC_SELF) acc += ac(x, z);
-  if (mk & BT_AC_ZM) acc += ac(x, z - 1);
-  if (mk & BT_AC_ZP) acc += ac(x, z + 1);
-  if (mk & BT_AC_XM) acc += ac(x - 1, z);
-  if (mk & BT_AC_XP) acc += ac(x + 1, z);
-  return acc;
-}
-
-// Adjoint of the one-way band update's reads of the previous level (level
-// m+1) at cell (x, z).  Zero unless edge_dist(x, z) <= pml.
-template <typename T>
-SEIS_HD inline T adj_oneway_prev_m(const Consts<T>& k, int nx, const AdjLevel<T>& L2, int x, int z, unsigned mk) {
-  const int pml = k.pml;
-  const long long ld = k.ld;
-  T acc = T(0);
-  if (mk & BT_AIN_SELF) acc += SEIS_GL(L2.ain)[fidx(ld, x, z)];
-  auto pin = [&](int bx, int bz) {  // (a1+a2) at = -(1-w) lam for the inner neighbour
-    const T w = sponge_weight<T>(bx, bz, nx, k.nz, pml);
-    return -(T(1) - w) * SEIS_GL(L2.lam)[fidx(ld, bx, bz)];
-  };
-  if (mk & BT_PIN_ZM) acc += pin(x, z - 1);
-  if (mk & BT_PIN_ZP) acc += pin(x, z + 1);
-  if (mk & BT_PIN_XM) acc += pin(x - 1, z);
-  if (mk & BT_PIN_XP) acc += pin(x + 1, z);
-  // tangent neighbours: z sides on the same row, x sides on the same column
-  auto at = [&](int xx, int zz) { return SEIS_GL(L2.at)[fidx(ld, xx, zz)]; };
-  if (mk & BT_TAN_XM) acc += at(x - 1, z);
-  if (mk & BT_TAN_ZM) acc += at(x, z - 1);
-  if (mk & BT_TAN_XP) acc += at(x + 1, z);
-  if (mk & BT_TAN_ZP) acc += at(x, z + 1);
-  return acc;
-}
-
-template <typename T>
-SEIS_HD inline T adj_oneway_cur(const Consts<T>& k, int NX, const AdjLevel<T>& L1, int x, int z) {
-  return adj_oneway_cur_m(k, L1, x, z, band_terms(x, z, NX, k.NZ, k.pml));
-}
-
-template <typename T>
-SEIS_HD inline T adj_oneway_prev(const Consts<T>& k, const Model<T>& m, int NX, const AdjLevel<T>& L2, int x, int z) {
-  return adj_oneway_prev_m(k, m.nx, L2, x, z, band_terms(x, z, NX, k.NZ, k.pml));
-}
-
-// lam^{m-1}(x, z) from level m (cur terms, L1) and level m+1 (prev terms, L2),
-// without the record injection.  ar1 / ar2 give adj_raw of a cell at level m /
-// m+1 (computed directly or read from a precomputed field).  Also returns the
-// adjoint of the raw value of the cell at level m (the gradient's imaging
-// factor).
+// lam^{m-1}(x, z) from level m and level m+1, without the record injection.
+// ar1 / ar2 give the raw adjoint of a cell at level m / m+1 (w lam plus the
+// products of the band cells that read its raw value), `ow` the cell's
+// one-way terms: the adjoint of the one-way updates' reads of cur(x, z) at
+// level m plus those of prev(x, z) at level m+1 (zero unless edge_dist <= pml).
+// Also returns the adjoint of the raw value of the cell at level m (the
+// gradient's imaging factor).
 template <typename T, class AR1, class AR2>
-SEIS_HD inline T adj_value(const Consts<T>& k, const Model<T>& m, int NX, const AdjLevel<T>& L1,
-                           const AdjLevel<T>& L2, const AR1& ar1, const AR2& ar2, int x, int z, T* raw_adj_out) {
+SEIS_HD inline T adj_value(const Consts<T>& k, const Model<T>& m, int NX, const AR1& ar1, const AR2& ar2, int x,
+                           int z, T* raw_adj_out, T ow) {
   const int NZ = k.NZ, pml = k.pml, nx = m.nx;
   const long long ld = k.ld;
   auto vt2 = [&](int xx, int zz) {
@@ -358,11 +268,10 @@ SEIS_HD inline T adj_value(const Consts<T>& k, const Model<T>& m, int NX, const 
   if (NZ - 1 - z <= 3) acc += ar1(x, NZ - 1) * extrap_coef<T>(NZ - 1 - z, vv(x, NZ - 2) * k.cz, vv(x, NZ - 3) * k.cz);
   if (x <= 3) acc += ar1(0, z) * extrap_coef<T>(x, vv(1, z) * k.cx, vv(2, z) * k.cx);
   if (NX - 1 - x <= 3) acc += ar1(NX - 1, z) * extrap_coef<T>(NX - 1 - x, vv(NX - 2, z) * k.cx, vv(NX - 3, z) * k.cx);
-  // (iii) one-way reads of cur, (iv) -prev of the stencil and ring base
-  // formula at level m+1, (v) one-way reads of prev at level m+1
-  acc += adj_oneway_cur(k, NX, L1, x, z);
+  // (iii) -prev of the stencil and ring base formula at level m+1, (iv) the
+  // one-way reads of cur and prev
   acc -= ar2(x, z);
-  acc += adj_oneway_prev(k, m, NX, L2, x, z);
+  acc += ow;
   return acc;
 }
 

@@ -27,33 +27,87 @@
 namespace seis {
 
 // ---------------------------------------------------------------------------
-// Field slots of one shot in the LSRTM pool.
-enum LsSlot { LS_FIELD = 0, LS_AIN = 3, LS_AT = 6, LS_AC = 9, LS_AR = 12, LS_OW = 14, LS_NSLOT = 15 };
+// Field slots of one shot in the LSRTM pool.  Slots below LS_NADJ hold the
+// adjoint's per-level state (cleared before every adjoint): three levels of
+// lam, two of the precomputed raw adjoints and the one-way terms.  The
+// LS_KIN..LS_KC slots hold the band coefficients of the shot (written once
+// per plan): a band cell b's products of a level are coefficient x lam(b).
+enum LsSlot {
+  LS_FIELD = 0,
+  LS_AR = 3,
+  LS_OW = 5,
+  LS_NADJ = 6,
+  LS_KIN = 6,
+  LS_KT = 7,
+  LS_KC = 8,
+  LS_NSLOT = 9
+};
 
-// Adjoint level m of a shot: slot m % 3 of the level fields and band products.
+// lam of adjoint level m of a shot: slot m % 3.
 template <typename T>
-__device__ __forceinline__ AdjLevel<T> level(const DevShot<T>& s, int lev) {
-  const int q = lev % 3;
-  const long long f = s.fstride;
-  return AdjLevel<T>{s.fld + (LS_FIELD + q) * f, s.fld + (LS_AIN + q) * f, s.fld + (LS_AT + q) * f,
-                     s.fld + (LS_AC + q) * f};
+__device__ __forceinline__ T* level(const DevShot<T>& s, int lev) {
+  return s.fld + (LS_FIELD + lev % 3) * s.fstride;
 }
 
-// Raw adjoints of level `lev` for the cells with a band reader or w < 1
-// (outside [pml+1, NX-pml-1) x [pml+1, NZ-pml-1)), stored for the step kernel.
-// Band-term masks (band_terms) of the ring cells, computed once per plan.
-__global__ void band_terms_kernel(Ring R, int NX, int NZ, int pml, unsigned* __restrict__ out) {
+// Planes of the ring-P coefficient table: the sponge weight w of the cell and
+// -(1 - w) of its neighbours z-1, z+1, x-1, x+1 (the factors of the one-way
+// prev terms of adj_prep_kernel).
+enum PrepCoef { PCOEF_W = 0, PCOEF_ZM = 1, PCOEF_ZP = 2, PCOEF_XM = 3, PCOEF_XP = 4, PCOEF_N = 5 };
+
+// Band-term masks (band_terms) and coefficients of the ring cells of one
+// window width, computed once per plan.
+template <typename T>
+__global__ void band_terms_kernel(Ring R, int NX, int NZ, int pml, unsigned* __restrict__ out,
+                                  T* __restrict__ coef) {
   const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= R.n) return;
   int x, z;
   ring_cell(R, NX, NZ, j, &x, &z);
   out[j] = band_terms(x, z, NX, NZ, pml);
+  const int nx = NX - 2 * pml, nz = NZ - 2 * pml;
+  auto pin = [&](int bx, int bz) { return -(T(1) - sponge_weight<T>(bx, bz, nx, nz, pml)); };
+  coef[PCOEF_W * R.n + j] = sponge_weight<T>(x, z, nx, nz, pml);
+  coef[PCOEF_ZM * R.n + j] = pin(x, z - 1);
+  coef[PCOEF_ZP * R.n + j] = pin(x, z + 1);
+  coef[PCOEF_XM * R.n + j] = pin(x - 1, z);
+  coef[PCOEF_XP * R.n + j] = pin(x + 1, z);
+}
+
+// Band coefficients of every band cell of a shot (edge_dist < pml): the
+// products of band_adj per unit adjoint, so that a step forms a band cell's
+// products as coefficient x lam(b) without the divisions of band_adj.
+template <typename T>
+__global__ void band_coef_kernel(Consts<T> k, const DevShot<T>* __restrict__ shots) {
+  const DevShot<T>& s = shots[blockIdx.y];
+  const Ring R = shot_split(k.pml, s.NX, k.NZ);
+  const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= R.n) return;
+  int x, z;
+  ring_cell(R, s.NX, k.NZ, j, &x, &z);
+  const Model<T> m{s.vel, s.nx, k.nz, k.pml};
+  const BandAdj<T> c = band_adj(k, m, x, z, classify(x, z, s.NX, k.NZ, k.pml), T(1));
+  const long long i = fidx(k.ld, x, z);
+  s.fld[LS_KIN * s.fstride + i] = c.in;
+  s.fld[LS_KT * s.fstride + i] = c.t;
+  s.fld[LS_KC * s.fstride + i] = c.c;
 }
 
 // Raw adjoints of level `lev` for the cells with a band reader or w < 1
 // (outside [pml+1, NX-pml-1) x [pml+1, NZ-pml-1)), and the one-way terms of
 // lam^{lev-1} at those cells (levels lev and lev+1 are complete now), both
-// stored for the step kernel.  `masks` holds band_terms of each ring cell.
+// stored for the step kernel.  A band cell b's products of a level are
+// ain = kin(b) lam(b), at = kt(b) lam(b), ac = kc(b) lam(b) (band_adj).  With
+// mk = band_terms of the cell (s.pmask):
+//   raw adjoint  w lam(c) + the products ain, at of level lev of the band
+//                cells that read raw(c), one per BT_AIN_* / BT_AT_* bit;
+//   one-way cur  the products ac of level lev of band cell c itself and of the
+//                side cells whose inner neighbour c is (BT_AC_* bits);
+//   one-way prev of level lev+1: ain(c) of a side cell c, -(1 - w) lam of the
+//                inner neighbours (BT_PIN_*), and the tangent products at of
+//                the side cells beside c (BT_TAN_*).
+// Every neighbour is loaded (all lie inside the field and its margin) and the
+// terms the mask leaves out are added as zero, in a fixed order: no load waits
+// on a branch.
 template <typename T>
 __global__ __launch_bounds__(256) void adj_prep_kernel(Consts<T> k, const DevShot<T>* __restrict__ shots, int step) {
   const DevShot<T>& s = shots[blockIdx.y];
@@ -65,11 +119,62 @@ __global__ __launch_bounds__(256) void adj_prep_kernel(Consts<T> k, const DevSho
   int x, z;
   ring_cell(R, s.NX, k.NZ, j, &x, &z);
   const unsigned mk = SEIS_GL(s.pmask)[j];
-  const AdjLevel<T> L = level(s, lev);
-  const long long i = fidx(k.ld, x, z);
-  SEIS_GL(s.fld + (LS_AR + lev % 2) * s.fstride)[i] = adj_raw_m(k, s.nx, L, x, z, mk);
-  SEIS_GL(s.fld)[LS_OW * s.fstride + i] =
-      adj_oneway_cur_m(k, L, x, z, mk) + adj_oneway_prev_m(k, s.nx, level(s, lev + 1), x, z, mk);
+  const long long ld = k.ld, f = s.fstride;
+  const long long i = fidx(ld, x, z);
+  const auto lam1 = SEIS_GL(level(s, lev)), lam2 = SEIS_GL(level(s, lev + 1));
+  const auto kin = SEIS_GL(s.fld + LS_KIN * f), kt = SEIS_GL(s.fld + LS_KT * f), kc = SEIS_GL(s.fld + LS_KC * f);
+  const auto cf = SEIS_GL(s.pcoef);
+  const long long zm = i - 1, zp = i + 1, xm = i - ld, xp = i + ld;
+  // level lev
+  const T l0 = lam1[i], l_zm = lam1[zm], l_zp = lam1[zp], l_xm = lam1[xm], l_xp = lam1[xp];
+  const T l_mm = lam1[xm - 1], l_pm = lam1[xp - 1], l_mp = lam1[xm + 1], l_pp = lam1[xp + 1];
+  // raw-adjoint terms
+  const T a_zm = kin[zm] * l_zm, a_zp = kin[zp] * l_zp, a_xm = kin[xm] * l_xm, a_xp = kin[xp] * l_xp;
+  const T t_mm = kt[xm - 1] * l_mm, t_pm = kt[xp - 1] * l_pm, t_mp = kt[xm + 1] * l_mp, t_pp = kt[xp + 1] * l_pp;
+  const T t_xm = kt[xm] * l_xm, t_xp = kt[xp] * l_xp;
+  // one-way cur terms
+  const T c_0 = kc[i] * l0, c_zm = kc[zm] * l_zm, c_zp = kc[zp] * l_zp, c_xm = kc[xm] * l_xm, c_xp = kc[xp] * l_xp;
+  // level lev+1: one-way prev terms
+  const T b_0 = kin[i] * lam2[i];
+  const T p_zm = lam2[zm], p_zp = lam2[zp], p_xm = lam2[xm], p_xp = lam2[xp];
+  const T u_xm = kt[xm] * p_xm, u_zm = kt[zm] * p_zm, u_xp = kt[xp] * p_xp, u_zp = kt[zp] * p_zp;
+  const T w = cf[PCOEF_W * R.n + j];
+  const T f_zm = cf[PCOEF_ZM * R.n + j], f_zp = cf[PCOEF_ZP * R.n + j];
+  const T f_xm = cf[PCOEF_XM * R.n + j], f_xp = cf[PCOEF_XP * R.n + j];
+  auto on = [&](unsigned bit, T v) { return (mk & bit) ? v : T(0); };
+  T r = w * l0;
+  r += on(BT_AIN_ZM, a_zm);
+  r += on(BT_AT_ZM_XM, t_mm);
+  r += on(BT_AT_ZM_XP, t_pm);
+  r += on(BT_AIN_ZP, a_zp);
+  r += on(BT_AT_ZP_XM, t_mp);
+  r += on(BT_AT_ZP_XP, t_pp);
+  r += on(BT_AIN_XM, a_xm);
+  r += on(BT_ATC_XM, t_xm);
+  r += on(BT_AT_XM_ZM, t_mm);
+  r += on(BT_AT_XM_ZP, t_mp);
+  r += on(BT_AIN_XP, a_xp);
+  r += on(BT_ATC_XP, t_xp);
+  r += on(BT_AT_XP_ZM, t_pm);
+  r += on(BT_AT_XP_ZP, t_pp);
+  T oc = T(0);
+  oc += on(BT_AC_SELF, c_0);
+  oc += on(BT_AC_ZM, c_zm);
+  oc += on(BT_AC_ZP, c_zp);
+  oc += on(BT_AC_XM, c_xm);
+  oc += on(BT_AC_XP, c_xp);
+  T op = T(0);
+  op += on(BT_AIN_SELF, b_0);
+  op += on(BT_PIN_ZM, f_zm * p_zm);
+  op += on(BT_PIN_ZP, f_zp * p_zp);
+  op += on(BT_PIN_XM, f_xm * p_xm);
+  op += on(BT_PIN_XP, f_xp * p_xp);
+  op += on(BT_TAN_XM, u_xm);
+  op += on(BT_TAN_ZM, u_zm);
+  op += on(BT_TAN_XP, u_xp);
+  op += on(BT_TAN_ZP, u_zp);
+  SEIS_GL(s.fld + (LS_AR + lev % 2) * s.fstride)[i] = r;
+  SEIS_GL(s.fld)[LS_OW * s.fstride + i] = oc + op;
 }
 
 // Reads a raw adjoint: precomputed near the band, the level itself elsewhere.
@@ -103,10 +208,171 @@ __device__ __forceinline__ T adj_finish(const Consts<T>& k, const DevShot<T>& s,
 
 constexpr int AWG = 256;
 
-// One reverse step: lam^{lev-1} for every cell, the band products of the new
-// level, and grad += raw-adjoint(lev) * S^{lev-1}.  Interior cells (no band
-// term within the stencil reach) are swept by wavefronts streaming along x
-// like the forward kernel; the others use the general gather.
+// One wavefront's sweep of the adjoint step over its strip [x0, x1) of 64
+// rows from z0: lam^{lev-1} = 2 a + (9-point gather of Q = a vt2 / h^2) -
+// lam^{lev+1}, with a the raw adjoint of level lev, then the record sample,
+// the gradient update and, near the edges, the one-way terms.  DEEP: every
+// cell the strip reads (4 beyond it on each side) lies inside
+// [pml+1, NX-pml-1) x [pml+1, NZ-pml-1), where the raw adjoint is the level
+// itself; then no cell of the strip takes a one-way term and every one is an
+// active model cell.  The caller decides it per wavefront.
+template <typename T, bool DEEP>
+__device__ __forceinline__ void adj_sweep(const Consts<T>& k, const DevShot<T>& s, int lev, int x0, int x1, int z0,
+                                          int zb, T* la, int lane) {
+  const int NX = s.NX, NZ = k.NZ, pml = k.pml;
+  const long long ld = k.ld;
+  const T* const lam1 = level(s, lev);
+  const T* const lam2 = level(s, lev + 1);
+  T* out = level(s, lev - 1);
+  const Model<T> m{s.vel, s.nx, k.nz, pml};
+  const RawAdj<T> A1{s.fld + (LS_AR + lev % 2) * s.fstride, lam1, ld, NX, NZ, pml};
+  const RawAdj<T> A2{s.fld + (LS_AR + (lev + 1) % 2) * s.fstride, lam2, ld, NX, NZ, pml};
+  auto raw1 = [&](int x, int z) { return DEEP ? SEIS_GL(lam1)[fidx(ld, x, z)] : A1(x, z); };
+  auto raw2 = [&](int x, int z) { return DEEP ? SEIS_GL(lam2)[fidx(ld, x, z)] : A2(x, z); };
+  auto velat = [&](int x, int zz) {
+    // clamped outside DEEP: idle lanes may sit past the model
+    return DEEP ? SEIS_GL(s.vel)[(long long)(x - pml) * k.nz + (zz - pml)] : m.vel(x, zz);
+  };
+  auto vt2of = [&](T v) {
+    const T a = k.dt * v;
+    return a * a;
+  };
+  auto vt2at = [&](int x, int zz) { return vt2of(velat(x, zz)); };
+  // the shot's record, snapshot and gradient arrays, held in registers for the
+  // whole sweep (the stores below would otherwise make the compiler reload
+  // them from the shot table every column)
+  const int* const gzp = s.gz;
+  const T* const recp = s.rec;
+  T* const gradp = s.grad;
+  const int nxm = s.nx, ntr = s.nt, snk = s.snap_kind;
+  const int itp = lev - 2 - s.ndel;                  // record sample injected now
+  const bool grad_on = lev - 1 <= ntr + s.ndel - 1;  // imaging with S^{lev-1}
+  // Every load of the sweep is issued for every lane and column, at an
+  // in-bounds address (clamped where its value goes unused), so that no
+  // loaded register is conditionally defined: the column-ahead loads then stay
+  // in flight until their use.
+  const long long snap0 = (long long)min(lev - 1, ntr + s.ndel - 1) * nxm * k.nz;  // step lev-1 of the snapshots
+  const T* const snap_n = reinterpret_cast<const T*>(s.snap) + snap0;
+  const unsigned short* const snap_h = reinterpret_cast<const unsigned short*>(s.snap) + snap0;
+  const int z = z0 + lane;
+  const bool act = DEEP || z < zb;
+  const int zl = DEEP ? z : min(z, NZ - 1);
+  const bool halo = lane < 4 || lane >= 60;
+  // halo rows z0-4..z0-1 (lanes 0-3) and z0+64..z0+67 (lanes 60-63); the other
+  // lanes load the nearest of them (the same cache lines)
+  const int hz = lane < 32 ? z0 - 4 + min(lane, 3) : min(z0 + 64 + max(lane - 60, 0), NZ - 1);
+  const int hslot = lane < 4 ? lane : lane + 8;
+  // queues along x: qx = a vt2 / dx^2 (rows x-4..x+4), qz and a (rows x..x+4)
+  T qx[9], qz[5], lq[5];
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    const int r = x0 - 4 + j;
+    const T l = raw1(r, zl);
+    const T v2 = vt2at(r, zl);
+    qx[j] = l * (v2 * k.idx2);
+    if (j >= 4) {
+      qz[j - 4] = l * (v2 * k.idz2);
+      lq[j - 4] = l;
+    }
+  }
+  // loads of column x+1 are issued before column x is computed, including
+  // the record sample, snapshot and gradient of its model cells
+  const int iz = z - pml;
+  const bool zin = DEEP || (iz >= 0 && iz < k.nz && act);
+  const int izc = DEEP ? iz : min(max(iz, 0), k.nz - 1);
+  const int itc = max(itp, 0);
+  // the snapshot of a column is loaded a column ahead and decoded when used
+  const T scl = (grad_on && snk != SNAP_NATIVE) ? (T)SEIS_GL(s.scale)[lev - 1] : T(1);
+  // the halo raw adjoint and velocity are combined when used
+  T na, nv, nhl, nhv, nprev, nrec, nsn = T(0), ngrad;
+  T now = T(0);  // outside DEEP: the one-way terms of the next cell
+  unsigned short nsh = 0;
+  int ngz;
+  const auto OW = SEIS_GL(s.fld + LS_OW * s.fstride);
+  auto fetch = [&](int xx) {
+    xx = min(xx, x1 - 1);
+    if (!DEEP) now = OW[fidx(ld, xx, zl)];
+    na = raw1(xx + 4, zl);
+    nv = velat(xx + 4, zl);
+    nhl = raw1(xx, hz);
+    nhv = velat(xx, hz);
+    nprev = raw2(xx, zl);
+    const int ixc = DEEP ? xx - pml : min(max(xx - pml, 0), nxm - 1);
+    ngz = SEIS_GL(gzp)[ixc];
+    nrec = SEIS_GL(recp)[(long long)ixc * ntr + itc];
+    const long long j = (long long)ixc * k.nz + izc;
+    if (snk == SNAP_NATIVE)
+      nsn = SEIS_GL(snap_n)[j];
+    else
+      nsh = SEIS_GL(snap_h)[j];
+    ngrad = SEIS_GL(gradp)[j];
+  };
+  fetch(x0);
+  for (int x = x0; x < x1; x++) {
+    const T nv2 = vt2of(nv);
+    qx[8] = na * (nv2 * k.idx2);
+    qz[4] = na * (nv2 * k.idz2);
+    lq[4] = na;
+    if (halo) la[hslot] = nhl * (vt2of(nhv) * k.idz2);
+    const T prev = nprev, rec_cur = nrec, grad_cur = ngrad;
+    const T sn_cur = nsn;
+    const unsigned short sh_cur = nsh;
+    const int gz_cur = ngz;
+    const T ow_cur = now;
+    fetch(x + 1);
+    la[lane + 4] = qz[0];
+    wave_sync();
+    T sz = la[lane] * k.c[4];
+    sz += la[lane + 1] * k.c[3];
+    sz += la[lane + 2] * k.c[2];
+    sz += la[lane + 3] * k.c[1];
+    sz += la[lane + 4] * k.c[0];
+    sz += la[lane + 5] * k.c[1];
+    sz += la[lane + 6] * k.c[2];
+    sz += la[lane + 7] * k.c[3];
+    sz += la[lane + 8] * k.c[4];
+    wave_sync();
+    T sx = qx[0] * k.c[4];
+    sx += qx[1] * k.c[3];
+    sx += qx[2] * k.c[2];
+    sx += qx[3] * k.c[1];
+    sx += qx[4] * k.c[0];
+    sx += qx[5] * k.c[1];
+    sx += qx[6] * k.c[2];
+    sx += qx[7] * k.c[3];
+    sx += qx[8] * k.c[4];
+    if (act) {
+      const long long i = fidx(ld, x, z);
+      T val = T(2) * lq[0] + sz + sx - prev;
+      const int ed = DEEP ? pml + 1 : edge_dist(x, z, NX, NZ);
+      if (!DEEP && ed <= pml) val += ow_cur;  // one-way terms (prep kernel)
+      // adjoint of the record extraction and the imaging condition (as adj_finish)
+      const int ix = x - pml;
+      if (DEEP || (ix >= 0 && ix < nxm && zin)) {
+        if (itp >= 0 && z == gz_cur + pml) val += rec_cur;
+        if (grad_on) {
+          const T snap_cur = snk == SNAP_NATIVE
+                                 ? sn_cur
+                                 : (T)((snk == SNAP_BF16 ? bf16_to_f32(sh_cur) : f16_to_f32(sh_cur)) * (float)scl);
+          SEIS_GL(gradp)[(long long)ix * k.nz + iz] = grad_cur + lq[0] * snap_cur;
+        }
+      }
+      SEIS_GL(out)[i] = val;
+    }
+#pragma unroll
+    for (int j = 0; j < 8; j++) qx[j] = qx[j + 1];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      qz[j] = qz[j + 1];
+      lq[j] = lq[j + 1];
+    }
+  }
+}
+
+// One reverse step: lam^{lev-1} for every cell and grad += raw-adjoint(lev) *
+// S^{lev-1}.  Interior cells (no band term within the stencil reach) are swept
+// by wavefronts streaming along x like the forward kernel; the others use the
+// general gather.
 template <typename T>
 __global__ __launch_bounds__(AWG) void adj_step_kernel(Consts<T> k, const DevShot<T>* __restrict__ shots, Layout F,
                                                        int step) {
@@ -115,152 +381,43 @@ __global__ __launch_bounds__(AWG) void adj_step_kernel(Consts<T> k, const DevSho
   const int lev = s.nt + s.ndel + 1 - step;
   if (lev < 1) return;
   const int NX = s.NX, NZ = k.NZ, pml = k.pml;
-  const long long ld = k.ld;
-  const AdjLevel<T> L1 = level(s, lev), L2 = level(s, lev + 1);
-  const int q0 = (lev - 1) % 3;
-  T* out = s.fld + (LS_FIELD + q0) * s.fstride;
-  const Model<T> m{s.vel, s.nx, k.nz, pml};
-  const RawAdj<T> A1{s.fld + (LS_AR + lev % 2) * s.fstride, L1.lam, ld, NX, NZ, pml};
-  const RawAdj<T> A2{s.fld + (LS_AR + (lev + 1) % 2) * s.fstride, L2.lam, ld, NX, NZ, pml};
   const Split sq = shot_split(F.m, NX, NZ);  // this shot's interior
   if ((int)blockIdx.x < F.nfwg) {
     // Cells at least 8 from the outer edge: every stencil reader within 4
     // cells uses the 9-point operator, so the stencil part is the 9-point
     // gather of Q = (raw adjoint) vt2 / h^2 of the readers; cells at most
-    // pml from the edge add the one-way terms and store their band products.
-    const int wave = threadIdx.x >> 6;
+    // pml from the edge add the one-way terms.  The wavefront's index is
+    // taken uniform across its lanes, so that the strip bounds, the loop and
+    // the column addresses stay in scalar registers.
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int item = xcd_tile(blockIdx.x, F.nfwg) * WAVES + wave;
-    if (item >= F.nzc * shot_strips(sq, F.bx)) return;
+    if (item >= F.nzc * shot_strips(sq, F.bx)) return;  // whole wavefront leaves; no workgroup barrier follows
     const int zc = item % F.nzc, xs = item / F.nzc;
     const int z0 = sq.za + zc * 64;
     const int x0 = sq.xa + xs * F.bx, x1 = min(x0 + F.bx, sq.xb);
     const int lane = threadIdx.x & 63;
-    const int z = z0 + lane;
-    const bool act = z < sq.zb;
-    const int zl = min(z, NZ - 1);
-    T* la = lds[wave];
-    const bool halo = lane < 4 || lane >= 60;
-    const int hz = lane < 4 ? z0 - 4 + lane : min(z0 + lane + 4, NZ - 1);
-    const int hslot = lane < 4 ? lane : lane + 8;
-    auto vt2at = [&](int x, int zz) {
-      const T a = k.dt * m.vel(x, zz);  // clamped: idle lanes may sit past the model
-      return a * a;
-    };
-    // queues along x: qx = a vt2 / dx^2 (rows x-4..x+4), qz and a (rows x..x+4)
-    T qx[9], qz[5], lq[5];
-#pragma unroll
-    for (int j = 0; j < 8; j++) {
-      const int r = x0 - 4 + j;
-      const T l = A1(r, zl);
-      const T v2 = vt2at(r, zl);
-      qx[j] = l * (v2 * k.idx2);
-      if (j >= 4) {
-        qz[j - 4] = l * (v2 * k.idz2);
-        lq[j - 4] = l;
-      }
-    }
-    // loads of column x+1 are issued before column x is computed, including
-    // the record sample, snapshot and gradient of its model cells
-    const int iz = z - pml;
-    const bool zin = iz >= 0 && iz < k.nz && act;
-    const int itp = lev - 2 - s.ndel;                 // record sample injected now
-    const bool grad_on = lev - 1 <= s.nt + s.ndel - 1;  // imaging with S^{lev-1}
-    T na, nv2, nh = T(0), nprev, nrec = T(0), nsnap = T(0), ngrad = T(0);
-    int ngz = -1;
-    auto fetch = [&](int xx) {
-      xx = min(xx, x1 - 1);
-      na = A1(xx + 4, zl);
-      nv2 = vt2at(xx + 4, zl);
-      if (halo) nh = A1(xx, hz) * (vt2at(xx, hz) * k.idz2);
-      nprev = A2(xx, zl);
-      const int ixx = xx - pml;
-      if (ixx >= 0 && ixx < s.nx) {
-        ngz = SEIS_GL(s.gz)[ixx];
-        if (itp >= 0) nrec = SEIS_GL(s.rec)[(long long)ixx * s.nt + itp];
-        if (grad_on && zin) {
-          const long long j = (long long)ixx * k.nz + iz;
-          nsnap = snap_load(s, k.nz, lev - 1, j);
-          ngrad = SEIS_GL(s.grad)[j];
-        }
-      } else {
-        ngz = -1;
-      }
-    };
-    fetch(x0);
-    for (int x = x0; x < x1; x++) {
-      qx[8] = na * (nv2 * k.idx2);
-      qz[4] = na * (nv2 * k.idz2);
-      lq[4] = na;
-      if (halo) la[hslot] = nh;
-      const T prev = nprev, rec_cur = nrec, snap_cur = nsnap, grad_cur = ngrad;
-      const int gz_cur = ngz;
-      fetch(x + 1);
-      la[lane + 4] = qz[0];
-      wave_sync();
-      T sz = la[lane] * k.c[4];
-      sz += la[lane + 1] * k.c[3];
-      sz += la[lane + 2] * k.c[2];
-      sz += la[lane + 3] * k.c[1];
-      sz += la[lane + 4] * k.c[0];
-      sz += la[lane + 5] * k.c[1];
-      sz += la[lane + 6] * k.c[2];
-      sz += la[lane + 7] * k.c[3];
-      sz += la[lane + 8] * k.c[4];
-      wave_sync();
-      T sx = qx[0] * k.c[4];
-      sx += qx[1] * k.c[3];
-      sx += qx[2] * k.c[2];
-      sx += qx[3] * k.c[1];
-      sx += qx[4] * k.c[0];
-      sx += qx[5] * k.c[1];
-      sx += qx[6] * k.c[2];
-      sx += qx[7] * k.c[3];
-      sx += qx[8] * k.c[4];
-      if (act) {
-        const long long i = fidx(ld, x, z);
-        T val = T(2) * lq[0] + sz + sx - prev;
-        const int ed = edge_dist(x, z, NX, NZ);
-        if (ed <= pml) val += SEIS_GL(s.fld)[LS_OW * s.fstride + i];  // one-way terms (prep kernel)
-        // adjoint of the record extraction and the imaging condition (as adj_finish)
-        const int ix = x - pml;
-        if (ix >= 0 && ix < s.nx && zin) {
-          if (itp >= 0 && z == gz_cur + pml) val += rec_cur;
-          if (grad_on) SEIS_GL(s.grad)[(long long)ix * k.nz + iz] = grad_cur + lq[0] * snap_cur;
-        }
-        SEIS_GL(out)[i] = val;
-        if (ed < pml) {
-          const BandAdj<T> ba = band_adj(k, m, x, z, classify(x, z, NX, NZ, pml), val);
-          SEIS_GL(s.fld)[(LS_AIN + q0) * s.fstride + i] = ba.in;
-          SEIS_GL(s.fld)[(LS_AT + q0) * s.fstride + i] = ba.t;
-          SEIS_GL(s.fld)[(LS_AC + q0) * s.fstride + i] = ba.c;
-        }
-      }
-#pragma unroll
-      for (int j = 0; j < 8; j++) qx[j] = qx[j + 1];
-#pragma unroll
-      for (int j = 0; j < 4; j++) {
-        qz[j] = qz[j + 1];
-        lq[j] = lq[j + 1];
-      }
-    }
+    const int lo = pml + 1, hix = NX - pml - 1, hiz = NZ - pml - 1;
+    const bool deep = x0 - 4 >= lo && x1 + 4 <= hix && z0 - 4 >= lo && z0 + 68 <= hiz && z0 + 64 <= sq.zb;
+    if (deep)
+      adj_sweep<T, true>(k, s, lev, x0, x1, z0, sq.zb, lds[wave], lane);
+    else
+      adj_sweep<T, false>(k, s, lev, x0, x1, z0, sq.zb, lds[wave], lane);
     return;
   }
+  T* out = level(s, lev - 1);
+  const Model<T> m{s.vel, s.nx, k.nz, pml};
+  const long long ld = k.ld;
+  const RawAdj<T> A1{s.fld + (LS_AR + lev % 2) * s.fstride, level(s, lev), ld, NX, NZ, pml};
+  const RawAdj<T> A2{s.fld + (LS_AR + (lev + 1) % 2) * s.fstride, level(s, lev + 1), ld, NX, NZ, pml};
   const long long j = (long long)(blockIdx.x - F.nfwg) * AWG + threadIdx.x;
   if (j >= sq.n) return;
   int x, z;
   ring_cell(sq, NX, NZ, j, &x, &z);
   T ar;
-  T val = adj_value(k, m, NX, L1, L2, A1, A2, x, z, &ar);
+  // one-way terms: stored by the prep kernel for ring P, zero elsewhere
+  T val = adj_value(k, m, NX, A1, A2, x, z, &ar, SEIS_GL(s.fld)[LS_OW * s.fstride + fidx(ld, x, z)]);
   val = adj_finish(k, s, lev, x, z, val, ar);
-  const long long i = fidx(ld, x, z);
-  out[i] = val;
-  const BandCell b = classify(x, z, NX, NZ, pml);
-  if (b.kind != 0) {
-    const BandAdj<T> ba = band_adj(k, m, x, z, b, val);
-    s.fld[(LS_AIN + q0) * s.fstride + i] = ba.in;
-    s.fld[(LS_AT + q0) * s.fstride + i] = ba.t;
-    s.fld[(LS_AC + q0) * s.fstride + i] = ba.c;
-  }
+  out[fidx(ld, x, z)] = val;
 }
 
 // Sum of the per-shot gradients in shot order into the image grid: cell j
@@ -394,6 +551,7 @@ struct LsrtmPlan {
   const T* weight = nullptr;  // optional image-space preconditioner W of CG (caller-owned, device)
   T* masked = nullptr;     // masked copy of an adjoint input
   unsigned* pmask = nullptr;  // band_terms of the ring cells P, one table per distinct width
+  T* pcoef = nullptr;         // their PCOEF_* planes, one table per distinct width
   Consts<T> k{};
   Layout F{};   // forward step split (interior sweep / band)
   Layout A{};   // adjoint step split
@@ -406,7 +564,7 @@ struct LsrtmPlan {
 
   ~LsrtmPlan() {
     for (void* p : {(void*)dev, (void*)pool, snaps, (void*)scales, (void*)grads, (void*)part, (void*)cg, (void*)cg64,
-                    (void*)masked, (void*)pmask})
+                    (void*)masked, (void*)pmask, (void*)pcoef})
       if (p) (void)hipFree(p);
   }
 
@@ -473,11 +631,12 @@ struct LsrtmPlan {
     }
     if (mtot > 0) {
       LS_CHECK(hipMalloc(&pmask, sizeof(unsigned) * (size_t)mtot));
+      LS_CHECK(hipMalloc(&pcoef, sizeof(T) * PCOEF_N * (size_t)mtot));
       for (size_t w = 0; w < widths.size(); w++) {
         const Ring R = shot_split(g.pml + 1, widths[w], NZ);
         if (R.n > 0)
-          hipLaunchKernelGGL(band_terms_kernel, dim3((unsigned)((R.n + 255) / 256)), dim3(256), 0, stream, R,
-                             widths[w], NZ, g.pml, pmask + moff[w]);
+          hipLaunchKernelGGL((band_terms_kernel<T>), dim3((unsigned)((R.n + 255) / 256)), dim3(256), 0, stream, R,
+                             widths[w], NZ, g.pml, pmask + moff[w], pcoef + PCOEF_N * moff[w]);
       }
     }
     host.resize(ns);
@@ -498,9 +657,9 @@ struct LsrtmPlan {
       d.fld = fp + origin;
       d.fstride = fe;
       fp += fe * LS_NSLOT;
-      // the forward passes propagate one field; the band-product slots are
-      // free then (the adjoint clears every slot before use)
-      d.raw = d.fld + (size_t)fe * LS_AIN;
+      // the forward passes propagate one field; the raw-adjoint slots hold
+      // their band cells' stencil values (the adjoint clears them before use)
+      d.raw = d.fld + (size_t)fe * LS_AR;
       d.snap = sp;
       d.scale = sc;
       d.grad = gp;
@@ -513,8 +672,9 @@ struct LsrtmPlan {
       d.ndel = q.ndel;
       d.sx = q.sx;
       d.sz = q.sz;
-      d.pmask = pmask ? pmask + moff[std::lower_bound(widths.begin(), widths.end(), NXs[s]) - widths.begin()]
-                      : nullptr;
+      const size_t wi = std::lower_bound(widths.begin(), widths.end(), NXs[s]) - widths.begin();
+      d.pmask = pmask ? pmask + moff[wi] : nullptr;
+      d.pcoef = pcoef ? pcoef + PCOEF_N * moff[wi] : nullptr;
       sp += snap_elem() * (size_t)(q.nt + q.ndel) * nx * g.nz;
       sc += q.nt + q.ndel;
     }
@@ -531,6 +691,12 @@ struct LsrtmPlan {
       };
       F = with_strip(F, tuned_strip("lsrtm_forward", NXmax, NZ, nshots, stream, fn, this));
     }
+    // band coefficients (the slots the forward passes leave alone)
+    const long long nband = shot_split(g.pml, NXmax, NZ).n;
+    if (nband > 0)
+      hipLaunchKernelGGL((band_coef_kernel<T>), dim3((unsigned)((nband + 255) / 256), ns), dim3(256), 0, stream, k,
+                         dev);
+    LS_CHECK(hipGetLastError());
     return make_snapshots();
   }
 
@@ -634,6 +800,10 @@ struct LsrtmPlan {
     return SEIS_OK;
   }
 
+  void launch_adj_step(const Layout& a, int st) {
+    hipLaunchKernelGGL((adj_step_kernel<T>), dim3(a.nfwg + a.nbwg, nshots), dim3(AWG), 0, stream, k, dev, a, st);
+  }
+
   // grad (device [nx][nz]) = L^T rec, summed over shots
   int adjoint_exact(const T* rec, T* grad) {
     if (mask) {
@@ -648,14 +818,12 @@ struct LsrtmPlan {
       // yet); the fields it writes are cleared below
       auto fn = [](void* c, int bx) {
         LsrtmPlan* p = static_cast<LsrtmPlan*>(c);
-        const Layout a = with_strip(p->A, bx);
-        hipLaunchKernelGGL((adj_step_kernel<T>), dim3(a.nfwg + a.nbwg, p->nshots), dim3(AWG), 0, p->stream, p->k,
-                           p->dev, a, 0);
+        p->launch_adj_step(with_strip(p->A, bx), 0);
       };
       A = with_strip(A, tuned_strip("lsrtm_adjoint", NXmax, NZ, nshots, stream, fn, this));
       adj_tuned = true;
     }
-    int rc = zero_fields(0, LS_NSLOT);
+    int rc = zero_fields(0, LS_NADJ);
     if (rc) return rc;
     LS_CHECK(hipMemsetAsync(grads, 0, sizeof(T) * (size_t)model_cells, stream));
     const double E = sizeof(T);
@@ -671,9 +839,7 @@ struct LsrtmPlan {
         hipLaunchKernelGGL((adj_prep_kernel<T>), dim3(P.nbwg, nshots), dim3(256), 0, stream, k, dev, st);
       });
       const double by = 4 * E * c + (snap_elem() + 2 * E) * mc;
-      timed_launch(stream, "lsrtm_adjoint", by, c, [&] {
-        hipLaunchKernelGGL((adj_step_kernel<T>), dim3(A.nfwg + A.nbwg, nshots), dim3(AWG), 0, stream, k, dev, A, st);
-      });
+      timed_launch(stream, "lsrtm_adjoint", by, c, [&] { launch_adj_step(A, st); });
     }
     hipLaunchKernelGGL((sum_grads_kernel<T>), dim3(256), dim3(256), 0, stream, dev, nshots, g.nz, model, grad);
     LS_CHECK(hipGetLastError());
