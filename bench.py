@@ -274,6 +274,8 @@ def c3_born_adjoint(nshots, pair_shots, seed=1003):
     out["born_gcell_updates_per_s"] = round(2.0 * sum(c * float(n) for c, n in zip(cells, nts)) / el / 1e9, 1)
     log("C3: Born batch of %d windowed shots in %.1f s" % (nshots, el))
     del d_rec
+    if pair_shots <= 0:
+        return out
     # the Born / adjoint pair with BF16 snapshots on a subset (receivers: the
     # spread columns of each window)
     from seis_amd.lsrtm import LsrtmPlan, SNAP_BF16

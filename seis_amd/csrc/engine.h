@@ -143,11 +143,18 @@ __device__ __forceinline__ SnapWord<T> snap_fetch(const DevShot<T>& s, int nz, i
   return w;
 }
 
+// Decoded snapshot word with the step's scale `sc` (s.scale[it]; unused for
+// native storage).  A sweep loads the scale once, before its loop.
 template <typename T>
-__device__ __forceinline__ T snap_decode(const DevShot<T>& s, int it, SnapWord<T> w) {
+__device__ __forceinline__ T snap_decode_scaled(const DevShot<T>& s, float sc, SnapWord<T> w) {
   if (s.snap_kind == SNAP_NATIVE) return w.v;
   const float f = s.snap_kind == SNAP_BF16 ? bf16_to_f32(w.h) : f16_to_f32(w.h);
-  return (T)(f * SEIS_GL(s.scale)[it]);
+  return (T)(f * sc);
+}
+
+template <typename T>
+__device__ __forceinline__ T snap_decode(const DevShot<T>& s, int it, SnapWord<T> w) {
+  return snap_decode_scaled(s, s.snap_kind == SNAP_NATIVE ? 0.f : SEIS_GL(s.scale)[it], w);
 }
 
 // Snapshot value of model cell j = ix*nz + iz at step it.

@@ -359,9 +359,12 @@ __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot
   T* la = lds;
   T* lb = lds + LROW;
   const bool halo = lane < 4 || lane >= 60;
-  const int hz = lane < 4 ? z0 - 4 + lane : min(z0 + lane + 4, NZ - 1);
+  // halo rows z0-4..z0-1 (lanes 0-3) and z0+64..z0+67 (lanes 60-63); every
+  // other lane loads the nearest of them (the same cache lines), so that the
+  // halo loads are not lane-conditional
+  const int hz = lane < 32 ? z0 - 4 + min(lane, 3) : min(z0 + 64 + max(lane - 60, 0), NZ - 1);
   const int hslot = lane < 4 ? lane : lane + 8;
-  const int iz = min(z - pml, k.nz - 1);
+  const int iz = min(z - pml, k.nz - 1);  // clamped: idle lanes past the model load in bounds
   const int sxp = s.sx + pml, szp = s.sz + pml;
   T qa[9], qb[9];
 #pragma unroll
@@ -382,6 +385,13 @@ __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot
   T nca, ncb = T(0), npa, npb = T(0), nv, nha = T(0), nhb = T(0), nrec = T(0), nss = T(0), ndv = T(0);
   SnapWord<T> nsn{T(0), 0};
   int ngz = -1;
+  // RTM_BWD: the saved boundary-strip value of the source field at this
+  // lane's cell of the next column (slot -1: not a strip cell), fetched with
+  // the other loads so that no load is issued and awaited inside a column
+  const bool restore = MODE == RTM_BWD && c.itp - 1 >= 0;
+  const auto STR = SEIS_GL(s.strips + (restore ? (long long)(c.itp - 1) * 8 * (s.nx + k.nz) : 0));
+  T nstr = T(0);
+  int nsl = -1;
   const bool acc = MODE == ILLUM && c.it > s.ndel && act;  // accumulate u^2 (interior cells are model cells)
   const auto SSA = SEIS_GL(s.img);
   auto fetch = [&](int xx) {
@@ -392,20 +402,23 @@ __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot
     npa = PA[r + zl];
     if (TWO) npb = PB[r + zl];
     nv = VEL[(long long)(xx - pml) * k.nz + iz];
-    if (halo) {
-      nha = CA[r + hz];
-      if (TWO) nhb = CB[r + hz];
-    }
+    nha = CA[r + hz];
+    if (TWO) nhb = CB[r + hz];
     if (GZ) ngz = GZA[xx - pml];
     if (RECIN && rec_t >= 0) nrec = REC[(long long)(xx - pml) * s.nt + rec_t];
-    if (acc) nss = SSA[r + zl];
-    if (MODE == BORN_S && act) {
-      // perturbation and Born-source snapshot of this model cell
+    if (MODE == ILLUM) nss = SSA[r + zl];  // used where acc
+    // (loads at the clamped model row of every lane; used by active lanes)
+    if (MODE == BORN_S || MODE == BORN) {
+      // perturbation (and, BORN_S, Born-source snapshot) of this model cell
       const long long j = (long long)(xx - pml) * k.nz + iz;
       ndv = SEIS_GL(s.dv)[j];
-      nsn = snap_fetch(s, k.nz, c.it, j);
+      if (MODE == BORN_S) nsn = snap_fetch(s, k.nz, c.it, j);
     }
-    if (MODE == ADJ_X && act) {
+    if (restore) {
+      nsl = act ? first_strip_slot(xx, zl, s.nx, k.nz, pml) : -1;
+      nstr = STR[max(nsl, 0)];
+    }
+    if (MODE == ADJ_X) {
       // gradient and Born-source snapshot of this model cell
       const long long j = (long long)(xx - pml) * k.nz + iz;
       nss = SEIS_GL(s.grad)[j];
@@ -414,6 +427,8 @@ __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot
   };
   fetch(x0);
   // per-step scalars
+  const float snap_sc =
+      ((MODE == BORN_S || MODE == ADJ_X) && s.snap_kind != SNAP_NATIVE) ? SEIS_GL(s.scale)[c.it] : 0.f;
   constexpr bool PSRC = (MODE != RTM_BWD && MODE != BORN_S && MODE != PW_BACK && MODE != ADJ_X);
   T amp = T(0);
   if (PSRC && c.it - 1 >= 0 && !s.line_src) amp = s.wav[c.it - 1];
@@ -443,6 +458,8 @@ __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot
     const T pa_cur = npa, pb_cur = npb, rec_cur = nrec, ss_cur = nss, dv_cur = ndv;
     const SnapWord<T> sn_cur = nsn;
     const int gzr = ngz + pml;  // receiver row of this column
+    const T str_cur = nstr;
+    const int sl_cur = nsl;
     fetch(x + 1);
     const T av = k.dt * v;
     const T vt2 = av * av;
@@ -499,7 +516,7 @@ __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot
       if (MODE == SNAP_MAX || MODE == SNAP_STORE) {
         NA[i] = ua;
       } else if (MODE == BORN_S) {
-        if (dv_cur != T(0)) ua = ua + snap_decode(s, c.it, sn_cur) * dv_cur;
+        if (dv_cur != T(0)) ua = ua + snap_decode_scaled(s, snap_sc, sn_cur) * dv_cur;
         NA[i] = ua;
         if (c.itp >= 0 && z == gzr) REC[(long long)ix * s.nt + c.itp] = ua;
       } else if (MODE == MODEL || MODE == RTM_FWD || MODE == ILLUM) {
@@ -515,7 +532,7 @@ __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot
           for (int q = 0; q < n; q++) st[sl[q]] = ua;
         }
       } else if (MODE == BORN) {
-        const T d = SEIS_GL(s.dv)[(long long)ix * k.nz + iz];
+        const T d = dv_cur;
         if (d != T(0)) {
           const T lap = (ua + pa - T(2) * qa[4]) * k.idt2;
           const T v3 = T(2) / (v * v * v);
@@ -542,19 +559,14 @@ __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot
       } else if (MODE == ADJ_X) {
         if (rec_t >= 0 && z == gzr) ua = ua + rec_cur * vt2;
         NA[i] = ua;
-        SEIS_GL(s.grad)[(long long)ix * k.nz + iz] = ss_cur + ua * (snap_decode(s, c.it, sn_cur) / vt2);
+        SEIS_GL(s.grad)[(long long)ix * k.nz + iz] = ss_cur + ua * (snap_decode_scaled(s, snap_sc, sn_cur) / vt2);
       } else {  // RTM_BWD: source field ua, receiver field ub
         if (z == gzr) ub = ub + rec_cur * vt2;
         const T lap = (pa + ua - T(2) * qa[4]) * k.idt2;
         const T v3 = T(2) / (v * v * v);
         SEIS_GL(c.aux)[i] = lap * v3;
         NB[i] = ub;
-        T out = ua;
-        if (c.itp - 1 >= 0) {
-          const int sl = first_strip_slot(x, z, s.nx, k.nz, pml);
-          if (sl >= 0) out = SEIS_GL(s.strips)[(long long)(c.itp - 1) * 8 * (s.nx + k.nz) + sl];
-        }
-        NA[i] = out;
+        NA[i] = (restore && sl_cur >= 0) ? str_cur : ua;
         // the band's one-way update reads the stencil value of this layer,
         // not the restored strip
         if (edge_dist(x, z, s.NX, NZ) == pml) SEIS_GL(s.raw)[i] = ua;
@@ -591,7 +603,9 @@ __global__ __launch_bounds__(WG) void step_kernel(Consts<T> k, const DevShot<T>*
   const int NX = s.NX, NZ = k.NZ;
   const Split q = shot_split(L.m, NX, NZ);  // this shot's interior (widths differ within a batch)
   if ((int)blockIdx.x < L.nfwg) {
-    const int wave = threadIdx.x >> 6;
+    // the wavefront's index, uniform across its lanes, keeps the strip
+    // bounds, the sweep's loop and its column addresses in scalar registers
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int item = xcd_tile(blockIdx.x, L.nfwg) * WAVES + wave;
     if (item >= L.nzc * shot_strips(q, L.bx)) return;  // whole wavefront leaves; no workgroup barrier follows
     const int zc = item % L.nzc, xs = item / L.nzc;
