@@ -43,11 +43,12 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Gcell-updates/s (batched shots) + LSRTM s/CG-iter vs plain FP32 at stated error"
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of the headline's kernels (26 shots per launch), from
-# rocprofv3 FETCH_SIZE / WRITE_SIZE calibrated against a 1 GiB copy; the
-# commands and counter values are in profiles/c5_rtm_hbm_traffic.md.  Scaled
-# by the shots of a launch.
-MEASURED_TRAFFIC = {"shots": 26, "bytes": {}}
+# HBM bytes per launch of the headline's kernels (26 shots per launch, strip
+# length 32), from rocprofv3 FETCH_SIZE / WRITE_SIZE calibrated against a 1 GiB
+# copy; the commands and counter values are in profiles/c5_rtm_hbm_traffic.md.
+# Scaled by the shots of a launch.
+MEASURED_TRAFFIC = {"shots": 26, "bytes": {"step_rtm_reverse": 2619605573, "step_rtm_forward": 1279442685,
+                                           "rtm_image": 1449277885}}
 
 
 def host_cpu_model():

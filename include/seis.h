@@ -353,7 +353,7 @@ int seis_lsrtm_cg_history(seis_lsrtm* plan, const void* dobs, int niter, double 
 /* Strip-length autotuning of the interior sweeps (no counterpart in the
  * reference).  On by default: the first run of a configuration (kind of
  * pass, padded grid, shots per launch, device) times a few trial launches of
- * each candidate strip length (8, 16, 32 columns; 1, 2, 4, 8, 16 when the
+ * each candidate strip length (8, 16, 32, 64 columns; 1, 2, 4, 8, 16 when the
  * padded grid times the shots is under 4 Mi cells) on its zeroed fields and
  * keeps the fastest for the process.  Results are identical for every strip
  * length.  seis_set_tuning(0) turns it off (strip length 16) and forgets the

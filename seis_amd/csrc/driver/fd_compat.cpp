@@ -3,6 +3,7 @@
 
 #include <cstdio>
 #include <cstring>
+#include <initializer_list>
 
 #include "../../../include/seis.h"
 
@@ -45,6 +46,19 @@ struct Flat {
   }
 };
 
+// Sizes and row tables are checked before any input row is read or an output
+// allocated: a negative size would become a huge size_t in Flat and alloc2.
+bool bad_args(const char* fn, int nt, int ndel, int nx, int nz, std::initializer_list<float**> rows) {
+  const char* why = nullptr;
+  if (nt <= 0 || nx <= 0 || nz <= 0) why = "nt, nx and nz must be positive";
+  else if (ndel < 0) why = "ndel must be non-negative";
+  for (float** r : rows)
+    if (!why && !r) why = "null input array";
+  if (!why) return false;
+  std::fprintf(stderr, "seis: %s: %s\n", fn, why);
+  return true;
+}
+
 float** finish(int rc, float** out) {
   if (rc == SEIS_OK) return out;
   std::fprintf(stderr, "seis: %s\n", seis_last_error());
@@ -56,6 +70,7 @@ float** finish(int rc, float** out) {
 
 float** modeling(float dt, float dx, float dz, int nt, int ndel, int nx, int nz, int pml, int sx, int sz, int* gz,
                  float* wav, float** vel) {
+  if (bad_args("modeling", nt, ndel, nx, nz, {vel})) return nullptr;
   Flat v(vel, nz, nx);
   float** rec = alloc2(nt, nx);
   if (!rec) return nullptr;
@@ -64,6 +79,7 @@ float** modeling(float dt, float dx, float dz, int nt, int ndel, int nx, int nz,
 
 float** forward(float dt, float dx, float dz, int nt, int ndel, int nx, int nz, int pml, int sx, int sz, int* gz,
                 float* wav, float** vel, float** dv) {
+  if (bad_args("forward", nt, ndel, nx, nz, {vel, dv})) return nullptr;
   Flat v(vel, nz, nx), d(dv, nz, nx);
   float** rec = alloc2(nt, nx);
   if (!rec) return nullptr;
@@ -72,6 +88,7 @@ float** forward(float dt, float dx, float dz, int nt, int ndel, int nx, int nz, 
 
 float** adjoint(float dt, float dx, float dz, int nt, int ndel, int nx, int nz, int pml, int sx, int sz, int* gz,
                 float* wav, float** vel, float** rec) {
+  if (bad_args("adjoint", nt, ndel, nx, nz, {vel, rec})) return nullptr;
   Flat v(vel, nz, nx), r(rec, nt, nx);
   float** img = alloc2(nz, nx);
   if (!img) return nullptr;
@@ -80,6 +97,7 @@ float** adjoint(float dt, float dx, float dz, int nt, int ndel, int nx, int nz, 
 
 float** illum(float dt, float dx, float dz, int nt, int ndel, int nx, int nz, int pml, int sx, int sz, float* wav,
               float** vel) {
+  if (bad_args("illum", nt, ndel, nx, nz, {vel})) return nullptr;
   Flat v(vel, nz, nx);
   float** ss = alloc2(nz, nx);
   if (!ss) return nullptr;
@@ -88,6 +106,7 @@ float** illum(float dt, float dx, float dz, int nt, int ndel, int nx, int nz, in
 
 float** forwordPlane(float dt, float dx, float dz, int nt, int ndel, int nx, int nz, int pml, int sz, int gz,
                      float** vel, float** sou, float** dv) {
+  if (bad_args("forwordPlane", nt, ndel, nx, nz, {vel, sou, dv})) return nullptr;
   Flat v(vel, nz, nx), s(sou, nt + ndel, nx), d(dv, nz, nx);
   float** rec = alloc2(nt, nx);
   if (!rec) return nullptr;
@@ -96,6 +115,7 @@ float** forwordPlane(float dt, float dx, float dz, int nt, int ndel, int nx, int
 
 float** adjointPlane(float dt, float dx, float dz, int nt, int ndel, int nx, int nz, int pml, int sz, int gz,
                      float** vel, float** sou, float** rec) {
+  if (bad_args("adjointPlane", nt, ndel, nx, nz, {vel, sou, rec})) return nullptr;
   Flat v(vel, nz, nx), s(sou, nt + ndel, nx), r(rec, nt, nx);
   float** img = alloc2(nz, nx);
   if (!img) return nullptr;
@@ -104,6 +124,7 @@ float** adjointPlane(float dt, float dx, float dz, int nt, int ndel, int nx, int
 
 float** illumPlane(float dt, float dx, float dz, int nt, int ndel, int nx, int nz, int pml, int sz, int gz,
                    float** vel, float** sou) {
+  if (bad_args("illumPlane", nt, ndel, nx, nz, {vel, sou})) return nullptr;
   Flat v(vel, nz, nx), s(sou, nt + ndel, nx);
   float** ss = alloc2(nz, nx);
   if (!ss) return nullptr;

@@ -14,7 +14,7 @@ namespace seis {
 namespace {
 
 constexpr int kDefaultStrip = 16;
-constexpr int kCandidates[] = {8, 16, 32};
+constexpr int kCandidates[] = {8, 16, 32, 64};
 // Small launches (fewer padded cells over all shots than kSmallCells): at a
 // strip of 8 columns they hold too few wavefronts to fill 256 CUs, and each
 // wavefront's serial sweep of its strip sets the kernel's duration, so

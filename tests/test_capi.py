@@ -125,3 +125,40 @@ def test_bench_refuses_more_than_one_gpu():
     assert r.returncode != 0
     assert "one GPU" in r.stderr + r.stdout
     assert r.stdout.strip() == ""  # no JSON line
+
+
+def test_fd_h_entry_points_reject_bad_sizes_without_gpu():
+    """The fd.h drop-in returns NULL for non-positive nt / nx / nz, negative
+    ndel or a null row table, before reading any input row or allocating an
+    output (no device call is reached)."""
+    import numpy as np
+    import seis_amd
+    lib = seis_amd.lib()
+    F, I, P = ctypes.c_float, ctypes.c_int, ctypes.c_void_p
+    PP = ctypes.POINTER(ctypes.POINTER(ctypes.c_float))
+    modeling = getattr(lib, "_Z8modelingfffiiiiiiiPiPfPS0_")
+    forward = getattr(lib, "_Z7forwardfffiiiiiiiPiPfPS0_S1_")
+    adjoint = getattr(lib, "_Z7adjointfffiiiiiiiPiPfPS0_S1_")
+    illum_plane = getattr(lib, "_Z10illumPlanefffiiiiiiiPPfS0_")
+    for f in (modeling, forward, adjoint, illum_plane):
+        f.restype = PP
+    a = np.ones((4, 4), np.float32)
+    rows = (ctypes.POINTER(ctypes.c_float) * 4)(*[ctypes.cast(a.ctypes.data + 16 * i, ctypes.POINTER(ctypes.c_float))
+                                                  for i in range(4)])
+    gz = np.zeros(4, np.int32)
+    wav = np.zeros(16, np.float32)
+    gzp, wavp = gz.ctypes.data_as(P), wav.ctypes.data_as(P)
+
+    def head(nt, ndel, nx, nz):
+        return (F(0.001), F(10.0), F(10.0), I(nt), I(ndel), I(nx), I(nz), I(2), I(1), I(1))
+
+    for nt, ndel, nx, nz in ((0, 0, 4, 4), (-5, 0, 4, 4), (8, -1, 4, 4), (8, 0, -4, 4), (8, 0, 4, 0)):
+        assert not modeling(*head(nt, ndel, nx, nz), gzp, wavp, rows)
+        assert not forward(*head(nt, ndel, nx, nz), gzp, wavp, rows, rows)
+        assert not adjoint(*head(nt, ndel, nx, nz), gzp, wavp, rows, rows)
+        assert not illum_plane(F(0.001), F(10.0), F(10.0), I(nt), I(ndel), I(nx), I(nz), I(2), I(1), I(1), rows,
+                               rows)
+    # null row tables
+    assert not modeling(*head(8, 0, 4, 4), gzp, wavp, None)
+    assert not forward(*head(8, 0, 4, 4), gzp, wavp, rows, None)
+    assert not adjoint(*head(8, 0, 4, 4), gzp, wavp, None, rows)
