@@ -576,18 +576,25 @@ def lsrtm_bench(iters, err_shots, plain_iters, xcorr_iters):
     out["snapshot_s"] = round(time.perf_counter() - t0, 3)
     out["snapshot_gb"] = round(2.0 * sum(n for _, n in wins) * nz * nt / 1e9, 1)
     d_obs = DeviceArray.from_host(dobs)
-    seis_amd.profile_start()
+    # the timed CG runs without the launch profiler (its event pair around
+    # every launch adds about 5 % here, as the plain path below has none);
+    # the kernel breakdown comes from a separate 2-iteration profiled run
     t0 = time.perf_counter()
     img, hist = plan.cg(d_obs, iters)
     el = time.perf_counter() - t0
-    st = seis_amd.profile_stop()
     done = len(hist) - 1
     out["cg_iterations"] = done
     out["s_per_cg_iter_bf16"] = round(el / max(done, 1), 3)
-    out["kernels"] = {k: {"launches": v_["launches"], "ms": round(v_["total_ms"], 1),
-                          "GB/s": round(v_["alg_bytes"] / max(v_["total_ms"], 1e-9) / 1e6, 1)} for k, v_ in st.items()}
     out["residual_rel"] = [round(float(np.sqrt(x / hist[0, 0])), 5) for x in hist[:, 0]]
     log("lsrtm: %d CG iterations (BF16 snapshots, exact adjoint) in %.1f s" % (done, el))
+    seis_amd.profile_start()
+    t0 = time.perf_counter()
+    plan.cg(d_obs, 2)
+    el_p = time.perf_counter() - t0
+    st = seis_amd.profile_stop()
+    out["kernels"] = {k: {"launches": v_["launches"], "ms": round(v_["total_ms"], 1),
+                          "GB/s": round(v_["alg_bytes"] / max(v_["total_ms"], 1e-9) / 1e6, 1)} for k, v_ in st.items()}
+    out["kernels_run"] = "2 CG iterations under the launch profiler, %.2f s" % el_p
     if xcorr_iters > 0:
         plan.set_adjoint(1)
         t0 = time.perf_counter()

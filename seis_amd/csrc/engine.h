@@ -39,8 +39,8 @@ struct DevShot {
                    // origin; second field at raw + fstride
   T* out;          // [nx][nz] destination of the cropped image (RTM, ILLUM)
   const unsigned* pmask;  // LSRTM: band_terms of the cells of ring P of this shot's width
-  const T* pcoef;         // LSRTM: sponge weights of those cells and their inner
-                          // neighbours (PCOEF_* planes of pmask's length)
+  const T* wtab;          // LSRTM: sponge weight by band index (sponge_weight's i):
+                          // wtab[0] = 1 (inside the model), wtab[i + 1], i <= pml
   int x0;          // LSRTM: first column of the shot's window in the image grid
 };
 

@@ -193,39 +193,56 @@ SEIS_HD inline bool high_order(int x, int z, int NX, int NZ) { return x >= 4 && 
 template <typename T, class AR1, class AR2>
 SEIS_HD inline T adj_value(const Consts<T>& k, const Model<T>& m, int NX, const AR1& ar1, const AR2& ar2, int x,
                            int z, T* raw_adj_out, T ow) {
-  const int NZ = k.NZ, pml = k.pml, nx = m.nx;
-  const long long ld = k.ld;
-  auto vt2 = [&](int xx, int zz) {
-    const T a = k.dt * m.vel(xx, zz);
+  const int NZ = k.NZ;
+  auto czc = [&](int zz) { return imin(imax(zz, 0), NZ - 1); };
+  auto cxc = [&](int xx) { return imin(imax(xx, 0), NX - 1); };
+  auto vt2 = [&](T v) {
+    const T a = k.dt * v;
     return a * a;
   };
-  auto vv = [&](int xx, int zz) {
-    const T v = m.vel(xx, zz);
-    return v * v;
-  };
+  // Every read first, at positions clamped into the grid; the sums below add
+  // the terms whose conditions hold, in a fixed order (no read waits on a
+  // branch).  Neighbour j = 2 (d - 1) + (e > 0) is (x, z - e d) for the z
+  // terms and (x - e d, z) for the x terms, d = 1..4, e = -1, 1.
+  const T a_self = ar1(x, z), v_self = m.vel(x, z);
+  T az[8], vz[8], ax[8], vx[8];
+  for (int d = 1; d <= 4; d++) {
+    for (int e = -1; e <= 1; e += 2) {
+      const int j = 2 * (d - 1) + (e > 0);
+      az[j] = ar1(x, czc(z - e * d));
+      vz[j] = m.vel(x, czc(z - e * d));
+      ax[j] = ar1(cxc(x - e * d), z);
+      vx[j] = m.vel(cxc(x - e * d), z);
+    }
+  }
+  const T top_a = ar1(x, 0), top_v1 = m.vel(x, 1), top_v2 = m.vel(x, 2);
+  const T bot_a = ar1(x, NZ - 1), bot_v1 = m.vel(x, NZ - 2), bot_v2 = m.vel(x, NZ - 3);
+  const T lft_a = ar1(0, z), lft_v1 = m.vel(1, z), lft_v2 = m.vel(2, z);
+  const T rgt_a = ar1(NX - 1, z), rgt_v1 = m.vel(NX - 2, z), rgt_v2 = m.vel(NX - 3, z);
+  const T a_prev = ar2(x, z);
   T acc = T(0);
-  const T a_self = ar1(x, z);
   if (raw_adj_out) *raw_adj_out = a_self;
   // (i) stencil reads of cur by non-ring cells
   if (!is_ring(x, z, NX, NZ)) {
-    const T v2 = vt2(x, z);
-    const T az = a_self * (v2 * k.idz2), ax = a_self * (v2 * k.idx2);
+    const T v2 = vt2(v_self);
+    const T az0 = a_self * (v2 * k.idz2), ax0 = a_self * (v2 * k.idx2);
     acc += T(2) * a_self;
     if (high_order(x, z, NX, NZ)) {
-      acc += az * k.c[0] + ax * k.c[0];
+      acc += az0 * k.c[0] + ax0 * k.c[0];
     } else {
-      acc -= T(2) * az;
-      acc -= T(2) * ax;
+      acc -= T(2) * az0;
+      acc -= T(2) * ax0;
     }
   }
   for (int d = 1; d <= 4; d++) {
     for (int e = -1; e <= 1; e += 2) {
+      const int j = 2 * (d - 1) + (e > 0);
       // z neighbour c = (x, z - e d) reads cur(x, z) at offset e d
       const int cz = z - e * d;
       if (cz >= 1 && cz <= NZ - 2 && x >= 1 && x <= NX - 2) {
         const bool hi = high_order(x, cz, NX, NZ);
         if (hi || d == 1) {
-          const T a = ar1(x, cz) * (vt2(x, cz) * k.idz2);
+          const T a = az[j] * (vt2(vz[j]) * k.idz2);
           acc += hi ? a * k.c[d] : a;
         }
       }
@@ -233,7 +250,7 @@ SEIS_HD inline T adj_value(const Consts<T>& k, const Model<T>& m, int NX, const 
       if (cx >= 1 && cx <= NX - 2 && z >= 1 && z <= NZ - 2) {
         const bool hi = high_order(cx, z, NX, NZ);
         if (hi || d == 1) {
-          const T a = ar1(cx, z) * (vt2(cx, z) * k.idx2);
+          const T a = ax[j] * (vt2(vx[j]) * k.idx2);
           acc += hi ? a * k.c[d] : a;
         }
       }
@@ -246,31 +263,27 @@ SEIS_HD inline T adj_value(const Consts<T>& k, const Model<T>& m, int NX, const 
       acc += T(2) * a_self;
     } else {
       const T f = ze ? k.cx : (x == 0 ? k.cz : k.cx);
-      acc += a_self * (T(2) - T(2) * (vv(x, z) * f));
+      acc += a_self * (T(2) - T(2) * ((v_self * v_self) * f));
     }
   }
-  if (z == 0 || z == NZ - 1) {  // edge-parallel neighbours on the top/bottom rows
-    for (int e = -1; e <= 1; e += 2) {
-      const int rx = x + e;
-      if (rx >= 1 && rx <= NX - 2) acc += ar1(rx, z) * (vv(rx, z) * k.cx);
-    }
+  if (z == 0 || z == NZ - 1) {  // edge-parallel neighbours on the top/bottom rows: x-1 (j 1), x+1 (j 0)
+    if (x - 1 >= 1 && x - 1 <= NX - 2) acc += ax[1] * ((vx[1] * vx[1]) * k.cx);
+    if (x + 1 >= 1 && x + 1 <= NX - 2) acc += ax[0] * ((vx[0] * vx[0]) * k.cx);
   }
-  if (x == 0 || x == NX - 1) {  // edge-parallel neighbours on the left/right columns
+  if (x == 0 || x == NX - 1) {  // edge-parallel neighbours on the left/right columns: z-1 (j 1), z+1 (j 0)
     const T f = (x == 0) ? k.cz : k.cx;
-    for (int e = -1; e <= 1; e += 2) {
-      const int rz = z + e;
-      if (rz >= 1 && rz <= NZ - 2) acc += ar1(x, rz) * (vv(x, rz) * f);
-    }
+    if (z - 1 >= 1 && z - 1 <= NZ - 2) acc += az[1] * ((vz[1] * vz[1]) * f);
+    if (z + 1 >= 1 && z + 1 <= NZ - 2) acc += az[0] * ((vz[0] * vz[0]) * f);
   }
   // extrapolated normal differences: ring cell (x, 0) reads cur(x, 0..3), etc.
   // (the coefficient of cur at distance j from the edge; straight-line form)
-  if (z <= 3) acc += ar1(x, 0) * extrap_coef<T>(z, vv(x, 1) * k.cz, vv(x, 2) * k.cz);
-  if (NZ - 1 - z <= 3) acc += ar1(x, NZ - 1) * extrap_coef<T>(NZ - 1 - z, vv(x, NZ - 2) * k.cz, vv(x, NZ - 3) * k.cz);
-  if (x <= 3) acc += ar1(0, z) * extrap_coef<T>(x, vv(1, z) * k.cx, vv(2, z) * k.cx);
-  if (NX - 1 - x <= 3) acc += ar1(NX - 1, z) * extrap_coef<T>(NX - 1 - x, vv(NX - 2, z) * k.cx, vv(NX - 3, z) * k.cx);
+  if (z <= 3) acc += top_a * extrap_coef<T>(z, (top_v1 * top_v1) * k.cz, (top_v2 * top_v2) * k.cz);
+  if (NZ - 1 - z <= 3) acc += bot_a * extrap_coef<T>(NZ - 1 - z, (bot_v1 * bot_v1) * k.cz, (bot_v2 * bot_v2) * k.cz);
+  if (x <= 3) acc += lft_a * extrap_coef<T>(x, (lft_v1 * lft_v1) * k.cx, (lft_v2 * lft_v2) * k.cx);
+  if (NX - 1 - x <= 3) acc += rgt_a * extrap_coef<T>(NX - 1 - x, (rgt_v1 * rgt_v1) * k.cx, (rgt_v2 * rgt_v2) * k.cx);
   // (iii) -prev of the stencil and ring base formula at level m+1, (iv) the
   // one-way reads of cur and prev
-  acc -= ar2(x, z);
+  acc -= a_prev;
   acc += ow;
   return acc;
 }

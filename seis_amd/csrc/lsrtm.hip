@@ -49,28 +49,27 @@ __device__ __forceinline__ T* level(const DevShot<T>& s, int lev) {
   return s.fld + (LS_FIELD + lev % 3) * s.fstride;
 }
 
-// Planes of the ring-P coefficient table: the sponge weight w of the cell and
-// -(1 - w) of its neighbours z-1, z+1, x-1, x+1 (the factors of the one-way
-// prev terms of adj_prep_kernel).
-enum PrepCoef { PCOEF_W = 0, PCOEF_ZM = 1, PCOEF_ZP = 2, PCOEF_XM = 3, PCOEF_XP = 4, PCOEF_N = 5 };
-
-// Band-term masks (band_terms) and coefficients of the ring cells of one
-// window width, computed once per plan.
-template <typename T>
-__global__ void band_terms_kernel(Ring R, int NX, int NZ, int pml, unsigned* __restrict__ out,
-                                  T* __restrict__ coef) {
+// Band-term masks (band_terms) of the ring cells of one window width,
+// computed once per plan.
+__global__ void band_terms_kernel(Ring R, int NX, int NZ, int pml, unsigned* __restrict__ out) {
   const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= R.n) return;
   int x, z;
   ring_cell(R, NX, NZ, j, &x, &z);
   out[j] = band_terms(x, z, NX, NZ, pml);
-  const int nx = NX - 2 * pml, nz = NZ - 2 * pml;
-  auto pin = [&](int bx, int bz) { return -(T(1) - sponge_weight<T>(bx, bz, nx, nz, pml)); };
-  coef[PCOEF_W * R.n + j] = sponge_weight<T>(x, z, nx, nz, pml);
-  coef[PCOEF_ZM * R.n + j] = pin(x, z - 1);
-  coef[PCOEF_ZP * R.n + j] = pin(x, z + 1);
-  coef[PCOEF_XM * R.n + j] = pin(x - 1, z);
-  coef[PCOEF_XP * R.n + j] = pin(x + 1, z);
+}
+
+// Sponge weight of (x, z) from the plan's table (DevShot::wtab): the band
+// index of sponge_weight, clamped so that a cell one beyond the padded grid
+// (read only where its term is masked out) stays inside the table.
+template <typename T>
+__device__ __forceinline__ T table_weight(const T* wt, int x, int z, int nx, int nz, int pml) {
+  int i = -1;
+  if (x < pml) i = pml - x - 1;
+  else if (x >= pml + nx) i = x - pml - nx;
+  else if (z < pml) i = pml - z - 1;
+  else if (z >= pml + nz) i = z - pml - nz;
+  return SEIS_GL(wt)[min(i, pml) + 1];
 }
 
 // Band coefficients of every band cell of a shot (edge_dist < pml): the
@@ -123,7 +122,6 @@ __global__ __launch_bounds__(256) void adj_prep_kernel(Consts<T> k, const DevSho
   const long long i = fidx(ld, x, z);
   const auto lam1 = SEIS_GL(level(s, lev)), lam2 = SEIS_GL(level(s, lev + 1));
   const auto kin = SEIS_GL(s.fld + LS_KIN * f), kt = SEIS_GL(s.fld + LS_KT * f), kc = SEIS_GL(s.fld + LS_KC * f);
-  const auto cf = SEIS_GL(s.pcoef);
   const long long zm = i - 1, zp = i + 1, xm = i - ld, xp = i + ld;
   // level lev
   const T l0 = lam1[i], l_zm = lam1[zm], l_zp = lam1[zp], l_xm = lam1[xm], l_xp = lam1[xp];
@@ -138,9 +136,11 @@ __global__ __launch_bounds__(256) void adj_prep_kernel(Consts<T> k, const DevSho
   const T b_0 = kin[i] * lam2[i];
   const T p_zm = lam2[zm], p_zp = lam2[zp], p_xm = lam2[xm], p_xp = lam2[xp];
   const T u_xm = kt[xm] * p_xm, u_zm = kt[zm] * p_zm, u_xp = kt[xp] * p_xp, u_zp = kt[zp] * p_zp;
-  const T w = cf[PCOEF_W * R.n + j];
-  const T f_zm = cf[PCOEF_ZM * R.n + j], f_zp = cf[PCOEF_ZP * R.n + j];
-  const T f_xm = cf[PCOEF_XM * R.n + j], f_xp = cf[PCOEF_XP * R.n + j];
+  // sponge weight of the cell, -(1 - w) of its four neighbours
+  const int nx = s.nx, nz = k.nz, pml = k.pml;
+  const T w = table_weight(s.wtab, x, z, nx, nz, pml);
+  auto pin = [&](int bx, int bz) { return -(T(1) - table_weight(s.wtab, bx, bz, nx, nz, pml)); };
+  const T f_zm = pin(x, z - 1), f_zp = pin(x, z + 1), f_xm = pin(x - 1, z), f_xp = pin(x + 1, z);
   auto on = [&](unsigned bit, T v) { return (mk & bit) ? v : T(0); };
   T r = w * l0;
   r += on(BT_AIN_ZM, a_zm);
@@ -215,8 +215,10 @@ constexpr int AWG = 256;
 // cell the strip reads (4 beyond it on each side) lies inside
 // [pml+1, NX-pml-1) x [pml+1, NZ-pml-1), where the raw adjoint is the level
 // itself; then no cell of the strip takes a one-way term and every one is an
-// active model cell.  The caller decides it per wavefront.
-template <typename T, bool DEEP>
+// active model cell.  The caller decides it per wavefront.  HALF: 16-bit
+// snapshots (BF16 or FP16), else native ones; a compile-time choice, so that
+// the snapshot load is one instruction into one register.
+template <typename T, bool DEEP, bool HALF>
 __device__ __forceinline__ void adj_sweep(const Consts<T>& k, const DevShot<T>& s, int lev, int x0, int x1, int z0,
                                           int zb, T* la, int lane) {
   const int NX = s.NX, NZ = k.NZ, pml = k.pml;
@@ -301,10 +303,10 @@ __device__ __forceinline__ void adj_sweep(const Consts<T>& k, const DevShot<T>& 
     ngz = SEIS_GL(gzp)[ixc];
     nrec = SEIS_GL(recp)[(long long)ixc * ntr + itc];
     const long long j = (long long)ixc * k.nz + izc;
-    if (snk == SNAP_NATIVE)
-      nsn = SEIS_GL(snap_n)[j];
-    else
+    if (HALF)
       nsh = SEIS_GL(snap_h)[j];
+    else
+      nsn = SEIS_GL(snap_n)[j];
     ngrad = SEIS_GL(gradp)[j];
   };
   fetch(x0);
@@ -351,9 +353,8 @@ __device__ __forceinline__ void adj_sweep(const Consts<T>& k, const DevShot<T>& 
       if (DEEP || (ix >= 0 && ix < nxm && zin)) {
         if (itp >= 0 && z == gz_cur + pml) val += rec_cur;
         if (grad_on) {
-          const T snap_cur = snk == SNAP_NATIVE
-                                 ? sn_cur
-                                 : (T)((snk == SNAP_BF16 ? bf16_to_f32(sh_cur) : f16_to_f32(sh_cur)) * (float)scl);
+          const T snap_cur = !HALF ? sn_cur
+                                   : (T)((snk == SNAP_BF16 ? bf16_to_f32(sh_cur) : f16_to_f32(sh_cur)) * (float)scl);
           SEIS_GL(gradp)[(long long)ix * k.nz + iz] = grad_cur + lq[0] * snap_cur;
         }
       }
@@ -370,9 +371,12 @@ __device__ __forceinline__ void adj_sweep(const Consts<T>& k, const DevShot<T>& 
 }
 
 // One reverse step: lam^{lev-1} for every cell and grad += raw-adjoint(lev) *
-// S^{lev-1}.  Interior cells (no band term within the stencil reach) are swept
-// by wavefronts streaming along x like the forward kernel; the others use the
-// general gather.
+// S^{lev-1}, in two kernels.  adj_step_kernel: the cells at least 8 from the
+// edge (no band term within the stencil reach of a reader), swept by
+// wavefronts streaming along x like the forward kernel.  adj_ring_kernel: the
+// outer 8 cells, by the general gather; its reads, all issued up front, need
+// more registers than the sweep, which keeps its occupancy in a kernel of its
+// own.
 template <typename T>
 __global__ __launch_bounds__(AWG) void adj_step_kernel(Consts<T> k, const DevShot<T>* __restrict__ shots, Layout F,
                                                        int step) {
@@ -382,34 +386,46 @@ __global__ __launch_bounds__(AWG) void adj_step_kernel(Consts<T> k, const DevSho
   if (lev < 1) return;
   const int NX = s.NX, NZ = k.NZ, pml = k.pml;
   const Split sq = shot_split(F.m, NX, NZ);  // this shot's interior
-  if ((int)blockIdx.x < F.nfwg) {
-    // Cells at least 8 from the outer edge: every stencil reader within 4
-    // cells uses the 9-point operator, so the stencil part is the 9-point
-    // gather of Q = (raw adjoint) vt2 / h^2 of the readers; cells at most
-    // pml from the edge add the one-way terms.  The wavefront's index is
-    // taken uniform across its lanes, so that the strip bounds, the loop and
-    // the column addresses stay in scalar registers.
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int item = xcd_tile(blockIdx.x, F.nfwg) * WAVES + wave;
-    if (item >= F.nzc * shot_strips(sq, F.bx)) return;  // whole wavefront leaves; no workgroup barrier follows
-    const int zc = item % F.nzc, xs = item / F.nzc;
-    const int z0 = sq.za + zc * 64;
-    const int x0 = sq.xa + xs * F.bx, x1 = min(x0 + F.bx, sq.xb);
-    const int lane = threadIdx.x & 63;
-    const int lo = pml + 1, hix = NX - pml - 1, hiz = NZ - pml - 1;
-    const bool deep = x0 - 4 >= lo && x1 + 4 <= hix && z0 - 4 >= lo && z0 + 68 <= hiz && z0 + 64 <= sq.zb;
-    if (deep)
-      adj_sweep<T, true>(k, s, lev, x0, x1, z0, sq.zb, lds[wave], lane);
-    else
-      adj_sweep<T, false>(k, s, lev, x0, x1, z0, sq.zb, lds[wave], lane);
-    return;
-  }
+  // Cells at least 8 from the outer edge: every stencil reader within 4 cells
+  // uses the 9-point operator, so the stencil part is the 9-point gather of
+  // Q = (raw adjoint) vt2 / h^2 of the readers; cells at most pml from the
+  // edge add the one-way terms.  The wavefront's index is taken uniform
+  // across its lanes, so that the strip bounds, the loop and the column
+  // addresses stay in scalar registers.
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int item = xcd_tile(blockIdx.x, F.nfwg) * WAVES + wave;
+  if (item >= F.nzc * shot_strips(sq, F.bx)) return;  // whole wavefront leaves; no workgroup barrier follows
+  const int zc = item % F.nzc, xs = item / F.nzc;
+  const int z0 = sq.za + zc * 64;
+  const int x0 = sq.xa + xs * F.bx, x1 = min(x0 + F.bx, sq.xb);
+  const int lane = threadIdx.x & 63;
+  const int lo = pml + 1, hix = NX - pml - 1, hiz = NZ - pml - 1;
+  const bool deep = x0 - 4 >= lo && x1 + 4 <= hix && z0 - 4 >= lo && z0 + 68 <= hiz && z0 + 64 <= sq.zb;
+  const bool half = s.snap_kind != SNAP_NATIVE;
+  if (deep && half)
+    adj_sweep<T, true, true>(k, s, lev, x0, x1, z0, sq.zb, lds[wave], lane);
+  else if (deep)
+    adj_sweep<T, true, false>(k, s, lev, x0, x1, z0, sq.zb, lds[wave], lane);
+  else if (half)
+    adj_sweep<T, false, true>(k, s, lev, x0, x1, z0, sq.zb, lds[wave], lane);
+  else
+    adj_sweep<T, false, false>(k, s, lev, x0, x1, z0, sq.zb, lds[wave], lane);
+}
+
+template <typename T>
+__global__ __launch_bounds__(AWG) void adj_ring_kernel(Consts<T> k, const DevShot<T>* __restrict__ shots, Layout F,
+                                                       int step) {
+  const DevShot<T>& s = shots[blockIdx.y];
+  const int lev = s.nt + s.ndel + 1 - step;
+  if (lev < 1) return;
+  const int NX = s.NX, NZ = k.NZ, pml = k.pml;
+  const Split sq = shot_split(F.m, NX, NZ);  // this shot's interior
   T* out = level(s, lev - 1);
   const Model<T> m{s.vel, s.nx, k.nz, pml};
   const long long ld = k.ld;
   const RawAdj<T> A1{s.fld + (LS_AR + lev % 2) * s.fstride, level(s, lev), ld, NX, NZ, pml};
   const RawAdj<T> A2{s.fld + (LS_AR + (lev + 1) % 2) * s.fstride, level(s, lev + 1), ld, NX, NZ, pml};
-  const long long j = (long long)(blockIdx.x - F.nfwg) * AWG + threadIdx.x;
+  const long long j = (long long)blockIdx.x * AWG + threadIdx.x;
   if (j >= sq.n) return;
   int x, z;
   ring_cell(sq, NX, NZ, j, &x, &z);
@@ -551,7 +567,7 @@ struct LsrtmPlan {
   const T* weight = nullptr;  // optional image-space preconditioner W of CG (caller-owned, device)
   T* masked = nullptr;     // masked copy of an adjoint input
   unsigned* pmask = nullptr;  // band_terms of the ring cells P, one table per distinct width
-  T* pcoef = nullptr;         // their PCOEF_* planes, one table per distinct width
+  T* wtab = nullptr;          // sponge weight by band index (DevShot::wtab)
   Consts<T> k{};
   Layout F{};   // forward step split (interior sweep / band)
   Layout A{};   // adjoint step split
@@ -564,7 +580,7 @@ struct LsrtmPlan {
 
   ~LsrtmPlan() {
     for (void* p : {(void*)dev, (void*)pool, snaps, (void*)scales, (void*)grads, (void*)part, (void*)cg, (void*)cg64,
-                    (void*)masked, (void*)pmask, (void*)pcoef})
+                    (void*)masked, (void*)pmask, (void*)wtab})
       if (p) (void)hipFree(p);
   }
 
@@ -629,14 +645,23 @@ struct LsrtmPlan {
       moff[w] = mtot;
       mtot += shot_split(g.pml + 1, widths[w], NZ).n;
     }
+    {
+      // sponge weights by band index, formed like sponge_weight's: entry 0
+      // inside the model, entry i + 1 for band index i (i = pml: one cell
+      // beyond the padded grid)
+      std::vector<T> wt(g.pml + 2);
+      wt[0] = sponge_weight<T>(g.pml, g.pml, 1, 1, g.pml);
+      for (int i = 0; i <= g.pml; i++) wt[i + 1] = sponge_weight<T>(g.pml - 1 - i, g.pml, 1, 1, g.pml);
+      LS_CHECK(hipMalloc(&wtab, sizeof(T) * wt.size()));
+      LS_CHECK(hipMemcpy(wtab, wt.data(), sizeof(T) * wt.size(), hipMemcpyHostToDevice));
+    }
     if (mtot > 0) {
       LS_CHECK(hipMalloc(&pmask, sizeof(unsigned) * (size_t)mtot));
-      LS_CHECK(hipMalloc(&pcoef, sizeof(T) * PCOEF_N * (size_t)mtot));
       for (size_t w = 0; w < widths.size(); w++) {
         const Ring R = shot_split(g.pml + 1, widths[w], NZ);
         if (R.n > 0)
-          hipLaunchKernelGGL((band_terms_kernel<T>), dim3((unsigned)((R.n + 255) / 256)), dim3(256), 0, stream, R,
-                             widths[w], NZ, g.pml, pmask + moff[w], pcoef + PCOEF_N * moff[w]);
+          hipLaunchKernelGGL(band_terms_kernel, dim3((unsigned)((R.n + 255) / 256)), dim3(256), 0, stream, R,
+                             widths[w], NZ, g.pml, pmask + moff[w]);
       }
     }
     host.resize(ns);
@@ -674,7 +699,7 @@ struct LsrtmPlan {
       d.sz = q.sz;
       const size_t wi = std::lower_bound(widths.begin(), widths.end(), NXs[s]) - widths.begin();
       d.pmask = pmask ? pmask + moff[wi] : nullptr;
-      d.pcoef = pcoef ? pcoef + PCOEF_N * moff[wi] : nullptr;
+      d.wtab = wtab;
       sp += snap_elem() * (size_t)(q.nt + q.ndel) * nx * g.nz;
       sc += q.nt + q.ndel;
     }
@@ -801,7 +826,10 @@ struct LsrtmPlan {
   }
 
   void launch_adj_step(const Layout& a, int st) {
-    hipLaunchKernelGGL((adj_step_kernel<T>), dim3(a.nfwg + a.nbwg, nshots), dim3(AWG), 0, stream, k, dev, a, st);
+    if (a.nfwg > 0)
+      hipLaunchKernelGGL((adj_step_kernel<T>), dim3(a.nfwg, nshots), dim3(AWG), 0, stream, k, dev, a, st);
+    if (a.nbwg > 0)
+      hipLaunchKernelGGL((adj_ring_kernel<T>), dim3(a.nbwg, nshots), dim3(AWG), 0, stream, k, dev, a, st);
   }
 
   // grad (device [nx][nz]) = L^T rec, summed over shots
