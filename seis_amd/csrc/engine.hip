@@ -681,13 +681,28 @@ __global__ __launch_bounds__(FQ_WG) void image_kernel(Consts<float> k, const Dev
   const float* U1 = s.aux + ((step + 1) % NAUX) * s.fstride + base;
   const float* R1 = s.fld + (3 + (step + 3) % 3) * s.fstride + base;
   const int nst = two ? 2 : 1;
-  // rows z0-29+q, q < 2 FQ_PAR, into A[st][f][q & 1][q >> 1] (coalesced reads)
-  for (int w = threadIdx.x; w < nst * 2 * 2 * FQ_PAR; w += FQ_WG) {
+  // rows z0-29+q, q < 2 FQ_PAR, into A[st][f][q & 1][q >> 1] (coalesced
+  // reads).  Every thread issues all of its loads before it stores any: each
+  // at a row clamped into the column (zero outside it), the second step's
+  // from the first step's fields when there is no second step (unused then).
+  constexpr int NLOAD = 2 * 2 * 2 * FQ_PAR / FQ_WG;
+  static_assert(NLOAD * FQ_WG == 2 * 2 * 2 * FQ_PAR, "the tile's inputs split evenly over the work-group");
+  const float* const U1s = two ? U1 : U0;
+  const float* const R1s = two ? R1 : R0;
+  float vals[NLOAD];
+#pragma unroll
+  for (int l = 0; l < NLOAD; l++) {
+    const int w = threadIdx.x + l * FQ_WG;
     const int q = w % (2 * FQ_PAR), sf = w / (2 * FQ_PAR);
-    const int st = sf >> 1, f = sf & 1;
-    const float* src = st == 0 ? (f == 0 ? U0 : R0) : (f == 0 ? U1 : R1);
+    const float* src = sf == 0 ? U0 : sf == 1 ? R0 : sf == 2 ? U1s : R1s;
+    vals[l] = SEIS_GL(src)[min(max(z0 - 29 + q, 0), NZ - 1)];
+  }
+#pragma unroll
+  for (int l = 0; l < NLOAD; l++) {
+    const int w = threadIdx.x + l * FQ_WG;
+    const int q = w % (2 * FQ_PAR), sf = w / (2 * FQ_PAR);
     const int r = z0 - 29 + q;
-    A[st][f][q & 1][q >> 1] = (r >= 0 && r < NZ) ? SEIS_GL(src)[r] : 0.f;
+    A[sf >> 1][sf & 1][q & 1][q >> 1] = (r >= 0 && r < NZ) ? vals[l] : 0.f;
   }
   __syncthreads();
   const int p = threadIdx.x >> 7;   // parity of this thread's outputs
@@ -722,11 +737,18 @@ __global__ __launch_bounds__(FQ_WG) void image_kernel(Consts<float> k, const Dev
   // img += U R (unless `plane`), then img += 1/2 (U R - H[U] H[R]), step by step
   const float scal = 2.0f / 4.0f;
   const auto IMG = SEIS_GL(s.img + base);
-  for (int q = threadIdx.x; q < FQ_TILE; q += FQ_WG) {
+  // the tile's image rows are all loaded first (clamped into the interior)
+  constexpr int NOUT = FQ_TILE / FQ_WG;
+  float imgv[NOUT];
+#pragma unroll
+  for (int i = 0; i < NOUT; i++) imgv[i] = IMG[min(z0 + (int)threadIdx.x + i * FQ_WG, zhi - 1)];
+#pragma unroll
+  for (int i = 0; i < NOUT; i++) {
+    const int q = threadIdx.x + i * FQ_WG;
     const int z = z0 + q;
     if (z >= zhi) break;
     const int pp = (q + 29) & 1, kk = (q + 29) >> 1;  // row z in the parity arrays
-    float img = IMG[z];
+    float img = imgv[i];
     const float ur0 = A[0][0][pp][kk] * A[0][1][pp][kk];
     if (!plane) img += ur0;
     img += scal * (ur0 - HP[0][q]);
