@@ -92,11 +92,11 @@ __global__ void band_coef_kernel(Consts<T> k, const DevShot<T>* __restrict__ sho
 }
 
 // Raw adjoints of level `lev` for the cells with a band reader or w < 1
-// (outside [pml+1, NX-pml-1) x [pml+1, NZ-pml-1)), and the one-way terms of
-// lam^{lev-1} at those cells (levels lev and lev+1 are complete now), both
-// stored for the step kernel.  A band cell b's products of a level are
-// ain = kin(b) lam(b), at = kt(b) lam(b), ac = kc(b) lam(b) (band_adj).  With
-// mk = band_terms of the cell (s.pmask):
+// (ring P: outside [pml+1, NX-pml-1) x [pml+1, NZ-pml-1)), and the one-way
+// terms of lam^{lev-1} at those cells (levels lev and lev+1 are complete now),
+// both stored for the band sweep and the ring.  A band cell b's products of a
+// level are ain = kin(b) lam(b), at = kt(b) lam(b), ac = kc(b) lam(b)
+// (band_adj).  With mk = band_terms of the cell (s.pmask):
 //   raw adjoint  w lam(c) + the products ain, at of level lev of the band
 //                cells that read raw(c), one per BT_AIN_* / BT_AT_* bit;
 //   one-way cur  the products ac of level lev of band cell c itself and of the
@@ -104,45 +104,39 @@ __global__ void band_coef_kernel(Consts<T> k, const DevShot<T>* __restrict__ sho
 //   one-way prev of level lev+1: ain(c) of a side cell c, -(1 - w) lam of the
 //                inner neighbours (BT_PIN_*), and the tangent products at of
 //                the side cells beside c (BT_TAN_*).
-// Every neighbour is loaded (all lie inside the field and its margin) and the
-// terms the mask leaves out are added as zero, in a fixed order: no load waits
-// on a branch.
+// The terms the mask leaves out are added as zero, in a fixed order.
+//
+// The neighbourhood of a ring-P cell (x, z) that these read: lam^lev (l*) and
+// kt on the 3 x 3 block around it, kin, kc and lam^{lev+1} (p*) on the cross;
+// suffixes: zm / zp = z -+ 1, xm / xp = x -+ 1, mm = (x-1, z-1), pm = (x+1,
+// z-1), mp = (x-1, z+1), pp = (x+1, z+1).
 template <typename T>
-__global__ __launch_bounds__(256) void adj_prep_kernel(Consts<T> k, const DevShot<T>* __restrict__ shots, int step) {
-  const DevShot<T>& s = shots[blockIdx.y];
-  const int lev = s.nt + s.ndel + 1 - step;
-  if (lev < 1) return;
-  const Ring R = shot_split(k.pml + 1, s.NX, k.NZ);  // ring P of this shot's width
-  const long long j = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (j >= R.n) return;
-  int x, z;
-  ring_cell(R, s.NX, k.NZ, j, &x, &z);
-  const unsigned mk = SEIS_GL(s.pmask)[j];
-  const long long ld = k.ld, f = s.fstride;
-  const long long i = fidx(ld, x, z);
-  const auto lam1 = SEIS_GL(level(s, lev)), lam2 = SEIS_GL(level(s, lev + 1));
-  const auto kin = SEIS_GL(s.fld + LS_KIN * f), kt = SEIS_GL(s.fld + LS_KT * f), kc = SEIS_GL(s.fld + LS_KC * f);
-  const long long zm = i - 1, zp = i + 1, xm = i - ld, xp = i + ld;
-  // level lev
-  const T l0 = lam1[i], l_zm = lam1[zm], l_zp = lam1[zp], l_xm = lam1[xm], l_xp = lam1[xp];
-  const T l_mm = lam1[xm - 1], l_pm = lam1[xp - 1], l_mp = lam1[xm + 1], l_pp = lam1[xp + 1];
+struct PrepIn {
+  T l0, l_zm, l_zp, l_xm, l_xp, l_mm, l_pm, l_mp, l_pp;
+  T kin0, kin_zm, kin_zp, kin_xm, kin_xp;
+  T kt_mm, kt_pm, kt_mp, kt_pp, kt_xm, kt_xp, kt_zm, kt_zp;
+  T kc0, kc_zm, kc_zp, kc_xm, kc_xp;
+  T p0, p_zm, p_zp, p_xm, p_xp;
+};
+
+// The raw adjoint *ar and the one-way terms *ow of a ring-P cell from its
+// neighbourhood, its band-term mask, its sponge weight w and -(1 - w) of its
+// four neighbours (f_*).
+template <typename T>
+__device__ __forceinline__ void prep_terms(const PrepIn<T>& n, unsigned mk, T w, T f_zm, T f_zp, T f_xm, T f_xp, T* ar,
+                                           T* ow) {
   // raw-adjoint terms
-  const T a_zm = kin[zm] * l_zm, a_zp = kin[zp] * l_zp, a_xm = kin[xm] * l_xm, a_xp = kin[xp] * l_xp;
-  const T t_mm = kt[xm - 1] * l_mm, t_pm = kt[xp - 1] * l_pm, t_mp = kt[xm + 1] * l_mp, t_pp = kt[xp + 1] * l_pp;
-  const T t_xm = kt[xm] * l_xm, t_xp = kt[xp] * l_xp;
+  const T a_zm = n.kin_zm * n.l_zm, a_zp = n.kin_zp * n.l_zp, a_xm = n.kin_xm * n.l_xm, a_xp = n.kin_xp * n.l_xp;
+  const T t_mm = n.kt_mm * n.l_mm, t_pm = n.kt_pm * n.l_pm, t_mp = n.kt_mp * n.l_mp, t_pp = n.kt_pp * n.l_pp;
+  const T t_xm = n.kt_xm * n.l_xm, t_xp = n.kt_xp * n.l_xp;
   // one-way cur terms
-  const T c_0 = kc[i] * l0, c_zm = kc[zm] * l_zm, c_zp = kc[zp] * l_zp, c_xm = kc[xm] * l_xm, c_xp = kc[xp] * l_xp;
+  const T c_0 = n.kc0 * n.l0, c_zm = n.kc_zm * n.l_zm, c_zp = n.kc_zp * n.l_zp, c_xm = n.kc_xm * n.l_xm,
+          c_xp = n.kc_xp * n.l_xp;
   // level lev+1: one-way prev terms
-  const T b_0 = kin[i] * lam2[i];
-  const T p_zm = lam2[zm], p_zp = lam2[zp], p_xm = lam2[xm], p_xp = lam2[xp];
-  const T u_xm = kt[xm] * p_xm, u_zm = kt[zm] * p_zm, u_xp = kt[xp] * p_xp, u_zp = kt[zp] * p_zp;
-  // sponge weight of the cell, -(1 - w) of its four neighbours
-  const int nx = s.nx, nz = k.nz, pml = k.pml;
-  const T w = table_weight(s.wtab, x, z, nx, nz, pml);
-  auto pin = [&](int bx, int bz) { return -(T(1) - table_weight(s.wtab, bx, bz, nx, nz, pml)); };
-  const T f_zm = pin(x, z - 1), f_zp = pin(x, z + 1), f_xm = pin(x - 1, z), f_xp = pin(x + 1, z);
+  const T b_0 = n.kin0 * n.p0;
+  const T u_xm = n.kt_xm * n.p_xm, u_zm = n.kt_zm * n.p_zm, u_xp = n.kt_xp * n.p_xp, u_zp = n.kt_zp * n.p_zp;
   auto on = [&](unsigned bit, T v) { return (mk & bit) ? v : T(0); };
-  T r = w * l0;
+  T r = w * n.l0;
   r += on(BT_AIN_ZM, a_zm);
   r += on(BT_AT_ZM_XM, t_mm);
   r += on(BT_AT_ZM_XP, t_pm);
@@ -165,16 +159,51 @@ __global__ __launch_bounds__(256) void adj_prep_kernel(Consts<T> k, const DevSho
   oc += on(BT_AC_XP, c_xp);
   T op = T(0);
   op += on(BT_AIN_SELF, b_0);
-  op += on(BT_PIN_ZM, f_zm * p_zm);
-  op += on(BT_PIN_ZP, f_zp * p_zp);
-  op += on(BT_PIN_XM, f_xm * p_xm);
-  op += on(BT_PIN_XP, f_xp * p_xp);
+  op += on(BT_PIN_ZM, f_zm * n.p_zm);
+  op += on(BT_PIN_ZP, f_zp * n.p_zp);
+  op += on(BT_PIN_XM, f_xm * n.p_xm);
+  op += on(BT_PIN_XP, f_xp * n.p_xp);
   op += on(BT_TAN_XM, u_xm);
   op += on(BT_TAN_ZM, u_zm);
   op += on(BT_TAN_XP, u_xp);
   op += on(BT_TAN_ZP, u_zp);
-  SEIS_GL(s.fld + (LS_AR + lev % 2) * s.fstride)[i] = r;
-  SEIS_GL(s.fld)[LS_OW * s.fstride + i] = oc + op;
+  *ar = r;
+  *ow = oc + op;
+}
+
+// Cell j of ring P of shot s at reverse step `step` (ring_cell order): every
+// neighbour is loaded (all lie inside the field and its margin), so that no
+// load waits on a branch.
+template <typename T>
+__device__ __forceinline__ void adj_prep(const Consts<T>& k, const DevShot<T>& s, int step, long long j) {
+  const int lev = s.nt + s.ndel + 1 - step;
+  if (lev < 1) return;
+  const Ring R = shot_split(k.pml + 1, s.NX, k.NZ);  // ring P of this shot's width
+  if (j >= R.n) return;
+  int x, z;
+  ring_cell(R, s.NX, k.NZ, j, &x, &z);
+  const unsigned mk = SEIS_GL(s.pmask)[j];
+  const long long ld = k.ld, f = s.fstride;
+  const long long i = fidx(ld, x, z);
+  const auto lam1 = SEIS_GL(level(s, lev)), lam2 = SEIS_GL(level(s, lev + 1));
+  const auto kin = SEIS_GL(s.fld + LS_KIN * f), kt = SEIS_GL(s.fld + LS_KT * f), kc = SEIS_GL(s.fld + LS_KC * f);
+  const long long zm = i - 1, zp = i + 1, xm = i - ld, xp = i + ld;
+  PrepIn<T> n;
+  n.l0 = lam1[i], n.l_zm = lam1[zm], n.l_zp = lam1[zp], n.l_xm = lam1[xm], n.l_xp = lam1[xp];
+  n.l_mm = lam1[xm - 1], n.l_pm = lam1[xp - 1], n.l_mp = lam1[xm + 1], n.l_pp = lam1[xp + 1];
+  n.kin0 = kin[i], n.kin_zm = kin[zm], n.kin_zp = kin[zp], n.kin_xm = kin[xm], n.kin_xp = kin[xp];
+  n.kt_mm = kt[xm - 1], n.kt_pm = kt[xp - 1], n.kt_mp = kt[xm + 1], n.kt_pp = kt[xp + 1];
+  n.kt_xm = kt[xm], n.kt_xp = kt[xp], n.kt_zm = kt[zm], n.kt_zp = kt[zp];
+  n.kc0 = kc[i], n.kc_zm = kc[zm], n.kc_zp = kc[zp], n.kc_xm = kc[xm], n.kc_xp = kc[xp];
+  n.p0 = lam2[i], n.p_zm = lam2[zm], n.p_zp = lam2[zp], n.p_xm = lam2[xm], n.p_xp = lam2[xp];
+  // sponge weight of the cell, -(1 - w) of its four neighbours
+  const int nx = s.nx, nz = k.nz, pml = k.pml;
+  auto pin = [&](int bx, int bz) { return -(T(1) - table_weight(s.wtab, bx, bz, nx, nz, pml)); };
+  T ar, ow;
+  prep_terms(n, mk, table_weight(s.wtab, x, z, nx, nz, pml), pin(x, z - 1), pin(x, z + 1), pin(x - 1, z),
+             pin(x + 1, z), &ar, &ow);
+  SEIS_GL(s.fld + (LS_AR + lev % 2) * f)[i] = ar;
+  SEIS_GL(s.fld + LS_OW * f)[i] = ow;
 }
 
 // Reads a raw adjoint: precomputed near the band, the level itself elsewhere.
@@ -370,52 +399,139 @@ __device__ __forceinline__ void adj_sweep(const Consts<T>& k, const DevShot<T>& 
   }
 }
 
-// One reverse step: lam^{lev-1} for every cell and grad += raw-adjoint(lev) *
-// S^{lev-1}, in two kernels.  adj_step_kernel: the cells at least 8 from the
-// edge (no band term within the stencil reach of a reader), swept by
-// wavefronts streaming along x like the forward kernel.  adj_ring_kernel: the
-// outer 8 cells, by the general gather; its reads, all issued up front, need
-// more registers than the sweep, which keeps its occupancy in a kernel of its
-// own.
-template <typename T>
-__global__ __launch_bounds__(AWG) void adj_step_kernel(Consts<T> k, const DevShot<T>* __restrict__ shots, Layout F,
-                                                       int step) {
-  __shared__ T lds[WAVES][LROW];
-  const DevShot<T>& s = shots[blockIdx.y];
-  const int lev = s.nt + s.ndel + 1 - step;
-  if (lev < 1) return;
-  const int NX = s.NX, NZ = k.NZ, pml = k.pml;
-  const Split sq = shot_split(F.m, NX, NZ);  // this shot's interior
-  // Cells at least 8 from the outer edge: every stencil reader within 4 cells
-  // uses the 9-point operator, so the stencil part is the 9-point gather of
-  // Q = (raw adjoint) vt2 / h^2 of the readers; cells at most pml from the
-  // edge add the one-way terms.  The wavefront's index is taken uniform
-  // across its lanes, so that the strip bounds, the loop and the column
-  // addresses stay in scalar registers.
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int item = xcd_tile(blockIdx.x, F.nfwg) * WAVES + wave;
-  if (item >= F.nzc * shot_strips(sq, F.bx)) return;  // whole wavefront leaves; no workgroup barrier follows
-  const int zc = item % F.nzc, xs = item / F.nzc;
+// Sweep items of the adjoint step: z chunk zc (64 rows from za + 64 zc) of
+// strip xs (columns from xa + bx xs) of a shot's interior sq.
+// An item is DEEP when its strip and every cell it reads (4 beyond it on each
+// side) lie inside [pml+1, NX-pml-1) x [pml+1, NZ-pml-1) and its chunk is
+// whole.
+SEIS_HD inline bool deep_item(const Split& sq, int zc, int xs, int bx, int NX, int NZ, int pml) {
+  const int z0 = sq.za + zc * 64;
+  const int x0 = sq.xa + xs * bx, x1 = imin(x0 + bx, sq.xb);
+  const int lo = pml + 1, hix = NX - pml - 1, hiz = NZ - pml - 1;
+  return x0 - 4 >= lo && x1 + 4 <= hix && z0 - 4 >= lo && z0 + 68 <= hiz && z0 + 64 <= sq.zb;
+}
+
+SEIS_HD inline int floor_div(int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
+SEIS_HD inline int ceil_div(int a, int b) { return -floor_div(-a, b); }
+
+// The DEEP items of a shot form a rectangle: chunks [zc0, zc0 + nzd) of
+// strips [xs0, xs0 + nxd) (empty: all zero).  Each bound of deep_item is
+// monotone in zc or xs; the plan checks this form against deep_item for
+// every width it launches.
+struct DeepBox {
+  int zc0, nzd, xs0, nxd;
+};
+
+SEIS_HD inline DeepBox deep_box(const Split& sq, int nzc, int bx, int NX, int NZ, int pml) {
+  const int nxs = shot_strips(sq, bx);
+  const int lo = pml + 1, hix = NX - pml - 1, hiz = NZ - pml - 1;
+  const int zlo = imax(0, ceil_div(lo + 4 - sq.za, 64));
+  const int zhi = imin(nzc - 1, floor_div(imin(hiz - 68, sq.zb - 64) - sq.za, 64));
+  const int xlo = imax(0, ceil_div(lo + 4 - sq.xa, bx));
+  // x1 = min(x0 + bx, xb): every strip's end qualifies when xb does
+  const int xhi = sq.xb + 4 <= hix ? nxs - 1 : imin(nxs - 1, floor_div(hix - 4 - bx - sq.xa, bx));
+  DeepBox d{zlo, zhi - zlo + 1, xlo, xhi - xlo + 1};
+  if (d.nzd <= 0 || d.nxd <= 0) d = DeepBox{0, 0, 0, 0};
+  return d;
+}
+
+// Item f of the items outside the deep box, in strip order: the strips before
+// the box (all chunks), the box's strips (chunks above and below it), the
+// strips after it.
+SEIS_HD inline void frame_item(const DeepBox& d, int nzc, int f, int* zc, int* xs) {
+  const int c1 = d.xs0 * nzc;
+  if (f < c1) {
+    *zc = f % nzc;
+    *xs = f / nzc;
+    return;
+  }
+  f -= c1;
+  const int per = nzc - d.nzd;
+  if (f < d.nxd * per) {
+    const int r = f % per;
+    *zc = r < d.zc0 ? r : r + d.nzd;
+    *xs = d.xs0 + f / per;
+    return;
+  }
+  f -= d.nxd * per;
+  *zc = f % nzc;
+  *xs = d.xs0 + d.nxd + f / nzc;
+}
+
+// One item's sweep, the snapshot form chosen by the shot.
+template <typename T, bool DEEP>
+__device__ __forceinline__ void adj_item(const Consts<T>& k, const DevShot<T>& s, const Split& sq, const Layout& F,
+                                         int lev, int zc, int xs, T* la) {
   const int z0 = sq.za + zc * 64;
   const int x0 = sq.xa + xs * F.bx, x1 = min(x0 + F.bx, sq.xb);
   const int lane = threadIdx.x & 63;
-  const int lo = pml + 1, hix = NX - pml - 1, hiz = NZ - pml - 1;
-  const bool deep = x0 - 4 >= lo && x1 + 4 <= hix && z0 - 4 >= lo && z0 + 68 <= hiz && z0 + 64 <= sq.zb;
-  const bool half = s.snap_kind != SNAP_NATIVE;
-  if (deep && half)
-    adj_sweep<T, true, true>(k, s, lev, x0, x1, z0, sq.zb, lds[wave], lane);
-  else if (deep)
-    adj_sweep<T, true, false>(k, s, lev, x0, x1, z0, sq.zb, lds[wave], lane);
-  else if (half)
-    adj_sweep<T, false, true>(k, s, lev, x0, x1, z0, sq.zb, lds[wave], lane);
+  if (s.snap_kind != SNAP_NATIVE)
+    adj_sweep<T, DEEP, true>(k, s, lev, x0, x1, z0, sq.zb, la, lane);
   else
-    adj_sweep<T, false, false>(k, s, lev, x0, x1, z0, sq.zb, lds[wave], lane);
+    adj_sweep<T, DEEP, false>(k, s, lev, x0, x1, z0, sq.zb, la, lane);
+}
+
+// One reverse step: lam^{lev-1} for every cell and grad += raw-adjoint(lev) *
+// S^{lev-1}, in two launches.
+//  * adj_deep_kernel: the raw adjoints and one-way terms of level lev in ring
+//    P (adj_prep, work-groups below npwg) and, beside them, the sweep of every
+//    DEEP item, which reads neither (fused, the two overlap: 182 us per step
+//    on the 128-shot 376 x 1151 plan against about 210 us as two launches).
+//  * adj_band_kernel: the sweep of the other items of the cells at least 8
+//    from the edge, reading the prep's results, and (work-groups from nfwg)
+//    the outer 8 cells by the general gather (adj_ring).
+// The sweeps: every stencil reader within 4 cells of a swept cell uses the
+// 9-point operator, so the stencil part is the 9-point gather of Q = (raw
+// adjoint) vt2 / h^2 of the readers; cells at most pml from the edge add the
+// one-way terms.  The wavefront's index is taken uniform across its lanes, so
+// that the strip bounds, the loop and the column addresses stay in scalar
+// registers; a wavefront past the last item leaves as a whole (no workgroup
+// barrier follows).
+template <typename T>
+__global__ __launch_bounds__(AWG) void adj_deep_kernel(Consts<T> k, const DevShot<T>* __restrict__ shots, Layout F,
+                                                       int npwg, int ndwg, int step) {
+  __shared__ T lds[WAVES][LROW];
+  const DevShot<T>& s = shots[blockIdx.y];
+  if ((int)blockIdx.x < npwg) {
+    adj_prep(k, s, step, (long long)blockIdx.x * AWG + threadIdx.x);
+    return;
+  }
+  const int lev = s.nt + s.ndel + 1 - step;
+  if (lev < 1) return;
+  const Split sq = shot_split(F.m, s.NX, k.NZ);  // this shot's interior
+  const DeepBox d = deep_box(sq, F.nzc, F.bx, s.NX, k.NZ, k.pml);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int item = xcd_tile(blockIdx.x - npwg, ndwg) * WAVES + wave;
+  if (item >= d.nzd * d.nxd) return;
+  adj_item<T, true>(k, s, sq, F, lev, d.zc0 + item % d.nzd, d.xs0 + item / d.nzd, lds[wave]);
 }
 
 template <typename T>
-__global__ __launch_bounds__(AWG) void adj_ring_kernel(Consts<T> k, const DevShot<T>* __restrict__ shots, Layout F,
-                                                       int step) {
+__device__ void adj_ring(const Consts<T>& k, const DevShot<T>& s, const Layout& F, int step, long long j);
+
+template <typename T>
+__global__ __launch_bounds__(AWG) void adj_band_kernel(Consts<T> k, const DevShot<T>* __restrict__ shots, Layout F,
+                                                       int nfwg, int step) {
+  __shared__ T lds[WAVES][LROW];
   const DevShot<T>& s = shots[blockIdx.y];
+  if ((int)blockIdx.x >= nfwg) {
+    adj_ring(k, s, F, step, (long long)(blockIdx.x - nfwg) * AWG + threadIdx.x);
+    return;
+  }
+  const int lev = s.nt + s.ndel + 1 - step;
+  if (lev < 1) return;
+  const Split sq = shot_split(F.m, s.NX, k.NZ);  // this shot's interior
+  const DeepBox d = deep_box(sq, F.nzc, F.bx, s.NX, k.NZ, k.pml);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int item = xcd_tile(blockIdx.x, nfwg) * WAVES + wave;
+  if (item >= F.nzc * shot_strips(sq, F.bx) - d.nzd * d.nxd) return;
+  int zc, xs;
+  frame_item(d, F.nzc, item, &zc, &xs);
+  adj_item<T, false>(k, s, sq, F, lev, zc, xs, lds[wave]);
+}
+
+template <typename T>
+__device__ void adj_ring(const Consts<T>& k, const DevShot<T>& s, const Layout& F, int step, long long j) {
   const int lev = s.nt + s.ndel + 1 - step;
   if (lev < 1) return;
   const int NX = s.NX, NZ = k.NZ, pml = k.pml;
@@ -425,7 +541,6 @@ __global__ __launch_bounds__(AWG) void adj_ring_kernel(Consts<T> k, const DevSho
   const long long ld = k.ld;
   const RawAdj<T> A1{s.fld + (LS_AR + lev % 2) * s.fstride, level(s, lev), ld, NX, NZ, pml};
   const RawAdj<T> A2{s.fld + (LS_AR + (lev + 1) % 2) * s.fstride, level(s, lev + 1), ld, NX, NZ, pml};
-  const long long j = (long long)blockIdx.x * AWG + threadIdx.x;
   if (j >= sq.n) return;
   int x, z;
   ring_cell(sq, NX, NZ, j, &x, &z);
@@ -825,11 +940,46 @@ struct LsrtmPlan {
     return SEIS_OK;
   }
 
+  // Work-group counts of the two sweep launches at strip length adj_bx: the
+  // most DEEP and the most other items of a shot.  deep_box is checked
+  // against deep_item on every item of every shot first.
+  int adj_bx = -1, ndwg = 0, nfrwg = 0, npwg = 0;
+  int adj_counts(const Layout& a) {
+    if (a.bx == adj_bx) return SEIS_OK;
+    int dmax = 0, fmax = 0;
+    for (int s = 0; s < nshots; s++) {
+      const Split sq = shot_split(a.m, NXs[s], NZ);
+      const int nxs = shot_strips(sq, a.bx);
+      const DeepBox d = deep_box(sq, a.nzc, a.bx, NXs[s], NZ, g.pml);
+      for (int xs = 0; xs < nxs; xs++)
+        for (int zc = 0; zc < a.nzc; zc++) {
+          const bool in = zc >= d.zc0 && zc < d.zc0 + d.nzd && xs >= d.xs0 && xs < d.xs0 + d.nxd;
+          if (in != deep_item(sq, zc, xs, a.bx, NXs[s], NZ, g.pml))
+            return set_error("lsrtm: adjoint sweep items do not split into a rectangle"), SEIS_ERR_ARG;
+        }
+      dmax = std::max(dmax, d.nzd * d.nxd);
+      fmax = std::max(fmax, a.nzc * nxs - d.nzd * d.nxd);
+    }
+    auto wgs = [](int items) {
+      const int w = (items + WAVES - 1) / WAVES;
+      return (w + NXCD - 1) / NXCD * NXCD;  // xcd_tile deals whole rounds
+    };
+    ndwg = wgs(dmax);
+    nfrwg = wgs(fmax);
+    npwg = P.nbwg;  // the ring-P cells of the widest shot, AWG per work-group
+    adj_bx = a.bx;
+    return SEIS_OK;
+  }
+
+  // The two launches of reverse step st at the counts of adj_counts(a): ring
+  // P with the DEEP items, the other items with the outer ring.
   void launch_adj_step(const Layout& a, int st) {
-    if (a.nfwg > 0)
-      hipLaunchKernelGGL((adj_step_kernel<T>), dim3(a.nfwg, nshots), dim3(AWG), 0, stream, k, dev, a, st);
-    if (a.nbwg > 0)
-      hipLaunchKernelGGL((adj_ring_kernel<T>), dim3(a.nbwg, nshots), dim3(AWG), 0, stream, k, dev, a, st);
+    if (npwg + ndwg > 0)
+      hipLaunchKernelGGL((adj_deep_kernel<T>), dim3(npwg + ndwg, nshots), dim3(AWG), 0, stream, k, dev, a, npwg, ndwg,
+                         st);
+    if (nfrwg + a.nbwg > 0)
+      hipLaunchKernelGGL((adj_band_kernel<T>), dim3(nfrwg + a.nbwg, nshots), dim3(AWG), 0, stream, k, dev, a, nfrwg,
+                         st);
   }
 
   // grad (device [nx][nz]) = L^T rec, summed over shots
@@ -841,17 +991,21 @@ struct LsrtmPlan {
     }
     for (int s = 0; s < nshots; s++) host[s].rec = const_cast<T*>(rec) + spec[s].rec;
     LS_CHECK(hipMemcpyAsync(dev, host.data(), sizeof(DevShot<T>) * nshots, hipMemcpyHostToDevice, stream));
+    int rc;
     if (!adj_tuned) {
       // strip length of the adjoint sweep: its step 0 (no gradient update
       // yet); the fields it writes are cleared below
       auto fn = [](void* c, int bx) {
         LsrtmPlan* p = static_cast<LsrtmPlan*>(c);
-        p->launch_adj_step(with_strip(p->A, bx), 0);
+        const Layout a = with_strip(p->A, bx);
+        if (p->adj_counts(a) == SEIS_OK) p->launch_adj_step(a, 0);
       };
       A = with_strip(A, tuned_strip("lsrtm_adjoint", NXmax, NZ, nshots, stream, fn, this));
       adj_tuned = true;
     }
-    int rc = zero_fields(0, LS_NADJ);
+    rc = adj_counts(A);
+    if (rc) return rc;
+    rc = zero_fields(0, LS_NADJ);
     if (rc) return rc;
     LS_CHECK(hipMemsetAsync(grads, 0, sizeof(T) * (size_t)model_cells, stream));
     const double E = sizeof(T);
@@ -863,10 +1017,8 @@ struct LsrtmPlan {
           c += (double)NXs[s] * NZ;
           mc += (double)(NXs[s] - 2 * g.pml) * g.nz;
         }
-      timed_launch(stream, "lsrtm_adjoint_prep", 8 * E * pc, 0, [&] {
-        hipLaunchKernelGGL((adj_prep_kernel<T>), dim3(P.nbwg, nshots), dim3(256), 0, stream, k, dev, st);
-      });
-      const double by = 4 * E * c + (snap_elem() + 2 * E) * mc;
+      // sweep: 16 B per cell plus snapshot and gradient; prep: 8 B per ring-P cell
+      const double by = 4 * E * c + (snap_elem() + 2 * E) * mc + 8 * E * pc;
       timed_launch(stream, "lsrtm_adjoint", by, c, [&] { launch_adj_step(A, st); });
     }
     hipLaunchKernelGGL((sum_grads_kernel<T>), dim3(256), dim3(256), 0, stream, dev, nshots, g.nz, model, grad);
