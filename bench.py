@@ -44,11 +44,15 @@ sys.path.insert(0, ROOT)
 METRIC = "Gcell-updates/s (batched shots) + LSRTM s/CG-iter vs plain FP32 at stated error"
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the headline's kernels (26 shots per launch, strip
-# length 32), from rocprofv3 FETCH_SIZE / WRITE_SIZE calibrated against a 1 GiB
-# copy; the commands and counter values are in profiles/c5_rtm_hbm_traffic.md.
-# Scaled by the shots of a launch.
-MEASURED_TRAFFIC = {"shots": 26, "bytes": {"step_rtm_reverse": 2619605573, "step_rtm_forward": 1279442685,
-                                           "rtm_image": 1449277885}}
+# length 16, the tuner's usual choice on this launch), from rocprofv3 FETCH_SIZE
+# / WRITE_SIZE calibrated against a 1 GiB copy and, equal to them, the
+# memory-side 32-byte request counters; the commands and counter values are
+# in profiles/c5_rtm_hbm_traffic.md.  step_rtm_reverse is the mean over the
+# reverse launches, every second of which also runs the imaging pass of the two
+# steps before it.  Scaled by the shots of a launch.
+MEASURED_TRAFFIC = {"shots": 26, "strip_length": 16,
+                    "bytes": {"step_rtm_reverse": 3350472666, "step_rtm_forward": 1328994835,
+                              "rtm_image": 1449277885}}
 
 
 def host_cpu_model():
@@ -685,6 +689,9 @@ def main():
     ap.add_argument("--plane-shots", type=int, default=96, help="shots of the plane-wave synthesis leg (0 skips it)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 8 device-to-device copies of 1 GiB (HBM counter calibration)")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="default strip length (16 columns) instead of the tuner's choice, for counter runs, "
+                         "whose serialised launches would mislead the tuner's timing")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the batch of shots is one GPU's share; "
@@ -694,6 +701,8 @@ def main():
     import seis_amd
     from seis_amd import DeviceArray
     from seis_amd._lib import check
+    if args.no_tune:
+        seis_amd.set_tuning(False)
     if args.copy_calibration:
         n = 1 << 28  # 1 GiB of float32
         a = DeviceArray((n,), np.float32, zero=True)
@@ -790,6 +799,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom[0], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic * S / MEASURED_TRAFFIC["shots"] if traffic else None,
+                     "traffic_strip_length": MEASURED_TRAFFIC["strip_length"],
                      "avg_launch_ms": round(dom[1]["total_ms"] / dom[1]["launches"], 4),
                      "alg_bytes_per_launch": dom[1]["alg_bytes"] / dom[1]["launches"]},
         "kernels": {k: {"launches": v["launches"], "ms": round(v["total_ms"], 2),

@@ -45,8 +45,9 @@ struct DevShot {
 };
 
 // U buffers of the RTM reverse pass: the imaging pass applies two
-// consecutive steps at once, after the second of them.
-constexpr int NAUX = 2;
+// consecutive steps at once, in the launch of the step after them, which
+// writes a third buffer (four, so that it never writes one being read).
+constexpr int NAUX = 4;
 
 // Step kernel modes.  MODEL / BORN / RTM_* follow the reference's modeling,
 // forward and adjoint (fd.cpp:15-177, 592-719); SNAP_* run the background of

@@ -584,11 +584,12 @@ __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot
   }
 }
 
+// One work-group of a step launch: block bx of shot `shot` (sweep strips
+// below L.nfwg, ring cells above).
 template <typename T, int MODE>
-__global__ __launch_bounds__(WG) void step_kernel(Consts<T> k, const DevShot<T>* __restrict__ shots, Layout L,
-                                                 int step) {
-  __shared__ T lds[WAVES][2 * LROW];
-  const DevShot<T>& s = shots[blockIdx.y];
+__device__ __forceinline__ void step_block(const Consts<T>& k, const DevShot<T>* __restrict__ shots, const Layout& L,
+                                           int step, int bx, int shot, T (*lds)[2 * LROW]) {
+  const DevShot<T>& s = shots[shot];
   StepCtx<T> c;
   c.it = step_it<T>(MODE, s, step);
   if (c.it < 0) return;
@@ -602,11 +603,11 @@ __global__ __launch_bounds__(WG) void step_kernel(Consts<T> k, const DevShot<T>*
   c.aux = s.aux ? s.aux + (step % NAUX) * s.fstride : nullptr;
   const int NX = s.NX, NZ = k.NZ;
   const Split q = shot_split(L.m, NX, NZ);  // this shot's interior (widths differ within a batch)
-  if ((int)blockIdx.x < L.nfwg) {
+  if (bx < L.nfwg) {
     // the wavefront's index, uniform across its lanes, keeps the strip
     // bounds, the sweep's loop and its column addresses in scalar registers
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int item = xcd_tile(blockIdx.x, L.nfwg) * WAVES + wave;
+    const int item = xcd_tile(bx, L.nfwg) * WAVES + wave;
     if (item >= L.nzc * shot_strips(q, L.bx)) return;  // whole wavefront leaves; no workgroup barrier follows
     const int zc = item % L.nzc, xs = item / L.nzc;
     const int z0 = q.za + zc * 64;
@@ -615,7 +616,7 @@ __global__ __launch_bounds__(WG) void step_kernel(Consts<T> k, const DevShot<T>*
     interior_sweep<T, MODE>(k, s, c, z0, q.zb, x0, x1, lds[wave]);
     return;
   }
-  const long long j = (long long)(blockIdx.x - L.nfwg) * WG + threadIdx.x;
+  const long long j = (long long)(bx - L.nfwg) * WG + threadIdx.x;
   if (j >= q.n) return;
   int x, z;
   ring_cell(q, NX, NZ, j, &x, &z);
@@ -623,6 +624,13 @@ __global__ __launch_bounds__(WG) void step_kernel(Consts<T> k, const DevShot<T>*
     band_raw<T, MODE>(k, s, c, x, z);
   else
     cell_update<T, MODE>(k, s, c, x, z);
+}
+
+template <typename T, int MODE>
+__global__ __launch_bounds__(WG) void step_kernel(Consts<T> k, const DevShot<T>* __restrict__ shots, Layout L,
+                                                 int step) {
+  __shared__ T lds[WAVES][2 * LROW];
+  step_block<T, MODE>(k, shots, L, step, blockIdx.x, blockIdx.y, lds);
 }
 
 // The imaging condition of one reverse step for one interior column x
@@ -659,21 +667,30 @@ constexpr int FQ_PAR = FQ_TILE / 2 + 32;  // inputs per parity array (multiple o
 // `plane`: step `step` of adjointPlane's forward pass instead (it = step+2,
 // imaged when it > ndel, FaQi term only; never paired).
 // Grid: (row tiles, interior columns of the widest shot, shots).
+// LDS of one imaging work-group: the parity-split inputs A[step][U, R][parity][k]
+// and H[U] H[R] per step and output.
+struct ImageLds {
+  float A[2][2][2][FQ_PAR];
+  float HP[2][FQ_TILE];
+};
+
+// One imaging work-group: row tile `tile` of interior column `col` of shot `shot`.
 template <int HALF>
-__global__ __launch_bounds__(FQ_WG) void image_kernel(Consts<float> k, const DevShot<float>* __restrict__ shots,
-                                                      int step, HilbertTaps taps, int pair, int plane) {
+__device__ __forceinline__ void image_block(const Consts<float>& k, const DevShot<float>* __restrict__ shots, int step,
+                                            const HilbertTaps& taps, int pair, int plane, int tile, int col, int shot,
+                                            ImageLds& sm) {
   static_assert(HALF == 30, "the tile arithmetic assumes FaQi's 61-tap filter");
-  __shared__ __align__(16) float A[2][2][2][FQ_PAR];  // [step][U, R][parity][k]
-  __shared__ float HP[2][FQ_TILE];                    // H[U] H[R] per step and output
-  const DevShot<float>& s = shots[blockIdx.z];
-  const int x = k.pml + blockIdx.y;
+  float (&A)[2][2][2][FQ_PAR] = sm.A;
+  float (&HP)[2][FQ_TILE] = sm.HP;
+  const DevShot<float>& s = shots[shot];
+  const int x = k.pml + col;
   if (x >= s.NX - k.pml) return;
   const int it = plane ? step + 2 : s.nt + s.ndel - 1 - step;
   if (plane ? (it >= s.nt + s.ndel || it - s.ndel <= 0) : it < s.ndel) return;
   const bool two = !plane && pair && it - 1 >= s.ndel;
   const int NZ = k.NZ;
   const int zhi = NZ - k.pml;
-  const int z0 = k.pml + blockIdx.x * FQ_TILE;
+  const int z0 = k.pml + tile * FQ_TILE;
   if (z0 >= zhi) return;
   const long long base = fidx(k.ld, x, 0);
   const float* U0 = s.aux + (step % NAUX) * s.fstride + base;
@@ -759,6 +776,43 @@ __global__ __launch_bounds__(FQ_WG) void image_kernel(Consts<float> k, const Dev
     }
     IMG[z] = img;
   }
+}
+
+template <int HALF>
+__global__ __launch_bounds__(FQ_WG) void image_kernel(Consts<float> k, const DevShot<float>* __restrict__ shots,
+                                                      int step, HilbertTaps taps, int pair, int plane) {
+  __shared__ __align__(16) ImageLds sm;
+  image_block<HALF>(k, shots, step, taps, pair, plane, blockIdx.x, blockIdx.y, blockIdx.z, sm);
+}
+
+// Reverse step `step` of every shot (the step kernel's work-groups) fused
+// with the imaging pass of reverse steps step-2 and step-1 (the last nimg
+// work-groups: tile t % ntile of interior column t / ntile).  The
+// two touch disjoint buffers: the pass reads U of steps step-2, step-1 (NAUX =
+// 4, so step writes another U buffer) and the receiver levels of those steps
+// (step reads them as its previous and current level and writes the third
+// slot), and writes only the image.  The imaging pass is bound by its
+// floating-point work and the step by memory, so in one launch each fills the
+// other's idle time.  The step's work-groups (sweep, then ring cells) come
+// first, so that the sweep keeps its XCD mapping, then the imaging pass's.
+// (Measured on the 26-shot survey launch at nt = 1000: 210.2 Gcell-updates/s
+// in this order, 209.2 with the imaging pass first, 182.8 with the two dealt
+// alternately to each XCD over the same columns.)
+__global__ __launch_bounds__(WG) void rtm_reverse_image_kernel(Consts<float> k, const DevShot<float>* __restrict__ shots,
+                                                               Layout L, int step, HilbertTaps taps, int ntile,
+                                                               int nimg) {
+  static_assert(FQ_WG == WG, "one work-group size for both parts");
+  static_assert(sizeof(ImageLds) >= sizeof(float) * WAVES * 2 * LROW, "the step's LDS rows fit");
+  __shared__ __align__(16) unsigned char sm[sizeof(ImageLds)];
+  const int nstep = (int)gridDim.x - nimg;  // the step's work-groups
+  const int b = blockIdx.x;
+  if (b >= nstep) {
+    const int t = b - nstep;
+    image_block<HILBERT_HALF>(k, shots, step - 2, taps, 1, 0, t % ntile, t / ntile, blockIdx.y,
+                              *reinterpret_cast<ImageLds*>(sm));
+    return;
+  }
+  step_block<float, RTM_BWD>(k, shots, L, step, b, blockIdx.y, reinterpret_cast<float (*)[2 * LROW]>(sm));
 }
 
 // Row tiles of the imaging pass for padded depth NZ.
@@ -1153,28 +1207,43 @@ int run_chunk(int mode, const GridSpec& g, int n, const ShotSpec* shots, T* cons
     constexpr int half = HILBERT_HALF;
     HilbertTaps taps{};
     hilbert_taps_host<float>(half, taps.h);
-    // two reverse steps per imaging pass
-    const int per = 2;
-    const dim3 igrid((unsigned)image_tiles(NZ, g.pml), (unsigned)nxmax, (unsigned)n);
-    // Imaging pass q covers reverse steps [per q, per q + per) and runs on the
-    // same stream right after the last of them, while their U buffers and
-    // receiver levels are intact.  (On a second stream beside the next reverse
-    // step it measured no faster: both are bound by HBM bandwidth, and the
-    // reverse step then loses the occupancy it needs to hide its latency.)
+    // Imaging pass q covers reverse steps 2q and 2q+1.  It runs inside the
+    // launch of reverse step 2q+2 (rtm_reverse_image_kernel), while their U
+    // buffers and receiver levels are intact; the last pair (or a last single
+    // step) in a launch of its own after the loop.  (As a separate launch on
+    // the same stream after step 2q+1 the pass cost 0.40 ms per pair on the
+    // 26-shot survey launch; on a second stream beside the next reverse step
+    // it measured no faster than that.)
+    const int ntile = image_tiles(NZ, g.pml);
+    const int nimg = ntile * nxmax;
+    // per step U and R over the interior columns; the image once per pair
+    auto image_bytes = [&](int first, int pair) {
+      double fq0, mc0, fq1 = 0;
+      cells_bwd(first, nullptr, &fq0, &mc0);
+      if (pair) cells_bwd(first + 1, nullptr, &fq1, nullptr);
+      return 2 * E * (fq0 + fq1) + 2 * E * mc0;
+    };
     for (int st = 0; st < maxbwd; st++) {
       double sb;
       const double a = cells_bwd(st, &sb, nullptr, nullptr);
-      launch(quiet, stream, "step_rtm_reverse", 8 * E * a + sb, 2 * a,
-             [&] { launch_step<T>(RTM_BWD, k, dev, L, B, n, st, stream); });
-      if ((st + 1) % per != 0 && st != maxbwd - 1) continue;
-      const int q = st / per, first = q * per;
-      const int pair = st > first;
-      double fq0, mc0, fq1 = 0;
-      cells_bwd(first, nullptr, &fq0, &mc0);
-      if (pair) cells_bwd(st, nullptr, &fq1, nullptr);
-      // per step U and R over the interior columns; the image once
-      launch(quiet, stream, "rtm_image", 2 * E * (fq0 + fq1) + 2 * E * mc0, 0, [&] {
-        hipLaunchKernelGGL((image_kernel<half>), igrid, dim3(FQ_WG), 0, stream, k, dev, first, taps, pair, 0);
+      const bool fused = st >= 2 && st % 2 == 0;
+      launch(quiet, stream, "step_rtm_reverse", 8 * E * a + sb + (fused ? image_bytes(st - 2, 1) : 0.0), 2 * a, [&] {
+        if (!fused) {
+          launch_step<T>(RTM_BWD, k, dev, L, B, n, st, stream);
+          return;
+        }
+        hipLaunchKernelGGL(rtm_reverse_image_kernel, dim3(nimg + L.nfwg + L.nbwg, n), dim3(WG), 0, stream, k, dev, L,
+                           st, taps, ntile, nimg);
+        if (B.nbwg > 0)
+          hipLaunchKernelGGL((band_kernel<T, RTM_BWD>), dim3((unsigned)B.nbwg, n), dim3(WG), 0, stream, k, dev, st);
+      });
+    }
+    if (maxbwd > 0) {
+      const int first = (maxbwd - 1) / 2 * 2;
+      const int pair = maxbwd - 1 > first;
+      launch(quiet, stream, "rtm_image", image_bytes(first, pair), 0, [&] {
+        hipLaunchKernelGGL((image_kernel<half>), dim3((unsigned)ntile, (unsigned)nxmax, (unsigned)n), dim3(FQ_WG), 0,
+                           stream, k, dev, first, taps, pair, 0);
       });
     }
     hipLaunchKernelGGL((crop_kernel<T>), dim3(64, n), dim3(256), 0, stream, k, dev);

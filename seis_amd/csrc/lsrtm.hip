@@ -670,6 +670,14 @@ struct LsrtmPlan {
   std::vector<int> NXs;    // padded width of each shot's window
   std::vector<DevShot<T>> host;
   DevShot<T>* dev = nullptr;
+  // The exact adjoint's view of the same fields: origins `adj_shift` elements
+  // later, so that row 8, where its sweep's 64-row chunks start, begins a
+  // 64-element line (the forward passes' chunks start at row max(pml, 4),
+  // which field_origin aligns).  Its slots are cleared or written by the
+  // adjoint alone (the band coefficients once, with this view).
+  std::vector<DevShot<T>> host_adj;
+  DevShot<T>* dev_adj = nullptr;
+  long long adj_shift = 0;
   T* pool = nullptr;       // LS_NSLOT fields per shot
   void* snaps = nullptr;   // snapshots of all shots
   float* scales = nullptr;
@@ -694,7 +702,7 @@ struct LsrtmPlan {
   hipStream_t stream = 0;
 
   ~LsrtmPlan() {
-    for (void* p : {(void*)dev, (void*)pool, snaps, (void*)scales, (void*)grads, (void*)part, (void*)cg, (void*)cg64,
+    for (void* p : {(void*)dev, (void*)dev_adj, (void*)pool, snaps, (void*)scales, (void*)grads, (void*)part, (void*)cg, (void*)cg64,
                     (void*)masked, (void*)pmask, (void*)wtab})
       if (p) (void)hipFree(p);
   }
@@ -714,7 +722,12 @@ struct LsrtmPlan {
     if (kind < 0 || kind > 2) return set_error("lsrtm: snapshot kind must be 0, 1 or 2"), SEIS_ERR_ARG;
     if (ns <= 0) return set_error("lsrtm: no shots"), SEIS_ERR_ARG;
     NZ = g.nz + 2 * g.pml;
-    ld = field_ld(NZ, g.pml);
+    {
+      const int mz = field_mz(g.pml), za = 8;  // the adjoint sweep's first row (make_layout margin below)
+      adj_shift = (64 - (mz + za) % 64) % 64;
+      // the column keeps at least 4 rows below the grid in the shifted view
+      ld = ((long long)mz + adj_shift + NZ + 4 + 63) / 64 * 64;
+    }
     model = (long long)g.nx * g.nz;
     spec.assign(shots, shots + ns);
     NXs.resize(ns);
@@ -820,6 +833,13 @@ struct LsrtmPlan {
     }
     LS_CHECK(hipMalloc(&dev, sizeof(DevShot<T>) * ns));
     LS_CHECK(hipMemcpy(dev, host.data(), sizeof(DevShot<T>) * ns, hipMemcpyHostToDevice));
+    host_adj = host;
+    for (DevShot<T>& d : host_adj) {
+      d.fld += adj_shift;
+      d.raw += adj_shift;
+    }
+    LS_CHECK(hipMalloc(&dev_adj, sizeof(DevShot<T>) * ns));
+    LS_CHECK(hipMemcpy(dev_adj, host_adj.data(), sizeof(DevShot<T>) * ns, hipMemcpyHostToDevice));
     {
       // strip length of the forward passes: the background step with the
       // max-abs reduction at step 0 on zeroed fields (writes zeros only)
@@ -835,7 +855,7 @@ struct LsrtmPlan {
     const long long nband = shot_split(g.pml, NXmax, NZ).n;
     if (nband > 0)
       hipLaunchKernelGGL((band_coef_kernel<T>), dim3((unsigned)((nband + 255) / 256), ns), dim3(256), 0, stream, k,
-                         dev);
+                         dev_adj);
     LS_CHECK(hipGetLastError());
     return make_snapshots();
   }
@@ -975,11 +995,11 @@ struct LsrtmPlan {
   // P with the DEEP items, the other items with the outer ring.
   void launch_adj_step(const Layout& a, int st) {
     if (npwg + ndwg > 0)
-      hipLaunchKernelGGL((adj_deep_kernel<T>), dim3(npwg + ndwg, nshots), dim3(AWG), 0, stream, k, dev, a, npwg, ndwg,
-                         st);
+      hipLaunchKernelGGL((adj_deep_kernel<T>), dim3(npwg + ndwg, nshots), dim3(AWG), 0, stream, k, dev_adj, a, npwg,
+                         ndwg, st);
     if (nfrwg + a.nbwg > 0)
-      hipLaunchKernelGGL((adj_band_kernel<T>), dim3(nfrwg + a.nbwg, nshots), dim3(AWG), 0, stream, k, dev, a, nfrwg,
-                         st);
+      hipLaunchKernelGGL((adj_band_kernel<T>), dim3(nfrwg + a.nbwg, nshots), dim3(AWG), 0, stream, k, dev_adj, a,
+                         nfrwg, st);
   }
 
   // grad (device [nx][nz]) = L^T rec, summed over shots
@@ -989,8 +1009,8 @@ struct LsrtmPlan {
       hipLaunchKernelGGL((mul_kernel<T>), dim3(1024), dim3(256), 0, stream, masked, rec, mask, nrec);
       rec = masked;
     }
-    for (int s = 0; s < nshots; s++) host[s].rec = const_cast<T*>(rec) + spec[s].rec;
-    LS_CHECK(hipMemcpyAsync(dev, host.data(), sizeof(DevShot<T>) * nshots, hipMemcpyHostToDevice, stream));
+    for (int s = 0; s < nshots; s++) host_adj[s].rec = host[s].rec = const_cast<T*>(rec) + spec[s].rec;
+    LS_CHECK(hipMemcpyAsync(dev_adj, host_adj.data(), sizeof(DevShot<T>) * nshots, hipMemcpyHostToDevice, stream));
     int rc;
     if (!adj_tuned) {
       // strip length of the adjoint sweep: its step 0 (no gradient update
@@ -1021,7 +1041,7 @@ struct LsrtmPlan {
       const double by = 4 * E * c + (snap_elem() + 2 * E) * mc + 8 * E * pc;
       timed_launch(stream, "lsrtm_adjoint", by, c, [&] { launch_adj_step(A, st); });
     }
-    hipLaunchKernelGGL((sum_grads_kernel<T>), dim3(256), dim3(256), 0, stream, dev, nshots, g.nz, model, grad);
+    hipLaunchKernelGGL((sum_grads_kernel<T>), dim3(256), dim3(256), 0, stream, dev_adj, nshots, g.nz, model, grad);
     LS_CHECK(hipGetLastError());
     LS_CHECK(hipStreamSynchronize(stream));
     return SEIS_OK;

@@ -13,7 +13,9 @@ from seis_amd.batch import (PLAN_BORN, PLAN_ILLUM, PLAN_MODEL, PLAN_RTM, PLAN_RT
                             born_batch, modeling_batch, pack, rtm_batch)
 from seis_amd._lib import SeisError
 
-FIELDS = {PLAN_MODEL: 4, PLAN_BORN: 8, PLAN_RTM: 11, PLAN_ILLUM: 5, PLAN_RTM_PLANE: 11}
+# fields per shot: three levels per propagated field, the band's stencil values,
+# and for RTM the padded image and four U buffers of the imaging condition
+FIELDS = {PLAN_MODEL: 4, PLAN_BORN: 8, PLAN_RTM: 13, PLAN_ILLUM: 5, PLAN_RTM_PLANE: 13}
 DESC_MAX = 512  # the descriptor is a few hundred bytes; the checks hold for any size up to this
 
 
