@@ -334,12 +334,18 @@ __device__ __forceinline__ T d2_z(const Consts<T>& k, const T* r) {
 }
 
 // Interior sweep of one wavefront: z = z0 + lane (cells with z >= zend idle),
-// x from x0 to x1-1.  The current level of each field is held in a 9-deep
-// register queue along x; the centre row is exchanged through LDS for the z
-// derivative.  Each cell's value is final (no absorbing band in here).
+// over columns [x0, x1), from x0 up, or from x1-1 down when `rev` (uniform
+// across the wavefront).  The current level of each field is held in a 9-deep
+// register queue in x order (qa[j] = column x-4+j) whichever way the sweep
+// runs, so the stencil sums keep their order; the centre row is exchanged
+// through LDS for the z derivative.  Each cell's value is final (no absorbing
+// band in here).  Neighbouring strips sweep in opposite directions, so the 8
+// columns two strips share are read by both at about the same time (both at
+// the start or both at the end of their sweeps) and the second read finds
+// them in L2.
 template <typename T, int MODE>
 __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot<T>& s, const StepCtx<T>& c, int z0,
-                                               int zend, int x0, int x1, T* lds) {
+                                               int zend, int x0, int x1, bool rev, T* lds) {
   constexpr bool TWO = (MODE == BORN || MODE == RTM_BWD || MODE == PW_FWD);
   const int lane = threadIdx.x & 63;
   const int z = z0 + lane;
@@ -366,13 +372,20 @@ __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot
   const int hslot = lane < 4 ? lane : lane + 8;
   const int iz = min(z - pml, k.nz - 1);  // clamped: idle lanes past the model load in bounds
   const int sxp = s.sx + pml, szp = s.sz + pml;
+  // first column and the column the sweep reaches next; the queue slot that
+  // the fetch fills (8 going up, 0 going down) and the fetched column's
+  // offset from the swept one
+  const int xs0 = rev ? x1 - 1 : x0;
+  const int dx = rev ? -1 : 1;
+  const long long ahead = (rev ? -4 : 4) * ld;
   T qa[9], qb[9];
 #pragma unroll
   for (int j = 0; j < 8; j++) {
-    qa[j] = CA[(long long)(x0 - 4 + j) * ld + zl];
-    if (TWO) qb[j] = CB[(long long)(x0 - 4 + j) * ld + zl];
+    const int slot = rev ? j + 1 : j;
+    qa[slot] = CA[(long long)(xs0 - 4 + slot) * ld + zl];
+    if (TWO) qb[slot] = CB[(long long)(xs0 - 4 + slot) * ld + zl];
   }
-  // The loads of column x+1 are issued before column x is computed (one
+  // The loads of the next column are issued before column x is computed (one
   // column of software pipelining; the last column re-reads itself).
   // The receiver depth of the column (and, in the reverse pass, its record
   // sample) is fetched the same way.
@@ -395,10 +408,10 @@ __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot
   const bool acc = MODE == ILLUM && c.it > s.ndel && act;  // accumulate u^2 (interior cells are model cells)
   const auto SSA = SEIS_GL(s.img);
   auto fetch = [&](int xx) {
-    xx = min(xx, x1 - 1);
+    xx = min(max(xx, x0), x1 - 1);
     const long long r = (long long)xx * ld;
-    nca = CA[r + 4 * ld + zl];
-    if (TWO) ncb = CB[r + 4 * ld + zl];
+    nca = CA[r + ahead + zl];
+    if (TWO) ncb = CB[r + ahead + zl];
     npa = PA[r + zl];
     if (TWO) npb = PB[r + zl];
     nv = VEL[(long long)(xx - pml) * k.nz + iz];
@@ -425,7 +438,7 @@ __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot
       nsn = snap_fetch(s, k.nz, c.it, j);
     }
   };
-  fetch(x0);
+  fetch(xs0);
   // per-step scalars
   const float snap_sc =
       ((MODE == BORN_S || MODE == ADJ_X) && s.snap_kind != SNAP_NATIVE) ? SEIS_GL(s.scale)[c.it] : 0.f;
@@ -443,10 +456,15 @@ __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot
   // SNAP_MAX: this lane's max |S| over the sweep; one atomic per wavefront
   // after it (non-negative floats order like their bit patterns)
   float smax = 0.f;
-  for (int x = x0; x < x1; x++) {
+  for (int x = xs0, n = x1 - x0; n > 0; x += dx, n--) {
     const long long row = (long long)x * ld;
-    qa[8] = nca;
-    if (TWO) qb[8] = ncb;
+    if (rev) {
+      qa[0] = nca;
+      if (TWO) qb[0] = ncb;
+    } else {
+      qa[8] = nca;
+      if (TWO) qb[8] = ncb;
+    }
     if (halo) {
       la[hslot] = nha;
       if (TWO) lb[hslot] = nhb;
@@ -460,7 +478,7 @@ __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot
     const int gzr = ngz + pml;  // receiver row of this column
     const T str_cur = nstr;
     const int sl_cur = nsl;
-    fetch(x + 1);
+    fetch(x + dx);
     const T av = k.dt * v;
     const T vt2 = av * av;
     const T vvzz = vt2 * k.idz2;
@@ -572,10 +590,18 @@ __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot
         if (edge_dist(x, z, s.NX, NZ) == pml) SEIS_GL(s.raw)[i] = ua;
       }
     }
+    if (rev) {
 #pragma unroll
-    for (int j = 0; j < 8; j++) {
-      qa[j] = qa[j + 1];
-      if (TWO) qb[j] = qb[j + 1];
+      for (int j = 8; j > 0; j--) {
+        qa[j] = qa[j - 1];
+        if (TWO) qb[j] = qb[j - 1];
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        qa[j] = qa[j + 1];
+        if (TWO) qb[j] = qb[j + 1];
+      }
     }
   }
   if (MODE == SNAP_MAX) {
@@ -613,7 +639,7 @@ __device__ __forceinline__ void step_block(const Consts<T>& k, const DevShot<T>*
     const int z0 = q.za + zc * 64;
     const int x0 = q.xa + xs * L.bx;
     const int x1 = min(x0 + L.bx, q.xb);
-    interior_sweep<T, MODE>(k, s, c, z0, q.zb, x0, x1, lds[wave]);
+    interior_sweep<T, MODE>(k, s, c, z0, q.zb, x0, x1, (xs & 1) != 0, lds[wave]);
     return;
   }
   const long long j = (long long)(bx - L.nfwg) * WG + threadIdx.x;
@@ -787,20 +813,29 @@ __global__ __launch_bounds__(FQ_WG) void image_kernel(Consts<float> k, const Dev
 
 // Reverse step `step` of every shot (the step kernel's work-groups) fused
 // with the imaging pass of reverse steps step-2 and step-1 (the last nimg
-// work-groups: tile t % ntile of interior column t / ntile).  The
-// two touch disjoint buffers: the pass reads U of steps step-2, step-1 (NAUX =
-// 4, so step writes another U buffer) and the receiver levels of those steps
-// (step reads them as its previous and current level and writes the third
-// slot), and writes only the image.  The imaging pass is bound by its
-// floating-point work and the step by memory, so in one launch each fills the
-// other's idle time.  The step's work-groups (sweep, then ring cells) come
-// first, so that the sweep keeps its XCD mapping, then the imaging pass's.
-// (Measured on the 26-shot survey launch at nt = 1000: 210.2 Gcell-updates/s
-// in this order, 209.2 with the imaging pass first, 182.8 with the two dealt
-// alternately to each XCD over the same columns.)
-__global__ __launch_bounds__(WG) void rtm_reverse_image_kernel(Consts<float> k, const DevShot<float>* __restrict__ shots,
-                                                               Layout L, int step, HilbertTaps taps, int ntile,
-                                                               int nimg) {
+// work-groups: tile t % ntile of interior column t / ntile).  The two touch
+// disjoint buffers: the pass reads U of steps step-2, step-1 (NAUX = 4, so
+// step writes another U buffer) and the receiver levels of those steps (step
+// reads them as its previous and current level and writes the third slot),
+// and writes only the image.  The imaging pass is bound by its floating-point
+// work and the step by memory, so in one launch each fills the other's idle
+// time:
+//  * the step's work-groups (sweep, then ring cells) come first, so that the
+//    sweep keeps its XCD mapping, then the imaging pass's;
+//  * the kernel is held to at most 80 VGPRs (6 waves per SIMD, as many as its
+//    LDS allows) so that the step keeps the waves it needs to hide memory
+//    latency: at 96 VGPRs (5 waves), where the compiler put it unbounded, the
+//    launch took 751 us, at 6 waves 693 us on the 26-shot survey launch at
+//    strip 16;
+//  * the step's waves issue ahead of the imaging waves that share their SIMD
+//    (686 us).
+// (Also measured there: the imaging work-groups spread evenly through the
+// sweep's on every XCD, each part keeping its own XCD mapping, so that every
+// XCD images the columns it sweeps: 790-816 us, the step then has too few
+// waves on each CU.)
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(6, 8))) void rtm_reverse_image_kernel(
+    Consts<float> k, const DevShot<float>* __restrict__ shots, Layout L, int step, HilbertTaps taps, int ntile,
+    int nimg) {
   static_assert(FQ_WG == WG, "one work-group size for both parts");
   static_assert(sizeof(ImageLds) >= sizeof(float) * WAVES * 2 * LROW, "the step's LDS rows fit");
   __shared__ __align__(16) unsigned char sm[sizeof(ImageLds)];
@@ -812,6 +847,7 @@ __global__ __launch_bounds__(WG) void rtm_reverse_image_kernel(Consts<float> k, 
                               *reinterpret_cast<ImageLds*>(sm));
     return;
   }
+  __builtin_amdgcn_s_setprio(3);
   step_block<float, RTM_BWD>(k, shots, L, step, b, blockIdx.y, reinterpret_cast<float (*)[2 * LROW]>(sm));
 }
 
