@@ -191,6 +191,18 @@ __device__ __forceinline__ int strip_slots(int x, int z, int nx, int nz, int pml
   return n;
 }
 
+// Stores v at every slot strip_slots lists for model cell (ix, iz), without
+// the slot list: one predicated store per strip, the side strips' conditions
+// depending on the column only.
+template <typename T, class Ptr>
+__device__ __forceinline__ void store_strip_cell(Ptr st, int ix, int iz, int nx, int nz, T v) {
+  if (ix < 0 || ix >= nx || iz < 0 || iz >= nz) return;
+  if (iz < 4) st[ix * 4 + iz] = v;
+  if (iz >= nz - 4) st[nx * 4 + ix * 4 + (nz - 1 - iz)] = v;
+  if (ix < 4) st[8 * nx + ix * nz + iz] = v;
+  if (ix >= nx - 4) st[8 * nx + 4 * nz + (nx - 1 - ix) * nz + iz] = v;
+}
+
 // The first slot strip_slots would return, or -1 (no private array).
 __device__ __forceinline__ int first_strip_slot(int x, int z, int nx, int nz, int pml) {
   const int ix = x - pml, iz = z - pml;

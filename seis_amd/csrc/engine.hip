@@ -334,8 +334,8 @@ __device__ __forceinline__ T d2_z(const Consts<T>& k, const T* r) {
 }
 
 // Interior sweep of one wavefront: z = z0 + lane (cells with z >= zend idle),
-// over columns [x0, x1), from x0 up, or from x1-1 down when `rev` (uniform
-// across the wavefront).  The current level of each field is held in a 9-deep
+// over columns [x0, x1), from x0 up, or from x1-1 down when REV (a template
+// parameter).  The current level of each field is held in a 9-deep
 // register queue in x order (qa[j] = column x-4+j) whichever way the sweep
 // runs, so the stencil sums keep their order; the centre row is exchanged
 // through LDS for the z derivative.  Each cell's value is final (no absorbing
@@ -343,9 +343,10 @@ __device__ __forceinline__ T d2_z(const Consts<T>& k, const T* r) {
 // columns two strips share are read by both at about the same time (both at
 // the start or both at the end of their sweeps) and the second read finds
 // them in L2.
-template <typename T, int MODE>
+template <typename T, int MODE, bool REV>
 __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot<T>& s, const StepCtx<T>& c, int z0,
-                                               int zend, int x0, int x1, bool rev, T* lds) {
+                                               int zend, int x0, int x1, T* lds) {
+  constexpr bool rev = REV;
   constexpr bool TWO = (MODE == BORN || MODE == RTM_BWD || MODE == PW_FWD);
   const int lane = threadIdx.x & 63;
   const int z = z0 + lane;
@@ -405,6 +406,12 @@ __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot
   const auto STR = SEIS_GL(s.strips + (restore ? (long long)(c.itp - 1) * 8 * (s.nx + k.nz) : 0));
   T nstr = T(0);
   int nsl = -1;
+  // first_strip_slot split into this lane's row part and the column's part
+  // (uniform across the wavefront): a top or bottom strip row gives slot
+  // ix 4 + zst, any other model row the side strips' column base + its row
+  const int izs = zl - pml;
+  const bool zst_in = act && izs >= 0 && izs < k.nz;
+  const int zst = !zst_in ? -2 : izs < 4 ? izs : izs >= k.nz - 4 ? 4 * s.nx + (k.nz - 1 - izs) : -1;
   const bool acc = MODE == ILLUM && c.it > s.ndel && act;  // accumulate u^2 (interior cells are model cells)
   const auto SSA = SEIS_GL(s.img);
   auto fetch = [&](int xx) {
@@ -428,7 +435,12 @@ __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot
       if (MODE == BORN_S) nsn = snap_fetch(s, k.nz, c.it, j);
     }
     if (restore) {
-      nsl = act ? first_strip_slot(xx, zl, s.nx, k.nz, pml) : -1;
+      const int ix = xx - pml;
+      const int side = ix < 0 || ix >= s.nx ? -1
+                       : ix < 4            ? 8 * s.nx + ix * k.nz
+                       : ix >= s.nx - 4    ? 8 * s.nx + 4 * k.nz + (s.nx - 1 - ix) * k.nz
+                                           : -1;
+      nsl = (zst == -2 || ix < 0 || ix >= s.nx) ? -1 : zst >= 0 ? ix * 4 + zst : side >= 0 ? side + izs : -1;
       nstr = STR[max(nsl, 0)];
     }
     if (MODE == ADJ_X) {
@@ -456,7 +468,7 @@ __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot
   // SNAP_MAX: this lane's max |S| over the sweep; one atomic per wavefront
   // after it (non-negative floats order like their bit patterns)
   float smax = 0.f;
-  for (int x = xs0, n = x1 - x0; n > 0; x += dx, n--) {
+  auto column = [&](const int x) {
     const long long row = (long long)x * ld;
     if (rev) {
       qa[0] = nca;
@@ -544,10 +556,7 @@ __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot
         } else if (MODE == MODEL) {
           if (c.itp >= 0 && z == gzr) REC[(long long)ix * s.nt + c.itp] = ua;
         } else if (c.itp >= 0) {
-          int sl[4];
-          const int n = strip_slots(x, z, s.nx, k.nz, pml, sl);
-          const auto st = SEIS_GL(s.strips + (long long)c.itp * 8 * (s.nx + k.nz));
-          for (int q = 0; q < n; q++) st[sl[q]] = ua;
+          store_strip_cell(SEIS_GL(s.strips + (long long)c.itp * 8 * (s.nx + k.nz)), ix, z - pml, s.nx, k.nz, ua);
         }
       } else if (MODE == BORN) {
         const T d = dv_cur;
@@ -562,12 +571,9 @@ __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot
       } else if (MODE == PW_BACK) {
         if (rec_t >= 0 && z == gzr) ua = ua + rec_cur * vt2;
         NA[i] = ua;
-        if (c.it >= 1) {
-          int sl[4];
-          const int n = strip_slots(x, z, s.nx, k.nz, pml, sl);
-          const auto st = SEIS_GL(s.strips + (long long)(c.it - 1) * 8 * (s.nx + k.nz));
-          for (int q = 0; q < n; q++) st[sl[q]] = ua;
-        }
+        if (c.it >= 1)
+          store_strip_cell(SEIS_GL(s.strips + (long long)(c.it - 1) * 8 * (s.nx + k.nz)), ix, z - pml, s.nx, k.nz,
+                           ua);
       } else if (MODE == PW_FWD) {
         const T lap = (ua + pa - T(2) * qa[4]) * k.idt2;
         const T v3 = T(2) / (v * v * v);
@@ -603,7 +609,8 @@ __device__ __forceinline__ void interior_sweep(const Consts<T>& k, const DevShot
         if (TWO) qb[j] = qb[j + 1];
       }
     }
-  }
+  };
+  for (int x = xs0, n = x1 - x0; n > 0; x += dx, n--) column(x);
   if (MODE == SNAP_MAX) {
     for (int o = 32; o >= 1; o >>= 1) smax = fmaxf(smax, __shfl_xor(smax, o, 64));
     if (lane == 0) atomicMax(reinterpret_cast<unsigned int*>(s.scale) + c.it, __float_as_uint(smax));
@@ -639,7 +646,12 @@ __device__ __forceinline__ void step_block(const Consts<T>& k, const DevShot<T>*
     const int z0 = q.za + zc * 64;
     const int x0 = q.xa + xs * L.bx;
     const int x1 = min(x0 + L.bx, q.xb);
-    interior_sweep<T, MODE>(k, s, c, z0, q.zb, x0, x1, (xs & 1) != 0, lds[wave]);
+    // the direction is a compile-time choice: a run-time one turns every
+    // shift of the register queues into selects
+    if (xs & 1)
+      interior_sweep<T, MODE, true>(k, s, c, z0, q.zb, x0, x1, lds[wave]);
+    else
+      interior_sweep<T, MODE, false>(k, s, c, z0, q.zb, x0, x1, lds[wave]);
     return;
   }
   const long long j = (long long)(bx - L.nfwg) * WG + threadIdx.x;
