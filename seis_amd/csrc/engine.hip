@@ -816,22 +816,24 @@ __device__ __forceinline__ void image_block(const Consts<float>& k, const DevSho
   }
 }
 
+// Grid: (row tiles, interior columns c0 .. c0 + gridDim.y - 1, shots).
 template <int HALF>
 __global__ __launch_bounds__(FQ_WG) void image_kernel(Consts<float> k, const DevShot<float>* __restrict__ shots,
-                                                      int step, HilbertTaps taps, int pair, int plane) {
+                                                      int step, HilbertTaps taps, int pair, int plane, int c0) {
   __shared__ __align__(16) ImageLds sm;
-  image_block<HALF>(k, shots, step, taps, pair, plane, blockIdx.x, blockIdx.y, blockIdx.z, sm);
+  image_block<HALF>(k, shots, step, taps, pair, plane, blockIdx.x, c0 + blockIdx.y, blockIdx.z, sm);
 }
 
 // Reverse step `step` of every shot (the step kernel's work-groups) fused
-// with the imaging pass of reverse steps step-2 and step-1 (the last nimg
-// work-groups: tile t % ntile of interior column t / ntile).  The two touch
-// disjoint buffers: the pass reads U of steps step-2, step-1 (NAUX = 4, so
-// step writes another U buffer) and the receiver levels of those steps (step
-// reads them as its previous and current level and writes the third slot),
-// and writes only the image.  The imaging pass is bound by its floating-point
-// work and the step by memory, so in one launch each fills the other's idle
-// time:
+// with the imaging pass of reverse steps pfirst and, when `pair`, pfirst + 1
+// over interior columns c0 .. c0 + nimg / ntile - 1 (the last nimg
+// work-groups: tile t % ntile of column c0 + t / ntile), where pfirst + pair
+// = step - 1.  The two touch disjoint buffers: the pass reads U of those
+// steps (NAUX = 4, so step writes another U buffer) and their receiver levels
+// (step reads them as its previous and current level and writes the third
+// slot), and writes only the image.  The imaging pass is bound by its
+// floating-point work and the step by memory, so in one launch each fills the
+// other's idle time:
 //  * the step's work-groups (sweep, then ring cells) come first, so that the
 //    sweep keeps its XCD mapping, then the imaging pass's;
 //  * the kernel is held to at most 80 VGPRs (6 waves per SIMD, as many as its
@@ -847,7 +849,7 @@ __global__ __launch_bounds__(FQ_WG) void image_kernel(Consts<float> k, const Dev
 // waves on each CU.)
 __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(6, 8))) void rtm_reverse_image_kernel(
     Consts<float> k, const DevShot<float>* __restrict__ shots, Layout L, int step, HilbertTaps taps, int ntile,
-    int nimg) {
+    int nimg, int pfirst, int pair, int c0) {
   static_assert(FQ_WG == WG, "one work-group size for both parts");
   static_assert(sizeof(ImageLds) >= sizeof(float) * WAVES * 2 * LROW, "the step's LDS rows fit");
   __shared__ __align__(16) unsigned char sm[sizeof(ImageLds)];
@@ -855,7 +857,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(6, 8))) void
   const int b = blockIdx.x;
   if (b >= nstep) {
     const int t = b - nstep;
-    image_block<HILBERT_HALF>(k, shots, step - 2, taps, 1, 0, t % ntile, t / ntile, blockIdx.y,
+    image_block<HILBERT_HALF>(k, shots, pfirst, taps, pair, 0, t % ntile, c0 + t / ntile, blockIdx.y,
                               *reinterpret_cast<ImageLds*>(sm));
     return;
   }
@@ -1255,43 +1257,68 @@ int run_chunk(int mode, const GridSpec& g, int n, const ShotSpec* shots, T* cons
     constexpr int half = HILBERT_HALF;
     HilbertTaps taps{};
     hilbert_taps_host<float>(half, taps.h);
-    // Imaging pass q covers reverse steps 2q and 2q+1.  It runs inside the
-    // launch of reverse step 2q+2 (rtm_reverse_image_kernel), while their U
-    // buffers and receiver levels are intact; the last pair (or a last single
-    // step) in a launch of its own after the loop.  (As a separate launch on
-    // the same stream after step 2q+1 the pass cost 0.40 ms per pair on the
-    // 26-shot survey launch; on a second stream beside the next reverse step
-    // it measured no faster than that.)
+    // The imaging pass runs inside the reverse launches
+    // (rtm_reverse_image_kernel), split by interior columns into two halves:
+    // launch st >= 2 images reverse steps st-2 and st-1 on half st % 2, and
+    // launch 1 images step 0 alone on half 1, each while those steps' U
+    // buffers and receiver levels are intact.  So half 0 takes the steps in
+    // pairs (0, 1), (2, 3), ... and half 1 takes step 0, then (1, 2), (3, 4),
+    // ...; every column is imaged step by step in the reference's order, its
+    // image read and written once per pair, and every launch carries the same
+    // share of the pass.  The steps a half has left after the loop (at most
+    // two) are imaged by image_kernel.  (With the whole pass of a pair in
+    // every second launch, the launches alternated between bound by memory
+    // and bound by the pass's floating-point work: 0.590 against 0.565 ms per
+    // reverse step on the 26-shot survey launch.  As a separate launch the
+    // pass cost 0.40 ms per pair there; on a second stream beside the next
+    // reverse step it measured no faster than that.)
     const int ntile = image_tiles(NZ, g.pml);
-    const int nimg = ntile * nxmax;
-    // per step U and R over the interior columns; the image once per pair
-    auto image_bytes = [&](int first, int pair) {
-      double fq0, mc0, fq1 = 0;
-      cells_bwd(first, nullptr, &fq0, &mc0);
-      if (pair) cells_bwd(first + 1, nullptr, &fq1, nullptr);
-      return 2 * E * (fq0 + fq1) + 2 * E * mc0;
+    const int hc = (nxmax + 1) / 2;  // columns of half 0; half 1 has the rest
+    auto half_c0 = [&](int h) { return h == 0 ? 0 : hc; };
+    auto half_cols = [&](int h) { return h == 0 ? hc : nxmax - hc; };
+    // U and R of each imaged step over the half's interior columns, and the
+    // half's image once
+    auto image_bytes = [&](int first, int pair, int h) {
+      double b = 0;
+      for (int s = 0; s < n; s++) {
+        const int w = NXs[s] - 2 * g.pml;
+        const int cols = std::max(0, std::min(w, half_c0(h) + half_cols(h)) - half_c0(h));
+        const int steps = (first < shots[s].nt) + (pair && first + 1 < shots[s].nt);
+        if (steps > 0) b += 2 * E * (double)cols * (steps * (double)NZ + g.nz);
+      }
+      return b;
     };
     for (int st = 0; st < maxbwd; st++) {
       double sb;
       const double a = cells_bwd(st, &sb, nullptr, nullptr);
-      const bool fused = st >= 2 && st % 2 == 0;
-      launch(quiet, stream, "step_rtm_reverse", 8 * E * a + sb + (fused ? image_bytes(st - 2, 1) : 0.0), 2 * a, [&] {
-        if (!fused) {
-          launch_step<T>(RTM_BWD, k, dev, L, B, n, st, stream);
-          return;
-        }
-        hipLaunchKernelGGL(rtm_reverse_image_kernel, dim3(nimg + L.nfwg + L.nbwg, n), dim3(WG), 0, stream, k, dev, L,
-                           st, taps, ntile, nimg);
-        if (B.nbwg > 0)
-          hipLaunchKernelGGL((band_kernel<T, RTM_BWD>), dim3((unsigned)B.nbwg, n), dim3(WG), 0, stream, k, dev, st);
-      });
+      const int h = st % 2, pfirst = st >= 2 ? st - 2 : 0, pair = st >= 2;
+      const bool fused = st >= 1 && half_cols(h) > 0;
+      launch(quiet, stream, "step_rtm_reverse", 8 * E * a + sb + (fused ? image_bytes(pfirst, pair, h) : 0.0), 2 * a,
+             [&] {
+               if (!fused) {
+                 launch_step<T>(RTM_BWD, k, dev, L, B, n, st, stream);
+                 return;
+               }
+               const int nimg = ntile * half_cols(h);
+               hipLaunchKernelGGL(rtm_reverse_image_kernel, dim3(nimg + L.nfwg + L.nbwg, n), dim3(WG), 0, stream, k,
+                                  dev, L, st, taps, ntile, nimg, pfirst, pair, half_c0(h));
+               if (B.nbwg > 0)
+                 hipLaunchKernelGGL((band_kernel<T, RTM_BWD>), dim3((unsigned)B.nbwg, n), dim3(WG), 0, stream, k, dev,
+                                    st);
+             });
     }
-    if (maxbwd > 0) {
-      const int first = (maxbwd - 1) / 2 * 2;
-      const int pair = maxbwd - 1 > first;
-      launch(quiet, stream, "rtm_image", image_bytes(first, pair), 0, [&] {
-        hipLaunchKernelGGL((image_kernel<half>), dim3((unsigned)ntile, (unsigned)nxmax, (unsigned)n), dim3(FQ_WG), 0,
-                           stream, k, dev, first, taps, pair, 0);
+    for (int h = 0; h < 2; h++) {
+      if (half_cols(h) == 0) continue;
+      // steps [0, done) of half h are imaged: by the last launch of parity h
+      // in the loop (from launch 2 for half 0, from launch 1 for half 1)
+      int last = maxbwd - 1;
+      if (last >= 0 && last % 2 != h) last--;
+      const int done = last >= (h == 0 ? 2 : 1) ? last : 0;
+      const int left = maxbwd - done;  // 0, 1 or 2 steps
+      if (left <= 0) continue;
+      launch(quiet, stream, "rtm_image", image_bytes(done, left == 2, h), 0, [&] {
+        hipLaunchKernelGGL((image_kernel<half>), dim3((unsigned)ntile, (unsigned)half_cols(h), (unsigned)n),
+                           dim3(FQ_WG), 0, stream, k, dev, done, taps, left == 2 ? 1 : 0, 0, half_c0(h));
       });
     }
     hipLaunchKernelGGL((crop_kernel<T>), dim3(64, n), dim3(256), 0, stream, k, dev);
@@ -1466,7 +1493,7 @@ int run_rtm_plane_chunk(const GridSpec& g, int n, const ShotSpec* shots, T* cons
       launch_step<T>(PW_FWD, k, dev, L, B, n, st, stream);
     });
     timed_launch(stream, "plane_image", 2 * E * a * faqi_cells + 2 * E * a * (double)g.nx * g.nz, 0, [&] {
-      hipLaunchKernelGGL((image_kernel<half>), igrid, dim3(FQ_WG), 0, stream, k, dev, st, taps, 0, 1);
+      hipLaunchKernelGGL((image_kernel<half>), igrid, dim3(FQ_WG), 0, stream, k, dev, st, taps, 0, 1, 0);
     });
     hipLaunchKernelGGL((restore_plane_strips_kernel<T>), dim3(16, n), dim3(256), 0, stream, k, dev, st);
   }
