@@ -515,7 +515,7 @@ def snapshot_vs_recompute(nshots, nt, seed=1007):
             "image_finite": bool(np.isfinite(img).all())}
 
 
-def lsrtm_bench(iters, err_shots, plain_iters, xcorr_iters):
+def lsrtm_bench(iters, err_shots, plain_iters, xcorr_iters, single_shots=16):
     """Least-squares RTM on the 376 x 1151 @ 8 m workload (the RTM model at
     half resolution, nt = 3000, dt = 0.8 ms): all 128 shots, 400 receivers
     each (every column from sx - 200 to sx + 199), each shot on its window
@@ -524,7 +524,9 @@ def lsrtm_bench(iters, err_shots, plain_iters, xcorr_iters):
     exact adjoint for `iters` iterations, the same with the cross-correlation
     adjoint, the plain FP32 GPU path (per iteration one Born batch and one RTM
     batch of the same shots, as the reference's forward_MPI + adjoint_MPI),
-    and, on `err_shots` of the shots, the image of every iteration with
+    the same operators one shot per call (the plain FP32 single-shot GPU path,
+    on `single_shots` of the shots, scaled to all), and, on `err_shots` of the
+    shots, the image of every iteration with
     FP32 / BF16 / FP16 snapshots against the FP64 plan."""
     import seis_amd
     from seis_amd import DeviceArray, models
@@ -632,6 +634,20 @@ def lsrtm_bench(iters, err_shots, plain_iters, xcorr_iters):
                                   RTM_ISOLATE_BAND))
     seis_amd.synchronize()
     plain = (time.perf_counter() - t0) / plain_iters
+    # the plain FP32 single-shot GPU path: the same Born and RTM operators with
+    # one shot per call, shot after shot (one iteration's work for
+    # `single_shots` shots spread over the survey, scaled to all of them)
+    single = None
+    if single_shots > 0:
+        idx = list(range(0, ns, max(1, ns // single_shots)))[:single_shots]
+        t0 = time.perf_counter()
+        for i in idx:
+            one = sub_array(arr, [i])
+            check(L.seis_born_batch(ctypes.byref(g), 1, one, d_v0.ptr, d_pert.ptr, d_w.ptr, d_g.ptr, d_rec.ptr))
+            check(L.seis_rtm_batch_ex(ctypes.byref(g), 1, one, d_v0.ptr, d_w.ptr, d_g.ptr, d_obs.ptr, d_img.ptr,
+                                      RTM_ISOLATE_BAND))
+        seis_amd.synchronize()
+        single = (time.perf_counter() - t0) * ns / len(idx)
     del d_img, d_rec, d_pert, d_obs
     out["s_per_iter_plain_fp32"] = round(plain, 3)
     out["plain_iterations_timed"] = plain_iters
@@ -640,6 +656,12 @@ def lsrtm_bench(iters, err_shots, plain_iters, xcorr_iters):
     if "s_per_cg_iter_bf16_xcorr" in out:
         out["speedup_vs_plain_fp32_xcorr"] = round(plain / out["s_per_cg_iter_bf16_xcorr"], 2)
     log("lsrtm: plain FP32 path, %.2f s per iteration (Born + RTM batches)" % plain)
+    if single is not None:
+        out["s_per_iter_plain_fp32_single_shot"] = round(single, 3)
+        out["plain_single_shot_sample"] = "%d of the %d shots, one Born and one RTM call each, scaled to all" % (
+            len(idx), ns)
+        out["speedup_vs_plain_fp32_single_shot"] = round(single / out["s_per_cg_iter_bf16"], 2)
+        log("lsrtm: plain FP32 single-shot path, %.2f s per iteration" % single)
     # --- image error per iteration against the FP64 plan, on a subset
     if err_shots > 0:
         esx = all_sx[4::128 // err_shots][:err_shots]
@@ -690,6 +712,8 @@ def main():
     ap.add_argument("--lsrtm-xcorr-iters", type=int, default=20)
     ap.add_argument("--lsrtm-plain-iters", type=int, default=1)
     ap.add_argument("--lsrtm-err-shots", type=int, default=8)
+    ap.add_argument("--lsrtm-single-shots", type=int, default=16,
+                    help="shots of the plain FP32 single-shot timing (scaled to all 128; 0 skips it)")
     ap.add_argument("--c1-calls", type=int, default=10, help="timed calls of the C1 single-shot case (0 skips it)")
     ap.add_argument("--c2", type=int, default=1, help="run the 96-shot RTM batch on the 751 x 2301 grid (0 skips)")
     ap.add_argument("--c3-shots", type=int, default=256, help="shots of the C3 Born batch (0 skips it)")
@@ -829,7 +853,7 @@ def main():
         out["c3_born_adjoint"] = c3_born_adjoint(args.c3_shots, args.c3_pair_shots)
     if args.lsrtm_iters > 0:
         out["lsrtm"] = lsrtm_bench(args.lsrtm_iters, args.lsrtm_err_shots, args.lsrtm_plain_iters,
-                                   args.lsrtm_xcorr_iters)
+                                   args.lsrtm_xcorr_iters, args.lsrtm_single_shots)
     if args.c1_calls > 0:
         log("C1 single-shot modeling, %d calls per mode" % args.c1_calls)
         out["c1_single_shot"] = c1_single_shot(args.c1_calls, not args.no_cpu, threads)
