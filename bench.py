@@ -45,14 +45,14 @@ METRIC = "Gcell-updates/s (batched shots) + LSRTM s/CG-iter vs plain FP32 at sta
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the headline's kernels (26 shots per launch, strip
 # length 16, the tuner's usual choice on this launch), from rocprofv3 FETCH_SIZE
-# / WRITE_SIZE calibrated against a 1 GiB copy and, equal to them, the
-# memory-side 32-byte request counters; the commands and counter values are
-# in profiles/c5_rtm_hbm_traffic.md.  step_rtm_reverse is the mean over the
-# reverse launches, every second of which also runs the imaging pass of the two
-# steps before it.  Scaled by the shots of a launch.
+# / WRITE_SIZE calibrated against a 1 GiB copy; the commands and counter values
+# are in profiles/c5_rtm_hbm_traffic.md.  step_rtm_reverse is the mean over the
+# reverse launches (each but the first also images half of the interior
+# columns), rtm_image the mean of the two imaging launches after the loop.
+# Scaled by the shots of a launch.
 MEASURED_TRAFFIC = {"shots": 26, "strip_length": 16,
-                    "bytes": {"step_rtm_reverse": 3350472666, "step_rtm_forward": 1328994835,
-                              "rtm_image": 1449277885}}
+                    "bytes": {"step_rtm_reverse": 3096269192, "step_rtm_forward": 1203995091,
+                              "rtm_image": 601327296}}
 
 
 def host_cpu_model():
