@@ -495,9 +495,12 @@ void born(T dt, T dx, T dz, int nt, int ndel, int nx, int nz, int pml, int sx, i
 //     receivers; image += lap*2/v^3 * R over the padded grid, plus the
 //     Hilbert term 0.5*(U R - H[U] H[R]) over the interior (FaQi).
 // img is [nx][nz].
+// `isolate`: the engine's SEIS_RTM_ISOLATE_BAND variant (not in the
+// reference), which holds the reconstructed source field at zero outside the
+// model area: each of its backward levels is zeroed there once it is formed.
 template <typename T>
 void rtm(T dt, T dx, T dz, int nt, int ndel, int nx, int nz, int pml, int sx, int sz, const int* gz,
-         const T* wav, const T* v, const T* rec, T* img) {
+         const T* wav, const T* v, const T* rec, T* img, bool isolate = false) {
   const int layer = 4, half = 30;
   Geom g(nx, nz, pml);
   Coefs<T> k(g, dt, dx, dz, v, true);
@@ -532,6 +535,10 @@ void rtm(T dt, T dx, T dz, int nt, int ndel, int nx, int nz, int pml, int sx, in
     stencil_step(g, u2, u1, u0, k.vvzz.data(), k.vvxx.data());
     habc_update(g, dt, dx, dz, k.vel.data(), ow.data(), u0, u1, u2);
     blend(g, u0, ow.data(), k.w.data());
+    if (isolate)
+      for (int ix = 0; ix < g.NX; ix++)
+        for (int iz = 0; iz < g.NZ; iz++)
+          if (ix < pml || ix >= pml + nx || iz < pml || iz >= pml + nz) u0[g.at(ix, iz)] = T(0);
     for (size_t i = 0; i < g.cells(); i++) lap[i] = (u2[i] + u0[i] - T(2) * u1[i]) * idt2;
     // receiver field
     stencil_step(g, r2, r1, r0, k.vvzz.data(), k.vvxx.data());
@@ -1047,6 +1054,12 @@ int oracle_get_threads(void) { return g_threads; }
   int oracle_rtm_##SUF(T dt, T dx, T dz, int nt, int ndel, int nx, int nz, int pml, int sx, int sz,   \
                        const int* gz, const T* wav, const T* vel, const T* rec, T* img) {            \
     rtm<T>(dt, dx, dz, nt, ndel, nx, nz, pml, sx, sz, gz, wav, vel, rec, img);                        \
+    return 0;                                                                                         \
+  }                                                                                                   \
+  int oracle_rtm_isolated_##SUF(T dt, T dx, T dz, int nt, int ndel, int nx, int nz, int pml, int sx,  \
+                                int sz, const int* gz, const T* wav, const T* vel, const T* rec,      \
+                                T* img) {                                                             \
+    rtm<T>(dt, dx, dz, nt, ndel, nx, nz, pml, sx, sz, gz, wav, vel, rec, img, true);                  \
     return 0;                                                                                         \
   }                                                                                                   \
   int oracle_born_adjoint_xcorr_##SUF(T dt, T dx, T dz, int nt, int ndel, int nx, int nz, int pml, int sx, \

@@ -286,20 +286,25 @@ __global__ __launch_bounds__(WG) void band_kernel(Consts<T> k, const DevShot<T>*
   const T* RA = s.raw;
   const T* RB = s.raw + s.fstride;
   if (MODE == RTM_BWD) {
-    T sub = T(0);
-    const bool on = it - 1 > 0;
-    if (on) {
-      const T as = k.dt * m.vel(s.sx + pml, s.sz + pml);
-      sub = (as * as) * s.wav[it - 1];
+    if (s.rtm_flags & SEIS_RTM_ISOLATE_BAND) {
+      // the source field stays zero throughout the band (the source cell
+      // lies in the model), so its new value and U there are +0 exactly
+      N[i] = T(0);
+      s.aux[(step % NAUX) * s.fstride + i] = T(0);
+    } else {
+      T sub = T(0);
+      const bool on = it - 1 > 0;
+      if (on) {
+        const T as = k.dt * m.vel(s.sx + pml, s.sz + pml);
+        sub = (as * as) * s.wav[it - 1];
+      }
+      const SourceRemoval<T> sr{fidx(ld, s.sx + pml, s.sz + pml), sub, on};
+      const T u = band_blend(k, m, NX, C, P, sr, x, z, RA[i], RawNeighbour<T>{RA, N, ld, NX, NZ, pml + 1});
+      const T lap = (sr.prev(P, i, x, z) + u - T(2) * C[i]) * k.idt2;
+      const T v = m.vel(x, z);
+      s.aux[(step % NAUX) * s.fstride + i] = lap * (T(2) / (v * v * v));
+      N[i] = u;
     }
-    const SourceRemoval<T> sr{fidx(ld, s.sx + pml, s.sz + pml), sub, on};
-    T u = T(0);
-    if (!(s.rtm_flags & SEIS_RTM_ISOLATE_BAND))
-      u = band_blend(k, m, NX, C, P, sr, x, z, RA[i], RawNeighbour<T>{RA, N, ld, NX, NZ, pml + 1});
-    const T lap = (sr.prev(P, i, x, z) + u - T(2) * C[i]) * k.idt2;
-    const T v = m.vel(x, z);
-    s.aux[(step % NAUX) * s.fstride + i] = lap * (T(2) / (v * v * v));
-    N[i] = u;
     N[f3 + i] = band_blend(k, m, NX, C + f3, P + f3, NoInject{}, x, z, RB[i],
                            RawNeighbour<T>{RB, N + f3, ld, NX, NZ, pml});
   } else if (MODE == PW_FWD) {

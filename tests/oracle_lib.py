@@ -76,8 +76,11 @@ class _Lib:
         f(*args, _ptr(wav), _ptr(vel), _ptr(dv), _ptr(rec))
         return rec
 
-    def rtm(self, dt, dx, dz, nt, ndel, nx, nz, pml, sx, sz, gz, wav, vel, rec, dtype=np.float32):
-        f, args, keep = self._geom_call("rtm", dtype, dt, dx, dz, nt, ndel, nx, nz, pml, sx, sz, gz)
+    def rtm(self, dt, dx, dz, nt, ndel, nx, nz, pml, sx, sz, gz, wav, vel, rec, dtype=np.float32, isolate=False):
+        """isolate: the engine's isolated-band variant (source field held at
+        zero outside the model area); the oracle library only."""
+        f, args, keep = self._geom_call("rtm_isolated" if isolate else "rtm", dtype, dt, dx, dz, nt, ndel, nx, nz,
+                                        pml, sx, sz, gz)
         wav = np.ascontiguousarray(wav, dtype)
         vel = np.ascontiguousarray(vel, dtype)
         rec = np.ascontiguousarray(rec, dtype)

@@ -104,7 +104,8 @@ def test_rtm_isolated_band_equals_reference_interior(oracle):
     the absorbing band.  The 4-layer strips shield the interior wavefields, so
     the image can differ from the reference's only through the strip cells and
     the 61-tap column Hilbert filter: nowhere beyond 4 + 30 cells of the model
-    edge, where it must be bit-identical."""
+    edge, where it must be bit-identical; the whole image is bit-identical
+    to the oracle's isolated-band RTM."""
     from seis_amd import ricker
     from seis_amd.batch import Grid, Shot, rtm_batch
     print("case rtm_isolated_band 100x90 nt=400", flush=True)
@@ -122,6 +123,9 @@ def test_rtm_isolated_band_equals_reference_interior(oracle):
     assert np.isfinite(img).all()
     assert np.array_equal(img[m:nx - m, m:nz - m], ref[m:nx - m, m:nz - m])
     assert not np.array_equal(img, ref)  # the edge rows do see the change
+    # the whole image, edges included, against the oracle's isolated variant
+    iso = oracle.rtm(dt, dx, dz, nt, 0, nx, nz, pml, 50, 2, gz, wav, v, rec, isolate=True)
+    assert np.array_equal(img, iso)
 
 
 def test_rtm_odd_record_lengths_match_oracle(engine, oracle):

@@ -120,3 +120,24 @@ def test_pad_velocity_is_clamped_model(oracle):
         for iz in range(v.shape[1] + 2 * pml):
             e = v[min(max(ix - pml, 0), v.shape[0] - 1), min(max(iz - pml, 0), v.shape[1] - 1)]
             assert vp[ix, iz] == e
+
+
+def test_isolated_band_rtm_matches_reference_away_from_edges(oracle):
+    """The oracle's isolated-band RTM (the engine's SEIS_RTM_ISOLATE_BAND
+    variant) zeroes the reconstructed source field outside the model area.
+    The 4-layer strips shield the interior wavefields from that change, so
+    the image equals the reference's RTM beyond 4 + 30 cells of the model
+    edge (strip layer + Hilbert half length) and differs at the edges."""
+    rng = np.random.default_rng(5)
+    nx, nz, pml, nt, dt = 80, 76, 10, 220, 0.001
+    v = (1800.0 + 900.0 * np.linspace(0, 1, nz)[None, :] + 40.0 * rng.standard_normal((nx, nz))).astype(np.float32)
+    gz = np.full(nx, 2, np.int32)
+    wav = oracle.ricker(dt, 20.0, 40, nt)
+    rec = rng.standard_normal((nx, nt)).astype(np.float32)
+    a = (dt, 10.0, 10.0, nt, 0, nx, nz, pml, 40, 2, gz, wav, v, rec)
+    ref = oracle.rtm(*a)
+    iso = oracle.rtm(*a, isolate=True)
+    m = 34
+    assert np.isfinite(iso).all()
+    assert np.array_equal(iso[m:nx - m, m:nz - m], ref[m:nx - m, m:nz - m])
+    assert not np.array_equal(iso, ref)
