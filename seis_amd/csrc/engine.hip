@@ -1182,9 +1182,14 @@ int run_chunk(int mode, const GridSpec& g, int n, const ShotSpec* shots, T* cons
   {
     // strip length: timed on the zeroed fields with this pass's step kernel
     // at step 0 (no injection yet, so the fields stay zero; records and strip
-    // samples written here are rewritten by the real step 0)
+    // samples written here are rewritten by the real step 0).  RTM times a
+    // forward and a reverse step together (the reverse, with two fields, is
+    // most of the time); the reverse trial writes only the new level of each
+    // field (strip samples restored from the not yet written strip store), U
+    // and the band's stencil values, all cleared before the reverse pass or
+    // written in full by the forward pass before they are read.
     struct Trial {
-      int mode;
+      int mode, mode2;
       const Consts<T>* k;
       const DevShot<T>* dev;
       Layout L, B;
@@ -1193,10 +1198,11 @@ int run_chunk(int mode, const GridSpec& g, int n, const ShotSpec* shots, T* cons
     };
     // (an ILLUM trial at step 0 accumulates nothing: step 0 never exceeds ndel)
     const int proxy = mode == RUN_MODEL ? MODEL : mode == RUN_BORN ? BORN : ilm ? ILLUM : RTM_FWD;
-    Trial tr{proxy, &k, dev, L, B, n, stream};
+    Trial tr{proxy, mode == RUN_RTM ? RTM_BWD : -1, &k, dev, L, B, n, stream};
     auto fn = [](void* c, int bx) {
       const Trial* t = static_cast<const Trial*>(c);
       launch_step<T>(t->mode, *t->k, t->dev, with_strip(t->L, bx), t->B, t->nshots, 0, t->s);
+      if (t->mode2 >= 0) launch_step<T>(t->mode2, *t->k, t->dev, with_strip(t->L, bx), t->B, t->nshots, 0, t->s);
     };
     const char* kind = mode == RUN_MODEL ? "forward" : mode == RUN_BORN ? "born" : ilm ? "illum" : "rtm";
     L = with_strip(L, quiet ? strip_lookup(kind, NXmax, NZ, n) : tuned_strip(kind, NXmax, NZ, n, stream, fn, &tr));
