@@ -51,8 +51,8 @@ HBM_PEAK_GBS = 8000.0
 # columns), rtm_image the mean of the two imaging launches after the loop.
 # Scaled by the shots of a launch.
 MEASURED_TRAFFIC = {"shots": 26, "strip_length": 16,
-                    "bytes": {"step_rtm_reverse": 3096269192, "step_rtm_forward": 1203995091,
-                              "rtm_image": 601327296}}
+                    "bytes": {"step_rtm_reverse": 3044062197, "step_rtm_forward": 1204190945,
+                              "rtm_image": 601327872}}
 
 
 def host_cpu_model():
